@@ -1,0 +1,146 @@
+/*
+ * plasticity_amd: MI355X (gfx950) crystal-plasticity Newton step behind a C ABI.
+ *
+ * The reference (PRISMS-Plasticity 1.5.0) has no FFI for this path: its boundary is the C++
+ * class pair ellipticBVP<dim> (include/ellipticBVP.h:26-226) and crystalPlasticity<dim>
+ * (include/crystalPlasticity.h:13-264).  Each entry point below names the reference member it
+ * stands in for.  Conventions:
+ *   - every function returns 0 on success or a PA_ERR_* code; pa_last_error() describes the
+ *     last failure of the calling thread; no C++ exception crosses this boundary;
+ *   - host buffers are caller-owned and only read/written during the call; device memory is
+ *     owned by the context; one context = one HIP stream on the current device;
+ *   - matrices are row-major 3x3 (9 doubles); dP/dF is 81 doubles with
+ *     dPdF[(3i+k)*9 + 3j+l] = dP_ik/dF_jl (the reference's dP_dF[i][k][j][l]);
+ *   - quadrature points are numbered Q = 8*e + q with elements e = ex + nx*(ey + ny*ez) and
+ *     q = qx + 2qy + 4qz of the 2x2x2 Gauss rule; nodes n = ix + (nx+1)*(iy + (ny+1)*iz);
+ *     dof = 3n + component.
+ */
+#ifndef PLASTICITY_AMD_H
+#define PLASTICITY_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PA_MAX_SLIP 48
+
+enum {
+  PA_OK = 0,
+  PA_ERR_ARG = 1,         /* invalid argument / size */
+  PA_ERR_UNSUPPORTED = 2, /* configuration outside what the kernels implement */
+  PA_ERR_HIP = 3,         /* HIP runtime error (no GPU, allocation, launch) */
+  PA_ERR_IO = 4,          /* unreadable input file (the reference exits with status 1) */
+  PA_ERR_STATE = 5        /* call out of order (e.g. no grains set) */
+};
+
+enum { PA_SOLVER_BICG = 0, PA_SOLVER_CG = 1 };
+enum { PA_PREC_FP64 = 0, PA_PREC_MIXED = 1 };
+
+/* Single-phase material tables, as crystalPlasticity<dim>::init() builds them from the prm
+ * inputs (src/materialModels/crystalPlasticity/init.cc:6-196, multiphaseInit.cc:4-67). */
+typedef struct {
+  int n_slip;                            /* Number of Slip Systems */
+  double m[PA_MAX_SLIP][3];              /* Slip Directions File rows (normalised by the library) */
+  double n[PA_MAX_SLIP][3];              /* Slip Normals File rows (normalised by the library) */
+  double q[PA_MAX_SLIP][PA_MAX_SLIP];    /* Latent Hardening Ratio file */
+  double C[6][6];                        /* Elastic Stiffness rows 1-6 (MPa, Voigt 11,22,33,23,13,12) */
+  double s0[PA_MAX_SLIP];                /* Initial Slip Resistance */
+  double h0[PA_MAX_SLIP];                /* Initial Hardening Modulus */
+  double ss[PA_MAX_SLIP];                /* Saturation Stress */
+  double pw[PA_MAX_SLIP];                /* Power Law Exponent */
+  double stress_tol;                     /* Stress Tolerance */
+  int max_slip_search;                   /* Max Slip Search Iterations (must be 1) */
+  int max_solver_iter;                   /* Max Solver Iterations */
+  int kinematic_hardening;               /* Enable Kinematic Hardening (must be 0) */
+  int twinning;                          /* Twinning enabled (must be 0) */
+} pa_material;
+
+/* Mesh, loading and solver inputs of ellipticBVP<dim> (userInputParameters.cc:17-235). */
+typedef struct {
+  int nx, ny, nz;                        /* elements per direction = Subdivisions * 2^Refine factor */
+  double span[3];                        /* Domain size X/Y/Z */
+  int total_increments;                  /* round(Total time / Time increments), ellipticBVP.cc:40 */
+  double delT;                           /* Time increments */
+  int max_newton;                        /* Maximum non linear iterations */
+  double lin_tol;                        /* Relative linear solver tolerance */
+  int lin_maxit;                         /* Maximum linear solver iterations */
+  int face_dof_constrained[6][3];        /* BC file rules (face 1..6 = x-,x+,y-,y+,z-,z+; dof 1..3) */
+  double face_total_u[6][3];             /* FinalDisplacement of each rule */
+  int cyclic;                            /* Enable cyclic loading */
+  int cyclic_face, cyclic_dof;           /* Cyclic loading face / direction (1-based) */
+  double quarter_cycle_time;             /* Quarter cycle time */
+} pa_problem;
+
+typedef struct pa_context pa_context;
+
+/* Last error message of the calling thread (never NULL). */
+const char *pa_last_error(void);
+
+/* Replaces crystalPlasticity<dim>::crystalPlasticity(userInputParameters&)
+ * (src/materialModels/crystalPlasticity/crystalPlasticity.cc:4-76) plus ellipticBVP::mesh()/init()
+ * (src/ellipticBVP/mesh.cc:7-124, init.cc:8-558): allocates per-qp state (Fp = I, s = s0),
+ * builds the 27-point block sparsity on the device. */
+int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out);
+int pa_destroy(pa_context *ctx);
+
+/* Replaces crystalOrientationsIO::getMaterialID over cell centres
+ * (src/utilityObjects/crystalOrientationsIO.cc:115-194, loadOrientations.cc:4-33), on the GPU.
+ * vox: [nv0][nv1][nv2] grain ids, z fastest (the grain-ID file order). */
+int pa_element_grains(pa_context *ctx, const unsigned *vox, const int nv[3], unsigned *elem_grain);
+
+/* Replaces crystalPlasticity::init's orientation load (init.cc:223-245) and
+ * loadOrientationVector (crystalOrientationsIO.cc:11-62): grain id of each element and the
+ * Rodrigues vector of each grain id (rows of 3, indexed by id). */
+int pa_set_grains(pa_context *ctx, const unsigned *elem_grain, const double *grain_rod, int n_grain_rows);
+
+/* Krylov method (BiCG = the reference's PETScWrappers::SolverBiCG, solveLinearSystem.cc:13-16;
+ * CG = Jacobi-PCG) and arithmetic form (FP64, or mixed: FP32 element tangents / matrix values,
+ * FP64 residual, return map and Krylov recurrences). */
+int pa_set_solver(pa_context *ctx, int method, int precision);
+
+/* One Newton iteration of solveNonLinearSystem (src/ellipticBVP/solveNonLinearSystem.cc:14-47):
+ * applyDirichletBCs + assemble (boundaryConditions.cc:452-514, assemble.cc:6-104,
+ * getElementalValues.cc:5-168) then solveLinearSystem (solveLinearSystem.cc:8-36).
+ * res_norm: ||R||_2 of the condensed residual; lin_iters: Krylov iterations. */
+int pa_newton_iteration(pa_context *ctx, int increment, int iteration, double *res_norm, int *lin_iters);
+
+/* Split halves of pa_newton_iteration, for timing and parity. */
+int pa_assemble(pa_context *ctx, int increment, int iteration, double *res_norm);
+int pa_linear_solve(pa_context *ctx, int increment, int iteration, int *lin_iters);
+
+/* updateAfterIncrement (src/materialModels/crystalPlasticity/updateAfterIncrement.cc:6-305,
+ * 710-771): stress-only qp pass, volume averages, reorient() (reorient.cc:4-100), conv <- iter.
+ * row: Exx Eyy Ezz Eyz Exz Exy Txx Tyy Tzz Tyz Txz Txy TwinRealVF TwinMade SlipTotal. */
+int pa_commit_increment(pa_context *ctx, double row[15]);
+
+/* solve() body for one increment (src/ellipticBVP/solve.cc:69-91): max_newton Newton iterations
+ * then pa_commit_increment.  lin_iters / res_norms (optional) receive max_newton entries. */
+int pa_run_increment(pa_context *ctx, int increment, double row[15], int *lin_iters, double *res_norms);
+
+/* State readback for parity.  which: 0 T (9/qp), 1 Fp_conv (9), 2 Fe_conv (9), 3 s_conv (n_slip),
+ * 4 rotnew (3), 5 slip_conv (n_slip), 6 u (3/node), 7 R (3/node), 8 Fp_iter, 9 Fe_iter, 10 s_iter.
+ * Output is qp-major (or node-major).  n = number of doubles the caller provides. */
+int pa_get_state(pa_context *ctx, int which, double *host, long long n);
+
+/* Sparsity of the Jacobian (DoFTools::make_sparsity_pattern over the full 27-node stencil,
+ * src/ellipticBVP/init.cc:183-199), built on the GPU: row_ptr (ndof+1), col_idx (nnz); values
+ * (optional, nnz) = the last assembled condensed Jacobian.  Sizes: pa_sizes. */
+int pa_get_csr(pa_context *ctx, long long *row_ptr, int *col_idx, double *values);
+int pa_sizes(pa_context *ctx, long long *nelem, long long *nqp, long long *ndof, long long *nnz);
+
+/* calculatePlasticity(cellID, q, flag) over a batch of quadrature points (calculatePlasticity.cc:13-959),
+ * host buffers in/out, with grain[i] selecting the orientation row set by pa_set_grains.
+ * F, Fp_conv: 9/qp; s_conv: n_slip/qp.  Outputs P, T, Fe, Fp: 9/qp; dPdF: 81/qp (flag 1, may be
+ * NULL for flag 0); s_out, slip_inc (x_beta_old): n_slip/qp; n_active: 1/qp. */
+int pa_qp_update(pa_context *ctx, long long nqp, const double *F, const double *Fp_conv, const double *s_conv,
+                 const unsigned *grain, int flag, double *P, double *T, double *dPdF, double *Fe, double *Fp,
+                 double *s_out, double *slip_inc, int *n_active);
+
+/* Timing of the last pa_newton_iteration / pa_run_increment phases on the context stream (ms):
+ * [0] assembly (qp + tangent + K_e + scatter), [1] Krylov, [2] post-increment pass. */
+int pa_last_timings(pa_context *ctx, double ms[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,110 @@
+/*
+ * CPU FP64 restatement of the PRISMS-Plasticity crystal-plasticity Newton step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity oracle: it is loaded
+ * by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg,
+ * and by nothing else.  The product (plasticity_amd/) never links or calls it.
+ *
+ * Every function restates the reference algorithm; the .cpp cites the
+ * reference file:line each block follows (paths relative to the reference
+ * repository root).
+ */
+#ifndef CP_ORACLE_H
+#define CP_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_MAXT 49 /* BCC-48 + the inert twin system */
+
+/* pinv_mode: how x = pinv(A_PA) b_PA is formed inside inactive_slip_removal
+ *   0 = Jacobi-SVD pseudo-inverse with threshold 0 (deal.II
+ *       LAPACKFullMatrix::compute_inverse_svd(0.0) semantics,
+ *       src/materialModels/crystalPlasticity/inactiveSlipRemoval.cc:26,99)
+ *   1 = LU with partial pivoting; falls back to the SVD pseudo-inverse with a
+ *       relative cut-off n*eps*sigma_max when a pivot is numerically zero
+ *       (the semantics the HIP kernel implements). */
+typedef struct {
+  int n_slip, n_twin, nT;          /* nT = n_slip + n_twin (n_twin=1 dummy when twinning is off) */
+  double m[OR_MAXT][3], n[OR_MAXT][3];  /* normalised slip directions / normals */
+  double q[OR_MAXT][OR_MAXT];       /* latent hardening ratios */
+  double C[6][6];                   /* crystal-frame elastic stiffness, Voigt (11,22,33,23,13,12), MPa */
+  double s0[OR_MAXT], h0[OR_MAXT], ss[OR_MAXT], pw[OR_MAXT];
+  double C1[OR_MAXT], C2[OR_MAXT];  /* kinematic hardening (0 when disabled) */
+  double tol;                       /* Stress Tolerance */
+  int max_slip_search, max_solver_iter;
+  int pinv_mode;
+} or_material;
+
+typedef struct {
+  int nx, ny, nz;                   /* elements per direction (after refinement) */
+  double span[3];
+  int n_increments;                 /* increments to run (<= total) */
+  int total_increments;             /* round(Total time / Time increments) */
+  double delT;
+  int max_newton;                   /* Maximum non linear iterations */
+  double lin_tol;                   /* Relative linear solver tolerance */
+  int lin_maxit;
+  int solver;                       /* 0 = Jacobi-BiCG (reference), 1 = Jacobi-PCG */
+  int face_dof_constrained[6][3];
+  double face_total_u[6][3];        /* FinalDisplacement of each (face,dof) rule */
+  int cyclic;                       /* Enable cyclic loading */
+  int cyclic_face, cyclic_dof;      /* 1-based as in prm */
+  double quarter_cycle_time;
+  int nthreads;                     /* OpenMP threads for the CPU baseline (<=0: all) */
+} or_problem;
+
+/* One calculatePlasticity(cellID,q,flag) call.  In/out arrays follow the
+ * reference member layout (row-major 3x3, dP_dF[i][k][j][l] flattened as
+ * (3i+k)*9 + 3j+l).  slip_iter is only written on the plastic path
+ * (calculatePlasticity.cc:454-456), so it is in/out.
+ * Returns the number of active systems after inactive-slip removal (0 = elastic). */
+int or_qp_update(const or_material *mat, const double rot[3], const double F[9],
+                 const double Fp_conv[9], const double *s_conv, const double *W_conv,
+                 const double *slip_conv, int flag,
+                 double P[9], double T[9], double dPdF[81], double Fe_iter[9],
+                 double Fp_iter[9], double *s_iter, double *W_iter, double *slip_iter);
+
+/* Nearest-voxel grain lookup of a point (crystalOrientationsIO.cc:115-194).
+ * vox is [nvx][nvy][nvz] (z fastest) as read from the grain-ID file. */
+unsigned or_material_id(const unsigned *vox, const int nv[3], const double span[3], const double p[3]);
+
+/* Grain id of every element (cell centre lookup, loadOrientations.cc:4-33),
+ * elements ordered ex + nx*(ey + ny*ez). */
+void or_element_grains(const unsigned *vox, const int nv[3], const or_problem *pb, unsigned *elem_grain);
+
+/* Full-pattern CSR of the structured Q1^3 mesh (init.cc:183-199); row_ptr has
+ * ndof+1 entries.  Returns nnz; col_idx may be NULL to query the size. */
+long long or_sparsity(const or_problem *pb, long long *row_ptr, int *col_idx);
+
+/* Run n_increments increments of the Newton loop (solve.cc:69-135) and write
+ * the 15-column stressstrain rows (updateAfterIncrement.cc:735-771).
+ * grain_rod is indexed by the grain id in elem_grain (rows of 3).
+ * lin_iters (optional) receives n_increments*max_newton Krylov iteration counts,
+ * res_norms (optional) the Newton residual norms.
+ * state_out (optional) receives the final per-qp state, qp-major:
+ *   [T 9 | Fp 9 | Fe 9 | s nT | rotnew 3 | slip n_slip] per qp.
+ * u_out (optional) receives the final displacement (3 per node). */
+int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *elem_grain,
+                const double *grain_rod, int n_grain_rows,
+                double *rows, int *lin_iters, double *res_norms, double *state_out,
+                double *u_out);
+
+/* Counts of inactive-slip-removal solves and of those whose A_PA was numerically
+ * singular (LU pivot below n*eps*max|A|) since the last call; resets the counters. */
+void or_pinv_stats(long long *calls, long long *singular);
+
+/* Single increment helpers exposed for property tests. */
+void or_odfpoint(const double r[3], double R[9]);
+void or_matrix_exponential(const double A[9], double E[9]);
+void or_polar_rotation(const double Fe[9], double R[9]);
+int  or_pinv_solve(int n, const double *A, const double *b, double *x, int mode);
+int  or_element_matrix(const or_material *mat, const double rot[3], const double h[3],
+                       const double ue[24], const double *Fp_conv, const double *s_conv,
+                       double Ke[576], double Re[24]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

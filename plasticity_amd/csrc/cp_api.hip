@@ -1,0 +1,1369 @@
+// plasticity_amd: C ABI, mesh/DoF integer work, assembly, post-increment and Krylov kernels
+// for gfx950.  Declarations and reference citations: include/plasticity_amd.h.
+//
+// Data layout in HBM (one context):
+//   per-qp state, structure of arrays, component-major:  X[c * nq + Q]  (Q = 8 e + q)
+//     Fp_conv/Fp_iter/Fe_conv/Fe_iter 9, s_conv/s_iter ns, slip_conv/slip_iter ns, rotnew 3, T 9
+//   Jacobian in 27-point node-block form: K[(slot * 9 + 3a + b) * nnode + n] is the 3x3 block
+//     coupling node n (rows 3n+a) to its neighbour n + off(slot) (cols 3m+b),
+//     slot = (dz+1)*9 + (dy+1)*3 + (dx+1).  Entries outside the mesh stay zero.
+//   vectors: dof = 3 n + component, FP64.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/plasticity_amd.h"
+#include "cp_qp.h"
+
+namespace {
+
+thread_local std::string g_err = "";
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) return fail(PA_ERR_HIP, std::string(#x ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int NSMAX = 48;
+constexpr int RED_BLOCK = 256;
+
+// ------------------------------------------------------------ reductions
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// Block sum of up to NV values (blockDim.x multiple of 64, <= 1024); result valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double *lds /* NV * 16 */) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const double s = wave_sum(v[k]);
+    if (lane == 0) lds[k * 16 + w] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0;
+      for (int i = 0; i < nw; ++i) s += lds[k * 16 + i];
+      v[k] = s;
+    }
+  }
+}
+
+// Deterministic grid reduction: each block writes its NV partial sums; the last block to
+// arrive sums the partials in block order.  Returns true in thread 0 of the last block,
+// with v holding the totals.
+template <int NV>
+__device__ bool grid_sum(double (&v)[NV], double *partials, unsigned *counter, double *lds) {
+  __shared__ bool last;
+  block_sum<NV>(v, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) partials[blockIdx.x * NV + k] = v[k];
+    __threadfence();
+    const unsigned t = atomicAdd(counter, 1u);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return false;
+  if (threadIdx.x == 0) {
+    __threadfence();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0;
+      for (unsigned b = 0; b < gridDim.x; ++b) s += __hip_atomic_load(&partials[b * NV + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      v[k] = s;
+    }
+    *counter = 0;
+  }
+  return threadIdx.x == 0;
+}
+
+// ------------------------------------------------------------ mesh helpers
+struct MeshDims {
+  int nx, ny, nz, nnx, nny, nnz;
+  long long nnode, nelem;
+};
+
+__device__ __forceinline__ void node_ijk(const MeshDims &m, long long n, int &i, int &j, int &k) {
+  i = (int)(n % m.nnx);
+  j = (int)((n / m.nnx) % m.nny);
+  k = (int)(n / ((long long)m.nnx * m.nny));
+}
+
+// Q1 gradient of vertex v at Gauss point q of a box cell with sides h
+__device__ __forceinline__ void shape_grad(int q, int v, const double *h, double g[3]) {
+  const double g0 = 0.5 - 0.5 / sqrt(3.0), g1 = 0.5 + 0.5 / sqrt(3.0);
+  double N[3], dN[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double xi = ((q >> d) & 1) ? g1 : g0;
+    const int b = (v >> d) & 1;
+    N[d] = b ? xi : 1.0 - xi;
+    dN[d] = (b ? 1.0 : -1.0) / h[d];
+  }
+  g[0] = dN[0] * N[1] * N[2];
+  g[1] = N[0] * dN[1] * N[2];
+  g[2] = N[0] * N[1] * dN[2];
+}
+
+// ---------------------------------------------------- grain ids (integer work)
+// Nearest voxel centre to the cell centre per axis, ties to the lower voxel
+// (crystalOrientationsIO.cc:121-193).  Arithmetic is spelled with _rn intrinsics so no
+// fused multiply-add changes a key or a centre.
+__device__ int nearest_voxel(double p, int nv, double span) {
+  const double st = __ddiv_rn(span, (double)nv);
+  const double half = __ddiv_rn(st, 2.0);
+  int lo = 0, hi = nv;  // lower_bound: first key >= p
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    const double key = __dadd_rn(__dmul_rn((double)mid, st), half);
+    if (key < p) lo = mid + 1; else hi = mid;
+  }
+  if (lo == nv) return nv - 1;
+  if (lo == 0) return 0;
+  const double kl = __dadd_rn(__dmul_rn((double)(lo - 1), st), half), ku = __dadd_rn(__dmul_rn((double)lo, st), half);
+  const double dl = fabs(__dsub_rn(kl, p)), du = fabs(__dsub_rn(ku, p));
+  return (du >= dl) ? lo - 1 : lo;
+}
+
+__global__ void k_element_grains(MeshDims m, double sx, double sy, double sz, const unsigned *__restrict__ vox, int nv0,
+                                 int nv1, int nv2, unsigned *__restrict__ eg) {
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= m.nelem) return;
+  const int ex = (int)(e % m.nx), ey = (int)((e / m.nx) % m.ny), ez = (int)(e / ((long long)m.nx * m.ny));
+  const double h[3] = {__ddiv_rn(sx, (double)m.nx), __ddiv_rn(sy, (double)m.ny), __ddiv_rn(sz, (double)m.nz)};
+  const int ie[3] = {ex, ey, ez};
+  const double span[3] = {sx, sy, sz};
+  const int nv[3] = {nv0, nv1, nv2};
+  int idx[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double c = __ddiv_rn(__dadd_rn(__dmul_rn((double)ie[d], h[d]), __dmul_rn((double)(ie[d] + 1), h[d])), 2.0);
+    idx[d] = nearest_voxel(c, nv[d], span[d]);
+  }
+  eg[e] = vox[((long long)idx[0] * nv1 + idx[1]) * nv2 + idx[2]];
+}
+
+// ------------------------------------------------------ sparsity (integer work)
+// Full 27-point pattern, columns ascending (DoFTools::make_sparsity_pattern, init.cc:183-199).
+// Entries before node (i,j,k) follow in closed form from per-axis neighbour counts
+// c(i) = 2 at the ends, 3 inside (1 for a single node) and prefix P(i) = sum_{i'<i} c(i').
+__device__ __forceinline__ long long cnt1(int i, int nn) { return (long long)(min(i + 1, nn - 1) - max(i - 1, 0) + 1); }
+__device__ __forceinline__ long long pre1(int i, int nn) {
+  long long s = 0;
+  if (i > 0) s = cnt1(0, nn) + (long long)(i - 1) * 3;
+  if (i == nn && nn >= 2) s = s - 3 + cnt1(nn - 1, nn);
+  return s;
+}
+__global__ void k_csr_rowptr(MeshDims m, long long *__restrict__ rp) {
+  const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long ndof = 3 * m.nnode;
+  if (r > ndof) return;
+  if (r == ndof) {
+    rp[r] = 9 * pre1(m.nnx, m.nnx) * pre1(m.nny, m.nny) * pre1(m.nnz, m.nnz);
+    return;
+  }
+  const long long n = r / 3;
+  const int c = (int)(r % 3);
+  int i, j, k;
+  node_ijk(m, n, i, j, k);
+  const long long Tx = pre1(m.nnx, m.nnx), Ty = pre1(m.nny, m.nny);
+  const long long before = pre1(k, m.nnz) * Ty * Tx + cnt1(k, m.nnz) * pre1(j, m.nny) * Tx + cnt1(k, m.nnz) * cnt1(j, m.nny) * pre1(i, m.nnx);
+  const long long len = 3 * cnt1(i, m.nnx) * cnt1(j, m.nny) * cnt1(k, m.nnz);
+  rp[r] = 9 * before + c * len;
+}
+// one thread per (row, position): column of each entry, and the SB source of its value
+__global__ void k_csr_cols(MeshDims m, const long long *__restrict__ rp, int *__restrict__ ci, const double *__restrict__ K,
+                           double *__restrict__ val) {
+  const long long r = blockIdx.x * (long long)blockDim.y + threadIdx.y;
+  if (r >= 3 * m.nnode) return;
+  const long long n = r / 3;
+  const int a = (int)(r % 3);
+  int i, j, k;
+  node_ijk(m, n, i, j, k);
+  const int p = threadIdx.x;  // 0..80
+  const int xl = (i > 0) ? -1 : 0, xh = (i < m.nnx - 1) ? 1 : 0;
+  const int yl = (j > 0) ? -1 : 0, yh = (j < m.nny - 1) ? 1 : 0;
+  const int zl = (k > 0) ? -1 : 0, zh = (k < m.nnz - 1) ? 1 : 0;
+  const int ax = xh - xl + 1, ay = yh - yl + 1, az = zh - zl + 1;
+  if (p >= 3 * ax * ay * az) return;
+  const int b = p % 3, nb = p / 3;
+  const int dx = xl + nb % ax, dy = yl + (nb / ax) % ay, dz = zl + nb / (ax * ay);
+  const long long mnode = n + dx + (long long)dy * m.nnx + (long long)dz * m.nnx * m.nny;
+  const long long pos = rp[r] + p;
+  if (ci) ci[pos] = (int)(3 * mnode + b);
+  if (val) {
+    const int slot = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+    val[pos] = K[(long long)(slot * 9 + 3 * a + b) * m.nnode + n];
+  }
+}
+
+// ------------------------------------------------------------ Dirichlet lines
+// setBoundaryValues + applyDirichletBCs (boundaryConditions.cc:6-114, 452-491): the first
+// matching face rule wins; cyclic rule first when enabled.
+struct BCArgs {
+  int fdc[6][3];
+  double dU[6][3];
+  int cyclic, cface, cdof;
+  double csign;
+};
+__global__ void k_dirichlet(MeshDims m, BCArgs bc, unsigned char *__restrict__ con, double *__restrict__ delu) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= m.nnode) return;
+  int i, j, k;
+  node_ijk(m, n, i, j, k);
+  const bool onb[6] = {i == 0, i == m.nx, j == 0, j == m.ny, k == 0, k == m.nz};
+  const bool bnd = onb[0] || onb[1] || onb[2] || onb[3] || onb[4] || onb[5];
+  for (int d = 0; d < 3; ++d) {
+    bool flag = false;
+    double val = 0.0;
+    if (bnd) {
+      if (bc.cyclic && d == bc.cdof - 1 && onb[bc.cface - 1]) {
+        flag = true;
+        val = bc.csign * bc.dU[bc.cface - 1][d];
+      } else {
+        for (int f = 0; f < 6; ++f)
+          if (bc.fdc[f][d] && onb[f]) { flag = true; val = bc.dU[f][d]; break; }
+      }
+    }
+    con[3 * n + d] = flag ? 1 : 0;
+    delu[3 * n + d] = flag ? val : 0.0;
+  }
+}
+
+// ------------------------------------------------------------ assembly
+struct AsmArgs {
+  MeshDims m;
+  double h[3], JxW;
+  int cx, cy, cz, ncx, ncy, ncz;  // colour offsets and element counts of this colour
+  const double *u;
+  const unsigned *egrain;
+  const double *gtab;
+  int gstride;
+  const double *mat;
+  int ns, max_solver_iter;
+  long long nq;
+  const double *Fp_conv, *s_conv, *slip_conv;
+  double *Fe_iter, *Fp_iter, *s_iter, *slip_iter;
+  const unsigned char *con;
+  const double *delu;
+  int iter0;
+  double *K, *R;
+};
+
+constexpr int ASM_EL = 8;                 // elements per 64-lane workgroup
+constexpr int DSTR = 81;                  // LDS stride of one qp's dP/dF
+
+// One colour of the element loop of assemble() (assemble.cc:64-90): lane = (element, qp)
+// runs calculatePlasticity(.,.,1) (getElementalValues.cc:141), then lane = (element,
+// vertex) forms 3 rows of K_e and R_e (:144-162) and adds them into the node-block
+// Jacobian with deal.II's condensation of Dirichlet lines
+// (AffineConstraints::distribute_local_to_global): free-free entries added, free rows get
+// -K_ij g_j for constrained columns at iteration 0, constrained rows get |K_ii| (or the mean
+// |diag| of K_e when K_ii = 0) on the diagonal and nothing in R.  Elements of one colour
+// share no node, so the read-modify-write needs no atomics and the sum order is fixed.
+template <int NS>
+__global__ void __launch_bounds__(64) k_assemble(AsmArgs a) {
+  __shared__ double sD[64 * DSTR];
+  __shared__ double sP[64 * 9];
+  __shared__ double sdiag[ASM_EL * 24];
+  const int t = threadIdx.x, le = t >> 3, sub = t & 7;
+  const long long ncol = (long long)a.ncx * a.ncy * a.ncz;
+  const long long ci = (long long)blockIdx.x * ASM_EL + le;
+  const bool active = ci < ncol;
+  long long e = 0;
+  int ex = 0, ey = 0, ez = 0;
+  if (active) {
+    ex = a.cx + 2 * (int)(ci % a.ncx);
+    ey = a.cy + 2 * (int)((ci / a.ncx) % a.ncy);
+    ez = a.cz + 2 * (int)(ci / ((long long)a.ncx * a.ncy));
+    e = ex + (long long)a.m.nx * (ey + (long long)a.m.ny * ez);
+  }
+  long long nodes[8];
+#pragma unroll
+  for (int v = 0; v < 8; ++v)
+    nodes[v] = (ex + (v & 1)) + (long long)a.m.nnx * ((ey + ((v >> 1) & 1)) + (long long)a.m.nny * (ez + ((v >> 2) & 1)));
+
+  // ---- phase 1: lane = qp
+  if (active) {
+    const int q = sub;
+    const long long Q = 8 * e + q;
+    double F[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      double g[3];
+      shape_grad(q, v, a.h, g);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double uc = a.u[3 * nodes[v] + c];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) F[3 * c + j] += uc * g[j];
+      }
+    }
+    F[0] += 1; F[4] += 1; F[8] += 1;
+    double Fp_t[9], s[NS], x[NS], P[9], T[9], Fe[9], Fp[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Fp_t[i] = a.Fp_conv[i * a.nq + Q];
+    for (int i = 0; i < a.ns; ++i) s[i] = a.s_conv[i * a.nq + Q];
+    const double *gt = a.gtab + (long long)a.egrain[e] * a.gstride;
+    const int na = cp::qp_update<NS>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 1, P, T, Fe, Fp, sD + t, 64);
+    (void)na;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) { a.Fe_iter[i * a.nq + Q] = Fe[i]; a.Fp_iter[i * a.nq + Q] = Fp[i]; sP[t * 9 + i] = P[i]; }
+    for (int i = 0; i < a.ns; ++i) a.s_iter[i * a.nq + Q] = s[i];
+    // slipfraction_iter is written only on the plastic path (calculatePlasticity.cc:454-456)
+    bool plastic = false;
+    for (int i = 0; i < a.ns; ++i) plastic |= (x[i] != 0.0);
+    if (plastic || na > 0)
+      for (int i = 0; i < a.ns; ++i) a.slip_iter[i * a.nq + Q] = a.slip_conv[i * a.nq + Q] + x[i];
+  }
+  __syncthreads();
+
+  // ---- phase 2: lane = vertex v1 of element le; rows 3 v1 + i
+  const int v1 = sub;
+  double Kr[3][24], Rr[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    Rr[i] = 0;
+#pragma unroll
+    for (int c = 0; c < 24; ++c) Kr[i][c] = 0;
+  }
+  if (active) {
+    for (int q = 0; q < 8; ++q) {
+      const double *Dq = sD + (le * 8 + q);
+      const double *Pq = sP + (le * 8 + q) * 9;
+      double g1[3];
+      shape_grad(q, v1, a.h, g1);
+      double g2[8][3];
+#pragma unroll
+      for (int v = 0; v < 8; ++v) shape_grad(q, v, a.h, g2[v]);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        Rr[i] -= g1[0] * Pq[3 * i] + g1[1] * Pq[3 * i + 1] + g1[2] * Pq[3 * i + 2];
+        double w[9];
+#pragma unroll
+        for (int jl = 0; jl < 9; ++jl)
+          w[jl] = g1[0] * Dq[((3 * i + 0) * 9 + jl) * 64] + g1[1] * Dq[((3 * i + 1) * 9 + jl) * 64] + g1[2] * Dq[((3 * i + 2) * 9 + jl) * 64];
+#pragma unroll
+        for (int v2 = 0; v2 < 8; ++v2)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) Kr[i][3 * v2 + j] += w[3 * j] * g2[v2][0] + w[3 * j + 1] * g2[v2][1] + w[3 * j + 2] * g2[v2][2];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      Rr[i] *= a.JxW;
+#pragma unroll
+      for (int c = 0; c < 24; ++c) Kr[i][c] *= a.JxW;
+      sdiag[le * 24 + 3 * v1 + i] = Kr[i][3 * v1 + i];
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  double avg = 0;
+  for (int d = 0; d < 24; ++d) avg += fabs(sdiag[le * 24 + d]);
+  avg /= 24.0;
+  const long long n1 = nodes[v1];
+  const long long nn = a.m.nnode;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const long long gi = 3 * n1 + i;
+    if (a.con[gi]) {
+      const double kd = fabs(Kr[i][3 * v1 + i]);
+      a.K[(long long)(13 * 9 + 4 * i) * nn + n1] += (kd != 0 ? kd : avg);
+      continue;
+    }
+    double rv = Rr[i];
+#pragma unroll
+    for (int v2 = 0; v2 < 8; ++v2) {
+      const int slot = (((v2 >> 2) & 1) - ((v1 >> 2) & 1) + 1) * 9 + (((v2 >> 1) & 1) - ((v1 >> 1) & 1) + 1) * 3 + ((v2 & 1) - (v1 & 1) + 1);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const long long gj = 3 * nodes[v2] + j;
+        if (a.con[gj]) {
+          if (a.iter0) {
+            const double g = a.delu[gj];
+            if (g != 0.0) rv -= Kr[i][3 * v2 + j] * g;
+          }
+        } else {
+          a.K[(long long)(slot * 9 + 3 * i + j) * nn + n1] += Kr[i][3 * v2 + j];
+        }
+      }
+    }
+    if (rv != 0.0) a.R[gi] += rv;
+  }
+}
+
+// ------------------------------------------------ post-increment qp pass
+struct PostArgs {
+  MeshDims m;
+  double h[3], JxW;
+  const double *u;
+  const unsigned *egrain;
+  const double *gtab;
+  int gstride;
+  const double *mat;
+  int ns, max_solver_iter;
+  long long nq;
+  const double *Fp_conv, *s_conv, *slip_conv, *Fe_conv;
+  double *Fe_iter, *Fp_iter, *s_iter, *slip_iter, *T, *rotnew;
+  double *partials;
+  unsigned *counter;
+  double *out;  // 13: E(6) T(6) slip
+};
+
+// symmetric 3x3 eigen-decomposition by cyclic Jacobi rotations
+__device__ void sym_eig3(const double *Cin, double lam[3], double V[9]) {
+  double C[9];
+  for (int i = 0; i < 9; ++i) { C[i] = Cin[i]; V[i] = (i % 4 == 0) ? 1.0 : 0.0; }
+  for (int sweep = 0; sweep < 50; ++sweep) {
+    const double off = C[1] * C[1] + C[2] * C[2] + C[5] * C[5];
+    if (off == 0.0) break;
+    for (int p = 0; p < 2; ++p)
+      for (int q = p + 1; q < 3; ++q) {
+        if (C[3 * p + q] == 0.0) continue;
+        const double th = (C[3 * q + q] - C[3 * p + p]) / (2.0 * C[3 * p + q]);
+        const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < 3; ++k) {
+          const double ckp = C[3 * k + p], ckq = C[3 * k + q];
+          C[3 * k + p] = c * ckp - s * ckq; C[3 * k + q] = s * ckp + c * ckq;
+        }
+        for (int k = 0; k < 3; ++k) {
+          const double cpk = C[3 * p + k], cqk = C[3 * q + k];
+          C[3 * p + k] = c * cpk - s * cqk; C[3 * q + k] = s * cpk + c * cqk;
+        }
+        for (int k = 0; k < 3; ++k) {
+          const double vkp = V[3 * k + p], vkq = V[3 * k + q];
+          V[3 * k + p] = c * vkp - s * vkq; V[3 * k + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  lam[0] = C[0]; lam[1] = C[4]; lam[2] = C[8];
+}
+// rotation of the polar decomposition Fe = R U (reorient.cc:33-57)
+__device__ void polar_R(const double *Fe, double *R) {
+  double C[9], lam[3], V[9], VL[9], U[9], Ui[9];
+  cp::m3_tmul(Fe, Fe, C);
+  sym_eig3(C, lam, V);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) VL[3 * i + j] = V[3 * i + j] * sqrt(lam[j]);
+  cp::m3_mult(VL, V, U);
+  cp::m3_inv(U, Ui);
+  cp::m3_mul(Fe, Ui, R);
+}
+
+// updateAfterIncrement's qp loop (updateAfterIncrement.cc:77-126) with calculatePlasticity(.,.,0),
+// fused with reorient() (reorient.cc:18-96) and the JxW-weighted sums.
+template <int NS>
+__global__ void __launch_bounds__(256) k_post(PostArgs a) {
+  __shared__ double lds[13 * 16];
+  const long long Q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  double acc[13];
+#pragma unroll
+  for (int i = 0; i < 13; ++i) acc[i] = 0.0;
+  if (Q < a.nq) {
+    const long long e = Q >> 3;
+    const int q = (int)(Q & 7);
+    const int ex = (int)(e % a.m.nx), ey = (int)((e / a.m.nx) % a.m.ny), ez = (int)(e / ((long long)a.m.nx * a.m.ny));
+    double F[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      const long long nd = (ex + (v & 1)) + (long long)a.m.nnx * ((ey + ((v >> 1) & 1)) + (long long)a.m.nny * (ez + ((v >> 2) & 1)));
+      double g[3];
+      shape_grad(q, v, a.h, g);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double uc = a.u[3 * nd + c];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) F[3 * c + j] += uc * g[j];
+      }
+    }
+    F[0] += 1; F[4] += 1; F[8] += 1;
+    double Fp_t[9], s[NS], x[NS], P[9], T[9], Fe[9], Fp[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Fp_t[i] = a.Fp_conv[i * a.nq + Q];
+    for (int i = 0; i < a.ns; ++i) s[i] = a.s_conv[i * a.nq + Q];
+    const double *gt = a.gtab + (long long)a.egrain[e] * a.gstride;
+    const int na = cp::qp_update<NS>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 0, P, T, Fe, Fp, nullptr, 0);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) { a.Fe_iter[i * a.nq + Q] = Fe[i]; a.Fp_iter[i * a.nq + Q] = Fp[i]; a.T[i * a.nq + Q] = T[i]; }
+    for (int i = 0; i < a.ns; ++i) a.s_iter[i * a.nq + Q] = s[i];
+    bool plastic = false;
+    for (int i = 0; i < a.ns; ++i) plastic |= (x[i] != 0.0);
+    double slip_sum = 0;
+    if (plastic || na > 0) {
+      for (int i = 0; i < a.ns; ++i) {
+        const double v = a.slip_conv[i * a.nq + Q] + x[i];
+        a.slip_iter[i * a.nq + Q] = v;
+        slip_sum += v * a.JxW;
+      }
+    } else {
+      for (int i = 0; i < a.ns; ++i) slip_sum += a.slip_iter[i * a.nq + Q] * a.JxW;
+    }
+    // Green-Lagrange strain and JxW-weighted sums (:114-126, :218-221)
+    double C[9];
+    cp::m3_tmul(F, F, C);
+    const int vi[6] = {0, 4, 8, 5, 2, 1};
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      acc[c] = 0.5 * (C[vi[c]] - ((vi[c] % 4 == 0) ? 1.0 : 0.0)) * a.JxW;
+      acc[6 + c] = T[vi[c]] * a.JxW;
+    }
+    acc[12] = slip_sum;
+    // reorient (reorient.cc:24-93) with Fe_old = Fe_conv, Fe_new = Fe_iter
+    double Fo[9], Ro[9], Rn[9], Om[9], Om2[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Fo[i] = a.Fe_conv[i * a.nq + Q];
+    polar_R(Fo, Ro);
+    polar_R(Fe, Rn);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Om[i] = Rn[i] - Ro[i];
+    cp::m3_mult(Om, Rn, Om2);
+    const double w[3] = {-0.5 * (Om2[5] - Om2[7]), 0.5 * (Om2[2] - Om2[6]), -0.5 * (Om2[1] - Om2[3])};
+    double r[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) r[i] = a.rotnew[i * a.nq + Q];
+    const double dt = w[0] * r[0] + w[1] * r[1] + w[2] * r[2];
+    const double cr[3] = {w[1] * r[2] - w[2] * r[1], w[2] * r[0] - w[0] * r[2], w[0] * r[1] - w[1] * r[0]};
+    double rn[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) rn[i] = r[i] + 0.5 * (w[i] + dt * r[i] + cr[i]);
+    const double nrm = sqrt(rn[0] * rn[0] + rn[1] * rn[1] + rn[2] * rn[2]);
+    if (nrm > 10000.0)
+      for (int i = 0; i < 3; ++i) rn[i] = rn[i] * 10000.0 / nrm;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) a.rotnew[i * a.nq + Q] = rn[i];
+  }
+  if (grid_sum<13>(acc, a.partials, a.counter, lds)) {
+#pragma unroll
+    for (int i = 0; i < 13; ++i) a.out[i] = acc[i];
+  }
+}
+
+// batch of calculatePlasticity calls with host-provided inputs (parity entry point)
+template <int NS>
+__global__ void __launch_bounds__(64) k_qp_batch(long long n, const double *__restrict__ F, const double *__restrict__ Fpc,
+                                                 const double *__restrict__ sc, const unsigned *__restrict__ grain,
+                                                 const double *gtab, int gstride, const double *mat, int ns, int msi, int flag,
+                                                 double *P, double *T, double *D, double *Fe, double *Fp, double *so, double *xo,
+                                                 int *na) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double f[9], fp[9], s[NS], x[NS], p[9], t[9], fe[9], fpo[9];
+  for (int k = 0; k < 9; ++k) { f[k] = F[9 * i + k]; fp[k] = Fpc[9 * i + k]; }
+  for (int k = 0; k < ns; ++k) s[k] = sc[(long long)ns * i + k];
+  const double *gt = gtab + (long long)grain[i] * gstride;
+  const int r = cp::qp_update<NS>(mat, gt, ns, msi, f, fp, s, x, flag, p, t, fe, fpo, flag ? D + 81 * i : nullptr, 1);
+  for (int k = 0; k < 9; ++k) { P[9 * i + k] = p[k]; T[9 * i + k] = t[k]; Fe[9 * i + k] = fe[k]; Fp[9 * i + k] = fpo[k]; }
+  for (int k = 0; k < ns; ++k) { so[(long long)ns * i + k] = s[k]; xo[(long long)ns * i + k] = x[k]; }
+  na[i] = r;
+}
+
+// ------------------------------------------------------------ Krylov (FP64)
+struct KState {
+  double beta, betaold, b, alpha, dp, tol, bnorm;
+  int it, done, maxit, breakdown;
+};
+
+__device__ __forceinline__ long long nb_node(const MeshDims &m, long long n, int i, int j, int k, int s, bool &ok) {
+  const int dx = s % 3 - 1, dy = (s / 3) % 3 - 1, dz = s / 9 - 1;
+  ok = (i + dx >= 0) && (i + dx < m.nnx) && (j + dy >= 0) && (j + dy < m.nny) && (k + dz >= 0) && (k + dz < m.nnz);
+  return n + dx + (long long)dy * m.nnx + (long long)dz * m.nnx * m.nny;
+}
+
+// y = K p, yt = K^T pt on the node-block matrix; partial dot(y, pl) for BiCG's dpi
+template <typename VT>
+__global__ void __launch_bounds__(256) k_bicg_spmv(MeshDims m, const VT *__restrict__ K, const double *__restrict__ Pr,
+                                                   const double *__restrict__ Pl, double *__restrict__ Zr, double *__restrict__ Zl,
+                                                   KState *st, double *partials, unsigned *counter) {
+  __shared__ double lds[16];
+  if (st->done) return;
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  double acc[1] = {0.0};
+  if (n < m.nnode) {
+    int i, j, k;
+    node_ijk(m, n, i, j, k);
+    double y[3] = {0, 0, 0}, yt[3] = {0, 0, 0};
+    const long long nn = m.nnode;
+#pragma unroll 3
+    for (int s = 0; s < 27; ++s) {
+      bool ok;
+      const long long mn = nb_node(m, n, i, j, k, s, ok);
+      if (!ok) continue;
+      const double x0 = Pr[3 * mn], x1 = Pr[3 * mn + 1], x2 = Pr[3 * mn + 2];
+      const double q0 = Pl[3 * mn], q1 = Pl[3 * mn + 1], q2 = Pl[3 * mn + 2];
+      const VT *Kb = K + (long long)(s * 9) * nn + n;
+      const VT *Kt = K + (long long)((26 - s) * 9) * nn + mn;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        y[r] += (double)Kb[(3 * r) * nn] * x0 + (double)Kb[(3 * r + 1) * nn] * x1 + (double)Kb[(3 * r + 2) * nn] * x2;
+        yt[r] += (double)Kt[(r) * nn] * q0 + (double)Kt[(3 + r) * nn] * q1 + (double)Kt[(6 + r) * nn] * q2;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) { Zr[3 * n + r] = y[r]; Zl[3 * n + r] = yt[r]; }
+    acc[0] = y[0] * Pl[3 * n] + y[1] * Pl[3 * n + 1] + y[2] * Pl[3 * n + 2];
+  }
+  if (grid_sum<1>(acc, partials, counter, lds)) {
+    st->alpha = st->beta / acc[0];
+  }
+}
+
+// p <- z (first iteration) or p <- z + b p
+__global__ void __launch_bounds__(256) k_bicg_p(long long N, const double *__restrict__ Zr, const double *__restrict__ Zl,
+                                                double *__restrict__ Pr, double *__restrict__ Pl, const KState *st) {
+  if (st->done) return;
+  const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (r >= N) return;
+  if (st->it == 0) { Pr[r] = Zr[r]; Pl[r] = Zl[r]; }
+  else { const double b = st->b; Pr[r] = Zr[r] + b * Pr[r]; Pl[r] = Zl[r] + b * Pl[r]; }
+}
+
+// x += a p; r -= a z; rl -= a zl; z = D^-1 r; zl = D^-1 rl; partial ||z||^2 and z.rl
+__global__ void __launch_bounds__(256) k_bicg_update(long long N, double *__restrict__ X, const double *__restrict__ Pr,
+                                                     double *__restrict__ Rr, double *__restrict__ Rl, double *__restrict__ Zr,
+                                                     double *__restrict__ Zl, const double *__restrict__ dinv, KState *st,
+                                                     double *partials, unsigned *counter) {
+  __shared__ double lds[32];
+  if (st->done) return;
+  const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+  if (r < N) {
+    const double a = st->alpha, d = dinv[r];
+    X[r] += a * Pr[r];
+    const double rr = Rr[r] - a * Zr[r];
+    const double rl = Rl[r] - a * Zl[r];
+    Rr[r] = rr; Rl[r] = rl;
+    const double zr = rr * d, zl = rl * d;
+    Zr[r] = zr; Zl[r] = zl;
+    acc[0] = zr * zr;
+    acc[1] = zr * rl;
+  }
+  if (grid_sum<2>(acc, partials, counter, lds)) {
+    st->dp = sqrt(acc[0]);
+    st->it += 1;
+    if (st->dp <= st->tol || st->it >= st->maxit) st->done = 1;
+    st->betaold = st->beta;
+    st->beta = acc[1];
+    st->b = st->beta / st->betaold;
+    if (!st->done && st->beta == 0.0) { st->done = 1; st->breakdown = 1; }
+  }
+}
+
+// r = rl = b; z = zl = D^-1 b; tol = lin_tol * ||b||; x = 0
+__global__ void __launch_bounds__(256) k_bicg_init(long long N, const double *__restrict__ B, double *__restrict__ X,
+                                                   double *__restrict__ Rr, double *__restrict__ Rl, double *__restrict__ Zr,
+                                                   double *__restrict__ Zl, const double *__restrict__ dinv, KState *st,
+                                                   double lin_tol, int maxit, double *partials, unsigned *counter) {
+  __shared__ double lds[48];
+  const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (r < N) {
+    const double b = B[r], z = b * dinv[r];
+    X[r] = 0.0; Rr[r] = b; Rl[r] = b; Zr[r] = z; Zl[r] = z;
+    acc[0] = b * b; acc[1] = z * z; acc[2] = z * b;
+  }
+  if (grid_sum<3>(acc, partials, counter, lds)) {
+    st->bnorm = sqrt(acc[0]);
+    st->tol = lin_tol * st->bnorm;
+    st->dp = sqrt(acc[1]);
+    st->beta = acc[2];
+    st->betaold = 1.0;
+    st->it = 0;
+    st->maxit = maxit;
+    st->breakdown = 0;
+    st->done = (st->dp <= st->tol) ? 1 : 0;
+    if (!st->done && st->beta == 0.0) { st->done = 1; st->breakdown = 1; }
+  }
+}
+
+// Jacobi-PCG (the option BASELINE names): q = K p and partial p.q
+template <typename VT>
+__global__ void __launch_bounds__(256) k_cg_spmv(MeshDims m, const VT *__restrict__ K, const double *__restrict__ P,
+                                                 double *__restrict__ Q, KState *st, double *partials, unsigned *counter) {
+  __shared__ double lds[16];
+  if (st->done) return;
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  double acc[1] = {0.0};
+  if (n < m.nnode) {
+    int i, j, k;
+    node_ijk(m, n, i, j, k);
+    double y[3] = {0, 0, 0};
+    const long long nn = m.nnode;
+#pragma unroll 3
+    for (int s = 0; s < 27; ++s) {
+      bool ok;
+      const long long mn = nb_node(m, n, i, j, k, s, ok);
+      if (!ok) continue;
+      const double x0 = P[3 * mn], x1 = P[3 * mn + 1], x2 = P[3 * mn + 2];
+      const VT *Kb = K + (long long)(s * 9) * nn + n;
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+        y[r] += (double)Kb[(3 * r) * nn] * x0 + (double)Kb[(3 * r + 1) * nn] * x1 + (double)Kb[(3 * r + 2) * nn] * x2;
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) Q[3 * n + r] = y[r];
+    acc[0] = y[0] * P[3 * n] + y[1] * P[3 * n + 1] + y[2] * P[3 * n + 2];
+  }
+  if (grid_sum<1>(acc, partials, counter, lds)) st->alpha = st->beta / acc[0];
+}
+__global__ void __launch_bounds__(256) k_cg_update(long long N, double *__restrict__ X, const double *__restrict__ P,
+                                                   double *__restrict__ Rr, const double *__restrict__ Q, double *__restrict__ Z,
+                                                   const double *__restrict__ dinv, KState *st, double *partials, unsigned *counter) {
+  __shared__ double lds[32];
+  if (st->done) return;
+  const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+  if (r < N) {
+    const double a = st->alpha;
+    X[r] += a * P[r];
+    const double rr = Rr[r] - a * Q[r];
+    Rr[r] = rr;
+    const double z = rr * dinv[r];
+    Z[r] = z;
+    acc[0] = z * z;
+    acc[1] = rr * z;
+  }
+  if (grid_sum<2>(acc, partials, counter, lds)) {
+    st->dp = sqrt(acc[0]);
+    st->it += 1;
+    if (st->dp <= st->tol || st->it >= st->maxit) st->done = 1;
+    st->betaold = st->beta;
+    st->beta = acc[1];
+    st->b = st->beta / st->betaold;
+  }
+}
+__global__ void __launch_bounds__(256) k_cg_p(long long N, const double *__restrict__ Z, double *__restrict__ P, const KState *st,
+                                              int first) {
+  if (st->done) return;
+  const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (r >= N) return;
+  P[r] = (first || st->it == 0) ? Z[r] : Z[r] + st->b * P[r];
+}
+
+__global__ void k_dinv(MeshDims m, const double *__restrict__ K, double *__restrict__ dinv) {
+  const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (r >= 3 * m.nnode) return;
+  const long long n = r / 3;
+  const int c = (int)(r % 3);
+  const double d = K[(long long)(13 * 9 + 4 * c) * m.nnode + n];
+  dinv[r] = (d == 0.0) ? 1.0 : 1.0 / d;
+}
+
+// constraints.distribute(du) and x += du (solveLinearSystem.cc:32-34)
+__global__ void k_apply_du(long long N, double *__restrict__ u, const double *__restrict__ du, const unsigned char *__restrict__ con,
+                           const double *__restrict__ delu, int iter0) {
+  const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (r >= N) return;
+  u[r] += con[r] ? (iter0 ? delu[r] : 0.0) : du[r];
+}
+
+__global__ void k_norm2(long long N, const double *__restrict__ v, double *out, double *partials, unsigned *counter) {
+  __shared__ double lds[16];
+  const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  double acc[1] = {r < N ? v[r] * v[r] : 0.0};
+  if (grid_sum<1>(acc, partials, counter, lds)) *out = sqrt(acc[0]);
+}
+
+__global__ void k_zero(long long N, double *__restrict__ v) {
+  const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (r < N) v[r] = 0.0;
+}
+
+// ------------------------------------------------------------ host side
+inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+struct pa_context {
+  pa_material mat;
+  pa_problem pb;
+  MeshDims m;
+  double h[3], JxW;
+  int ns = 0;
+  long long nq = 0, ndof = 0, nnz = 0;
+  int method = PA_SOLVER_BICG, precision = PA_PREC_FP64;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double last_ms[3] = {0, 0, 0};
+  bool grains_set = false;
+  int n_grain_rows = 0;
+  // device memory
+  double *d_mat = nullptr, *d_gtab = nullptr;
+  int gstride = 0;
+  unsigned *d_egrain = nullptr;
+  double *Fp_conv = nullptr, *Fp_iter = nullptr, *Fe_conv = nullptr, *Fe_iter = nullptr, *s_conv = nullptr, *s_iter = nullptr;
+  double *slip_conv = nullptr, *slip_iter = nullptr, *rotnew = nullptr, *T = nullptr;
+  double *u = nullptr, *R = nullptr, *K = nullptr, *dinv = nullptr;
+  unsigned char *con = nullptr;
+  double *delu = nullptr;
+  double *kv[8] = {nullptr};  // Krylov vectors: X Rr Rl Zr Zl Pr Pl spare
+  KState *kst = nullptr;
+  double *partials = nullptr;
+  unsigned *counters = nullptr;
+  double *d_out = nullptr;
+  long long *d_rp = nullptr;
+  int bc_inc = -1;
+  std::vector<void *> allocs;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(pa_context *c, T **p, size_t count) {
+  void *q = nullptr;
+  HIPCHK(hipMalloc(&q, count * sizeof(T) + 16));
+  c->allocs.push_back(q);
+  *p = (T *)q;
+  return 0;
+}
+
+// per-grain constants: rotated Schmid tensors, Dmat (shear columns doubled), TM
+// (calculatePlasticity.cc:39-112, rotationOperations.cc:4-51, matrixOperations.cc:189-234)
+void grain_table(const pa_material &mat, const double mv[][3], const double nv[][3], const double r[3], int ns, int NS,
+                 double *out) {
+  double rr = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+  const double t1 = 1.0 - rr, t2 = 1.0 + rr;
+  double R[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) R[i][j] = (i == j ? t1 : 0.0) + 2.0 * r[i] * r[j];
+  R[0][1] -= 2.0 * r[2]; R[0][2] += 2.0 * r[1]; R[1][2] -= 2.0 * r[0];
+  R[1][0] += 2.0 * r[2]; R[2][0] -= 2.0 * r[1]; R[2][1] += 2.0 * r[0];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) R[i][j] *= 1.0 / t2;
+  double N[6][6];
+  const int pi[3][2] = {{1, 2}, {2, 0}, {0, 1}};
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) N[i][j] = R[i][j] * R[i][j];
+    for (int j = 0; j < 3; ++j) N[i][3 + j] = 2.0 * R[i][pi[j][0]] * R[i][pi[j][1]];
+  }
+  for (int i = 0; i < 3; ++i) {
+    const int a = pi[i][0], b = pi[i][1];
+    for (int j = 0; j < 3; ++j) N[3 + i][j] = R[a][j] * R[b][j];
+    for (int j = 0; j < 3; ++j) {
+      const int c = pi[j][0], d = pi[j][1];
+      N[3 + i][3 + j] = R[a][c] * R[b][d] + R[a][d] * R[b][c];
+    }
+  }
+  double Tm[6][6], D2[6][6];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) { double s = 0; for (int k = 0; k < 6; ++k) s += N[i][k] * mat.C[k][j]; Tm[i][j] = s; }
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) { double s = 0; for (int k = 0; k < 6; ++k) s += Tm[i][k] * N[j][k]; D2[i][j] = s; }
+  double *S = out, *D = out + 9 * NS, *TM = out + 9 * NS + 36;
+  for (int a = 0; a < ns; ++a) {
+    double mn[3][3], t[3][3];
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k) mn[j][k] = mv[a][j] * nv[a][k];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) { double s = 0; for (int k = 0; k < 3; ++k) s += R[i][k] * mn[k][j]; t[i][j] = s; }
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) { double s = 0; for (int k = 0; k < 3; ++k) s += t[i][k] * R[j][k]; S[9 * a + 3 * i + j] = s; }
+  }
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) D[6 * i + j] = (j >= 3 ? 2 * D2[i][j] : D2[i][j]);
+  const int v2[9] = {0, 5, 4, 5, 1, 3, 4, 3, 2};
+  for (int i = 0; i < 9; ++i)
+    for (int j = 0; j < 9; ++j) TM[9 * i + j] = D2[v2[i]][v2[j]];
+}
+
+int ns_template(int ns) { return ns <= 12 ? 12 : (ns <= 24 ? 24 : 48); }
+
+int launch_assemble(pa_context *c, int iteration) {
+  AsmArgs a;
+  a.m = c->m;
+  for (int d = 0; d < 3; ++d) a.h[d] = c->h[d];
+  a.JxW = c->JxW;
+  a.u = c->u; a.egrain = c->d_egrain; a.gtab = c->d_gtab; a.gstride = c->gstride; a.mat = c->d_mat;
+  a.ns = c->ns; a.max_solver_iter = c->mat.max_solver_iter; a.nq = c->nq;
+  a.Fp_conv = c->Fp_conv; a.s_conv = c->s_conv; a.slip_conv = c->slip_conv;
+  a.Fe_iter = c->Fe_iter; a.Fp_iter = c->Fp_iter; a.s_iter = c->s_iter; a.slip_iter = c->slip_iter;
+  a.con = c->con; a.delu = c->delu; a.iter0 = (iteration == 0);
+  a.K = c->K; a.R = c->R;
+  for (int col = 0; col < 8; ++col) {
+    a.cx = col & 1; a.cy = (col >> 1) & 1; a.cz = (col >> 2) & 1;
+    a.ncx = (c->m.nx - a.cx + 1) / 2; a.ncy = (c->m.ny - a.cy + 1) / 2; a.ncz = (c->m.nz - a.cz + 1) / 2;
+    const long long ncol = (long long)a.ncx * a.ncy * a.ncz;
+    if (ncol == 0) continue;
+    const unsigned grid = nblk(ncol, ASM_EL);
+    const int NS = ns_template(c->ns);
+    if (NS == 12) hipLaunchKernelGGL(k_assemble<12>, dim3(grid), dim3(64), 0, c->stream, a);
+    else if (NS == 24) hipLaunchKernelGGL(k_assemble<24>, dim3(grid), dim3(64), 0, c->stream, a);
+    else hipLaunchKernelGGL(k_assemble<48>, dim3(grid), dim3(64), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+  }
+  return 0;
+}
+
+int set_dirichlet(pa_context *c, int inc) {
+  if (c->bc_inc == inc) return 0;
+  BCArgs bc;
+  std::memset(&bc, 0, sizeof(bc));
+  for (int f = 0; f < 6; ++f)
+    for (int d = 0; d < 3; ++d) {
+      bc.fdc[f][d] = c->pb.face_dof_constrained[f][d];
+      bc.dU[f][d] = c->pb.face_total_u[f][d] / c->pb.total_increments;  // deluConstraint, init.cc:341
+    }
+  bc.cyclic = c->pb.cyclic;
+  bc.csign = 1.0;
+  if (c->pb.cyclic) {
+    bc.cface = c->pb.cyclic_face; bc.cdof = c->pb.cyclic_dof;
+    const int cf = bc.cface - 1, cd = bc.cdof - 1;
+    bc.dU[cf][cd] = bc.dU[cf][cd] * c->pb.total_increments * c->pb.delT / c->pb.quarter_cycle_time;  // init.cc:343-345
+    const double qt = c->pb.quarter_cycle_time, t = std::fmod(inc * c->pb.delT, 4 * qt);
+    bc.csign = (t < qt) ? 1.0 : (t < 3 * qt ? -1.0 : 1.0);
+  }
+  hipLaunchKernelGGL(k_dirichlet, dim3(nblk(c->m.nnode, 256)), dim3(256), 0, c->stream, c->m, bc, c->con, c->delu);
+  HIPCHK(hipGetLastError());
+  c->bc_inc = inc;
+  return 0;
+}
+
+int do_assemble(pa_context *c, int inc, int it, double *res_norm) {
+  int rc = set_dirichlet(c, inc);
+  if (rc) return rc;
+  HIPCHK(hipMemsetAsync(c->K, 0, sizeof(double) * 243 * (size_t)c->m.nnode, c->stream));
+  HIPCHK(hipMemsetAsync(c->R, 0, sizeof(double) * (size_t)c->ndof, c->stream));
+  rc = launch_assemble(c, it);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_dinv, dim3(nblk(c->ndof, 256)), dim3(256), 0, c->stream, c->m, c->K, c->dinv);
+  HIPCHK(hipGetLastError());
+  if (res_norm) {
+    hipLaunchKernelGGL(k_norm2, dim3(nblk(c->ndof, 256)), dim3(256), 0, c->stream, c->ndof, c->R, c->d_out + 16, c->partials,
+                       c->counters + 3);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(res_norm, c->d_out + 16, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return 0;
+}
+
+int do_solve(pa_context *c, int it, int *iters) {
+  const long long N = c->ndof;
+  double *X = c->kv[0], *Rr = c->kv[1], *Rl = c->kv[2], *Zr = c->kv[3], *Zl = c->kv[4], *Pr = c->kv[5], *Pl = c->kv[6];
+  const unsigned gv = nblk(N, 256), gn = nblk(c->m.nnode, 256);
+  hipLaunchKernelGGL(k_bicg_init, dim3(gv), dim3(256), 0, c->stream, N, c->R, X, Rr, Rl, Zr, Zl, c->dinv, c->kst, c->pb.lin_tol,
+                     c->pb.lin_maxit, c->partials, c->counters + 0);
+  HIPCHK(hipGetLastError());
+  KState hs;
+  const int chunk = 16;
+  for (int base = 0;; base += chunk) {
+    for (int k = 0; k < chunk; ++k) {
+      if (c->method == PA_SOLVER_CG) {
+        hipLaunchKernelGGL(k_cg_p, dim3(gv), dim3(256), 0, c->stream, N, Zr, Pr, (const KState *)c->kst, 0);
+        hipLaunchKernelGGL(k_cg_spmv<double>, dim3(gn), dim3(256), 0, c->stream, c->m, (const double *)c->K, (const double *)Pr, Zl,
+                           c->kst, c->partials, c->counters + 1);
+        hipLaunchKernelGGL(k_cg_update, dim3(gv), dim3(256), 0, c->stream, N, X, (const double *)Pr, Rr, (const double *)Zl, Zr,
+                           (const double *)c->dinv, c->kst, c->partials, c->counters + 2);
+      } else {
+        hipLaunchKernelGGL(k_bicg_p, dim3(gv), dim3(256), 0, c->stream, N, (const double *)Zr, (const double *)Zl, Pr, Pl,
+                           (const KState *)c->kst);
+        hipLaunchKernelGGL(k_bicg_spmv<double>, dim3(gn), dim3(256), 0, c->stream, c->m, (const double *)c->K, (const double *)Pr,
+                           (const double *)Pl, Zr, Zl, c->kst, c->partials, c->counters + 1);
+        hipLaunchKernelGGL(k_bicg_update, dim3(gv), dim3(256), 0, c->stream, N, X, (const double *)Pr, Rr, Rl, Zr, Zl,
+                           (const double *)c->dinv, c->kst, c->partials, c->counters + 2);
+      }
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&hs, c->kst, sizeof(KState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (hs.done) break;
+  }
+  if (iters) *iters = hs.it;
+  hipLaunchKernelGGL(k_apply_du, dim3(gv), dim3(256), 0, c->stream, N, c->u, (const double *)X, (const unsigned char *)c->con,
+                     (const double *)c->delu, (int)(it == 0));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *pa_last_error(void) { return g_err.c_str(); }
+
+int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out) {
+  if (!mat || !pb || !out) return fail(PA_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (mat->n_slip < 1 || mat->n_slip > PA_MAX_SLIP) return fail(PA_ERR_ARG, "n_slip out of range");
+  if (mat->max_slip_search != 1) return fail(PA_ERR_UNSUPPORTED, "Max Slip Search Iterations must be 1");
+  if (mat->max_solver_iter < 1) return fail(PA_ERR_ARG, "Max Solver Iterations must be >= 1");
+  if (mat->kinematic_hardening) return fail(PA_ERR_UNSUPPORTED, "kinematic hardening is not implemented");
+  if (mat->twinning) return fail(PA_ERR_UNSUPPORTED, "twinning is not implemented");
+  if (pb->nx < 1 || pb->ny < 1 || pb->nz < 1) return fail(PA_ERR_ARG, "mesh size must be >= 1");
+  if (pb->total_increments < 1 || pb->max_newton < 1 || pb->lin_maxit < 1) return fail(PA_ERR_ARG, "bad increment/solver counts");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(PA_ERR_HIP, "no HIP device available");
+  pa_context *c = new pa_context();
+  c->mat = *mat;
+  c->pb = *pb;
+  c->ns = mat->n_slip;
+  MeshDims &m = c->m;
+  m.nx = pb->nx; m.ny = pb->ny; m.nz = pb->nz;
+  m.nnx = m.nx + 1; m.nny = m.ny + 1; m.nnz = m.nz + 1;
+  m.nnode = (long long)m.nnx * m.nny * m.nnz;
+  m.nelem = (long long)m.nx * m.ny * m.nz;
+  c->nq = 8 * m.nelem;
+  c->ndof = 3 * m.nnode;
+  for (int d = 0; d < 3; ++d) c->h[d] = pb->span[d] / (d == 0 ? m.nx : d == 1 ? m.ny : m.nz);
+  c->JxW = c->h[0] * c->h[1] * c->h[2] / 8.0;
+  auto cnt = [](int i, int nn) { return (long long)(std::min(i + 1, nn - 1) - std::max(i - 1, 0) + 1); };
+  auto tot = [&](int nn) { long long s = 0; for (int i = 0; i < nn; ++i) s += cnt(i, nn); return s; };
+  c->nnz = 9 * tot(m.nnx) * tot(m.nny) * tot(m.nnz);
+  if (c->nnz >= (1LL << 31)) { delete c; return fail(PA_ERR_ARG, "mesh too large for int32 column indices"); }
+  int rc = 0;
+  if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return fail(PA_ERR_HIP, "hipStreamCreate failed"); }
+  for (int i = 0; i < 4; ++i) hipEventCreate(&c->ev[i]);
+  const int NS = ns_template(c->ns);
+  // material block
+  {
+    std::vector<double> mb(3 * NS + NS * NS + 1, 0.0);
+    for (int a = 0; a < c->ns; ++a) {
+      mb[a] = mat->h0[a]; mb[NS + a] = mat->ss[a]; mb[2 * NS + a] = mat->pw[a];
+      for (int b = 0; b < c->ns; ++b) mb[3 * NS + a * NS + b] = mat->q[a][b];
+    }
+    mb[3 * NS + NS * NS] = mat->stress_tol;
+    if ((rc = dalloc(c, &c->d_mat, mb.size()))) { pa_destroy(c); return rc; }
+    if (hipMemcpy(c->d_mat, mb.data(), mb.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) { pa_destroy(c); return fail(PA_ERR_HIP, "copy material"); }
+  }
+  c->gstride = 9 * NS + 36 + 81;
+  const long long nq = c->nq, nn = m.nnode, N = c->ndof;
+  const int ns = c->ns;
+  if ((rc = dalloc(c, &c->Fp_conv, 9 * nq)) || (rc = dalloc(c, &c->Fp_iter, 9 * nq)) || (rc = dalloc(c, &c->Fe_conv, 9 * nq)) ||
+      (rc = dalloc(c, &c->Fe_iter, 9 * nq)) || (rc = dalloc(c, &c->s_conv, ns * nq)) || (rc = dalloc(c, &c->s_iter, ns * nq)) ||
+      (rc = dalloc(c, &c->slip_conv, ns * nq)) || (rc = dalloc(c, &c->slip_iter, ns * nq)) || (rc = dalloc(c, &c->rotnew, 3 * nq)) ||
+      (rc = dalloc(c, &c->T, 9 * nq)) || (rc = dalloc(c, &c->u, N)) || (rc = dalloc(c, &c->R, N)) ||
+      (rc = dalloc(c, &c->K, 243 * nn)) || (rc = dalloc(c, &c->dinv, N)) || (rc = dalloc(c, &c->con, N)) ||
+      (rc = dalloc(c, &c->delu, N)) || (rc = dalloc(c, &c->d_egrain, m.nelem)) || (rc = dalloc(c, &c->kst, 1)) ||
+      (rc = dalloc(c, &c->partials, 16 * (size_t)nblk(std::max(N, nq), 64))) || (rc = dalloc(c, &c->counters, 16)) ||
+      (rc = dalloc(c, &c->d_out, 64))) {
+    pa_destroy(c);
+    return rc;
+  }
+  for (int i = 0; i < 7; ++i)
+    if ((rc = dalloc(c, &c->kv[i], N))) { pa_destroy(c); return rc; }
+  // initial state (crystalPlasticity::init, init.cc:302-330): Fp = Fe = I, s = s0, slip = 0
+  {
+    std::vector<double> hbuf(9 * nq, 0.0);
+    for (int i = 0; i < 3; ++i) std::fill(hbuf.begin() + (4 * i) * nq, hbuf.begin() + (4 * i + 1) * nq, 1.0);
+    if (hipMemcpy(c->Fp_conv, hbuf.data(), 9 * nq * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->Fe_conv, hbuf.data(), 9 * nq * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->Fp_iter, hbuf.data(), 9 * nq * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->Fe_iter, hbuf.data(), 9 * nq * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+      pa_destroy(c);
+      return fail(PA_ERR_HIP, "state init copy");
+    }
+    std::vector<double> sb((size_t)ns * nq);
+    for (int a = 0; a < ns; ++a) std::fill(sb.begin() + a * nq, sb.begin() + (a + 1) * nq, mat->s0[a]);
+    if (hipMemcpy(c->s_conv, sb.data(), sb.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->s_iter, sb.data(), sb.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+      pa_destroy(c);
+      return fail(PA_ERR_HIP, "state init copy");
+    }
+  }
+  if (hipMemset(c->slip_conv, 0, sizeof(double) * ns * nq) != hipSuccess || hipMemset(c->slip_iter, 0, sizeof(double) * ns * nq) != hipSuccess ||
+      hipMemset(c->u, 0, sizeof(double) * N) != hipSuccess || hipMemset(c->T, 0, sizeof(double) * 9 * nq) != hipSuccess ||
+      hipMemset(c->counters, 0, sizeof(unsigned) * 16) != hipSuccess || hipMemset(c->kst, 0, sizeof(KState)) != hipSuccess) {
+    pa_destroy(c);
+    return fail(PA_ERR_HIP, "memset");
+  }
+  *out = c;
+  return 0;
+}
+
+int pa_destroy(pa_context *c) {
+  if (!c) return 0;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (void *p : c->allocs) hipFree(p);
+  for (int i = 0; i < 4; ++i)
+    if (c->ev[i]) hipEventDestroy(c->ev[i]);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+int pa_sizes(pa_context *c, long long *nelem, long long *nqp, long long *ndof, long long *nnz) {
+  if (!c) return fail(PA_ERR_ARG, "null context");
+  if (nelem) *nelem = c->m.nelem;
+  if (nqp) *nqp = c->nq;
+  if (ndof) *ndof = c->ndof;
+  if (nnz) *nnz = c->nnz;
+  return 0;
+}
+
+int pa_element_grains(pa_context *c, const unsigned *vox, const int nv[3], unsigned *elem_grain) {
+  if (!c || !vox || !nv || !elem_grain) return fail(PA_ERR_ARG, "null argument");
+  if (nv[0] < 1 || nv[1] < 1 || nv[2] < 1) return fail(PA_ERR_ARG, "voxel counts must be >= 1");
+  const size_t nvox = (size_t)nv[0] * nv[1] * nv[2];
+  unsigned *dv = nullptr, *de = nullptr;
+  HIPCHK(hipMalloc(&dv, nvox * sizeof(unsigned)));
+  if (hipMalloc(&de, c->m.nelem * sizeof(unsigned)) != hipSuccess) { hipFree(dv); return fail(PA_ERR_HIP, "hipMalloc"); }
+  int rc = 0;
+  if (hipMemcpy(dv, vox, nvox * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess) rc = fail(PA_ERR_HIP, "copy voxels");
+  if (!rc) {
+    hipLaunchKernelGGL(k_element_grains, dim3(nblk(c->m.nelem, 256)), dim3(256), 0, c->stream, c->m, c->pb.span[0], c->pb.span[1],
+                       c->pb.span[2], (const unsigned *)dv, nv[0], nv[1], nv[2], de);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(PA_ERR_HIP, "grain kernel");
+  }
+  if (!rc && hipMemcpy(elem_grain, de, c->m.nelem * sizeof(unsigned), hipMemcpyDeviceToHost) != hipSuccess) rc = fail(PA_ERR_HIP, "copy grains");
+  hipFree(dv);
+  hipFree(de);
+  return rc;
+}
+
+int pa_set_grains(pa_context *c, const unsigned *elem_grain, const double *grain_rod, int n_grain_rows) {
+  if (!c || !elem_grain || !grain_rod || n_grain_rows < 1) return fail(PA_ERR_ARG, "null argument");
+  for (long long e = 0; e < c->m.nelem; ++e)
+    if ((int)elem_grain[e] >= n_grain_rows) return fail(PA_ERR_ARG, "grain id without an orientation row");
+  const int NS = ns_template(c->ns);
+  // normalised slip vectors (init.cc:73-123)
+  std::vector<double> mv(3 * c->ns), nv(3 * c->ns);
+  for (int a = 0; a < c->ns; ++a) {
+    double mn = 0, nn = 0;
+    for (int j = 0; j < 3; ++j) { mn += c->mat.m[a][j] * c->mat.m[a][j]; nn += c->mat.n[a][j] * c->mat.n[a][j]; }
+    mn = std::sqrt(mn); nn = std::sqrt(nn);
+    for (int j = 0; j < 3; ++j) { mv[3 * a + j] = c->mat.m[a][j] / mn; nv[3 * a + j] = c->mat.n[a][j] / nn; }
+  }
+  std::vector<double> tab((size_t)n_grain_rows * c->gstride, 0.0);
+  for (int g = 0; g < n_grain_rows; ++g)
+    grain_table(c->mat, (const double(*)[3])mv.data(), (const double(*)[3])nv.data(), grain_rod + 3 * g, c->ns, NS, &tab[(size_t)g * c->gstride]);
+  if (c->d_gtab) hipFree(c->d_gtab);
+  c->d_gtab = nullptr;
+  HIPCHK(hipMalloc(&c->d_gtab, tab.size() * sizeof(double)));
+  HIPCHK(hipMemcpy(c->d_gtab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_egrain, elem_grain, c->m.nelem * sizeof(unsigned), hipMemcpyHostToDevice));
+  // rotnew starts at the grain orientation of each qp (init.cc:225-245)
+  std::vector<double> rn(3 * c->nq);
+  for (long long e = 0; e < c->m.nelem; ++e)
+    for (int q = 0; q < 8; ++q)
+      for (int i = 0; i < 3; ++i) rn[i * c->nq + 8 * e + q] = grain_rod[3 * elem_grain[e] + i];
+  HIPCHK(hipMemcpy(c->rotnew, rn.data(), rn.size() * sizeof(double), hipMemcpyHostToDevice));
+  c->n_grain_rows = n_grain_rows;
+  c->grains_set = true;
+  return 0;
+}
+
+int pa_set_solver(pa_context *c, int method, int precision) {
+  if (!c) return fail(PA_ERR_ARG, "null context");
+  if (method != PA_SOLVER_BICG && method != PA_SOLVER_CG) return fail(PA_ERR_ARG, "unknown solver");
+  if (precision != PA_PREC_FP64) return fail(PA_ERR_UNSUPPORTED, "only the FP64 form is available");
+  c->method = method;
+  c->precision = precision;
+  return 0;
+}
+
+int pa_assemble(pa_context *c, int inc, int it, double *res_norm) {
+  if (!c) return fail(PA_ERR_ARG, "null context");
+  if (!c->grains_set) return fail(PA_ERR_STATE, "pa_set_grains has not been called");
+  HIPCHK(hipEventRecord(c->ev[0], c->stream));
+  int rc = do_assemble(c, inc, it, res_norm);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(c->ev[1], c->stream));
+  return 0;
+}
+
+int pa_linear_solve(pa_context *c, int inc, int it, int *iters) {
+  if (!c) return fail(PA_ERR_ARG, "null context");
+  (void)inc;
+  HIPCHK(hipEventRecord(c->ev[2], c->stream));
+  int rc = do_solve(c, it, iters);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(c->ev[3], c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) c->last_ms[0] = ms;
+  if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) c->last_ms[1] = ms;
+  return 0;
+}
+
+int pa_newton_iteration(pa_context *c, int inc, int it, double *res_norm, int *iters) {
+  int rc = pa_assemble(c, inc, it, res_norm);
+  if (rc) return rc;
+  return pa_linear_solve(c, inc, it, iters);
+}
+
+int pa_commit_increment(pa_context *c, double row[15]) {
+  if (!c) return fail(PA_ERR_ARG, "null context");
+  if (!c->grains_set) return fail(PA_ERR_STATE, "pa_set_grains has not been called");
+  PostArgs a;
+  a.m = c->m;
+  for (int d = 0; d < 3; ++d) a.h[d] = c->h[d];
+  a.JxW = c->JxW;
+  a.u = c->u; a.egrain = c->d_egrain; a.gtab = c->d_gtab; a.gstride = c->gstride; a.mat = c->d_mat;
+  a.ns = c->ns; a.max_solver_iter = c->mat.max_solver_iter; a.nq = c->nq;
+  a.Fp_conv = c->Fp_conv; a.s_conv = c->s_conv; a.slip_conv = c->slip_conv; a.Fe_conv = c->Fe_conv;
+  a.Fe_iter = c->Fe_iter; a.Fp_iter = c->Fp_iter; a.s_iter = c->s_iter; a.slip_iter = c->slip_iter; a.T = c->T; a.rotnew = c->rotnew;
+  a.partials = c->partials; a.counter = c->counters + 4; a.out = c->d_out;
+  HIPCHK(hipEventRecord(c->ev[0], c->stream));
+  const int NS = ns_template(c->ns);
+  const unsigned grid = nblk(c->nq, 256);
+  if (NS == 12) hipLaunchKernelGGL(k_post<12>, dim3(grid), dim3(256), 0, c->stream, a);
+  else if (NS == 24) hipLaunchKernelGGL(k_post<24>, dim3(grid), dim3(256), 0, c->stream, a);
+  else hipLaunchKernelGGL(k_post<48>, dim3(grid), dim3(256), 0, c->stream, a);
+  HIPCHK(hipGetLastError());
+  // _conv <- _iter (updateAfterIncrement.cc:298-305); _iter keeps its values
+  const size_t b9 = 9 * c->nq * sizeof(double), bs = (size_t)c->ns * c->nq * sizeof(double);
+  HIPCHK(hipMemcpyAsync(c->Fe_conv, c->Fe_iter, b9, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->Fp_conv, c->Fp_iter, b9, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->s_conv, c->s_iter, bs, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->slip_conv, c->slip_iter, bs, hipMemcpyDeviceToDevice, c->stream));
+  double h[13];
+  HIPCHK(hipMemcpyAsync(h, c->d_out, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipEventRecord(c->ev[1], c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) c->last_ms[2] = ms;
+  double vol = 0;
+  for (long long e = 0; e < 8 * c->m.nelem; ++e) vol += c->JxW;  // microvol = sum of JxW (updateAfterIncrement.cc:221)
+  if (row) {
+    for (int k = 0; k < 12; ++k) row[k] = h[k] / vol;
+    row[12] = 0.0;  // TwinRealVF: no twin system is ever active
+    row[13] = 0.0;  // TwinMade
+    row[14] = h[12] / vol;
+  }
+  c->bc_inc = -1;
+  return 0;
+}
+
+int pa_run_increment(pa_context *c, int inc, double row[15], int *lin_iters, double *res_norms) {
+  if (!c) return fail(PA_ERR_ARG, "null context");
+  double t_asm = 0, t_kry = 0;
+  for (int it = 0; it < c->pb.max_newton; ++it) {
+    double rn = 0;
+    int its = 0;
+    int rc = pa_newton_iteration(c, inc, it, res_norms ? &rn : nullptr, &its);
+    if (rc) return rc;
+    if (res_norms) res_norms[it] = rn;
+    if (lin_iters) lin_iters[it] = its;
+    t_asm += c->last_ms[0];
+    t_kry += c->last_ms[1];
+  }
+  int rc = pa_commit_increment(c, row);
+  c->last_ms[0] = t_asm;
+  c->last_ms[1] = t_kry;
+  return rc;
+}
+
+int pa_last_timings(pa_context *c, double ms[3]) {
+  if (!c || !ms) return fail(PA_ERR_ARG, "null argument");
+  for (int i = 0; i < 3; ++i) ms[i] = c->last_ms[i];
+  return 0;
+}
+
+int pa_get_state(pa_context *c, int which, double *host, long long n) {
+  if (!c || !host) return fail(PA_ERR_ARG, "null argument");
+  const long long nq = c->nq;
+  const double *src = nullptr;
+  int comps = 0;
+  bool node = false;
+  switch (which) {
+    case 0: src = c->T; comps = 9; break;
+    case 1: src = c->Fp_conv; comps = 9; break;
+    case 2: src = c->Fe_conv; comps = 9; break;
+    case 3: src = c->s_conv; comps = c->ns; break;
+    case 4: src = c->rotnew; comps = 3; break;
+    case 5: src = c->slip_conv; comps = c->ns; break;
+    case 6: src = c->u; comps = 3; node = true; break;
+    case 7: src = c->R; comps = 3; node = true; break;
+    case 8: src = c->Fp_iter; comps = 9; break;
+    case 9: src = c->Fe_iter; comps = 9; break;
+    case 10: src = c->s_iter; comps = c->ns; break;
+    default: return fail(PA_ERR_ARG, "unknown state id");
+  }
+  const long long cnt = node ? c->ndof : comps * nq;
+  if (n < cnt) return fail(PA_ERR_ARG, "host buffer too small");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (node) {
+    HIPCHK(hipMemcpy(host, src, cnt * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+  }
+  std::vector<double> tmp(cnt);
+  HIPCHK(hipMemcpy(tmp.data(), src, cnt * sizeof(double), hipMemcpyDeviceToHost));
+  for (long long Q = 0; Q < nq; ++Q)
+    for (int k = 0; k < comps; ++k) host[Q * comps + k] = tmp[k * nq + Q];
+  return 0;
+}
+
+int pa_get_csr(pa_context *c, long long *row_ptr, int *col_idx, double *values) {
+  if (!c || !row_ptr || !col_idx) return fail(PA_ERR_ARG, "null argument");
+  long long *drp = nullptr;
+  int *dci = nullptr;
+  double *dval = nullptr;
+  HIPCHK(hipMalloc(&drp, (c->ndof + 1) * sizeof(long long)));
+  int rc = 0;
+  if (hipMalloc(&dci, c->nnz * sizeof(int)) != hipSuccess) rc = fail(PA_ERR_HIP, "hipMalloc");
+  if (!rc && values && hipMalloc(&dval, c->nnz * sizeof(double)) != hipSuccess) rc = fail(PA_ERR_HIP, "hipMalloc");
+  if (!rc) {
+    hipLaunchKernelGGL(k_csr_rowptr, dim3(nblk(c->ndof + 1, 256)), dim3(256), 0, c->stream, c->m, drp);
+    hipLaunchKernelGGL(k_csr_cols, dim3(nblk(c->ndof, 4)), dim3(81, 4), 0, c->stream, c->m, (const long long *)drp, dci,
+                       (const double *)c->K, dval);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(PA_ERR_HIP, "csr kernels");
+  }
+  if (!rc && hipMemcpy(row_ptr, drp, (c->ndof + 1) * sizeof(long long), hipMemcpyDeviceToHost) != hipSuccess) rc = fail(PA_ERR_HIP, "copy");
+  if (!rc && hipMemcpy(col_idx, dci, c->nnz * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) rc = fail(PA_ERR_HIP, "copy");
+  if (!rc && values && hipMemcpy(values, dval, c->nnz * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) rc = fail(PA_ERR_HIP, "copy");
+  hipFree(drp);
+  hipFree(dci);
+  if (dval) hipFree(dval);
+  return rc;
+}
+
+int pa_qp_update(pa_context *c, long long n, const double *F, const double *Fp_conv, const double *s_conv, const unsigned *grain,
+                 int flag, double *P, double *T, double *dPdF, double *Fe, double *Fp, double *s_out, double *slip_inc, int *n_active) {
+  if (!c || n < 0 || !F || !Fp_conv || !s_conv || !grain || !P || !T || !Fe || !Fp || !s_out || !slip_inc || !n_active)
+    return fail(PA_ERR_ARG, "null argument");
+  if (flag && !dPdF) return fail(PA_ERR_ARG, "dPdF required for flag 1");
+  if (!c->grains_set) return fail(PA_ERR_STATE, "pa_set_grains has not been called");
+  for (long long i = 0; i < n; ++i)
+    if ((int)grain[i] >= c->n_grain_rows) return fail(PA_ERR_ARG, "grain index out of range");
+  if (n == 0) return 0;
+  const int ns = c->ns;
+  const size_t b9 = 9 * n * sizeof(double), bs = (size_t)ns * n * sizeof(double);
+  double *dF, *dFp, *ds, *dP, *dT, *dD = nullptr, *dFe, *dFpo, *dso, *dx;
+  unsigned *dg;
+  int *dna;
+  std::vector<void *> tmp;
+  auto al = [&](void **p, size_t b) { if (hipMalloc(p, b) != hipSuccess) return false; tmp.push_back(*p); return true; };
+  bool ok = al((void **)&dF, b9) && al((void **)&dFp, b9) && al((void **)&ds, bs) && al((void **)&dP, b9) && al((void **)&dT, b9) &&
+            al((void **)&dFe, b9) && al((void **)&dFpo, b9) && al((void **)&dso, bs) && al((void **)&dx, bs) &&
+            al((void **)&dg, n * sizeof(unsigned)) && al((void **)&dna, n * sizeof(int)) && (!flag || al((void **)&dD, 81 * n * sizeof(double)));
+  int rc = 0;
+  if (!ok) rc = fail(PA_ERR_HIP, "hipMalloc");
+  if (!rc && (hipMemcpy(dF, F, b9, hipMemcpyHostToDevice) != hipSuccess || hipMemcpy(dFp, Fp_conv, b9, hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy(ds, s_conv, bs, hipMemcpyHostToDevice) != hipSuccess || hipMemcpy(dg, grain, n * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess))
+    rc = fail(PA_ERR_HIP, "copy inputs");
+  if (!rc) {
+    const int NS = ns_template(ns);
+    const unsigned grid = nblk(n, 64);
+    if (NS == 12) hipLaunchKernelGGL(k_qp_batch<12>, dim3(grid), dim3(64), 0, c->stream, n, dF, dFp, ds, dg, c->d_gtab, c->gstride, c->d_mat, ns, c->mat.max_solver_iter, flag, dP, dT, dD, dFe, dFpo, dso, dx, dna);
+    else if (NS == 24) hipLaunchKernelGGL(k_qp_batch<24>, dim3(grid), dim3(64), 0, c->stream, n, dF, dFp, ds, dg, c->d_gtab, c->gstride, c->d_mat, ns, c->mat.max_solver_iter, flag, dP, dT, dD, dFe, dFpo, dso, dx, dna);
+    else hipLaunchKernelGGL(k_qp_batch<48>, dim3(grid), dim3(64), 0, c->stream, n, dF, dFp, ds, dg, c->d_gtab, c->gstride, c->d_mat, ns, c->mat.max_solver_iter, flag, dP, dT, dD, dFe, dFpo, dso, dx, dna);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(PA_ERR_HIP, "qp kernel");
+  }
+  if (!rc && (hipMemcpy(P, dP, b9, hipMemcpyDeviceToHost) != hipSuccess || hipMemcpy(T, dT, b9, hipMemcpyDeviceToHost) != hipSuccess ||
+              hipMemcpy(Fe, dFe, b9, hipMemcpyDeviceToHost) != hipSuccess || hipMemcpy(Fp, dFpo, b9, hipMemcpyDeviceToHost) != hipSuccess ||
+              hipMemcpy(s_out, dso, bs, hipMemcpyDeviceToHost) != hipSuccess || hipMemcpy(slip_inc, dx, bs, hipMemcpyDeviceToHost) != hipSuccess ||
+              hipMemcpy(n_active, dna, n * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+              (flag && hipMemcpy(dPdF, dD, 81 * n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)))
+    rc = fail(PA_ERR_HIP, "copy outputs");
+  for (void *p : tmp) hipFree(p);
+  return rc;
+}
+
+}  // extern "C"

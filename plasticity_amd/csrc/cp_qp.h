@@ -1,0 +1,551 @@
+// Per-quadrature-point rate-independent crystal-plasticity update and consistent
+// tangent (FP64), device side.  Restates calculatePlasticity(cellID, q, flag) of
+// src/materialModels/crystalPlasticity/calculatePlasticity.cc:13-959 for one qp
+// held by one lane:
+//   * per-grain constants (rotated Schmid tensors, stiffness in Voigt form with
+//     doubled shear columns, 9x9 TM) come from a table built once per grain
+//     (the reference rebuilds them on every call from the grain's orientation,
+//     :39-112; the values are the same);
+//   * only the n_slip real systems are carried.  The inert twin system the
+//     reference appends when twinning is off (s0 = 1e15, zero latent-hardening
+//     row/column, userInputParameters.cc:300-311, init.cc:178-185) can never
+//     enter the potentially-active set (b = |tau| - 1e15 < 0), so it changes
+//     nothing and is dropped;
+//   * Max Slip Search Iterations is 1, so the tangent's "previous" terms
+//     delFp_delF_prev and dels_delF_prev are zero (:147-148, :476-477) and
+//     dels_delF stays zero (:579); Max Solver Iterations may be > 1;
+//   * kinematic hardening is off (C_1 = C_2 = 0, backstress W = 0).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace cp {
+
+#define CP_DEV __device__ __forceinline__
+
+// ---------------------------------------------------------------- 3x3 helpers
+CP_DEV void m3_mul(const double *A, const double *B, double *C) {  // C = A B
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) C[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
+}
+CP_DEV void m3_tmul(const double *A, const double *B, double *C) {  // C = A^T B
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) C[3 * i + j] = A[i] * B[j] + A[3 + i] * B[3 + j] + A[6 + i] * B[6 + j];
+}
+CP_DEV void m3_mult(const double *A, const double *B, double *C) {  // C = A B^T
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) C[3 * i + j] = A[3 * i] * B[3 * j] + A[3 * i + 1] * B[3 * j + 1] + A[3 * i + 2] * B[3 * j + 2];
+}
+CP_DEV double m3_det(const double *A) {
+  return A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) + A[2] * (A[3] * A[7] - A[4] * A[6]);
+}
+CP_DEV void m3_inv(const double *A, double *B) {
+  const double r = 1.0 / m3_det(A);
+  B[0] = (A[4] * A[8] - A[5] * A[7]) * r;
+  B[1] = (A[2] * A[7] - A[1] * A[8]) * r;
+  B[2] = (A[1] * A[5] - A[2] * A[4]) * r;
+  B[3] = (A[5] * A[6] - A[3] * A[8]) * r;
+  B[4] = (A[0] * A[8] - A[2] * A[6]) * r;
+  B[5] = (A[2] * A[3] - A[0] * A[5]) * r;
+  B[6] = (A[3] * A[7] - A[4] * A[6]) * r;
+  B[7] = (A[1] * A[6] - A[0] * A[7]) * r;
+  B[8] = (A[0] * A[4] - A[1] * A[3]) * r;
+}
+CP_DEV double m3_fro(const double *A) {
+  double s = 0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) s += A[i] * A[i];
+  return sqrt(s);
+}
+CP_DEV double m3_ddot(const double *A, const double *B) {
+  double s = 0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) s += A[i] * B[i];
+  return s;
+}
+CP_DEV void m3_symm(double *A) {
+  double t = 0.5 * (A[1] + A[3]); A[1] = A[3] = t;
+  t = 0.5 * (A[2] + A[6]); A[2] = A[6] = t;
+  t = 0.5 * (A[5] + A[7]); A[5] = A[7] = t;
+}
+// T = matform(Dmat . vecform(E)), Voigt order (11,22,33,23,13,12)
+// (src/materialModels/crystalPlasticity/matrixOperations.cc:133-162)
+CP_DEV void dmat_apply(const double *__restrict__ D, const double *E, double *T) {
+  const double v[6] = {E[0], E[4], E[8], E[5], E[2], E[1]};
+  double t[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double s = 0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) s += D[6 * i + j] * v[j];
+    t[i] = s;
+  }
+  T[0] = t[0]; T[4] = t[1]; T[8] = t[2];
+  T[5] = T[7] = t[3]; T[2] = T[6] = t[4]; T[1] = T[3] = t[5];
+}
+// Taylor series exp with the reference's stop rule ||term||_F <= 1e-10 (matrixOperations.cc:165-186)
+CP_DEV void m3_expm(const double *A, double *E) {
+  double t[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, t2[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) E[i] = t[i];
+  double count = 1;
+  for (int it = 0; it < 64 && m3_fro(t) > 1.0e-10; ++it) {
+    m3_mul(t, A, t2);
+    const double r = 1 / count;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) { t[i] = t2[i] * r; E[i] += t[i]; }
+    count += 1.0;
+  }
+}
+
+// ------------------------------------------------------- small dense solvers
+// LU with partial pivoting on an n x n row-major matrix (ld = NS) with m right-hand
+// sides (row-major, ld = m).  Returns false when a pivot is <= n*eps*max|A|.
+template <int NS>
+CP_DEV bool lu_solve(int n, double *A, double *B, int m) {
+  double amax = 0;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) amax = fmax(amax, fabs(A[i * NS + j]));
+  const double tiny = n * 2.220446049250313e-16 * amax;
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    double pv = fabs(A[k * NS + k]);
+    for (int i = k + 1; i < n; ++i) {
+      const double v = fabs(A[i * NS + k]);
+      if (v > pv) { pv = v; p = i; }
+    }
+    if (!(pv > tiny)) return false;
+    if (p != k) {
+      for (int j = 0; j < n; ++j) { const double t = A[k * NS + j]; A[k * NS + j] = A[p * NS + j]; A[p * NS + j] = t; }
+      for (int j = 0; j < m; ++j) { const double t = B[k * m + j]; B[k * m + j] = B[p * m + j]; B[p * m + j] = t; }
+    }
+    const double inv = 1.0 / A[k * NS + k];
+    for (int i = k + 1; i < n; ++i) {
+      const double f = A[i * NS + k] * inv;
+      if (f == 0.0) continue;
+      for (int j = k + 1; j < n; ++j) A[i * NS + j] -= f * A[k * NS + j];
+      for (int j = 0; j < m; ++j) B[i * m + j] -= f * B[k * m + j];
+    }
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    const double inv = 1.0 / A[k * NS + k];
+    for (int j = 0; j < m; ++j) {
+      double s = B[k * m + j];
+      for (int c = k + 1; c < n; ++c) s -= A[k * NS + c] * B[c * m + j];
+      B[k * m + j] = s * inv;
+    }
+  }
+  return true;
+}
+
+// x = pinv(A) b by one-sided Jacobi SVD, singular values <= n*eps*sigma_max dropped.
+// Used only when LU finds A_PA numerically singular.
+template <int NS>
+CP_DEV void svd_pinv_solve(int n, const double *A, const double *b, double *x) {
+  double U[NS * NS], V[NS * NS];
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) { U[i * NS + j] = A[i * NS + j]; V[i * NS + j] = (i == j) ? 1.0 : 0.0; }
+  for (int sweep = 0; sweep < 80; ++sweep) {
+    bool rotated = false;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        double al = 0, be = 0, ga = 0;
+        for (int i = 0; i < n; ++i) {
+          const double up = U[i * NS + p], uq = U[i * NS + q];
+          al += up * up; be += uq * uq; ga += up * uq;
+        }
+        if (ga == 0.0 || fabs(ga) <= 1e-15 * sqrt(al * be)) continue;
+        rotated = true;
+        const double zeta = (be - al) / (2.0 * ga);
+        const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+        for (int i = 0; i < n; ++i) {
+          const double up = U[i * NS + p], uq = U[i * NS + q];
+          U[i * NS + p] = c * up - s * uq; U[i * NS + q] = s * up + c * uq;
+          const double vp = V[i * NS + p], vq = V[i * NS + q];
+          V[i * NS + p] = c * vp - s * vq; V[i * NS + q] = s * vp + c * vq;
+        }
+      }
+    if (!rotated) break;
+  }
+  double sig[NS], smax = 0;
+  for (int j = 0; j < n; ++j) {
+    double s = 0;
+    for (int i = 0; i < n; ++i) s += U[i * NS + j] * U[i * NS + j];
+    sig[j] = sqrt(s);
+    if (sig[j] > 0)
+      for (int i = 0; i < n; ++i) U[i * NS + j] /= sig[j];
+    smax = fmax(smax, sig[j]);
+  }
+  const double lim = smax * (n * 2.220446049250313e-16);
+  for (int i = 0; i < n; ++i) x[i] = 0;
+  for (int k = 0; k < n; ++k) {
+    if (!(sig[k] > lim)) continue;
+    double ub = 0;
+    for (int i = 0; i < n; ++i) ub += U[i * NS + k] * b[i];
+    ub /= sig[k];
+    for (int i = 0; i < n; ++i) x[i] += V[i * NS + k] * ub;
+  }
+}
+
+// ------------------------------------------------------------ table layouts
+// material block (doubles): h0[NS] ss[NS] pw[NS] q[NS*NS] tol
+template <int NS> struct MatLayout {
+  static constexpr int h0 = 0, ss = NS, pw = 2 * NS, q = 3 * NS, tol = 3 * NS + NS * NS, size = tol + 1;
+};
+// grain block (doubles): S[NS][9] Dmat[36] TM[81]
+template <int NS> struct GrainLayout {
+  static constexpr int S = 0, D = 9 * NS, TM = 9 * NS + 36, size = 9 * NS + 36 + 81;
+};
+
+// dexp series of calculatePlasticity.cc:645-676 (sign = -1) and :748-779 (sign = +1):
+// out = S + sum_{k=1..count3} c_k sum_{l=0..k} X^l S X^(k-l), c_k = (+-1)^k / (k+1)!
+#define CP_MAXPOW 24
+CP_DEV void dexp_series(const double (*Xp)[9], const double *S, int count3, bool neg, double *out) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) out[i] = S[i];
+  double fact = 1.0;  // (k+1)!
+  for (int k = 1; k <= count3; ++k) {
+    fact *= (double)(k + 1);
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, t5[9], t6[9];
+    for (int l = 0; l <= k; ++l) {
+      m3_mul(Xp[l], S, t5);
+      m3_mul(t5, Xp[k - l], t6);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) acc[i] += t6[i];
+    }
+    const double c = ((neg && (k & 1)) ? -1.0 : 1.0) / fact;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) out[i] += c * acc[i];
+  }
+}
+
+// Return: number of systems left active after inactive-slip removal (0 = elastic).
+// s holds s_conv on entry and s_tau on exit; x receives x_beta_old (the slip increments).
+// D (flag = 1) receives dP/dF as D[(3i+k)*ldD ...] with element stride `dstride`:
+// D[((3i+k)*9 + 3j+l) * dstride] = dP_ik / dF_jl.
+template <int NS>
+CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ gt, int ns, int max_solver_iter,
+                     const double *F, const double *Fp_t, double *s, double *x, int flag, double *P, double *T,
+                     double *Fe_out, double *Fp_out, double *D, int dstride) {
+  typedef MatLayout<NS> ML;
+  typedef GrainLayout<NS> GL;
+  const double tol1 = mat[ML::tol];
+  const double *__restrict__ Dm = gt + GL::D;
+  const double *__restrict__ TM = gt + GL::TM;
+  const double *__restrict__ Sg = gt + GL::S;
+
+  double Fpinv[9], FE[9], CE[9], E[9], Tst[9], tmp[9], tmp2[9];
+  m3_inv(Fp_t, Fpinv);                                   // :142, :160-161
+  m3_mul(F, Fpinv, FE);                                  // :143, :164
+  m3_tmul(FE, FE, CE);                                   // :168
+#pragma unroll
+  for (int i = 0; i < 9; ++i) E[i] = 0.5 * (CE[i] - ((i % 4 == 0) ? 1.0 : 0.0));
+  dmat_apply(Dm, E, Tst);                                // :184-186
+  double Fp[9], Ce[9], TstTau[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) { Fp[i] = Fp_t[i]; Ce[i] = CE[i]; TstTau[i] = Tst[i]; }
+
+  double tau_tr[NS], b[NS];
+  int PA[NS];
+  int nPA = 0;
+  m3_mul(CE, Tst, tmp);                                  // :203-240
+  for (int a = 0; a < ns; ++a) {
+    const double r = m3_ddot(tmp, Sg + 9 * a);
+    tau_tr[a] = r;
+    b[a] = fabs(r) - s[a];
+    x[a] = 0.0;
+    if (b[a] >= tol1) PA[nPA++] = a;
+  }
+  const bool plastic = nPA > 0;
+  double detFE;
+  int n_active = 0;
+
+  // ---- plastic correction (:250-468)
+  double A[NS * NS];  // A over the initial potentially-active set, indexed by positions in PA0
+  int PA0[NS];
+  int n0 = nPA;
+  double h[NS];
+  double XdelFp[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (plastic) {
+    for (int i = 0; i < n0; ++i) PA0[i] = PA[i];
+    for (int a = 0; a < ns; ++a) h[a] = mat[ML::h0 + a] * pow((1 - s[a] / mat[ML::ss + a]), mat[ML::pw + a]);
+    for (int jj = 0; jj < n0; ++jj) {                   // :273-306 restricted to PA x PA
+      const int j = PA0[jj];
+      double t2[9], t1[9], t3[9];
+      m3_mul(CE, Sg + 9 * j, t2);
+      m3_symm(t2);
+      dmat_apply(Dm, t2, t3);
+      m3_mul(CE, t3, t1);
+      m3_mul(t2, Tst, t3);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) t1[k] += 2.0 * t3[k];
+      for (int ii = 0; ii < n0; ++ii) {
+        const int i = PA0[ii];
+        const double d = m3_ddot(Sg + 9 * i, t1);
+        const bool opp = (tau_tr[i] * tau_tr[j]) < 0.0;
+        A[ii * NS + jj] = mat[ML::q + i * NS + j] * h[j] + (opp ? -d : d);
+      }
+    }
+    // PA as positions into PA0 from here on
+    int pos[NS];
+    for (int i = 0; i < n0; ++i) pos[i] = i;
+    int npos = n0;
+    auto linf_b = [&]() {
+      double m = 0;
+      for (int i = 0; i < npos; ++i) m = fmax(m, fabs(b[PA0[pos[i]]]));
+      return m;
+    };
+    int count1 = 0;
+    while (linf_b() > tol1) {                            // :322
+      count1++;
+      if (count1 > max_solver_iter) break;
+      // inactive_slip_removal (inactiveSlipRemoval.cc:4-124)
+      double xb[NS];
+      for (int a = 0; a < ns; ++a) xb[a] = 0.0;
+      while (true) {
+        double Ap[NS * NS], bp[NS];
+        for (int i = 0; i < npos; ++i) {
+          bp[i] = b[PA0[pos[i]]];
+          for (int j = 0; j < npos; ++j) Ap[i * NS + j] = A[pos[i] * NS + pos[j]];
+        }
+        double xs[NS];
+        for (int i = 0; i < npos; ++i) xs[i] = bp[i];
+        if (npos > 0 && !lu_solve<NS>(npos, Ap, xs, 1)) {
+          for (int i = 0; i < npos; ++i)
+            for (int j = 0; j < npos; ++j) Ap[i * NS + j] = A[pos[i] * NS + pos[j]];
+          svd_pinv_solve<NS>(npos, Ap, bp, xs);
+        }
+        for (int a = 0; a < ns; ++a) xb[a] = 0.0;
+        for (int i = 0; i < npos; ++i) xb[PA0[pos[i]]] = xs[i];
+        int keep = 0;
+        bool removed = false;
+        for (int i = 0; i < npos; ++i) {
+          const int a = PA0[pos[i]];
+          if ((x[a] + xb[a]) < 0) removed = true; else pos[keep++] = pos[i];
+        }
+        if (!removed) break;
+        npos = keep;
+      }
+      for (int a = 0; a < ns; ++a) x[a] += xb[a];        // :119
+      for (int i = 0; i < 9; ++i) XdelFp[i] = 0.0;      // :338-357
+      for (int a = 0; a < ns; ++a) {
+        const double g = (tau_tr[a] > 0) ? x[a] : -x[a];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) XdelFp[k] += g * Sg[9 * a + k];
+      }
+      double ex[9];
+      m3_expm(XdelFp, ex);
+      m3_mul(ex, Fp_t, Fp);                              // :359
+      m3_inv(Fp, tmp);                                   // :362-373
+      m3_mul(F, tmp, FE);
+      m3_tmul(FE, FE, Ce);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) E[i] = 0.5 * (Ce[i] - ((i % 4 == 0) ? 1.0 : 0.0));
+      dmat_apply(Dm, E, TstTau);                         // :376-377
+      m3_mul(Ce, TstTau, tmp);
+      for (int a = 0; a < ns; ++a) {                     // :380-440
+        double h1 = 0;
+        for (int c = 0; c < ns; ++c) h1 += mat[ML::q + a * NS + c] * h[c] * xb[c];
+        double sa = s[a] + h1;
+        if (sa > mat[ML::ss + a]) sa = mat[ML::ss + a];
+        s[a] = sa;
+        b[a] = fabs(m3_ddot(tmp, Sg + 9 * a)) - sa;
+      }
+    }
+    n_active = npos;
+    // final active set as system indices
+    for (int i = 0; i < npos; ++i) PA[i] = PA0[pos[i]];
+    nPA = npos;
+    m3_inv(Fp, Fpinv);                                   // :458-460
+    m3_mul(F, Fpinv, FE);
+  }
+
+  // ---- stresses (:188-196 elastic, :458-468 plastic)
+  detFE = m3_det(FE);
+  m3_mul(FE, TstTau, tmp);
+  {
+    const double r = 1.0 / detFE;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) tmp[i] *= r;
+  }
+  m3_mult(tmp, FE, T);
+  double Finv[9];
+  m3_inv(F, Finv);
+  m3_mult(T, Finv, tmp2);
+  const double pf = plastic ? m3_det(F) : detFE;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) P[i] = pf * tmp2[i];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) { Fe_out[i] = FE[i]; Fp_out[i] = Fp[i]; }
+  if (!flag) return n_active;
+
+  // ---- tangent: in-loop block (:473-828) with delFp_delF_prev = 0 and dels_delF = 0
+  double dFp[81];
+#pragma unroll
+  for (int i = 0; i < 81; ++i) dFp[i] = 0.0;
+  if (plastic && nPA > 0) {
+    // count3 (:607-619)
+    double Xp[CP_MAXPOW][9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Xp[0][i] = (i % 4 == 0) ? 1.0 : 0.0;
+    int count3 = 0;
+    {
+      double t2[9], t1[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) t2[i] = Xp[0][i];
+      double tol2 = 1.0;
+      const double lim = fmax(tol1 / 1e4, 1e-12);
+      while (tol2 > lim && count3 < CP_MAXPOW - 1) {
+        count3++;
+        m3_mul(XdelFp, t2, t1);
+        const double r = 1.0 / count3;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) t1[i] *= r;
+        tol2 = m3_fro(t1);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) t2[i] = t1[i];
+      }
+      for (int p = 1; p <= count3; ++p) m3_mul(Xp[p - 1], XdelFp, Xp[p]);
+    }
+    // delb (:581-604): delb(k, kl) = +- S_k : deltau_delF(:, kl), streamed per column kl
+    double rhs[NS * 9];
+    for (int kl = 0; kl < 9; ++kl) {
+      const int k = kl / 3, l = kl % 3;
+      double dE[9], dT[9], dtau[9];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) dE[3 * i + j] = 0.5 * (Fpinv[3 * l + i] * FE[3 * k + j] + Fpinv[3 * l + j] * FE[3 * k + i]);
+#pragma unroll
+      for (int r = 0; r < 9; ++r) {
+        double sacc = 0;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) sacc += TM[9 * r + c] * dE[c];
+        dT[r] = sacc;
+      }
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          double sacc = 0;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) sacc += 2 * dE[3 * i + a] * TstTau[3 * a + j] + Ce[3 * i + a] * dT[3 * a + j];
+          dtau[3 * i + j] = sacc;
+        }
+      for (int kk = 0; kk < nPA; ++kk) {
+        const int a = PA[kk];
+        const double v = m3_ddot(dtau, Sg + 9 * a);
+        rhs[kk * 9 + kl] = (tau_tr[a] < 0) ? -v : v;
+      }
+    }
+    // A2 (:622-709)
+    double A2[NS * NS];
+    double eneg[9], Xneg[9], t6[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Xneg[i] = -XdelFp[i];
+    m3_expm(Xneg, eneg);
+    m3_mul(CE, eneg, t6);
+    for (int jj = 0; jj < nPA; ++jj) {
+      const int j = PA[jj];
+      double diff[9], t2[9], t3[9], tm[9];
+      dexp_series(Xp, Sg + 9 * j, count3, true, diff);
+      m3_tmul(diff, t6, t2);
+      m3_symm(t2);
+      dmat_apply(Dm, t2, t3);
+      m3_mul(Ce, t3, tm);
+      m3_mul(t2, TstTau, t3);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) tm[k] += 2.0 * t3[k];
+      const bool nj = tau_tr[j] < 0.0;
+      for (int ii = 0; ii < nPA; ++ii) {
+        const int i = PA[ii];
+        const double d = m3_ddot(Sg + 9 * i, tm);
+        const bool ni = tau_tr[i] < 0.0;
+        A2[ii * NS + jj] = mat[ML::q + i * NS + j] * h[j] + ((ni ^ nj) ? -d : d);
+      }
+    }
+    // delgamma = A2^-1 (delb - dels_PA) with dels = 0 (:712-722)
+    if (!lu_solve<NS>(nPA, A2, rhs, 9)) {
+      for (int i = 0; i < nPA * 9; ++i) rhs[i] = 0.0;  // singular A2: no plastic tangent contribution
+    }
+    // S_PA and delFp_delF = S_PA delgamma (:734-821)
+    for (int jj = 0; jj < nPA; ++jj) {
+      const int j = PA[jj];
+      double diff[9], t1[9];
+      dexp_series(Xp, Sg + 9 * j, count3, false, diff);
+      m3_mul(diff, Fp_t, t1);
+      const double sg = (tau_tr[j] < 0) ? -1.0 : 1.0;
+#pragma unroll
+      for (int r = 0; r < 9; ++r) {
+        const double sv = sg * t1[r];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) dFp[9 * r + c] += sv * rhs[jj * 9 + c];
+      }
+    }
+  }
+
+  // ---- final tangent (:835-940), streamed per column kl
+  double q1[9], q2[9], q3[9], q5[9];
+  m3_mul(Finv, FE, q2);                                  // temp = temp2 = F^-1 Fe
+  m3_mult(TstTau, q2, q1);                               // T* (F^-1 Fe)^T
+  m3_mul(FE, TstTau, q3);
+  m3_mult(q3, F, q5);
+  for (int kl = 0; kl < 9; ++kl) {
+    const int k = kl / 3, l = kl % 3;
+    double dFe[9], W[9], dTs[9];
+    // dFe(ij,kl) = d_ik Fpinv(l,j) - sum_ab FE(i,a) dFp(ab,kl) Fpinv(b,j)
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double sacc = 0;
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb) sacc += dFp[9 * (3 * a + bb) + kl] * Fpinv[3 * bb + j];
+        W[3 * a + j] = sacc;  // (dFp . Fpinv)(a, j)
+      }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double sacc = (i == k) ? Fpinv[3 * l + j] : 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) sacc -= FE[3 * i + a] * W[3 * a + j];
+        dFe[3 * i + j] = sacc;
+      }
+    // dTs(ij,kl) = sum_ab TM(ij,ab) (dFe^T FE)(a,b)
+    double G[9];
+    m3_tmul(dFe, FE, G);
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+      double sacc = 0;
+#pragma unroll
+      for (int c = 0; c < 9; ++c) sacc += TM[9 * r + c] * G[c];
+      dTs[r] = sacc;
+    }
+    // PK(ij,kl) = (dFe q1)(i,j) + (FE dTs q2^T)(i,j) - (q3 dFe Finv^T)(i,j) - sum_a q5(i,a) Finv(j,k) Finv(l,a)
+    double t1[9], t2[9], t3[9], t4[9];
+    m3_mul(dFe, q1, t1);
+    m3_mul(FE, dTs, t4);
+    m3_mult(t4, q2, t2);
+    m3_mul(q3, dFe, t4);
+    m3_mult(t4, Finv, t3);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double qa = 0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) qa += q5[3 * i + a] * Finv[3 * l + a];
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        D[((3 * i + j) * 9 + kl) * dstride] = t1[3 * i + j] + t2[3 * i + j] - t3[3 * i + j] - qa * Finv[3 * j + k];
+    }
+  }
+  return n_active;
+}
+
+}  // namespace cp

@@ -1,0 +1,321 @@
+"""Readers for the reference's input formats and a ctypes driver for the oracle.
+
+Test infrastructure only.  File formats follow the reference readers:
+  prm.prm            deal.II ParameterHandler "set Key = value" (src/userInputParameters/userInputParameters.cc:17-545)
+  slip/twin files    one vector per line, normalised on read (src/materialModels/crystalPlasticity/init.cc:73-123)
+  latent hardening   n_slip x n_slip table (init.cc:51-69)
+  grain-ID voxels    header lines, then for x: for y: one line of z values (src/utilityObjects/crystalOrientationsIO.cc:85-105)
+  orientations       1 header line, then "id r1 r2 r3" (crystalOrientationsIO.cc:24-53)
+  BC file            header lines, then "face dof totalU" (src/ellipticBVP/init.cc:327-348)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+MAXT = 49
+
+
+def parse_prm(path):
+    out = {}
+    with open(path) as f:
+        for line in f:
+            line = line.split("#", 1)[0].strip()
+            if not line.startswith("set "):
+                continue
+            k, v = line[4:].split("=", 1)
+            out[" ".join(k.split())] = v.strip()
+    return out
+
+
+def _floats(s):
+    return [float(x) for x in s.replace(",", " ").split()]
+
+
+def read_vectors(path, n):
+    rows = []
+    with open(path) as f:
+        for line in f:
+            vals = line.split()
+            if len(vals) < 3:
+                continue
+            v = np.array([float(x) for x in vals[:3]])
+            rows.append(v / np.sqrt((v * v).sum()))
+            if len(rows) == n:
+                break
+    return np.array(rows)
+
+
+def read_latent(path, n):
+    rows = []
+    with open(path) as f:
+        for line in f:
+            vals = line.split()
+            if not vals:
+                continue
+            rows.append([float(x) for x in vals[:n]])
+            if len(rows) == n:
+                break
+    return np.array(rows)
+
+
+def read_voxels(path, header, nv):
+    with open(path) as f:
+        lines = f.read().split("\n")[header:]
+    vox = np.zeros(nv, dtype=np.uint32)
+    li = 0
+    for x in range(nv[0]):
+        for y in range(nv[1]):
+            vals = lines[li].split()
+            li += 1
+            vox[x, y, :] = [int(v) for v in vals[: nv[2]]]
+    return vox
+
+
+def read_orientations(path):
+    table = {}
+    with open(path) as f:
+        f.readline()
+        for line in f:
+            vals = line.split()
+            if len(vals) >= 4:
+                table[int(vals[0])] = [float(v) for v in vals[1:4]]
+    n = max(table) + 1
+    rod = np.zeros((n, 3))
+    for k, v in table.items():
+        rod[k] = v
+    return rod
+
+
+def read_bcs(path, header, count):
+    with open(path) as f:
+        lines = [l for l in f.read().split("\n")]
+    rules = []
+    for line in lines[header: header + count]:
+        v = line.split()
+        rules.append((int(v[0]), int(v[1]), float(v[2])))
+    return rules
+
+
+class Material(ctypes.Structure):
+    _fields_ = [
+        ("n_slip", ctypes.c_int), ("n_twin", ctypes.c_int), ("nT", ctypes.c_int),
+        ("m", ctypes.c_double * 3 * MAXT), ("n", ctypes.c_double * 3 * MAXT),
+        ("q", ctypes.c_double * MAXT * MAXT), ("C", ctypes.c_double * 6 * 6),
+        ("s0", ctypes.c_double * MAXT), ("h0", ctypes.c_double * MAXT),
+        ("ss", ctypes.c_double * MAXT), ("pw", ctypes.c_double * MAXT),
+        ("C1", ctypes.c_double * MAXT), ("C2", ctypes.c_double * MAXT),
+        ("tol", ctypes.c_double), ("max_slip_search", ctypes.c_int),
+        ("max_solver_iter", ctypes.c_int), ("pinv_mode", ctypes.c_int),
+    ]
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("span", ctypes.c_double * 3), ("n_increments", ctypes.c_int),
+        ("total_increments", ctypes.c_int), ("delT", ctypes.c_double),
+        ("max_newton", ctypes.c_int), ("lin_tol", ctypes.c_double),
+        ("lin_maxit", ctypes.c_int), ("solver", ctypes.c_int),
+        ("face_dof_constrained", ctypes.c_int * 3 * 6),
+        ("face_total_u", ctypes.c_double * 3 * 6),
+        ("cyclic", ctypes.c_int), ("cyclic_face", ctypes.c_int), ("cyclic_dof", ctypes.c_int),
+        ("quarter_cycle_time", ctypes.c_double), ("nthreads", ctypes.c_int),
+    ]
+
+
+class Case:
+    """Material + problem + grains, from explicit arrays (the oracle's view of a prm case)."""
+
+    def __init__(self, m, n, q, C, s0, h0, ss, pw, tol=1e-9, max_slip_search=1,
+                 max_solver_iter=1, nx=8, ny=8, nz=8, span=(1.0, 1.0, 1.0), delT=1.0,
+                 total_time=10.0, bcs=((1, 1, 0.0), (1, 2, 0.0), (1, 3, 0.0), (2, 1, 0.01)),
+                 max_newton=4, lin_tol=1e-10, lin_maxit=50000, elem_grain=None, grain_rod=None,
+                 solver=0):
+        self.m = np.asarray(m, float)
+        self.n = np.asarray(n, float)
+        self.q = np.asarray(q, float)
+        self.C = np.asarray(C, float)
+        self.n_slip = len(self.m)
+        self.s0, self.h0, self.ss, self.pw = [np.asarray(x, float) for x in (s0, h0, ss, pw)]
+        self.tol, self.max_slip_search, self.max_solver_iter = tol, max_slip_search, max_solver_iter
+        self.nx, self.ny, self.nz, self.span = nx, ny, nz, tuple(span)
+        self.delT, self.total_time = delT, total_time
+        self.total_increments = int(round(total_time / delT))
+        self.bcs = list(bcs)
+        self.max_newton, self.lin_tol, self.lin_maxit, self.solver = max_newton, lin_tol, lin_maxit, solver
+        self.elem_grain = None if elem_grain is None else np.asarray(elem_grain, np.uint32)
+        self.grain_rod = None if grain_rod is None else np.asarray(grain_rod, float)
+        self.cyclic = (0, 1, 1, 1.0)
+
+    # --- reference defaults for the inert twin system (userInputParameters.cc:300-311)
+    def material(self, pinv_mode=1):
+        mat = Material()
+        ns = self.n_slip
+        mat.n_slip, mat.n_twin, mat.nT = ns, 1, ns + 1
+        for a in range(ns):
+            for j in range(3):
+                mat.m[a][j] = self.m[a, j]
+                mat.n[a][j] = self.n[a, j]
+            mat.s0[a], mat.h0[a], mat.ss[a], mat.pw[a] = self.s0[a], self.h0[a], self.ss[a], self.pw[a]
+            for b in range(ns):
+                mat.q[a][b] = self.q[a, b]
+        # dummy twin: n=(1,0,0), m=(0,1,0), s0=10e14, h0=1, a=0, s_s=10e15 (init.cc:178-185)
+        mat.n[ns][0], mat.m[ns][1] = 1.0, 1.0
+        mat.s0[ns], mat.h0[ns], mat.pw[ns], mat.ss[ns] = 10e14, 1.0, 0.0, 10e15
+        for i in range(6):
+            for j in range(6):
+                mat.C[i][j] = self.C[i, j]
+        mat.tol, mat.max_slip_search, mat.max_solver_iter = self.tol, self.max_slip_search, self.max_solver_iter
+        mat.pinv_mode = pinv_mode
+        return mat
+
+    def problem(self, n_increments=None, nthreads=1):
+        pb = Problem()
+        pb.nx, pb.ny, pb.nz = self.nx, self.ny, self.nz
+        for d in range(3):
+            pb.span[d] = self.span[d]
+        pb.total_increments = self.total_increments
+        pb.n_increments = self.total_increments if n_increments is None else n_increments
+        pb.delT = self.delT
+        pb.max_newton, pb.lin_tol, pb.lin_maxit, pb.solver = self.max_newton, self.lin_tol, self.lin_maxit, self.solver
+        for f, d, u in self.bcs:
+            pb.face_dof_constrained[f - 1][d - 1] = 1
+            pb.face_total_u[f - 1][d - 1] = u
+        pb.cyclic, pb.cyclic_face, pb.cyclic_dof, pb.quarter_cycle_time = self.cyclic
+        pb.nthreads = nthreads
+        return pb
+
+    @property
+    def nT(self):
+        return self.n_slip + 1
+
+
+def load_reference_case(folder):
+    """A Case built from one of the reference application folders copied under tests/golden."""
+    d = os.path.join(GOLDEN, folder)
+    prm = parse_prm(os.path.join(d, "prm.prm"))
+    ns = int(prm["Number of Slip Systems"])
+    C = np.array([_floats(prm["Elastic Stiffness row %d" % (i + 1)]) for i in range(6)])
+    m = read_vectors(os.path.join(d, prm["Slip Directions File"]), ns)
+    n = read_vectors(os.path.join(d, prm["Slip Normals File"]), ns)
+    q = read_latent(os.path.join(d, prm["Latent Hardening Ratio filename"]), ns)
+    refine = int(prm.get("Refine factor", "0"))
+    sub = [int(prm.get("Subdivisions %s" % a, "1")) for a in "XYZ"]
+    nel = [s * 2 ** refine for s in sub]
+    span = [float(prm["Domain size %s" % a]) for a in "XYZ"]
+    nv = [int(float(prm["Voxels in %s direction" % a])) for a in "XYZ"]
+    vox = read_voxels(os.path.join(d, prm["Grain ID file name"]), int(prm.get("Header Lines GrainID File", "0")), nv)
+    rod = read_orientations(os.path.join(d, prm["Orientations file name"]))
+    bcs = read_bcs(os.path.join(d, prm["Boundary condition filename"]), int(prm["BC file number of header lines"]),
+                   int(prm["Number of boundary conditions"]))
+    case = Case(m, n, q, C, _floats(prm["Initial Slip Resistance"]), _floats(prm["Initial Hardening Modulus"]),
+                _floats(prm["Saturation Stress"]), _floats(prm["Power Law Exponent"]),
+                tol=float(prm["Stress Tolerance"]), max_slip_search=int(prm["Max Slip Search Iterations"]),
+                max_solver_iter=int(prm["Max Solver Iterations"]), nx=nel[0], ny=nel[1], nz=nel[2], span=span,
+                delT=float(prm["Time increments"]), total_time=float(prm["Total time"]), bcs=bcs,
+                max_newton=int(prm["Maximum non linear iterations"]),
+                lin_tol=float(prm["Relative linear solver tolerance"]),
+                lin_maxit=int(prm["Maximum linear solver iterations"]))
+    case.vox, case.nv = vox, nv
+    case.grain_rod = rod
+    case.elem_grain = oracle_element_grains(case)
+    golden = os.path.join(d, "correct_stressstrain.txt")
+    case.golden = np.loadtxt(golden, skiprows=1) if os.path.exists(golden) else None
+    return case
+
+
+# ------------------------------------------------------------------ oracle
+_ORACLE = None
+
+
+def oracle():
+    global _ORACLE
+    if _ORACLE is None:
+        path = os.path.join(ROOT, "oracle", "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        lib = ctypes.CDLL(path)
+        D, I, P = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), ctypes.c_void_p
+        lib.or_qp_update.restype = ctypes.c_int
+        lib.or_simulate.restype = ctypes.c_int
+        lib.or_sparsity.restype = ctypes.c_longlong
+        lib.or_material_id.restype = ctypes.c_uint
+        _ORACLE = lib
+    return _ORACLE
+
+
+def ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def oracle_element_grains(case):
+    lib = oracle()
+    nel = case.nx * case.ny * case.nz
+    out = np.zeros(nel, np.uint32)
+    vox = np.ascontiguousarray(case.vox, dtype=np.uint32)
+    nv = np.array(case.nv, np.int32)
+    pb = case.problem()
+    lib.or_element_grains(ptr(vox), ptr(nv), ctypes.byref(pb), ptr(out))
+    return out
+
+
+def oracle_simulate(case, n_increments, pinv_mode=1, nthreads=1, want_state=False, solver=None):
+    lib = oracle()
+    if solver is not None:
+        case.solver = solver
+    mat = case.material(pinv_mode)
+    pb = case.problem(n_increments, nthreads)
+    rows = np.zeros((n_increments, 15))
+    its = np.zeros(n_increments * case.max_newton, np.int32)
+    res = np.zeros(n_increments * case.max_newton)
+    nq = case.nx * case.ny * case.nz * 8
+    W = 27 + case.nT + 3 + case.n_slip
+    state = np.zeros((nq, W)) if want_state else None
+    u = np.zeros(3 * (case.nx + 1) * (case.ny + 1) * (case.nz + 1)) if want_state else None
+    rod = np.ascontiguousarray(case.grain_rod, float)
+    eg = np.ascontiguousarray(case.elem_grain, np.uint32)
+    rc = lib.or_simulate(ctypes.byref(mat), ctypes.byref(pb), ptr(eg), ptr(rod), ctypes.c_int(len(rod)),
+                         ptr(rows), ptr(its), ptr(res), ptr(state) if want_state else None,
+                         ptr(u) if want_state else None)
+    assert rc == 0, rc
+    out = {"rows": rows, "lin_iters": its, "res_norms": res}
+    if want_state:
+        out["state"], out["u"] = state, u
+    return out
+
+
+def oracle_qp(case, rot, F, Fp=None, s=None, flag=1, pinv_mode=1, slip=None):
+    lib = oracle()
+    mat = case.material(pinv_mode)
+    nT, ns = case.nT, case.n_slip
+    Fp = np.eye(3) if Fp is None else np.asarray(Fp, float)
+    s = np.array(list(case.s0) + [10e14]) if s is None else np.asarray(s, float)
+    W = np.zeros(nT)
+    slip_c = np.zeros(ns) if slip is None else np.asarray(slip, float)
+    P, T, D, Fe, Fpo = np.zeros(9), np.zeros(9), np.zeros(81), np.zeros(9), np.zeros(9)
+    so, Wo, sl = np.zeros(nT), np.zeros(nT), slip_c.copy()
+    F = np.ascontiguousarray(np.asarray(F, float).reshape(9))
+    Fp = np.ascontiguousarray(Fp.reshape(9))
+    rot = np.ascontiguousarray(np.asarray(rot, float))
+    na = lib.or_qp_update(ctypes.byref(mat), ptr(rot), ptr(F), ptr(Fp), ptr(s), ptr(W), ptr(slip_c), ctypes.c_int(flag),
+                          ptr(P), ptr(T), ptr(D), ptr(Fe), ptr(Fpo), ptr(so), ptr(Wo), ptr(sl))
+    return {"P": P.reshape(3, 3), "T": T.reshape(3, 3), "dPdF": D.reshape(9, 9), "Fe": Fe.reshape(3, 3),
+            "Fp": Fpo.reshape(3, 3), "s": so, "slip": sl, "n_active": na}
+
+
+def fcc_case(**kw):
+    """The FCC material of the 8^3 tension workload (cubic 170/124/75 GPa, s0 16, h0 180, s_s 148, a 2.25, q 1.4)."""
+    d = os.path.join(GOLDEN, "fcc_simpleTension")
+    m = read_vectors(os.path.join(d, "slipDirections.txt"), 12)
+    n = read_vectors(os.path.join(d, "slipNormals.txt"), 12)
+    q = read_latent(os.path.join(d, "LatentHardeningRatio.txt"), 12)
+    C = np.zeros((6, 6))
+    C[:3, :3] = 124e3
+    np.fill_diagonal(C[:3, :3], 170e3)
+    C[3, 3] = C[4, 4] = C[5, 5] = 75e3
+    args = dict(s0=[16.0] * 12, h0=[180.0] * 12, ss=[148.0] * 12, pw=[2.25] * 12)
+    args.update(kw)
+    return Case(m, n, q, C, **args)
