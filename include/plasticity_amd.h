@@ -140,6 +140,14 @@ int pa_qp_update(pa_context *ctx, long long nqp, const double *F, const double *
  * [0] assembly (qp + tangent + K_e + scatter), [1] Krylov, [2] post-increment pass. */
 int pa_last_timings(pa_context *ctx, double ms[3]);
 
+/* Instrumentation (no counterpart in the reference): times `reps` launches of one kernel
+ * between HIP events on the context stream, on the context's current data.  which:
+ *   0 = BiCG operator pair  Zr = K Pr, Zl = K^T Pl  (the Krylov hot kernel)
+ *   1 = one full assembly sweep (8 colour launches of the qp update + K_e + scatter)
+ *   2 = Jacobi-PCG operator  Q = K P
+ * avg_ms receives the mean time of one launch (one sweep for which = 1). */
+int pa_time_kernel(pa_context *ctx, int which, int reps, double *avg_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1366,4 +1366,37 @@ int pa_qp_update(pa_context *c, long long n, const double *F, const double *Fp_c
   return rc;
 }
 
+int pa_time_kernel(pa_context *c, int which, int reps, double *avg_ms) {
+  if (!c || !avg_ms || reps < 1) return fail(PA_ERR_ARG, "bad argument");
+  if (!c->grains_set) return fail(PA_ERR_STATE, "pa_set_grains has not been called");
+  const unsigned gn = nblk(c->m.nnode, 256);
+  // keep the solver state "not done" so the operator kernels run in full
+  KState ks;
+  std::memset(&ks, 0, sizeof(ks));
+  ks.beta = 1.0;
+  HIPCHK(hipMemcpyAsync(c->kst, &ks, sizeof(ks), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipEventRecord(c->ev[0], c->stream));
+  for (int r = 0; r < reps; ++r) {
+    if (which == 0) {
+      hipLaunchKernelGGL(k_bicg_spmv<double>, dim3(gn), dim3(256), 0, c->stream, c->m, (const double *)c->K, (const double *)c->kv[5],
+                         (const double *)c->kv[6], c->kv[3], c->kv[4], c->kst, c->partials, c->counters + 1);
+    } else if (which == 1) {
+      int rc = launch_assemble(c, 1);
+      if (rc) return rc;
+    } else if (which == 2) {
+      hipLaunchKernelGGL(k_cg_spmv<double>, dim3(gn), dim3(256), 0, c->stream, c->m, (const double *)c->K, (const double *)c->kv[5],
+                         c->kv[4], c->kst, c->partials, c->counters + 1);
+    } else {
+      return fail(PA_ERR_ARG, "unknown kernel id");
+    }
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c->ev[1], c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  *avg_ms = ms / reps;
+  return 0;
+}
+
 }  // extern "C"

@@ -1,0 +1,378 @@
+"""Host-side mirror of the reference's crystalPlasticity<3> / ellipticBVP<3> / userInputParameters API
+over the C ABI (include/plasticity_amd.h).
+
+Reference surfaces mirrored (paths relative to the reference repository root):
+  userInputParameters(file)                     src/userInputParameters/userInputParameters.cc:5-545
+  crystalOrientationsIO.loadOrientations(...)   src/utilityObjects/crystalOrientationsIO.cc:66-111
+  crystalOrientationsIO.loadOrientationVector   src/utilityObjects/crystalOrientationsIO.cc:11-62
+  crystalPlasticity(userInputs).run()           include/crystalPlasticity.h:20, src/ellipticBVP/run.cc:9-30
+  applications/crystalPlasticity/main.cc:10-68  -> plasticity_amd.model.main(prm)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import native
+from .native import PaMaterial, PaProblem, check, f64, ptr
+
+
+# ------------------------------------------------------------------ prm input
+class userInputParameters:
+    """The prm keys this path reads, with the reference's declared defaults
+    (userInputParameters.cc:546-968).  Unknown keys are kept in .raw and ignored."""
+
+    def __init__(self, path):
+        self.path = os.path.abspath(path)
+        self.dir = os.path.dirname(self.path)
+        raw = {}
+        with open(path) as f:
+            for line in f:
+                line = line.split("#", 1)[0].strip()
+                if not line.startswith("set "):
+                    continue
+                k, v = line[4:].split("=", 1)
+                raw[" ".join(k.split())] = v.strip()
+        self.raw = raw
+        g = lambda k, d=None: raw.get(k, d)
+        fl = lambda s: [float(x) for x in s.replace(",", " ").split()] if s else []
+        b = lambda k, d: (g(k, d).strip().lower() == "true")
+        self.dim = int(g("Number of dimensions", "3"))
+        if self.dim != 3:
+            raise native.PlasticityError("only 3-d problems are supported")
+        self.feOrder = int(g("Order of finite elements", "-1"))
+        self.quadOrder = int(g("Order of quadrature", "-1"))
+        if self.feOrder != 1 or self.quadOrder != 2:
+            raise native.PlasticityError("only Q1 elements with QGauss(2) are implemented")
+        self.span = [float(g("Domain size %s" % a, "-1")) for a in "XYZ"]
+        self.subdivisions = [int(g("Subdivisions %s" % a, "1")) for a in "XYZ"]
+        self.meshRefineFactor = int(g("Refine factor", "-1"))
+        self.readExternalMesh = b("Use external mesh", "false")
+        self.externalMeshParameter = float(g("External mesh parameter", "0"))
+        self.writeOutput = b("Write Output", "false")
+        self.outputDirectory = g("Output Directory", "results")
+        self.delT = float(g("Time increments", "-1"))
+        self.totalTime = float(g("Total time", "-1"))
+        self.enableSimpleBCs = b("Use Simple BCs", "true")
+        self.BCfilename = g("Boundary condition filename", "")
+        self.BCheaderLines = int(g("BC file number of header lines", "0"))
+        self.NumberofBCs = int(g("Number of boundary conditions", "0"))
+        self.maxLinearSolverIterations = int(g("Maximum linear solver iterations", "-1"))
+        self.relLinearSolverTolerance = float(g("Relative linear solver tolerance", "-1"))
+        self.maxNonLinearIterations = int(g("Maximum non linear iterations", "-1"))
+        self.elasticStiffness1 = [fl(g("Elastic Stiffness row %d" % (i + 1), "")) for i in range(6)]
+        self.numSlipSystems1 = int(g("Number of Slip Systems", "0"))
+        self.latentHardeningRatioFileName1 = g("Latent Hardening Ratio filename", "")
+        self.initialSlipResistance1 = fl(g("Initial Slip Resistance", ""))
+        self.initialHardeningModulus1 = fl(g("Initial Hardening Modulus", ""))
+        self.powerLawExponent1 = fl(g("Power Law Exponent", ""))
+        self.saturationStress1 = fl(g("Saturation Stress", ""))
+        self.slipDirectionsFile1 = g("Slip Directions File", "")
+        self.slipNormalsFile1 = g("Slip Normals File", "")
+        self.enableKinematicHardening1 = b("Enable Kinematic Hardening", "false")
+        self.enableTwinning1 = b("Twinning enabled", "false")
+        self.enableMultiphase = b("Enable Multiphase", "false")
+        self.modelStressTolerance = float(g("Stress Tolerance", "-1"))
+        self.modelMaxSlipSearchIterations = int(g("Max Slip Search Iterations", "-1"))
+        self.modelMaxSolverIterations = int(g("Max Solver Iterations", "-1"))
+        self.grainIDFile = g("Grain ID file name", "")
+        self.numPts = [int(float(g("Voxels in %s direction" % a, "0"))) for a in "XYZ"]
+        self.grainOrientationsFile = g("Orientations file name", "")
+        self.headerLinesGrainIDFile = int(g("Header Lines GrainID File", "0"))
+        self.enableCyclicLoading = b("Enable cyclic loading", "false")
+        self.cyclicLoadingFace = int(g("Cyclic loading face", "1"))
+        self.cyclicLoadingDOF = int(g("Cyclic loading direction", "1"))
+        self.quarterCycleTime = float(g("Quarter cycle time", "1"))
+        for key in ("Use Periodic BCs", "Use Tabular BCs", "Use velocity gradient BC", "Use Neumann BCs",
+                    "Use Indentation BCs", "Use Torsion BCs", "Flag To Use Taylor Model", "Use Element Deletion"):
+            if b(key, "false"):
+                raise native.PlasticityError("'%s' is outside the implemented path" % key)
+        if self.enableMultiphase or self.readExternalMesh:
+            raise native.PlasticityError("multiphase and external meshes are outside the implemented path")
+
+    def file(self, name):
+        return name if os.path.isabs(name) else os.path.join(self.dir, name)
+
+
+def read_vectors(path, n):
+    """Slip/twin vector files (init.cc:73-123); normalisation happens in the library."""
+    rows = []
+    with open(path) as f:
+        for line in f:
+            v = line.split()
+            if len(v) >= 3:
+                rows.append([float(x) for x in v[:3]])
+            if len(rows) == n:
+                break
+    if len(rows) < n:
+        raise native.PlasticityError("%s: expected %d vectors" % (path, n))
+    return np.array(rows)
+
+
+def read_table(path, n):
+    rows = []
+    with open(path) as f:
+        for line in f:
+            v = line.split()
+            if v:
+                rows.append([float(x) for x in v[:n]])
+            if len(rows) == n:
+                break
+    if len(rows) < n:
+        raise native.PlasticityError("%s: expected %d rows" % (path, n))
+    return np.array(rows)
+
+
+class crystalOrientationsIO:
+    """Voxel grain map and id -> Rodrigues table (crystalOrientationsIO.cc)."""
+
+    def __init__(self):
+        self.vox = None
+        self.numPts = None
+        self.eulerAngles = None
+
+    def loadOrientations(self, voxelFile, headerLines, orientationFile, numPts, span):
+        with open(voxelFile) as f:
+            lines = f.read().split("\n")[headerLines:]
+        nv = [int(x) for x in numPts]
+        vox = np.zeros(nv, dtype=np.uint32)
+        li = 0
+        for x in range(nv[0]):
+            for y in range(nv[1]):
+                vals = lines[li].split()
+                li += 1
+                vox[x, y, :] = [int(v) for v in vals[: nv[2]]]
+        self.vox, self.numPts = vox, nv
+
+    def loadOrientationVector(self, eulerFile, enableMultiphase=False, numVoxelData=0):
+        table = {}
+        with open(eulerFile) as f:
+            f.readline()
+            for line in f:
+                v = line.split()
+                if len(v) >= 4:
+                    table[int(v[0])] = [float(x) for x in v[1:4]]
+        rod = np.zeros((max(table) + 1, 3))
+        for k, v in table.items():
+            rod[k] = v
+        self.eulerAngles = rod
+
+
+# ------------------------------------------------------------------ context
+class Context:
+    """One device context (C-ABI handle)."""
+
+    def __init__(self, mat: PaMaterial, pb: PaProblem):
+        self._h = ctypes.c_void_p()
+        L = native.lib()
+        check(L.pa_create(ctypes.byref(mat), ctypes.byref(pb), ctypes.byref(self._h)))
+        self.mat, self.pb = mat, pb
+        ne, nq, nd, nz = (ctypes.c_longlong() for _ in range(4))
+        check(L.pa_sizes(self._h, ctypes.byref(ne), ctypes.byref(nq), ctypes.byref(nd), ctypes.byref(nz)))
+        self.nelem, self.nqp, self.ndof, self.nnz = ne.value, nq.value, nd.value, nz.value
+        self.n_slip = mat.n_slip
+
+    def close(self):
+        if self._h:
+            native.lib().pa_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def element_grains(self, vox):
+        vox = np.ascontiguousarray(vox, dtype=np.uint32)
+        nv = np.array(vox.shape, dtype=np.int32)
+        out = np.zeros(self.nelem, dtype=np.uint32)
+        check(native.lib().pa_element_grains(self._h, ptr(vox), ptr(nv), ptr(out)))
+        return out
+
+    def set_grains(self, elem_grain, grain_rod):
+        eg = np.ascontiguousarray(elem_grain, dtype=np.uint32)
+        rod = f64(grain_rod)
+        check(native.lib().pa_set_grains(self._h, ptr(eg), ptr(rod), ctypes.c_int(len(rod))))
+
+    def set_solver(self, method=native.PA_SOLVER_BICG, precision=native.PA_PREC_FP64):
+        check(native.lib().pa_set_solver(self._h, ctypes.c_int(method), ctypes.c_int(precision)))
+
+    def assemble(self, inc, it):
+        rn = ctypes.c_double()
+        check(native.lib().pa_assemble(self._h, ctypes.c_int(inc), ctypes.c_int(it), ctypes.byref(rn)))
+        return rn.value
+
+    def linear_solve(self, inc, it):
+        n = ctypes.c_int()
+        check(native.lib().pa_linear_solve(self._h, ctypes.c_int(inc), ctypes.c_int(it), ctypes.byref(n)))
+        return n.value
+
+    def newton_iteration(self, inc, it):
+        rn, n = ctypes.c_double(), ctypes.c_int()
+        check(native.lib().pa_newton_iteration(self._h, ctypes.c_int(inc), ctypes.c_int(it), ctypes.byref(rn), ctypes.byref(n)))
+        return rn.value, n.value
+
+    def commit_increment(self):
+        row = np.zeros(15)
+        check(native.lib().pa_commit_increment(self._h, ptr(row)))
+        return row
+
+    def run_increment(self, inc):
+        row = np.zeros(15)
+        its = np.zeros(self.pb.max_newton, dtype=np.int32)
+        res = np.zeros(self.pb.max_newton)
+        check(native.lib().pa_run_increment(self._h, ctypes.c_int(inc), ptr(row), ptr(its), ptr(res)))
+        return row, its, res
+
+    STATE = {"T": (0, 9), "Fp": (1, 9), "Fe": (2, 9), "s": (3, None), "rotnew": (4, 3), "slip": (5, None),
+             "u": (6, None), "R": (7, None), "Fp_iter": (8, 9), "Fe_iter": (9, 9), "s_iter": (10, None)}
+
+    def get_state(self, name):
+        which, comps = self.STATE[name]
+        if name in ("u", "R"):
+            out = np.zeros(self.ndof)
+        else:
+            comps = self.n_slip if comps is None else comps
+            out = np.zeros((self.nqp, comps))
+        check(native.lib().pa_get_state(self._h, ctypes.c_int(which), ptr(out), ctypes.c_longlong(out.size)))
+        return out
+
+    def get_csr(self, values=False):
+        rp = np.zeros(self.ndof + 1, dtype=np.int64)
+        ci = np.zeros(self.nnz, dtype=np.int32)
+        val = np.zeros(self.nnz) if values else None
+        check(native.lib().pa_get_csr(self._h, ptr(rp), ptr(ci), ptr(val)))
+        return (rp, ci, val) if values else (rp, ci)
+
+    def last_timings(self):
+        ms = np.zeros(3)
+        check(native.lib().pa_last_timings(self._h, ptr(ms)))
+        return ms
+
+    def qp_update(self, F, Fp, s, grain, flag=1):
+        F = f64(F).reshape(-1, 9)
+        n = len(F)
+        Fp = f64(Fp).reshape(n, 9)
+        s = f64(s).reshape(n, self.n_slip)
+        grain = np.ascontiguousarray(grain, dtype=np.uint32).reshape(n)
+        out = {k: np.zeros((n, 9)) for k in ("P", "T", "Fe", "Fp")}
+        out["dPdF"] = np.zeros((n, 81)) if flag else None
+        out["s"] = np.zeros((n, self.n_slip))
+        out["slip_inc"] = np.zeros((n, self.n_slip))
+        out["n_active"] = np.zeros(n, dtype=np.int32)
+        check(native.lib().pa_qp_update(self._h, ctypes.c_longlong(n), ptr(F), ptr(Fp), ptr(s), ptr(grain), ctypes.c_int(flag),
+                                        ptr(out["P"]), ptr(out["T"]), ptr(out["dPdF"]), ptr(out["Fe"]), ptr(out["Fp"]),
+                                        ptr(out["s"]), ptr(out["slip_inc"]), ptr(out["n_active"])))
+        return out
+
+
+def make_material(m, n, q, C, s0, h0, ss, pw, tol, max_slip_search=1, max_solver_iter=1):
+    mat = PaMaterial()
+    ns = len(m)
+    if ns > native.MAX_SLIP:
+        raise native.PlasticityError("at most %d slip systems" % native.MAX_SLIP)
+    mat.n_slip = ns
+    for a in range(ns):
+        for j in range(3):
+            mat.m[a][j] = m[a][j]
+            mat.n[a][j] = n[a][j]
+        mat.s0[a], mat.h0[a], mat.ss[a], mat.pw[a] = s0[a], h0[a], ss[a], pw[a]
+        for b in range(ns):
+            mat.q[a][b] = q[a][b]
+    for i in range(6):
+        for j in range(6):
+            mat.C[i][j] = C[i][j]
+    mat.stress_tol = tol
+    mat.max_slip_search = max_slip_search
+    mat.max_solver_iter = max_solver_iter
+    return mat
+
+
+def make_problem(nel, span, total_increments, delT, bcs, max_newton=4, lin_tol=1e-10, lin_maxit=50000, cyclic=None):
+    pb = PaProblem()
+    pb.nx, pb.ny, pb.nz = nel
+    for d in range(3):
+        pb.span[d] = span[d]
+    pb.total_increments, pb.delT = total_increments, delT
+    pb.max_newton, pb.lin_tol, pb.lin_maxit = max_newton, lin_tol, lin_maxit
+    for f, d, u in bcs:
+        pb.face_dof_constrained[f - 1][d - 1] = 1
+        pb.face_total_u[f - 1][d - 1] = u
+    if cyclic:
+        pb.cyclic = 1
+        pb.cyclic_face, pb.cyclic_dof, pb.quarter_cycle_time = cyclic
+    return pb
+
+
+class crystalPlasticity:
+    """crystalPlasticity<3>(userInputs) (include/crystalPlasticity.h:13-264) over the device context."""
+
+    def __init__(self, userInputs: userInputParameters):
+        self.userInputs = ui = userInputs
+        ns = ui.numSlipSystems1
+        m = read_vectors(ui.file(ui.slipDirectionsFile1), ns)
+        n = read_vectors(ui.file(ui.slipNormalsFile1), ns)
+        q = read_table(ui.file(ui.latentHardeningRatioFileName1), ns)
+        self.material = make_material(m, n, q, ui.elasticStiffness1, ui.initialSlipResistance1, ui.initialHardeningModulus1,
+                                      ui.saturationStress1, ui.powerLawExponent1, ui.modelStressTolerance,
+                                      ui.modelMaxSlipSearchIterations, ui.modelMaxSolverIterations)
+        self.material.kinematic_hardening = int(ui.enableKinematicHardening1)
+        self.material.twinning = int(ui.enableTwinning1)
+        nel = [s * 2 ** ui.meshRefineFactor for s in ui.subdivisions]
+        self.totalIncrements = int(round(ui.totalTime / ui.delT))  # ellipticBVP.cc:40
+        bcs = []
+        if ui.enableSimpleBCs or ui.enableCyclicLoading:  # init.cc:327-348
+            with open(ui.file(ui.BCfilename)) as f:
+                lines = f.read().split("\n")[ui.BCheaderLines: ui.BCheaderLines + ui.NumberofBCs]
+            for line in lines:
+                v = line.split()
+                bcs.append((int(v[0]), int(v[1]), float(v[2])))
+        cyc = (ui.cyclicLoadingFace, ui.cyclicLoadingDOF, ui.quarterCycleTime) if ui.enableCyclicLoading else None
+        self.problem = make_problem(nel, ui.span, self.totalIncrements, ui.delT, bcs, ui.maxNonLinearIterations,
+                                    ui.relLinearSolverTolerance, ui.maxLinearSolverIterations, cyc)
+        self.orientations = crystalOrientationsIO()
+        self.ctx = None
+
+    def _init(self):
+        self.ctx = Context(self.material, self.problem)
+        eg = self.ctx.element_grains(self.orientations.vox)
+        self.ctx.set_grains(eg, self.orientations.eulerAngles)
+        self.cellOrientationMap = eg
+
+    def run(self, n_increments=None, write=True, log=None):
+        """solve() (src/ellipticBVP/solve.cc:69-135); returns the stressstrain rows."""
+        if self.ctx is None:
+            self._init()
+        n = self.totalIncrements if n_increments is None else n_increments
+        rows = np.zeros((n, 15))
+        self.lin_iters = np.zeros((n, self.problem.max_newton), dtype=np.int32)
+        for inc in range(n):
+            rows[inc], self.lin_iters[inc], res = self.ctx.run_increment(inc)
+            if log:
+                log("increment %d: residuals %s, linear iterations %s" % (inc, res, self.lin_iters[inc]))
+        if write:
+            outdir = self.userInputs.file(self.userInputs.outputDirectory)
+            os.makedirs(outdir, exist_ok=True)
+            write_stressstrain(os.path.join(outdir, "stressstrain.txt"), rows)
+        return rows
+
+
+def write_stressstrain(path, rows):
+    """results/stressstrain.txt with the reference header and default 6-significant-digit
+    ostream formatting (updateAfterIncrement.cc:735-771)."""
+    head = "Exx\tEyy\tEzz\tEyz\tExz\tExy\tTxx\tTyy\tTzz\tTyz\tTxz\tTxy\tTwinRealVF\tTwinMade\tSlipTotal\n"
+    with open(path, "w") as f:
+        f.write(head)
+        for r in rows:
+            f.write("\t".join("%.6g" % v for v in r) + "\n")
+
+
+def main(prm):
+    """applications/crystalPlasticity/main.cc:10-68: read prm, load the microstructure, run."""
+    ui = userInputParameters(prm)
+    problem = crystalPlasticity(ui)
+    problem.orientations.loadOrientations(ui.file(ui.grainIDFile), ui.headerLinesGrainIDFile, ui.file(ui.grainOrientationsFile),
+                                          ui.numPts, ui.span)
+    problem.orientations.loadOrientationVector(ui.file(ui.grainOrientationsFile))
+    return problem.run()

@@ -1,0 +1,78 @@
+"""ctypes binding of the C ABI in include/plasticity_amd.h (libplasticity_amd.so, built in-tree).
+
+The library is required: every entry point here calls the HIP implementation and raises when the
+shared object is missing or a call fails.  There is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libplasticity_amd.so")
+MAX_SLIP = 48
+
+PA_SOLVER_BICG, PA_SOLVER_CG = 0, 1
+PA_PREC_FP64, PA_PREC_MIXED = 0, 1
+
+# every symbol include/plasticity_amd.h declares
+EXPORTS = [
+    "pa_last_error", "pa_create", "pa_destroy", "pa_element_grains", "pa_set_grains", "pa_set_solver",
+    "pa_newton_iteration", "pa_assemble", "pa_linear_solve", "pa_commit_increment", "pa_run_increment",
+    "pa_get_state", "pa_get_csr", "pa_sizes", "pa_qp_update", "pa_last_timings", "pa_time_kernel",
+]
+
+
+class PaMaterial(ctypes.Structure):
+    _fields_ = [
+        ("n_slip", ctypes.c_int),
+        ("m", ctypes.c_double * 3 * MAX_SLIP), ("n", ctypes.c_double * 3 * MAX_SLIP),
+        ("q", ctypes.c_double * MAX_SLIP * MAX_SLIP), ("C", ctypes.c_double * 6 * 6),
+        ("s0", ctypes.c_double * MAX_SLIP), ("h0", ctypes.c_double * MAX_SLIP),
+        ("ss", ctypes.c_double * MAX_SLIP), ("pw", ctypes.c_double * MAX_SLIP),
+        ("stress_tol", ctypes.c_double), ("max_slip_search", ctypes.c_int), ("max_solver_iter", ctypes.c_int),
+        ("kinematic_hardening", ctypes.c_int), ("twinning", ctypes.c_int),
+    ]
+
+
+class PaProblem(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("span", ctypes.c_double * 3),
+        ("total_increments", ctypes.c_int), ("delT", ctypes.c_double), ("max_newton", ctypes.c_int),
+        ("lin_tol", ctypes.c_double), ("lin_maxit", ctypes.c_int),
+        ("face_dof_constrained", ctypes.c_int * 3 * 6), ("face_total_u", ctypes.c_double * 3 * 6),
+        ("cyclic", ctypes.c_int), ("cyclic_face", ctypes.c_int), ("cyclic_dof", ctypes.c_int),
+        ("quarter_cycle_time", ctypes.c_double),
+    ]
+
+
+class PlasticityError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def lib():
+    """Load libplasticity_amd.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise PlasticityError("%s is missing: build it with __graft_entry__.build()" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        L.pa_last_error.restype = ctypes.c_char_p
+        _LIB = L
+    return _LIB
+
+
+def check(rc):
+    if rc != 0:
+        raise PlasticityError("plasticity_amd error %d: %s" % (rc, lib().pa_last_error().decode()))
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)

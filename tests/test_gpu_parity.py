@@ -1,0 +1,119 @@
+"""GPU parity of the HIP path against the oracle (through the C ABI).
+
+Tolerances: per-qp stress, tangent, CRSS and Fp/Fe 1e-10 relative (to the largest entry of each
+quantity); load-displacement rows 1e-9 relative; integer work (grain ids, CSR pattern) bit-exact.
+"""
+import numpy as np
+import pytest
+
+import plasticity_amd as pa
+import refcase
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(case, nel=None, n_grain=None):
+    mat = pa.make_material(case.m, case.n, case.q, case.C, case.s0, case.h0, case.ss, case.pw, case.tol,
+                           case.max_slip_search, case.max_solver_iter)
+    pb = pa.make_problem(nel or (case.nx, case.ny, case.nz), case.span, case.total_increments, case.delT, case.bcs,
+                         case.max_newton, case.lin_tol, case.lin_maxit)
+    return pa.Context(mat, pb)
+
+
+def _rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _random_states(case, n, rng, strain):
+    F = np.eye(3) + rng.normal(scale=strain, size=(n, 3, 3))
+    Fp = np.eye(3) + rng.normal(scale=strain / 5, size=(n, 3, 3))
+    Fp /= np.cbrt(np.linalg.det(Fp))[:, None, None]
+    s = rng.uniform(16.0, 60.0, size=(n, case.n_slip))
+    return F, Fp, s
+
+
+@pytest.mark.parametrize("strain", [2e-4, 2e-3, 1e-2])
+def test_qp_update_matches_oracle(strain):
+    case = refcase.fcc_case()
+    rng = np.random.default_rng(11)
+    rod = rng.uniform(-0.5, 0.5, size=(8, 3))
+    n = 256
+    F, Fp, s = _random_states(case, n, rng, strain)
+    grain = rng.integers(0, 8, size=n).astype(np.uint32)
+    ctx = _ctx(case, nel=(1, 1, 1))
+    ctx.set_grains(np.zeros(1, np.uint32), rod)
+    for flag in (0, 1):
+        g = ctx.qp_update(F.reshape(n, 9), Fp.reshape(n, 9), s, grain, flag)
+        nplastic = 0
+        for i in range(n):
+            o = refcase.oracle_qp(case, rod[grain[i]], F[i], Fp[i], np.append(s[i], 10e14), flag)
+            assert g["n_active"][i] == o["n_active"], (i, g["n_active"][i], o["n_active"])
+            nplastic += o["n_active"] > 0
+            for k_g, k_o in (("P", "P"), ("T", "T"), ("Fe", "Fe"), ("Fp", "Fp")):
+                assert _rel(g[k_g][i], o[k_o].ravel()) < 1e-10, (k_g, i, _rel(g[k_g][i], o[k_o].ravel()))
+            assert _rel(g["s"][i], o["s"][:12]) < 1e-10
+            if flag:
+                assert _rel(g["dPdF"][i], o["dPdF"].ravel()) < 1e-10, (i, _rel(g["dPdF"][i], o["dPdF"].ravel()))
+        if strain >= 2e-3:
+            assert nplastic > n // 4  # the plastic branch is exercised
+    ctx.close()
+
+
+def test_element_grains_bit_exact():
+    case = refcase.load_reference_case("fcc_simpleTension")
+    ctx = _ctx(case)
+    eg = ctx.element_grains(case.vox)
+    assert np.array_equal(eg, case.elem_grain)
+    ctx.close()
+    case = refcase.load_reference_case("bcc_simpleTension")
+    ctx = _ctx(case)
+    assert np.array_equal(ctx.element_grains(case.vox), case.elem_grain)
+    ctx.close()
+
+
+@pytest.mark.parametrize("nel", [(1, 1, 1), (2, 3, 4), (8, 8, 8), (5, 8, 10)])
+def test_csr_pattern_bit_exact(nel):
+    import ctypes
+    case = refcase.fcc_case(nx=nel[0], ny=nel[1], nz=nel[2])
+    pb = case.problem()
+    lib = refcase.oracle()
+    nnz = lib.or_sparsity(ctypes.byref(pb), None, None)
+    rp = np.zeros((nel[0] + 1) * (nel[1] + 1) * (nel[2] + 1) * 3 + 1, np.int64)
+    ci = np.zeros(nnz, np.int32)
+    lib.or_sparsity(ctypes.byref(pb), refcase.ptr(rp), refcase.ptr(ci))
+    ctx = _ctx(case)
+    assert ctx.nnz == nnz
+    grp, gci = ctx.get_csr()
+    assert np.array_equal(grp, rp) and np.array_equal(gci, ci)
+    ctx.close()
+
+
+def test_newton_curve_matches_oracle_fcc_tension():
+    """8^3 mesh with the fcc/simpleTension inputs: 6 increments through yield."""
+    case = refcase.load_reference_case("fcc_simpleTension")
+    n = 6
+    ref = refcase.oracle_simulate(case, n, nthreads=8, want_state=True)
+    ctx = _ctx(case)
+    ctx.set_grains(case.elem_grain, case.grain_rod)
+    rows, its = [], []
+    for inc in range(n):
+        r, it, res = ctx.run_increment(inc)
+        rows.append(r)
+        its.append(it)
+    rows = np.array(rows)
+    its = np.array(its).ravel()
+    print("lin iters gpu", its, "oracle", ref["lin_iters"])
+    assert np.abs(its - ref["lin_iters"]).max() <= 1
+    for c in range(15):
+        scale = max(np.abs(ref["rows"][:, c]).max(), 1e-300)
+        if scale > 1e-12:
+            assert np.abs(rows[:, c] - ref["rows"][:, c]).max() / scale < 1e-9, (c, rows[:, c], ref["rows"][:, c])
+    st = ref["state"]
+    nT = case.nT
+    T = ctx.get_state("T")
+    assert _rel(T, st[:, 0:9]) < 1e-9
+    assert _rel(ctx.get_state("s"), st[:, 27:27 + 12]) < 1e-10
+    assert _rel(ctx.get_state("rotnew"), st[:, 27 + nT:30 + nT]) < 1e-10
+    assert _rel(ctx.get_state("u"), ref["u"]) < 1e-9
+    ctx.close()
