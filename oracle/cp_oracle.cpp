@@ -207,12 +207,16 @@ void svd_pinv_solve(int n, const double *A, const double *b, double *x, double r
   }
 }
 
+// Relative level below which a pivot (LU) or singular value (SVD fallback) counts as zero.
+// A structurally rank-deficient A_PA shows pivots ~1e-13 max|A|, well under it.
+const double kPinvRtol = 1e-10;
+
 // LU with partial pivoting, multiple right-hand sides (row-major B, n x m, overwritten by X).
-// Returns false if a pivot is numerically zero relative to max|A|.
+// Returns false if a pivot is <= kPinvRtol * max|A|.
 bool lu_solve(int n, std::vector<double> A, double *B, int m) {
   double amax = 0;
   for (double v : A) amax = std::max(amax, std::fabs(v));
-  const double tiny = n * 2.220446049250313e-16 * amax;
+  const double tiny = kPinvRtol * amax;
   for (int k = 0; k < n; ++k) {
     int p = k;
     double pv = std::fabs(A[k * n + k]);
@@ -251,7 +255,7 @@ void pinv_solve(int n, const double *A, const double *b, double *x, int mode) {
     for (int i = 0; i < n; ++i) x[i] = b[i];
     if (lu_solve(n, Av, x, 1)) return;
     g_pinv_singular++;
-    svd_pinv_solve(n, A, b, x, n * 2.220446049250313e-16);
+    svd_pinv_solve(n, A, b, x, kPinvRtol);
     return;
   }
   svd_pinv_solve(n, A, b, x, 0.0);
@@ -980,6 +984,32 @@ long long or_sparsity(const or_problem *pb, long long *row_ptr, int *col_idx) {
   return A.rp[m.ndof];
 }
 
+static double *g_dump_K = nullptr, *g_dump_R = nullptr, *g_dump_du = nullptr;
+
+int or_krylov(int method, long long n, const long long *rp, const int *ci, const double *val, const double *b,
+              double *x, double tol, int maxit) {
+  CSR A;
+  A.rp.assign(rp, rp + n + 1);
+  A.ci.assign(ci, ci + rp[n]);
+  A.val.assign(val, val + rp[n]);
+  std::vector<double> bv(b, b + n), xv;
+  const int it = (method == 1) ? pcg(A, bv, xv, tol, maxit) : bicg(A, bv, xv, tol, maxit);
+  std::copy(xv.begin(), xv.end(), x);
+  return it;
+}
+
+int or_first_newton(const or_material *mat, const or_problem *pb, const unsigned *elem_grain, const double *grain_rod,
+                    int n_grain_rows, double *K_vals, double *R, double *du) {
+  g_dump_K = K_vals; g_dump_R = R; g_dump_du = du;
+  or_problem p1 = *pb;
+  p1.n_increments = 1;
+  p1.max_newton = 1;
+  double rows[15];
+  const int rc = or_simulate(mat, &p1, elem_grain, grain_rod, n_grain_rows, rows, nullptr, nullptr, nullptr, nullptr);
+  g_dump_K = g_dump_R = g_dump_du = nullptr;
+  return rc;
+}
+
 int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *elem_grain, const double *grain_rod,
                 int n_grain_rows, double *rows, int *lin_iters, double *res_norms, double *state_out, double *u_out) {
 #ifdef _OPENMP
@@ -1089,11 +1119,14 @@ int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *el
           }
         }
       }
+      if (g_dump_K) std::copy(A.val.begin(), A.val.end(), g_dump_K);
+      if (g_dump_R) std::copy(R.begin(), R.end(), g_dump_R);
       const double rn = std::sqrt(dot(R, R));
       if (res_norms) res_norms[inc * pb->max_newton + it] = rn;
       const int its = (pb->solver == 1) ? pcg(A, R, du, pb->lin_tol * rn, pb->lin_maxit) : bicg(A, R, du, pb->lin_tol * rn, pb->lin_maxit);
       if (lin_iters) lin_iters[inc * pb->max_newton + it] = its;
       for (long long r = 0; r < m.ndof; ++r) if (con[r]) du[r] = (it == 0) ? delu[r] : 0.0;  // constraints.distribute
+      if (g_dump_du) std::copy(du.begin(), du.end(), g_dump_du);
       for (long long r = 0; r < m.ndof; ++r) x[r] += du[r];
     }
     // updateAfterIncrement (CP updateAfterIncrement.cc:6-305, 710-771)

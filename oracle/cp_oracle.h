@@ -22,8 +22,8 @@ extern "C" {
  *   0 = Jacobi-SVD pseudo-inverse with threshold 0 (deal.II
  *       LAPACKFullMatrix::compute_inverse_svd(0.0) semantics,
  *       src/materialModels/crystalPlasticity/inactiveSlipRemoval.cc:26,99)
- *   1 = LU with partial pivoting; falls back to the SVD pseudo-inverse with a
- *       relative cut-off n*eps*sigma_max when a pivot is numerically zero
+ *   1 = LU with partial pivoting; when a pivot is <= 1e-10 max|A| falls back to
+ *       the SVD pseudo-inverse with singular values <= 1e-10 sigma_max dropped
  *       (the semantics the HIP kernel implements). */
 typedef struct {
   int n_slip, n_twin, nT;          /* nT = n_slip + n_twin (n_twin=1 dummy when twinning is off) */
@@ -91,8 +91,18 @@ int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *el
                 double *rows, int *lin_iters, double *res_norms, double *state_out,
                 double *u_out);
 
+/* Increment 0, Newton iteration 0 only: the condensed CSR values (pattern of or_sparsity),
+ * the condensed residual and the Newton correction du (after constraints.distribute). */
+int or_first_newton(const or_material *mat, const or_problem *pb, const unsigned *elem_grain,
+                    const double *grain_rod, int n_grain_rows, double *K_vals, double *R, double *du);
+
+/* Jacobi-BiCG (method 0) / Jacobi-PCG (1) on a CSR system with absolute tolerance tol on the
+ * preconditioned residual norm; returns the iteration count. */
+int or_krylov(int method, long long n, const long long *rp, const int *ci, const double *val, const double *b,
+              double *x, double tol, int maxit);
+
 /* Counts of inactive-slip-removal solves and of those whose A_PA was numerically
- * singular (LU pivot below n*eps*max|A|) since the last call; resets the counters. */
+ * singular (LU pivot <= 1e-10 max|A|) since the last call; resets the counters. */
 void or_pinv_stats(long long *calls, long long *singular);
 
 /* Single increment helpers exposed for property tests. */

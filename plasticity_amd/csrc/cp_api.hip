@@ -10,6 +10,7 @@
 //   vectors: dof = 3 n + component, FP64.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -18,6 +19,7 @@
 
 #include "../../include/plasticity_amd.h"
 #include "cp_qp.h"
+#include "cp_tables.h"
 
 namespace {
 
@@ -65,8 +67,8 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *lds /* NV * 1
 }
 
 // Deterministic grid reduction: each block writes its NV partial sums; the last block to
-// arrive sums the partials in block order.  Returns true in thread 0 of the last block,
-// with v holding the totals.
+// arrive sums the partials, every thread a fixed strided subset, then a block sum.  Returns
+// true in thread 0 of the last block, with v holding the totals.
 template <int NV>
 __device__ bool grid_sum(double (&v)[NV], double *partials, unsigned *counter, double *lds) {
   __shared__ bool last;
@@ -80,16 +82,17 @@ __device__ bool grid_sum(double (&v)[NV], double *partials, unsigned *counter, d
   }
   __syncthreads();
   if (!last) return false;
-  if (threadIdx.x == 0) {
-    __threadfence();
+  __threadfence();
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      double s = 0;
-      for (unsigned b = 0; b < gridDim.x; ++b) s += __hip_atomic_load(&partials[b * NV + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      v[k] = s;
-    }
-    *counter = 0;
+  for (int k = 0; k < NV; ++k) {
+    double s = 0;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
+      s += __hip_atomic_load(&partials[b * NV + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v[k] = s;
   }
+  __syncthreads();  // lds reuse
+  block_sum<NV>(v, lds);
+  if (threadIdx.x == 0) *counter = 0;
   return threadIdx.x == 0;
 }
 
@@ -321,15 +324,13 @@ __global__ void __launch_bounds__(64) k_assemble(AsmArgs a) {
     for (int i = 0; i < 9; ++i) Fp_t[i] = a.Fp_conv[i * a.nq + Q];
     for (int i = 0; i < a.ns; ++i) s[i] = a.s_conv[i * a.nq + Q];
     const double *gt = a.gtab + (long long)a.egrain[e] * a.gstride;
-    const int na = cp::qp_update<NS>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 1, P, T, Fe, Fp, sD + t, 64);
-    (void)na;
+    bool plastic = false;
+    cp::qp_update<NS>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 1, P, T, Fe, Fp, sD + t, 64, &plastic);
 #pragma unroll
     for (int i = 0; i < 9; ++i) { a.Fe_iter[i * a.nq + Q] = Fe[i]; a.Fp_iter[i * a.nq + Q] = Fp[i]; sP[t * 9 + i] = P[i]; }
     for (int i = 0; i < a.ns; ++i) a.s_iter[i * a.nq + Q] = s[i];
     // slipfraction_iter is written only on the plastic path (calculatePlasticity.cc:454-456)
-    bool plastic = false;
-    for (int i = 0; i < a.ns; ++i) plastic |= (x[i] != 0.0);
-    if (plastic || na > 0)
+    if (plastic)
       for (int i = 0; i < a.ns; ++i) a.slip_iter[i * a.nq + Q] = a.slip_conv[i * a.nq + Q] + x[i];
   }
   __syncthreads();
@@ -500,14 +501,13 @@ __global__ void __launch_bounds__(256) k_post(PostArgs a) {
     for (int i = 0; i < 9; ++i) Fp_t[i] = a.Fp_conv[i * a.nq + Q];
     for (int i = 0; i < a.ns; ++i) s[i] = a.s_conv[i * a.nq + Q];
     const double *gt = a.gtab + (long long)a.egrain[e] * a.gstride;
-    const int na = cp::qp_update<NS>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 0, P, T, Fe, Fp, nullptr, 0);
+    bool plastic = false;
+    cp::qp_update<NS>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 0, P, T, Fe, Fp, nullptr, 0, &plastic);
 #pragma unroll
     for (int i = 0; i < 9; ++i) { a.Fe_iter[i * a.nq + Q] = Fe[i]; a.Fp_iter[i * a.nq + Q] = Fp[i]; a.T[i * a.nq + Q] = T[i]; }
     for (int i = 0; i < a.ns; ++i) a.s_iter[i * a.nq + Q] = s[i];
-    bool plastic = false;
-    for (int i = 0; i < a.ns; ++i) plastic |= (x[i] != 0.0);
     double slip_sum = 0;
-    if (plastic || na > 0) {
+    if (plastic) {
       for (int i = 0; i < a.ns; ++i) {
         const double v = a.slip_conv[i * a.nq + Q] + x[i];
         a.slip_iter[i * a.nq + Q] = v;
@@ -642,18 +642,19 @@ __global__ void __launch_bounds__(256) k_bicg_update(long long N, double *__rest
                                                      double *partials, unsigned *counter) {
   __shared__ double lds[32];
   if (st->done) return;
-  const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
   double acc[2] = {0.0, 0.0};
-  if (r < N) {
-    const double a = st->alpha, d = dinv[r];
+  const double a = st->alpha;
+  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < N; r += stride) {
+    const double d = dinv[r];
     X[r] += a * Pr[r];
     const double rr = Rr[r] - a * Zr[r];
     const double rl = Rl[r] - a * Zl[r];
     Rr[r] = rr; Rl[r] = rl;
     const double zr = rr * d, zl = rl * d;
     Zr[r] = zr; Zl[r] = zl;
-    acc[0] = zr * zr;
-    acc[1] = zr * rl;
+    acc[0] += zr * zr;
+    acc[1] += zr * rl;
   }
   if (grid_sum<2>(acc, partials, counter, lds)) {
     st->dp = sqrt(acc[0]);
@@ -672,12 +673,12 @@ __global__ void __launch_bounds__(256) k_bicg_init(long long N, const double *__
                                                    double *__restrict__ Zl, const double *__restrict__ dinv, KState *st,
                                                    double lin_tol, int maxit, double *partials, unsigned *counter) {
   __shared__ double lds[48];
-  const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
   double acc[3] = {0.0, 0.0, 0.0};
-  if (r < N) {
+  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < N; r += stride) {
     const double b = B[r], z = b * dinv[r];
     X[r] = 0.0; Rr[r] = b; Rl[r] = b; Zr[r] = z; Zl[r] = z;
-    acc[0] = b * b; acc[1] = z * z; acc[2] = z * b;
+    acc[0] += b * b; acc[1] += z * z; acc[2] += z * b;
   }
   if (grid_sum<3>(acc, partials, counter, lds)) {
     st->bnorm = sqrt(acc[0]);
@@ -728,17 +729,17 @@ __global__ void __launch_bounds__(256) k_cg_update(long long N, double *__restri
                                                    const double *__restrict__ dinv, KState *st, double *partials, unsigned *counter) {
   __shared__ double lds[32];
   if (st->done) return;
-  const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
   double acc[2] = {0.0, 0.0};
-  if (r < N) {
-    const double a = st->alpha;
+  const double a = st->alpha;
+  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < N; r += stride) {
     X[r] += a * P[r];
     const double rr = Rr[r] - a * Q[r];
     Rr[r] = rr;
     const double z = rr * dinv[r];
     Z[r] = z;
-    acc[0] = z * z;
-    acc[1] = rr * z;
+    acc[0] += z * z;
+    acc[1] += rr * z;
   }
   if (grid_sum<2>(acc, partials, counter, lds)) {
     st->dp = sqrt(acc[0]);
@@ -776,8 +777,9 @@ __global__ void k_apply_du(long long N, double *__restrict__ u, const double *__
 
 __global__ void k_norm2(long long N, const double *__restrict__ v, double *out, double *partials, unsigned *counter) {
   __shared__ double lds[16];
-  const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  double acc[1] = {r < N ? v[r] * v[r] : 0.0};
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  double acc[1] = {0.0};
+  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < N; r += stride) acc[0] += v[r] * v[r];
   if (grid_sum<1>(acc, partials, counter, lds)) *out = sqrt(acc[0]);
 }
 
@@ -788,6 +790,10 @@ __global__ void k_zero(long long N, double *__restrict__ v) {
 
 // ------------------------------------------------------------ host side
 inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+// grid of the reducing vector kernels: grid-stride loops over at most this many 256-lane blocks,
+// which bounds the partial sums the last block adds up
+constexpr unsigned VGRID_MAX = 1024;
+inline unsigned vgrid(long long n) { return std::min(nblk(n, 256), VGRID_MAX); }
 
 }  // namespace
 
@@ -832,55 +838,6 @@ int dalloc(pa_context *c, T **p, size_t count) {
   c->allocs.push_back(q);
   *p = (T *)q;
   return 0;
-}
-
-// per-grain constants: rotated Schmid tensors, Dmat (shear columns doubled), TM
-// (calculatePlasticity.cc:39-112, rotationOperations.cc:4-51, matrixOperations.cc:189-234)
-void grain_table(const pa_material &mat, const double mv[][3], const double nv[][3], const double r[3], int ns, int NS,
-                 double *out) {
-  double rr = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-  const double t1 = 1.0 - rr, t2 = 1.0 + rr;
-  double R[3][3];
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) R[i][j] = (i == j ? t1 : 0.0) + 2.0 * r[i] * r[j];
-  R[0][1] -= 2.0 * r[2]; R[0][2] += 2.0 * r[1]; R[1][2] -= 2.0 * r[0];
-  R[1][0] += 2.0 * r[2]; R[2][0] -= 2.0 * r[1]; R[2][1] += 2.0 * r[0];
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) R[i][j] *= 1.0 / t2;
-  double N[6][6];
-  const int pi[3][2] = {{1, 2}, {2, 0}, {0, 1}};
-  for (int i = 0; i < 3; ++i) {
-    for (int j = 0; j < 3; ++j) N[i][j] = R[i][j] * R[i][j];
-    for (int j = 0; j < 3; ++j) N[i][3 + j] = 2.0 * R[i][pi[j][0]] * R[i][pi[j][1]];
-  }
-  for (int i = 0; i < 3; ++i) {
-    const int a = pi[i][0], b = pi[i][1];
-    for (int j = 0; j < 3; ++j) N[3 + i][j] = R[a][j] * R[b][j];
-    for (int j = 0; j < 3; ++j) {
-      const int c = pi[j][0], d = pi[j][1];
-      N[3 + i][3 + j] = R[a][c] * R[b][d] + R[a][d] * R[b][c];
-    }
-  }
-  double Tm[6][6], D2[6][6];
-  for (int i = 0; i < 6; ++i)
-    for (int j = 0; j < 6; ++j) { double s = 0; for (int k = 0; k < 6; ++k) s += N[i][k] * mat.C[k][j]; Tm[i][j] = s; }
-  for (int i = 0; i < 6; ++i)
-    for (int j = 0; j < 6; ++j) { double s = 0; for (int k = 0; k < 6; ++k) s += Tm[i][k] * N[j][k]; D2[i][j] = s; }
-  double *S = out, *D = out + 9 * NS, *TM = out + 9 * NS + 36;
-  for (int a = 0; a < ns; ++a) {
-    double mn[3][3], t[3][3];
-    for (int j = 0; j < 3; ++j)
-      for (int k = 0; k < 3; ++k) mn[j][k] = mv[a][j] * nv[a][k];
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) { double s = 0; for (int k = 0; k < 3; ++k) s += R[i][k] * mn[k][j]; t[i][j] = s; }
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) { double s = 0; for (int k = 0; k < 3; ++k) s += t[i][k] * R[j][k]; S[9 * a + 3 * i + j] = s; }
-  }
-  for (int i = 0; i < 6; ++i)
-    for (int j = 0; j < 6; ++j) D[6 * i + j] = (j >= 3 ? 2 * D2[i][j] : D2[i][j]);
-  const int v2[9] = {0, 5, 4, 5, 1, 3, 4, 3, 2};
-  for (int i = 0; i < 9; ++i)
-    for (int j = 0; j < 9; ++j) TM[9 * i + j] = D2[v2[i]][v2[j]];
 }
 
 int ns_template(int ns) { return ns <= 12 ? 12 : (ns <= 24 ? 24 : 48); }
@@ -945,7 +902,7 @@ int do_assemble(pa_context *c, int inc, int it, double *res_norm) {
   hipLaunchKernelGGL(k_dinv, dim3(nblk(c->ndof, 256)), dim3(256), 0, c->stream, c->m, c->K, c->dinv);
   HIPCHK(hipGetLastError());
   if (res_norm) {
-    hipLaunchKernelGGL(k_norm2, dim3(nblk(c->ndof, 256)), dim3(256), 0, c->stream, c->ndof, c->R, c->d_out + 16, c->partials,
+    hipLaunchKernelGGL(k_norm2, dim3(vgrid(c->ndof)), dim3(256), 0, c->stream, c->ndof, c->R, c->d_out + 16, c->partials,
                        c->counters + 3);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(res_norm, c->d_out + 16, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -957,8 +914,8 @@ int do_assemble(pa_context *c, int inc, int it, double *res_norm) {
 int do_solve(pa_context *c, int it, int *iters) {
   const long long N = c->ndof;
   double *X = c->kv[0], *Rr = c->kv[1], *Rl = c->kv[2], *Zr = c->kv[3], *Zl = c->kv[4], *Pr = c->kv[5], *Pl = c->kv[6];
-  const unsigned gv = nblk(N, 256), gn = nblk(c->m.nnode, 256);
-  hipLaunchKernelGGL(k_bicg_init, dim3(gv), dim3(256), 0, c->stream, N, c->R, X, Rr, Rl, Zr, Zl, c->dinv, c->kst, c->pb.lin_tol,
+  const unsigned gv = nblk(N, 256), gr = vgrid(N), gn = nblk(c->m.nnode, 256);
+  hipLaunchKernelGGL(k_bicg_init, dim3(gr), dim3(256), 0, c->stream, N, c->R, X, Rr, Rl, Zr, Zl, c->dinv, c->kst, c->pb.lin_tol,
                      c->pb.lin_maxit, c->partials, c->counters + 0);
   HIPCHK(hipGetLastError());
   KState hs;
@@ -969,14 +926,14 @@ int do_solve(pa_context *c, int it, int *iters) {
         hipLaunchKernelGGL(k_cg_p, dim3(gv), dim3(256), 0, c->stream, N, Zr, Pr, (const KState *)c->kst, 0);
         hipLaunchKernelGGL(k_cg_spmv<double>, dim3(gn), dim3(256), 0, c->stream, c->m, (const double *)c->K, (const double *)Pr, Zl,
                            c->kst, c->partials, c->counters + 1);
-        hipLaunchKernelGGL(k_cg_update, dim3(gv), dim3(256), 0, c->stream, N, X, (const double *)Pr, Rr, (const double *)Zl, Zr,
+        hipLaunchKernelGGL(k_cg_update, dim3(gr), dim3(256), 0, c->stream, N, X, (const double *)Pr, Rr, (const double *)Zl, Zr,
                            (const double *)c->dinv, c->kst, c->partials, c->counters + 2);
       } else {
         hipLaunchKernelGGL(k_bicg_p, dim3(gv), dim3(256), 0, c->stream, N, (const double *)Zr, (const double *)Zl, Pr, Pl,
                            (const KState *)c->kst);
         hipLaunchKernelGGL(k_bicg_spmv<double>, dim3(gn), dim3(256), 0, c->stream, c->m, (const double *)c->K, (const double *)Pr,
                            (const double *)Pl, Zr, Zl, c->kst, c->partials, c->counters + 1);
-        hipLaunchKernelGGL(k_bicg_update, dim3(gv), dim3(256), 0, c->stream, N, X, (const double *)Pr, Rr, Rl, Zr, Zl,
+        hipLaunchKernelGGL(k_bicg_update, dim3(gr), dim3(256), 0, c->stream, N, X, (const double *)Pr, Rr, Rl, Zr, Zl,
                            (const double *)c->dinv, c->kst, c->partials, c->counters + 2);
       }
     }

@@ -20,7 +20,7 @@
 
 namespace cp {
 
-#define CP_DEV __device__ __forceinline__
+#define CP_DEV __host__ __device__ __forceinline__
 
 // ---------------------------------------------------------------- 3x3 helpers
 CP_DEV void m3_mul(const double *A, const double *B, double *C) {  // C = A B
@@ -104,14 +104,19 @@ CP_DEV void m3_expm(const double *A, double *E) {
 }
 
 // ------------------------------------------------------- small dense solvers
+// Relative level below which a pivot (LU) or singular value (SVD fallback) counts as zero.
+// A structurally rank-deficient A_PA (e.g. ten potentially active FCC systems: the latent
+// hardening matrix has rank 4, the elastic coupling rank <= 6) shows pivots ~1e-13 max|A|.
+#define CP_PINV_RTOL 1e-10
+
 // LU with partial pivoting on an n x n row-major matrix (ld = NS) with m right-hand
-// sides (row-major, ld = m).  Returns false when a pivot is <= n*eps*max|A|.
+// sides (row-major, ld = m).  Returns false when a pivot is <= CP_PINV_RTOL * max|A|.
 template <int NS>
 CP_DEV bool lu_solve(int n, double *A, double *B, int m) {
   double amax = 0;
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) amax = fmax(amax, fabs(A[i * NS + j]));
-  const double tiny = n * 2.220446049250313e-16 * amax;
+  const double tiny = CP_PINV_RTOL * amax;
   for (int k = 0; k < n; ++k) {
     int p = k;
     double pv = fabs(A[k * NS + k]);
@@ -143,7 +148,7 @@ CP_DEV bool lu_solve(int n, double *A, double *B, int m) {
   return true;
 }
 
-// x = pinv(A) b by one-sided Jacobi SVD, singular values <= n*eps*sigma_max dropped.
+// x = pinv(A) b by one-sided Jacobi SVD, singular values <= CP_PINV_RTOL * sigma_max dropped.
 // Used only when LU finds A_PA numerically singular.
 template <int NS>
 CP_DEV void svd_pinv_solve(int n, const double *A, const double *b, double *x) {
@@ -182,7 +187,7 @@ CP_DEV void svd_pinv_solve(int n, const double *A, const double *b, double *x) {
       for (int i = 0; i < n; ++i) U[i * NS + j] /= sig[j];
     smax = fmax(smax, sig[j]);
   }
-  const double lim = smax * (n * 2.220446049250313e-16);
+  const double lim = smax * CP_PINV_RTOL;
   for (int i = 0; i < n; ++i) x[i] = 0;
   for (int k = 0; k < n; ++k) {
     if (!(sig[k] > lim)) continue;
@@ -227,12 +232,14 @@ CP_DEV void dexp_series(const double (*Xp)[9], const double *S, int count3, bool
 
 // Return: number of systems left active after inactive-slip removal (0 = elastic).
 // s holds s_conv on entry and s_tau on exit; x receives x_beta_old (the slip increments).
+// *plastic_path (optional) is set when the potentially-active set was non-empty, the case in
+// which the reference writes slipfraction_iter (calculatePlasticity.cc:454-456).
 // D (flag = 1) receives dP/dF as D[(3i+k)*ldD ...] with element stride `dstride`:
 // D[((3i+k)*9 + 3j+l) * dstride] = dP_ik / dF_jl.
 template <int NS>
 CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ gt, int ns, int max_solver_iter,
                      const double *F, const double *Fp_t, double *s, double *x, int flag, double *P, double *T,
-                     double *Fe_out, double *Fp_out, double *D, int dstride) {
+                     double *Fe_out, double *Fp_out, double *D, int dstride, bool *plastic_path = nullptr) {
   typedef MatLayout<NS> ML;
   typedef GrainLayout<NS> GL;
   const double tol1 = mat[ML::tol];
@@ -263,6 +270,7 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
     if (b[a] >= tol1) PA[nPA++] = a;
   }
   const bool plastic = nPA > 0;
+  if (plastic_path) *plastic_path = plastic;
   double detFE;
   int n_active = 0;
 
