@@ -58,7 +58,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--grains", type=int, default=256)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
     ap.add_argument("--solver", choices=["bicg", "cg"], default="bicg")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -73,12 +73,59 @@ def main():
 
     assert torch.cuda.is_available(), "no GPU"
     n = args.n
+    forms = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    results = {}
+    for form in forms:
+        results[form] = run_form(args, form, pa, native, synth, torch)
+    head = results[forms[-1]]
+    nq = head["nq"]
+    out = {
+        "metric": METRIC,
+        "value": nq / (head["ms_per_step"] * 1e-3),
+        "unit": "qp updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if forms[-1] == "fp64" else "f64 (return map, residual, Krylov) + f32 (tangents, matrix)",
+        "data": "synthetic (seeded Voronoi grains, random orientations)",
+        "config": {"workload": "FCC uniaxial tension, %d^3 Q1 hex (%d qps, %d dofs), %d grains, 5e-4 strain/step, "
+                               "4 Newton iterations/step, Jacobi-%s 1e-10" % (n, nq, head["ndof"], args.grains, args.solver.upper()),
+                   "form": forms[-1], "mesh": [n, n, n], "nnz": head["nnz"], "parallelism": "single GPU"},
+        "roofline": head["roofline"],
+    }
+    for form in forms:
+        r = results[form]
+        out["config"][form] = {k: r[k] for k in ("ms_per_step", "qp_updates_per_s", "krylov_iters_per_step", "assembly_ms_per_step",
+                                                 "krylov_ms_per_step", "post_ms_per_increment", "qp_tangent_ms", "element_ms",
+                                                 "spmv_pair_us")}
+    if len(forms) == 2:
+        a, b = results["fp64"], results["mixed"]
+        out["config"]["speedup_mixed_vs_fp64"] = a["ms_per_step"] / b["ms_per_step"]
+        rows_a, rows_b = np.array(a["rows"]), np.array(b["rows"])
+        err = {}
+        for name, cols in (("avg_stress", slice(6, 12)), ("avg_strain", slice(0, 6)), ("slip_total", slice(14, 15))):
+            err[name] = float(np.abs(rows_b[:, cols] - rows_a[:, cols]).max() / np.abs(rows_a[:, cols]).max())
+        err["qp_cauchy_stress"] = float(np.abs(b["T"] - a["T"]).max() / np.abs(a["T"]).max())
+        err["crss"] = float(np.abs(b["s"] - a["s"]).max() / np.abs(a["s"]).max())
+        err["displacement"] = float(np.abs(b["u"] - a["u"]).max() / np.abs(a["u"]).max())
+        out["config"]["mixed_error_vs_fp64_gpu"] = err
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args, quick=(n < 32))
+    print(json.dumps(out))
+
+
+def run_form(args, form, pa, native, synth, torch):
+    n = args.n
     mat, pb, vox, rod, bcs, _ = synth.fcc_workload(n, args.grains, 20, 5e-4, seed=2024)
     ctx = pa.Context(mat, pb)
     eg = ctx.element_grains(vox)
     ctx.set_grains(eg, rod)
     ctx.set_solver(native.PA_SOLVER_CG if args.solver == "cg" else native.PA_SOLVER_BICG,
-                   native.PA_PREC_MIXED if args.precision == "mixed" else native.PA_PREC_FP64)
+                   native.PA_PREC_MIXED if form == "mixed" else native.PA_PREC_FP64)
     nmax = pb.max_newton
     state = {"step": 0}
     log = []
@@ -94,8 +141,8 @@ def main():
             rec["post_ms"] = ctx.last_timings()[2]
         state["step"] += 1
         log.append(rec)
-        print("step %d (increment %d, iteration %d): |R| %.3e, %d Krylov iterations, assembly %.1f ms, Krylov %.1f ms"
-              % (s, inc, it, rn, its, asm_ms, kry_ms), file=sys.stderr, flush=True)
+        print("[%s] step %d (increment %d, iteration %d): |R| %.3e, %d Krylov iterations, assembly %.1f ms, Krylov %.1f ms"
+              % (form, s, inc, it, rn, its, asm_ms, kry_ms), file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
         newton_step()
@@ -107,51 +154,39 @@ def main():
     dt = time.perf_counter() - t0
     timed = log[args.warmup:]
     ms_per_step = 1000.0 * dt / args.steps
-    nq = ctx.nqp
-
     # roofline of the dominant kernel: the BiCG operator pair (K p and K^T p~), HBM-bound.
-    # algorithmic bytes per launch: 2 reads of the nnz matrix values (8 B each) + Pr, Pl read and
-    # Zr, Zl written once (8 B per dof each)
-    reps = 20
+    # Algorithmic bytes per launch: two reads of the nnz matrix values + Pr, Pl read and Zr, Zl
+    # written once (8 B per dof each).
+    vb = 4 if form == "mixed" else 8
     avg = ctypes.c_double()
-    native.check(native.lib().pa_time_kernel(ctx._h, 0, reps, ctypes.byref(avg)))
+    native.check(native.lib().pa_time_kernel(ctx._h, 0, 20, ctypes.byref(avg)))
     spmv_ms = avg.value
-    bytes_spmv = 2 * ctx.nnz * 8 + 4 * ctx.ndof * 8
+    bytes_spmv = 2 * ctx.nnz * vb + 4 * ctx.ndof * 8
     achieved = bytes_spmv / (spmv_ms * 1e-3) / 1e9
-    native.check(native.lib().pa_time_kernel(ctx._h, 1, 1, ctypes.byref(avg)))
-    asm_sweep_ms = avg.value
-    lin = [r["lin_iters"] for r in timed]
-    out = {
-        "metric": METRIC,
-        "value": nq / (ms_per_step * 1e-3),
-        "unit": "qp updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f64+f32",
-        "data": "synthetic (seeded Voronoi grains, random orientations)",
-        "config": {"workload": "FCC uniaxial tension, %d^3 Q1 hex (%d qps, %d dofs), %d grains, 5e-4 strain/step, "
-                               "4 Newton iterations/step, Jacobi-%s 1e-10" % (n, nq, ctx.ndof, args.grains, args.solver.upper()),
-                   "form": "plain FP64" if args.precision == "fp64" else "mixed precision",
-                   "mesh": [n, n, n], "nnz": ctx.nnz,
-                   "krylov_iters_per_step": float(np.mean(lin)),
-                   "assembly_ms_per_step": float(np.mean([r["asm_ms"] for r in timed])),
-                   "krylov_ms_per_step": float(np.mean([r["krylov_ms"] for r in timed])),
-                   "assembly_sweep_ms": asm_sweep_ms,
-                   "parallelism": "single GPU"},
-        "roofline": {"kernel": "k_bicg_spmv<double> (K p and K^T p~, 27-point node-block FP64)",
+    native.check(native.lib().pa_time_kernel(ctx._h, 3, 1, ctypes.byref(avg)))
+    qp_ms = avg.value
+    native.check(native.lib().pa_time_kernel(ctx._h, 4, 1, ctypes.byref(avg)))
+    el_ms = avg.value
+    posts = [r["post_ms"] for r in timed if "post_ms" in r]
+    res = {
+        "nq": ctx.nqp, "ndof": ctx.ndof, "nnz": ctx.nnz,
+        "ms_per_step": ms_per_step, "qp_updates_per_s": ctx.nqp / (ms_per_step * 1e-3),
+        "krylov_iters_per_step": float(np.mean([r["lin_iters"] for r in timed])),
+        "assembly_ms_per_step": float(np.mean([r["asm_ms"] for r in timed])),
+        "krylov_ms_per_step": float(np.mean([r["krylov_ms"] for r in timed])),
+        "post_ms_per_increment": float(np.mean(posts)) if posts else None,
+        "qp_tangent_ms": qp_ms, "element_ms": el_ms, "spmv_pair_us": spmv_ms * 1e3,
+        "rows": [r["row"] for r in log if "row" in r],
+        "roofline": {"kernel": "k_bicg_spmv<%s> (K p and K^T p~, 27-point node-block)" % ("float" if form == "mixed" else "double"),
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": spmv_ms,
                      "algorithmic_bytes": bytes_spmv},
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args, quick=(n < 32))
+    res["T"] = ctx.get_state("T")
+    res["s"] = ctx.get_state("s")
+    res["u"] = ctx.get_state("u")
     ctx.close()
-    print(json.dumps(out))
+    return res
 
 
 if __name__ == "__main__":

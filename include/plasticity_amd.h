@@ -143,9 +143,12 @@ int pa_last_timings(pa_context *ctx, double ms[3]);
 /* Instrumentation (no counterpart in the reference): times `reps` launches of one kernel
  * between HIP events on the context stream, on the context's current data.  which:
  *   0 = BiCG operator pair  Zr = K Pr, Zl = K^T Pl  (the Krylov hot kernel)
- *   1 = one full assembly sweep (8 colour launches of the qp update + K_e + scatter)
+ *   1 = one full assembly (qp-tangent launch + 8 colour launches of K_e + scatter)
  *   2 = Jacobi-PCG operator  Q = K P
- * avg_ms receives the mean time of one launch (one sweep for which = 1). */
+ *   3 = the qp-tangent launch alone (qp update + dP/dF of every qp)
+ *   4 = the 8 element colour launches alone (K_e, R_e, condensed scatter)
+ * avg_ms receives the mean time of one launch (of one sweep for which = 1 and 4).  The matrix
+ * is not re-zeroed between repetitions (timing only). */
 int pa_time_kernel(pa_context *ctx, int which, int reps, double *avg_ms);
 
 #ifdef __cplusplus

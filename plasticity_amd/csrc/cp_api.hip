@@ -191,7 +191,8 @@ __global__ void k_csr_rowptr(MeshDims m, long long *__restrict__ rp) {
   rp[r] = 9 * before + c * len;
 }
 // one thread per (row, position): column of each entry, and the SB source of its value
-__global__ void k_csr_cols(MeshDims m, const long long *__restrict__ rp, int *__restrict__ ci, const double *__restrict__ K,
+template <typename KT>
+__global__ void k_csr_cols(MeshDims m, const long long *__restrict__ rp, int *__restrict__ ci, const KT *__restrict__ K,
                            double *__restrict__ val) {
   const long long r = blockIdx.x * (long long)blockDim.y + threadIdx.y;
   if (r >= 3 * m.nnode) return;
@@ -212,7 +213,7 @@ __global__ void k_csr_cols(MeshDims m, const long long *__restrict__ rp, int *__
   if (ci) ci[pos] = (int)(3 * mnode + b);
   if (val) {
     const int slot = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
-    val[pos] = K[(long long)(slot * 9 + 3 * a + b) * m.nnode + n];
+    val[pos] = (double)K[(long long)(slot * 9 + 3 * a + b) * m.nnode + n];
   }
 }
 
@@ -250,10 +251,11 @@ __global__ void k_dirichlet(MeshDims m, BCArgs bc, unsigned char *__restrict__ c
 }
 
 // ------------------------------------------------------------ assembly
-struct AsmArgs {
+// ---- assembly, stage 1: one lane per qp.  F from u, calculatePlasticity(.,.,1)
+// (getElementalValues.cc:129-141); writes the iter history and JxW * dP/dF (as DT) and JxW * P.
+struct QpArgs {
   MeshDims m;
   double h[3], JxW;
-  int cx, cy, cz, ncx, ncy, ncz;  // colour offsets and element counts of this colour
   const double *u;
   const unsigned *egrain;
   const double *gtab;
@@ -263,26 +265,77 @@ struct AsmArgs {
   long long nq;
   const double *Fp_conv, *s_conv, *slip_conv;
   double *Fe_iter, *Fp_iter, *s_iter, *slip_iter;
-  const unsigned char *con;
-  const double *delu;
-  int iter0;
-  double *K, *R;
+  void *D;    // [81][nq] of DT
+  double *P;  // [9][nq]
 };
 
-constexpr int ASM_EL = 8;                 // elements per 64-lane workgroup
-constexpr int DSTR = 81;                  // LDS stride of one qp's dP/dF
+template <int NS, typename DT>
+__global__ void __launch_bounds__(256) k_qp_tangent(QpArgs a) {
+  const long long Q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (Q >= a.nq) return;
+  const long long e = Q >> 3;
+  const int q = (int)(Q & 7);
+  const int ex = (int)(e % a.m.nx), ey = (int)((e / a.m.nx) % a.m.ny), ez = (int)(e / ((long long)a.m.nx * a.m.ny));
+  double F[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    const long long nd = (ex + (v & 1)) + (long long)a.m.nnx * ((ey + ((v >> 1) & 1)) + (long long)a.m.nny * (ez + ((v >> 2) & 1)));
+    double g[3];
+    shape_grad(q, v, a.h, g);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double uc = a.u[3 * nd + c];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) F[3 * c + j] += uc * g[j];
+    }
+  }
+  F[0] += 1; F[4] += 1; F[8] += 1;
+  double Fp_t[9], s[NS], x[NS], P[9], T[9], Fe[9], Fp[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) Fp_t[i] = a.Fp_conv[i * a.nq + Q];
+  for (int i = 0; i < a.ns; ++i) s[i] = a.s_conv[i * a.nq + Q];
+  const double *gt = a.gtab + (long long)a.egrain[e] * a.gstride;
+  bool plastic = false;
+  cp::qp_update<NS, DT>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 1, P, T, Fe, Fp, (DT *)a.D + Q, a.nq, &plastic, a.JxW);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    a.Fe_iter[i * a.nq + Q] = Fe[i];
+    a.Fp_iter[i * a.nq + Q] = Fp[i];
+    a.P[i * a.nq + Q] = a.JxW * P[i];
+  }
+  for (int i = 0; i < a.ns; ++i) a.s_iter[i * a.nq + Q] = s[i];
+  // slipfraction_iter is written only on the plastic path (calculatePlasticity.cc:454-456)
+  if (plastic)
+    for (int i = 0; i < a.ns; ++i) a.slip_iter[i * a.nq + Q] = a.slip_conv[i * a.nq + Q] + x[i];
+}
 
-// One colour of the element loop of assemble() (assemble.cc:64-90): lane = (element, qp)
-// runs calculatePlasticity(.,.,1) (getElementalValues.cc:141), then lane = (element,
-// vertex) forms 3 rows of K_e and R_e (:144-162) and adds them into the node-block
+// ---- assembly, stage 2: one colour of the element loop of assemble() (assemble.cc:64-90).
+// Lane = (element, vertex v1) forms 3 rows of K_e and R_e (getElementalValues.cc:144-162) from
+// the staged JxW dP/dF (KT arithmetic) and JxW P (FP64), and adds them into the node-block
 // Jacobian with deal.II's condensation of Dirichlet lines
 // (AffineConstraints::distribute_local_to_global): free-free entries added, free rows get
 // -K_ij g_j for constrained columns at iteration 0, constrained rows get |K_ii| (or the mean
-// |diag| of K_e when K_ii = 0) on the diagonal and nothing in R.  Elements of one colour
-// share no node, so the read-modify-write needs no atomics and the sum order is fixed.
-template <int NS>
-__global__ void __launch_bounds__(64) k_assemble(AsmArgs a) {
-  __shared__ double sD[64 * DSTR];
+// |diag| of K_e when K_ii = 0) on the diagonal and nothing in R.  Elements of one colour share
+// no node, so the read-modify-write needs no atomics and the sum order is fixed.
+struct ElemArgs {
+  MeshDims m;
+  double h[3];
+  int cx, cy, cz, ncx, ncy, ncz;
+  long long nq;
+  const void *D;
+  const double *P;
+  const unsigned char *con;
+  const double *delu;
+  int iter0;
+  void *K;
+  double *R;
+};
+
+constexpr int ASM_EL = 8;  // elements per 64-lane workgroup
+
+template <typename KT>
+__global__ void __launch_bounds__(64) k_element(ElemArgs a) {
+  __shared__ KT sD[64 * 81];
   __shared__ double sP[64 * 9];
   __shared__ double sdiag[ASM_EL * 24];
   const int t = threadIdx.x, le = t >> 3, sub = t & 7;
@@ -297,47 +350,23 @@ __global__ void __launch_bounds__(64) k_assemble(AsmArgs a) {
     ez = a.cz + 2 * (int)(ci / ((long long)a.ncx * a.ncy));
     e = ex + (long long)a.m.nx * (ey + (long long)a.m.ny * ez);
   }
+  // stage the 8 elements' 64 qps: lane t loads qp (le, sub)
+  const KT *Dg = (const KT *)a.D;
+  if (active) {
+    const long long Q = 8 * e + sub;
+#pragma unroll 9
+    for (int c = 0; c < 81; ++c) sD[c * 64 + t] = Dg[c * a.nq + Q];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) sP[t * 9 + c] = a.P[c * a.nq + Q];
+  }
+  __syncthreads();
   long long nodes[8];
 #pragma unroll
   for (int v = 0; v < 8; ++v)
     nodes[v] = (ex + (v & 1)) + (long long)a.m.nnx * ((ey + ((v >> 1) & 1)) + (long long)a.m.nny * (ez + ((v >> 2) & 1)));
-
-  // ---- phase 1: lane = qp
-  if (active) {
-    const int q = sub;
-    const long long Q = 8 * e + q;
-    double F[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int v = 0; v < 8; ++v) {
-      double g[3];
-      shape_grad(q, v, a.h, g);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const double uc = a.u[3 * nodes[v] + c];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) F[3 * c + j] += uc * g[j];
-      }
-    }
-    F[0] += 1; F[4] += 1; F[8] += 1;
-    double Fp_t[9], s[NS], x[NS], P[9], T[9], Fe[9], Fp[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) Fp_t[i] = a.Fp_conv[i * a.nq + Q];
-    for (int i = 0; i < a.ns; ++i) s[i] = a.s_conv[i * a.nq + Q];
-    const double *gt = a.gtab + (long long)a.egrain[e] * a.gstride;
-    bool plastic = false;
-    cp::qp_update<NS>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 1, P, T, Fe, Fp, sD + t, 64, &plastic);
-#pragma unroll
-    for (int i = 0; i < 9; ++i) { a.Fe_iter[i * a.nq + Q] = Fe[i]; a.Fp_iter[i * a.nq + Q] = Fp[i]; sP[t * 9 + i] = P[i]; }
-    for (int i = 0; i < a.ns; ++i) a.s_iter[i * a.nq + Q] = s[i];
-    // slipfraction_iter is written only on the plastic path (calculatePlasticity.cc:454-456)
-    if (plastic)
-      for (int i = 0; i < a.ns; ++i) a.slip_iter[i * a.nq + Q] = a.slip_conv[i * a.nq + Q] + x[i];
-  }
-  __syncthreads();
-
-  // ---- phase 2: lane = vertex v1 of element le; rows 3 v1 + i
   const int v1 = sub;
-  double Kr[3][24], Rr[3];
+  KT Kr[3][24];
+  double Rr[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     Rr[i] = 0;
@@ -346,17 +375,22 @@ __global__ void __launch_bounds__(64) k_assemble(AsmArgs a) {
   }
   if (active) {
     for (int q = 0; q < 8; ++q) {
-      const double *Dq = sD + (le * 8 + q);
+      const KT *Dq = sD + (le * 8 + q);
       const double *Pq = sP + (le * 8 + q) * 9;
-      double g1[3];
-      shape_grad(q, v1, a.h, g1);
-      double g2[8][3];
+      double g1d[3];
+      shape_grad(q, v1, a.h, g1d);
+      KT g1[3] = {(KT)g1d[0], (KT)g1d[1], (KT)g1d[2]};
+      KT g2[8][3];
 #pragma unroll
-      for (int v = 0; v < 8; ++v) shape_grad(q, v, a.h, g2[v]);
+      for (int v = 0; v < 8; ++v) {
+        double g[3];
+        shape_grad(q, v, a.h, g);
+        g2[v][0] = (KT)g[0]; g2[v][1] = (KT)g[1]; g2[v][2] = (KT)g[2];
+      }
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        Rr[i] -= g1[0] * Pq[3 * i] + g1[1] * Pq[3 * i + 1] + g1[2] * Pq[3 * i + 2];
-        double w[9];
+        Rr[i] -= g1d[0] * Pq[3 * i] + g1d[1] * Pq[3 * i + 1] + g1d[2] * Pq[3 * i + 2];
+        KT w[9];
 #pragma unroll
         for (int jl = 0; jl < 9; ++jl)
           w[jl] = g1[0] * Dq[((3 * i + 0) * 9 + jl) * 64] + g1[1] * Dq[((3 * i + 1) * 9 + jl) * 64] + g1[2] * Dq[((3 * i + 2) * 9 + jl) * 64];
@@ -367,12 +401,7 @@ __global__ void __launch_bounds__(64) k_assemble(AsmArgs a) {
       }
     }
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      Rr[i] *= a.JxW;
-#pragma unroll
-      for (int c = 0; c < 24; ++c) Kr[i][c] *= a.JxW;
-      sdiag[le * 24 + 3 * v1 + i] = Kr[i][3 * v1 + i];
-    }
+    for (int i = 0; i < 3; ++i) sdiag[le * 24 + 3 * v1 + i] = (double)Kr[i][3 * v1 + i];
   }
   __syncthreads();
   if (!active) return;
@@ -381,12 +410,13 @@ __global__ void __launch_bounds__(64) k_assemble(AsmArgs a) {
   avg /= 24.0;
   const long long n1 = nodes[v1];
   const long long nn = a.m.nnode;
+  KT *K = (KT *)a.K;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     const long long gi = 3 * n1 + i;
     if (a.con[gi]) {
-      const double kd = fabs(Kr[i][3 * v1 + i]);
-      a.K[(long long)(13 * 9 + 4 * i) * nn + n1] += (kd != 0 ? kd : avg);
+      const double kd = fabs((double)Kr[i][3 * v1 + i]);
+      K[(long long)(13 * 9 + 4 * i) * nn + n1] += (KT)(kd != 0 ? kd : avg);
       continue;
     }
     double rv = Rr[i];
@@ -399,10 +429,10 @@ __global__ void __launch_bounds__(64) k_assemble(AsmArgs a) {
         if (a.con[gj]) {
           if (a.iter0) {
             const double g = a.delu[gj];
-            if (g != 0.0) rv -= Kr[i][3 * v2 + j] * g;
+            if (g != 0.0) rv -= (double)Kr[i][3 * v2 + j] * g;
           }
         } else {
-          a.K[(long long)(slot * 9 + 3 * i + j) * nn + n1] += Kr[i][3 * v2 + j];
+          K[(long long)(slot * 9 + 3 * i + j) * nn + n1] += Kr[i][3 * v2 + j];
         }
       }
     }
@@ -502,7 +532,7 @@ __global__ void __launch_bounds__(256) k_post(PostArgs a) {
     for (int i = 0; i < a.ns; ++i) s[i] = a.s_conv[i * a.nq + Q];
     const double *gt = a.gtab + (long long)a.egrain[e] * a.gstride;
     bool plastic = false;
-    cp::qp_update<NS>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 0, P, T, Fe, Fp, nullptr, 0, &plastic);
+    cp::qp_update<NS, double>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 0, P, T, Fe, Fp, (double *)nullptr, 0, &plastic);
 #pragma unroll
     for (int i = 0; i < 9; ++i) { a.Fe_iter[i * a.nq + Q] = Fe[i]; a.Fp_iter[i * a.nq + Q] = Fp[i]; a.T[i * a.nq + Q] = T[i]; }
     for (int i = 0; i < a.ns; ++i) a.s_iter[i * a.nq + Q] = s[i];
@@ -758,12 +788,13 @@ __global__ void __launch_bounds__(256) k_cg_p(long long N, const double *__restr
   P[r] = (first || st->it == 0) ? Z[r] : Z[r] + st->b * P[r];
 }
 
-__global__ void k_dinv(MeshDims m, const double *__restrict__ K, double *__restrict__ dinv) {
+template <typename KT>
+__global__ void k_dinv(MeshDims m, const KT *__restrict__ K, double *__restrict__ dinv) {
   const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (r >= 3 * m.nnode) return;
   const long long n = r / 3;
   const int c = (int)(r % 3);
-  const double d = K[(long long)(13 * 9 + 4 * c) * m.nnode + n];
+  const double d = (double)K[(long long)(13 * 9 + 4 * c) * m.nnode + n];
   dinv[r] = (d == 0.0) ? 1.0 : 1.0 / d;
 }
 
@@ -816,7 +847,10 @@ struct pa_context {
   unsigned *d_egrain = nullptr;
   double *Fp_conv = nullptr, *Fp_iter = nullptr, *Fe_conv = nullptr, *Fe_iter = nullptr, *s_conv = nullptr, *s_iter = nullptr;
   double *slip_conv = nullptr, *slip_iter = nullptr, *rotnew = nullptr, *T = nullptr;
-  double *u = nullptr, *R = nullptr, *K = nullptr, *dinv = nullptr;
+  double *u = nullptr, *R = nullptr, *dinv = nullptr;
+  void *K = nullptr;   // node-block Jacobian, double (FP64 form) or float (mixed form)
+  void *Dq = nullptr;  // [81][nq] JxW dP/dF of the last assembly, double or float
+  double *Pq = nullptr;  // [9][nq] JxW P
   unsigned char *con = nullptr;
   double *delu = nullptr;
   double *kv[8] = {nullptr};  // Krylov vectors: X Rr Rl Zr Zl Pr Pl spare
@@ -842,8 +876,10 @@ int dalloc(pa_context *c, T **p, size_t count) {
 
 int ns_template(int ns) { return ns <= 12 ? 12 : (ns <= 24 ? 24 : 48); }
 
-int launch_assemble(pa_context *c, int iteration) {
-  AsmArgs a;
+bool mixed(const pa_context *c) { return c->precision == PA_PREC_MIXED; }
+
+int launch_qp_tangent(pa_context *c) {
+  QpArgs a;
   a.m = c->m;
   for (int d = 0; d < 3; ++d) a.h[d] = c->h[d];
   a.JxW = c->JxW;
@@ -851,6 +887,30 @@ int launch_assemble(pa_context *c, int iteration) {
   a.ns = c->ns; a.max_solver_iter = c->mat.max_solver_iter; a.nq = c->nq;
   a.Fp_conv = c->Fp_conv; a.s_conv = c->s_conv; a.slip_conv = c->slip_conv;
   a.Fe_iter = c->Fe_iter; a.Fp_iter = c->Fp_iter; a.s_iter = c->s_iter; a.slip_iter = c->slip_iter;
+  a.D = c->Dq; a.P = c->Pq;
+  const unsigned grid = nblk(c->nq, 256);
+  const int NS = ns_template(c->ns);
+  const bool f = mixed(c);
+  if (NS == 12) {
+    if (f) hipLaunchKernelGGL((k_qp_tangent<12, float>), dim3(grid), dim3(256), 0, c->stream, a);
+    else hipLaunchKernelGGL((k_qp_tangent<12, double>), dim3(grid), dim3(256), 0, c->stream, a);
+  } else if (NS == 24) {
+    if (f) hipLaunchKernelGGL((k_qp_tangent<24, float>), dim3(grid), dim3(256), 0, c->stream, a);
+    else hipLaunchKernelGGL((k_qp_tangent<24, double>), dim3(grid), dim3(256), 0, c->stream, a);
+  } else {
+    if (f) hipLaunchKernelGGL((k_qp_tangent<48, float>), dim3(grid), dim3(256), 0, c->stream, a);
+    else hipLaunchKernelGGL((k_qp_tangent<48, double>), dim3(grid), dim3(256), 0, c->stream, a);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int launch_elements(pa_context *c, int iteration) {
+  ElemArgs a;
+  a.m = c->m;
+  for (int d = 0; d < 3; ++d) a.h[d] = c->h[d];
+  a.nq = c->nq;
+  a.D = c->Dq; a.P = c->Pq;
   a.con = c->con; a.delu = c->delu; a.iter0 = (iteration == 0);
   a.K = c->K; a.R = c->R;
   for (int col = 0; col < 8; ++col) {
@@ -859,14 +919,20 @@ int launch_assemble(pa_context *c, int iteration) {
     const long long ncol = (long long)a.ncx * a.ncy * a.ncz;
     if (ncol == 0) continue;
     const unsigned grid = nblk(ncol, ASM_EL);
-    const int NS = ns_template(c->ns);
-    if (NS == 12) hipLaunchKernelGGL(k_assemble<12>, dim3(grid), dim3(64), 0, c->stream, a);
-    else if (NS == 24) hipLaunchKernelGGL(k_assemble<24>, dim3(grid), dim3(64), 0, c->stream, a);
-    else hipLaunchKernelGGL(k_assemble<48>, dim3(grid), dim3(64), 0, c->stream, a);
+    if (mixed(c)) hipLaunchKernelGGL(k_element<float>, dim3(grid), dim3(64), 0, c->stream, a);
+    else hipLaunchKernelGGL(k_element<double>, dim3(grid), dim3(64), 0, c->stream, a);
     HIPCHK(hipGetLastError());
   }
   return 0;
 }
+
+int launch_assemble(pa_context *c, int iteration) {
+  int rc = launch_qp_tangent(c);
+  if (rc) return rc;
+  return launch_elements(c, iteration);
+}
+
+size_t kbytes(const pa_context *c) { return (mixed(c) ? sizeof(float) : sizeof(double)) * 243 * (size_t)c->m.nnode; }
 
 int set_dirichlet(pa_context *c, int inc) {
   if (c->bc_inc == inc) return 0;
@@ -895,11 +961,12 @@ int set_dirichlet(pa_context *c, int inc) {
 int do_assemble(pa_context *c, int inc, int it, double *res_norm) {
   int rc = set_dirichlet(c, inc);
   if (rc) return rc;
-  HIPCHK(hipMemsetAsync(c->K, 0, sizeof(double) * 243 * (size_t)c->m.nnode, c->stream));
+  HIPCHK(hipMemsetAsync(c->K, 0, kbytes(c), c->stream));
   HIPCHK(hipMemsetAsync(c->R, 0, sizeof(double) * (size_t)c->ndof, c->stream));
   rc = launch_assemble(c, it);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_dinv, dim3(nblk(c->ndof, 256)), dim3(256), 0, c->stream, c->m, c->K, c->dinv);
+  if (mixed(c)) hipLaunchKernelGGL(k_dinv<float>, dim3(nblk(c->ndof, 256)), dim3(256), 0, c->stream, c->m, (const float *)c->K, c->dinv);
+  else hipLaunchKernelGGL(k_dinv<double>, dim3(nblk(c->ndof, 256)), dim3(256), 0, c->stream, c->m, (const double *)c->K, c->dinv);
   HIPCHK(hipGetLastError());
   if (res_norm) {
     hipLaunchKernelGGL(k_norm2, dim3(vgrid(c->ndof)), dim3(256), 0, c->stream, c->ndof, c->R, c->d_out + 16, c->partials,
@@ -924,15 +991,23 @@ int do_solve(pa_context *c, int it, int *iters) {
     for (int k = 0; k < chunk; ++k) {
       if (c->method == PA_SOLVER_CG) {
         hipLaunchKernelGGL(k_cg_p, dim3(gv), dim3(256), 0, c->stream, N, Zr, Pr, (const KState *)c->kst, 0);
-        hipLaunchKernelGGL(k_cg_spmv<double>, dim3(gn), dim3(256), 0, c->stream, c->m, (const double *)c->K, (const double *)Pr, Zl,
-                           c->kst, c->partials, c->counters + 1);
+        if (mixed(c))
+          hipLaunchKernelGGL(k_cg_spmv<float>, dim3(gn), dim3(256), 0, c->stream, c->m, (const float *)c->K, (const double *)Pr, Zl,
+                             c->kst, c->partials, c->counters + 1);
+        else
+          hipLaunchKernelGGL(k_cg_spmv<double>, dim3(gn), dim3(256), 0, c->stream, c->m, (const double *)c->K, (const double *)Pr, Zl,
+                             c->kst, c->partials, c->counters + 1);
         hipLaunchKernelGGL(k_cg_update, dim3(gr), dim3(256), 0, c->stream, N, X, (const double *)Pr, Rr, (const double *)Zl, Zr,
                            (const double *)c->dinv, c->kst, c->partials, c->counters + 2);
       } else {
         hipLaunchKernelGGL(k_bicg_p, dim3(gv), dim3(256), 0, c->stream, N, (const double *)Zr, (const double *)Zl, Pr, Pl,
                            (const KState *)c->kst);
-        hipLaunchKernelGGL(k_bicg_spmv<double>, dim3(gn), dim3(256), 0, c->stream, c->m, (const double *)c->K, (const double *)Pr,
-                           (const double *)Pl, Zr, Zl, c->kst, c->partials, c->counters + 1);
+        if (mixed(c))
+          hipLaunchKernelGGL(k_bicg_spmv<float>, dim3(gn), dim3(256), 0, c->stream, c->m, (const float *)c->K, (const double *)Pr,
+                             (const double *)Pl, Zr, Zl, c->kst, c->partials, c->counters + 1);
+        else
+          hipLaunchKernelGGL(k_bicg_spmv<double>, dim3(gn), dim3(256), 0, c->stream, c->m, (const double *)c->K, (const double *)Pr,
+                             (const double *)Pl, Zr, Zl, c->kst, c->partials, c->counters + 1);
         hipLaunchKernelGGL(k_bicg_update, dim3(gr), dim3(256), 0, c->stream, N, X, (const double *)Pr, Rr, Rl, Zr, Zl,
                            (const double *)c->dinv, c->kst, c->partials, c->counters + 2);
       }
@@ -1006,7 +1081,8 @@ int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out) {
       (rc = dalloc(c, &c->Fe_iter, 9 * nq)) || (rc = dalloc(c, &c->s_conv, ns * nq)) || (rc = dalloc(c, &c->s_iter, ns * nq)) ||
       (rc = dalloc(c, &c->slip_conv, ns * nq)) || (rc = dalloc(c, &c->slip_iter, ns * nq)) || (rc = dalloc(c, &c->rotnew, 3 * nq)) ||
       (rc = dalloc(c, &c->T, 9 * nq)) || (rc = dalloc(c, &c->u, N)) || (rc = dalloc(c, &c->R, N)) ||
-      (rc = dalloc(c, &c->K, 243 * nn)) || (rc = dalloc(c, &c->dinv, N)) || (rc = dalloc(c, &c->con, N)) ||
+      (rc = dalloc(c, (double **)&c->K, 243 * nn)) || (rc = dalloc(c, (double **)&c->Dq, 81 * nq)) ||
+      (rc = dalloc(c, &c->Pq, 9 * nq)) || (rc = dalloc(c, &c->dinv, N)) || (rc = dalloc(c, &c->con, N)) ||
       (rc = dalloc(c, &c->delu, N)) || (rc = dalloc(c, &c->d_egrain, m.nelem)) || (rc = dalloc(c, &c->kst, 1)) ||
       (rc = dalloc(c, &c->partials, 16 * (size_t)nblk(std::max(N, nq), 64))) || (rc = dalloc(c, &c->counters, 16)) ||
       (rc = dalloc(c, &c->d_out, 64))) {
@@ -1119,7 +1195,7 @@ int pa_set_grains(pa_context *c, const unsigned *elem_grain, const double *grain
 int pa_set_solver(pa_context *c, int method, int precision) {
   if (!c) return fail(PA_ERR_ARG, "null context");
   if (method != PA_SOLVER_BICG && method != PA_SOLVER_CG) return fail(PA_ERR_ARG, "unknown solver");
-  if (precision != PA_PREC_FP64) return fail(PA_ERR_UNSUPPORTED, "only the FP64 form is available");
+  if (precision != PA_PREC_FP64 && precision != PA_PREC_MIXED) return fail(PA_ERR_ARG, "unknown precision");
   c->method = method;
   c->precision = precision;
   return 0;
@@ -1268,8 +1344,12 @@ int pa_get_csr(pa_context *c, long long *row_ptr, int *col_idx, double *values) 
   if (!rc && values && hipMalloc(&dval, c->nnz * sizeof(double)) != hipSuccess) rc = fail(PA_ERR_HIP, "hipMalloc");
   if (!rc) {
     hipLaunchKernelGGL(k_csr_rowptr, dim3(nblk(c->ndof + 1, 256)), dim3(256), 0, c->stream, c->m, drp);
-    hipLaunchKernelGGL(k_csr_cols, dim3(nblk(c->ndof, 4)), dim3(81, 4), 0, c->stream, c->m, (const long long *)drp, dci,
-                       (const double *)c->K, dval);
+    if (mixed(c))
+      hipLaunchKernelGGL(k_csr_cols<float>, dim3(nblk(c->ndof, 4)), dim3(81, 4), 0, c->stream, c->m, (const long long *)drp, dci,
+                         (const float *)c->K, dval);
+    else
+      hipLaunchKernelGGL(k_csr_cols<double>, dim3(nblk(c->ndof, 4)), dim3(81, 4), 0, c->stream, c->m, (const long long *)drp, dci,
+                         (const double *)c->K, dval);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(PA_ERR_HIP, "csr kernels");
   }
   if (!rc && hipMemcpy(row_ptr, drp, (c->ndof + 1) * sizeof(long long), hipMemcpyDeviceToHost) != hipSuccess) rc = fail(PA_ERR_HIP, "copy");
@@ -1334,18 +1414,31 @@ int pa_time_kernel(pa_context *c, int which, int reps, double *avg_ms) {
   HIPCHK(hipMemcpyAsync(c->kst, &ks, sizeof(ks), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipEventRecord(c->ev[0], c->stream));
   for (int r = 0; r < reps; ++r) {
+    int rc = 0;
     if (which == 0) {
-      hipLaunchKernelGGL(k_bicg_spmv<double>, dim3(gn), dim3(256), 0, c->stream, c->m, (const double *)c->K, (const double *)c->kv[5],
-                         (const double *)c->kv[6], c->kv[3], c->kv[4], c->kst, c->partials, c->counters + 1);
+      if (mixed(c))
+        hipLaunchKernelGGL(k_bicg_spmv<float>, dim3(gn), dim3(256), 0, c->stream, c->m, (const float *)c->K, (const double *)c->kv[5],
+                           (const double *)c->kv[6], c->kv[3], c->kv[4], c->kst, c->partials, c->counters + 1);
+      else
+        hipLaunchKernelGGL(k_bicg_spmv<double>, dim3(gn), dim3(256), 0, c->stream, c->m, (const double *)c->K, (const double *)c->kv[5],
+                           (const double *)c->kv[6], c->kv[3], c->kv[4], c->kst, c->partials, c->counters + 1);
     } else if (which == 1) {
-      int rc = launch_assemble(c, 1);
-      if (rc) return rc;
+      rc = launch_assemble(c, 1);
     } else if (which == 2) {
-      hipLaunchKernelGGL(k_cg_spmv<double>, dim3(gn), dim3(256), 0, c->stream, c->m, (const double *)c->K, (const double *)c->kv[5],
-                         c->kv[4], c->kst, c->partials, c->counters + 1);
+      if (mixed(c))
+        hipLaunchKernelGGL(k_cg_spmv<float>, dim3(gn), dim3(256), 0, c->stream, c->m, (const float *)c->K, (const double *)c->kv[5],
+                           c->kv[4], c->kst, c->partials, c->counters + 1);
+      else
+        hipLaunchKernelGGL(k_cg_spmv<double>, dim3(gn), dim3(256), 0, c->stream, c->m, (const double *)c->K, (const double *)c->kv[5],
+                           c->kv[4], c->kst, c->partials, c->counters + 1);
+    } else if (which == 3) {
+      rc = launch_qp_tangent(c);
+    } else if (which == 4) {
+      rc = launch_elements(c, 1);
     } else {
       return fail(PA_ERR_ARG, "unknown kernel id");
     }
+    if (rc) return rc;
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(c->ev[1], c->stream));

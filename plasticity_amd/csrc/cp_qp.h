@@ -151,7 +151,7 @@ CP_DEV bool lu_solve(int n, double *A, double *B, int m) {
 // x = pinv(A) b by one-sided Jacobi SVD, singular values <= CP_PINV_RTOL * sigma_max dropped.
 // Used only when LU finds A_PA numerically singular.
 template <int NS>
-CP_DEV void svd_pinv_solve(int n, const double *A, const double *b, double *x) {
+__host__ __device__ __attribute__((noinline)) void svd_pinv_solve(int n, const double *A, const double *b, double *x) {
   double U[NS * NS], V[NS * NS];
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) { U[i * NS + j] = A[i * NS + j]; V[i * NS + j] = (i == j) ? 1.0 : 0.0; }
@@ -209,24 +209,23 @@ template <int NS> struct GrainLayout {
 };
 
 // dexp series of calculatePlasticity.cc:645-676 (sign = -1) and :748-779 (sign = +1):
-// out = S + sum_{k=1..count3} c_k sum_{l=0..k} X^l S X^(k-l), c_k = (+-1)^k / (k+1)!
-#define CP_MAXPOW 24
-CP_DEV void dexp_series(const double (*Xp)[9], const double *S, int count3, bool neg, double *out) {
+// out = S + sum_{k=1..count3} c_k W_k, W_k = sum_{l=0..k} X^l S X^(k-l), c_k = (+-1)^k / (k+1)!.
+// W_k is formed by the recurrence W_k = X W_{k-1} + S X^k (W_0 = S), so only X^k and W_k are kept.
+CP_DEV void dexp_series(const double *X, const double *S, int count3, bool neg, double *out) {
+  double W[9], Xk[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, t1[9], t2[9];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) out[i] = S[i];
+  for (int i = 0; i < 9; ++i) { out[i] = S[i]; W[i] = S[i]; }
   double fact = 1.0;  // (k+1)!
   for (int k = 1; k <= count3; ++k) {
     fact *= (double)(k + 1);
-    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, t5[9], t6[9];
-    for (int l = 0; l <= k; ++l) {
-      m3_mul(Xp[l], S, t5);
-      m3_mul(t5, Xp[k - l], t6);
+    m3_mul(Xk, X, t1);
 #pragma unroll
-      for (int i = 0; i < 9; ++i) acc[i] += t6[i];
-    }
+    for (int i = 0; i < 9; ++i) Xk[i] = t1[i];
+    m3_mul(X, W, t1);
+    m3_mul(S, Xk, t2);
     const double c = ((neg && (k & 1)) ? -1.0 : 1.0) / fact;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) out[i] += c * acc[i];
+    for (int i = 0; i < 9; ++i) { W[i] = t1[i] + t2[i]; out[i] += c * W[i]; }
   }
 }
 
@@ -234,12 +233,13 @@ CP_DEV void dexp_series(const double (*Xp)[9], const double *S, int count3, bool
 // s holds s_conv on entry and s_tau on exit; x receives x_beta_old (the slip increments).
 // *plastic_path (optional) is set when the potentially-active set was non-empty, the case in
 // which the reference writes slipfraction_iter (calculatePlasticity.cc:454-456).
-// D (flag = 1) receives dP/dF as D[(3i+k)*ldD ...] with element stride `dstride`:
-// D[((3i+k)*9 + 3j+l) * dstride] = dP_ik / dF_jl.
-template <int NS>
+// D (flag = 1) receives dscale * dP/dF with element stride `dstride`:
+// D[((3i+k)*9 + 3j+l) * dstride] = dscale * dP_ik / dF_jl, stored as DT.
+template <int NS, typename DT = double>
 CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ gt, int ns, int max_solver_iter,
                      const double *F, const double *Fp_t, double *s, double *x, int flag, double *P, double *T,
-                     double *Fe_out, double *Fp_out, double *D, int dstride, bool *plastic_path = nullptr) {
+                     double *Fe_out, double *Fp_out, DT *D, long long dstride, bool *plastic_path = nullptr,
+                     double dscale = 1.0) {
   typedef MatLayout<NS> ML;
   typedef GrainLayout<NS> GL;
   const double tol1 = mat[ML::tol];
@@ -398,18 +398,13 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
 #pragma unroll
   for (int i = 0; i < 81; ++i) dFp[i] = 0.0;
   if (plastic && nPA > 0) {
-    // count3 (:607-619)
-    double Xp[CP_MAXPOW][9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) Xp[0][i] = (i % 4 == 0) ? 1.0 : 0.0;
+    // count3 (:607-619): number of series terms until ||X^k / k!||_F <= max(tol/1e4, 1e-12)
     int count3 = 0;
     {
-      double t2[9], t1[9];
-#pragma unroll
-      for (int i = 0; i < 9; ++i) t2[i] = Xp[0][i];
+      double t2[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, t1[9];
       double tol2 = 1.0;
       const double lim = fmax(tol1 / 1e4, 1e-12);
-      while (tol2 > lim && count3 < CP_MAXPOW - 1) {
+      while (tol2 > lim && count3 < 64) {
         count3++;
         m3_mul(XdelFp, t2, t1);
         const double r = 1.0 / count3;
@@ -419,7 +414,6 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
 #pragma unroll
         for (int i = 0; i < 9; ++i) t2[i] = t1[i];
       }
-      for (int p = 1; p <= count3; ++p) m3_mul(Xp[p - 1], XdelFp, Xp[p]);
     }
     // delb (:581-604): delb(k, kl) = +- S_k : deltau_delF(:, kl), streamed per column kl
     double rhs[NS * 9];
@@ -462,7 +456,7 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
     for (int jj = 0; jj < nPA; ++jj) {
       const int j = PA[jj];
       double diff[9], t2[9], t3[9], tm[9];
-      dexp_series(Xp, Sg + 9 * j, count3, true, diff);
+      dexp_series(XdelFp, Sg + 9 * j, count3, true, diff);
       m3_tmul(diff, t6, t2);
       m3_symm(t2);
       dmat_apply(Dm, t2, t3);
@@ -486,7 +480,7 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
     for (int jj = 0; jj < nPA; ++jj) {
       const int j = PA[jj];
       double diff[9], t1[9];
-      dexp_series(Xp, Sg + 9 * j, count3, false, diff);
+      dexp_series(XdelFp, Sg + 9 * j, count3, false, diff);
       m3_mul(diff, Fp_t, t1);
       const double sg = (tau_tr[j] < 0) ? -1.0 : 1.0;
 #pragma unroll
@@ -550,7 +544,7 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
       for (int a = 0; a < 3; ++a) qa += q5[3 * i + a] * Finv[3 * l + a];
 #pragma unroll
       for (int j = 0; j < 3; ++j)
-        D[((3 * i + j) * 9 + kl) * dstride] = t1[3 * i + j] + t2[3 * i + j] - t3[3 * i + j] - qa * Finv[3 * j + k];
+        D[((3 * i + j) * 9 + kl) * dstride] = (DT)(dscale * (t1[3 * i + j] + t2[3 * i + j] - t3[3 * i + j] - qa * Finv[3 * j + k]));
     }
   }
   return n_active;
