@@ -4,9 +4,11 @@
 // Data layout in HBM (one context):
 //   per-qp state, structure of arrays, component-major:  X[c * nq + Q]  (Q = 8 e + q)
 //     Fp_conv/Fp_iter/Fe_conv/Fe_iter 9, s_conv/s_iter ns, slip_conv/slip_iter ns, rotnew 3, T 9
-//   Jacobian in 27-point node-block form: K[(slot * 9 + 3a + b) * nnode + n] is the 3x3 block
+//   Jacobian in 27-point node-block form: K[(slot * nnode + n) * 9 + 3a + b] is the 3x3 block
 //     coupling node n (rows 3n+a) to its neighbour n + off(slot) (cols 3m+b),
-//     slot = (dz+1)*9 + (dy+1)*3 + (dx+1).  Entries outside the mesh stay zero.
+//     slot = (dz+1)*9 + (dy+1)*3 + (dx+1).  Entries outside the mesh stay zero.  Each block is
+//     72 (FP64) or 36 (FP32) contiguous bytes, so one lane loads it with wide loads and the 64
+//     lanes of a wave, holding consecutive nodes, read one contiguous span per slot.
 //   vectors: dof = 3 n + component, FP64.
 #include <hip/hip_runtime.h>
 
@@ -213,7 +215,7 @@ __global__ void k_csr_cols(MeshDims m, const long long *__restrict__ rp, int *__
   if (ci) ci[pos] = (int)(3 * mnode + b);
   if (val) {
     const int slot = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
-    val[pos] = (double)K[(long long)(slot * 9 + 3 * a + b) * m.nnode + n];
+    val[pos] = (double)K[((long long)slot * m.nnode + n) * 9 + 3 * a + b];
   }
 }
 
@@ -416,7 +418,7 @@ __global__ void __launch_bounds__(64) k_element(ElemArgs a) {
     const long long gi = 3 * n1 + i;
     if (a.con[gi]) {
       const double kd = fabs((double)Kr[i][3 * v1 + i]);
-      K[(long long)(13 * 9 + 4 * i) * nn + n1] += (KT)(kd != 0 ? kd : avg);
+      K[(13 * nn + n1) * 9 + 4 * i] += (KT)(kd != 0 ? kd : avg);
       continue;
     }
     double rv = Rr[i];
@@ -432,7 +434,7 @@ __global__ void __launch_bounds__(64) k_element(ElemArgs a) {
             if (g != 0.0) rv -= (double)Kr[i][3 * v2 + j] * g;
           }
         } else {
-          K[(long long)(slot * 9 + 3 * i + j) * nn + n1] += Kr[i][3 * v2 + j];
+          K[((long long)slot * nn + n1) * 9 + 3 * i + j] += Kr[i][3 * v2 + j];
         }
       }
     }
@@ -617,11 +619,51 @@ __device__ __forceinline__ long long nb_node(const MeshDims &m, long long n, int
   return n + dx + (long long)dy * m.nnx + (long long)dz * m.nnx * m.nny;
 }
 
-// y = K p, yt = K^T pt on the node-block matrix; partial dot(y, pl) for BiCG's dpi
+// Block sums written to partials[blockIdx.x * NV + k]; consumed by the next kernel (kernel
+// boundaries order the writes), which sums them in block order in every block.
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&v)[NV], double *partials, double *lds) {
+  block_sum<NV>(v, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) partials[blockIdx.x * NV + k] = v[k];
+  }
+}
+// Sum of nparts block partials (one value each), identical in every block; returned to all lanes.
+__device__ __forceinline__ double sum_partials(const double *partials, unsigned nparts, double *lds) {
+  double v[1] = {0.0};
+  for (unsigned b = threadIdx.x; b < nparts; b += blockDim.x) v[0] += partials[b];
+  block_sum<1>(v, lds);
+  __shared__ double total;
+  if (threadIdx.x == 0) total = v[0];
+  __syncthreads();
+  return total;
+}
+
+// 3x3 block product of one slot: y += B x (B row-major, 9 contiguous values)
+template <typename VT>
+__device__ __forceinline__ void block_mv(const VT *__restrict__ B, double x0, double x1, double x2, double y[3]) {
+  VT b[9];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) b[c] = B[c];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) y[r] += (double)b[3 * r] * x0 + (double)b[3 * r + 1] * x1 + (double)b[3 * r + 2] * x2;
+}
+// y += B^T x
+template <typename VT>
+__device__ __forceinline__ void block_tmv(const VT *__restrict__ B, double x0, double x1, double x2, double y[3]) {
+  VT b[9];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) b[c] = B[c];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) y[r] += (double)b[r] * x0 + (double)b[3 + r] * x1 + (double)b[6 + r] * x2;
+}
+
+// Zr = K Pr, Zl = K^T Pl on the node-block matrix (lane = node); block partials of Zr.Pl (BiCG's dpi)
 template <typename VT>
 __global__ void __launch_bounds__(256) k_bicg_spmv(MeshDims m, const VT *__restrict__ K, const double *__restrict__ Pr,
                                                    const double *__restrict__ Pl, double *__restrict__ Zr, double *__restrict__ Zl,
-                                                   KState *st, double *partials, unsigned *counter) {
+                                                   const KState *st, double *partials) {
   __shared__ double lds[16];
   if (st->done) return;
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -636,23 +678,14 @@ __global__ void __launch_bounds__(256) k_bicg_spmv(MeshDims m, const VT *__restr
       bool ok;
       const long long mn = nb_node(m, n, i, j, k, s, ok);
       if (!ok) continue;
-      const double x0 = Pr[3 * mn], x1 = Pr[3 * mn + 1], x2 = Pr[3 * mn + 2];
-      const double q0 = Pl[3 * mn], q1 = Pl[3 * mn + 1], q2 = Pl[3 * mn + 2];
-      const VT *Kb = K + (long long)(s * 9) * nn + n;
-      const VT *Kt = K + (long long)((26 - s) * 9) * nn + mn;
-#pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        y[r] += (double)Kb[(3 * r) * nn] * x0 + (double)Kb[(3 * r + 1) * nn] * x1 + (double)Kb[(3 * r + 2) * nn] * x2;
-        yt[r] += (double)Kt[(r) * nn] * q0 + (double)Kt[(3 + r) * nn] * q1 + (double)Kt[(6 + r) * nn] * q2;
-      }
+      block_mv(K + ((long long)s * nn + n) * 9, Pr[3 * mn], Pr[3 * mn + 1], Pr[3 * mn + 2], y);
+      block_tmv(K + ((long long)(26 - s) * nn + mn) * 9, Pl[3 * mn], Pl[3 * mn + 1], Pl[3 * mn + 2], yt);
     }
 #pragma unroll
     for (int r = 0; r < 3; ++r) { Zr[3 * n + r] = y[r]; Zl[3 * n + r] = yt[r]; }
     acc[0] = y[0] * Pl[3 * n] + y[1] * Pl[3 * n + 1] + y[2] * Pl[3 * n + 2];
   }
-  if (grid_sum<1>(acc, partials, counter, lds)) {
-    st->alpha = st->beta / acc[0];
-  }
+  block_partials<1>(acc, partials, lds);
 }
 
 // p <- z (first iteration) or p <- z + b p
@@ -665,16 +698,18 @@ __global__ void __launch_bounds__(256) k_bicg_p(long long N, const double *__res
   else { const double b = st->b; Pr[r] = Zr[r] + b * Pr[r]; Pl[r] = Zl[r] + b * Pl[r]; }
 }
 
-// x += a p; r -= a z; rl -= a zl; z = D^-1 r; zl = D^-1 rl; partial ||z||^2 and z.rl
+// a = beta / dpi (dpi from the operator kernel's partials); x += a p; r -= a z; rl -= a zl;
+// z = D^-1 r; zl = D^-1 rl; ||z|| and z.rl -> convergence test and the next beta
 __global__ void __launch_bounds__(256) k_bicg_update(long long N, double *__restrict__ X, const double *__restrict__ Pr,
                                                      double *__restrict__ Rr, double *__restrict__ Rl, double *__restrict__ Zr,
                                                      double *__restrict__ Zl, const double *__restrict__ dinv, KState *st,
-                                                     double *partials, unsigned *counter) {
+                                                     const double *spmv_partials, unsigned n_spmv_partials, double *partials,
+                                                     unsigned *counter) {
   __shared__ double lds[32];
   if (st->done) return;
+  const double a = st->beta / sum_partials(spmv_partials, n_spmv_partials, lds);
   const long long stride = (long long)gridDim.x * blockDim.x;
   double acc[2] = {0.0, 0.0};
-  const double a = st->alpha;
   for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < N; r += stride) {
     const double d = dinv[r];
     X[r] += a * Pr[r];
@@ -686,7 +721,9 @@ __global__ void __launch_bounds__(256) k_bicg_update(long long N, double *__rest
     acc[0] += zr * zr;
     acc[1] += zr * rl;
   }
+  __syncthreads();
   if (grid_sum<2>(acc, partials, counter, lds)) {
+    st->alpha = a;
     st->dp = sqrt(acc[0]);
     st->it += 1;
     if (st->dp <= st->tol || st->it >= st->maxit) st->done = 1;
@@ -724,10 +761,10 @@ __global__ void __launch_bounds__(256) k_bicg_init(long long N, const double *__
   }
 }
 
-// Jacobi-PCG (the option BASELINE names): q = K p and partial p.q
+// Jacobi-PCG (the option BASELINE names): q = K p and block partials of p.q
 template <typename VT>
 __global__ void __launch_bounds__(256) k_cg_spmv(MeshDims m, const VT *__restrict__ K, const double *__restrict__ P,
-                                                 double *__restrict__ Q, KState *st, double *partials, unsigned *counter) {
+                                                 double *__restrict__ Q, const KState *st, double *partials) {
   __shared__ double lds[16];
   if (st->done) return;
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -742,26 +779,23 @@ __global__ void __launch_bounds__(256) k_cg_spmv(MeshDims m, const VT *__restric
       bool ok;
       const long long mn = nb_node(m, n, i, j, k, s, ok);
       if (!ok) continue;
-      const double x0 = P[3 * mn], x1 = P[3 * mn + 1], x2 = P[3 * mn + 2];
-      const VT *Kb = K + (long long)(s * 9) * nn + n;
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-        y[r] += (double)Kb[(3 * r) * nn] * x0 + (double)Kb[(3 * r + 1) * nn] * x1 + (double)Kb[(3 * r + 2) * nn] * x2;
+      block_mv(K + ((long long)s * nn + n) * 9, P[3 * mn], P[3 * mn + 1], P[3 * mn + 2], y);
     }
 #pragma unroll
     for (int r = 0; r < 3; ++r) Q[3 * n + r] = y[r];
     acc[0] = y[0] * P[3 * n] + y[1] * P[3 * n + 1] + y[2] * P[3 * n + 2];
   }
-  if (grid_sum<1>(acc, partials, counter, lds)) st->alpha = st->beta / acc[0];
+  block_partials<1>(acc, partials, lds);
 }
 __global__ void __launch_bounds__(256) k_cg_update(long long N, double *__restrict__ X, const double *__restrict__ P,
                                                    double *__restrict__ Rr, const double *__restrict__ Q, double *__restrict__ Z,
-                                                   const double *__restrict__ dinv, KState *st, double *partials, unsigned *counter) {
+                                                   const double *__restrict__ dinv, KState *st, const double *spmv_partials,
+                                                   unsigned n_spmv_partials, double *partials, unsigned *counter) {
   __shared__ double lds[32];
   if (st->done) return;
+  const double a = st->beta / sum_partials(spmv_partials, n_spmv_partials, lds);
   const long long stride = (long long)gridDim.x * blockDim.x;
   double acc[2] = {0.0, 0.0};
-  const double a = st->alpha;
   for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < N; r += stride) {
     X[r] += a * P[r];
     const double rr = Rr[r] - a * Q[r];
@@ -771,7 +805,9 @@ __global__ void __launch_bounds__(256) k_cg_update(long long N, double *__restri
     acc[0] += z * z;
     acc[1] += rr * z;
   }
+  __syncthreads();
   if (grid_sum<2>(acc, partials, counter, lds)) {
+    st->alpha = a;
     st->dp = sqrt(acc[0]);
     st->it += 1;
     if (st->dp <= st->tol || st->it >= st->maxit) st->done = 1;
@@ -794,7 +830,7 @@ __global__ void k_dinv(MeshDims m, const KT *__restrict__ K, double *__restrict_
   if (r >= 3 * m.nnode) return;
   const long long n = r / 3;
   const int c = (int)(r % 3);
-  const double d = (double)K[(long long)(13 * 9 + 4 * c) * m.nnode + n];
+  const double d = (double)K[(13 * m.nnode + n) * 9 + 4 * c];
   dinv[r] = (d == 0.0) ? 1.0 : 1.0 / d;
 }
 
@@ -823,7 +859,7 @@ __global__ void k_zero(long long N, double *__restrict__ v) {
 inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 // grid of the reducing vector kernels: grid-stride loops over at most this many 256-lane blocks,
 // which bounds the partial sums the last block adds up
-constexpr unsigned VGRID_MAX = 1024;
+constexpr unsigned VGRID_MAX = 256;
 inline unsigned vgrid(long long n) { return std::min(nblk(n, 256), VGRID_MAX); }
 
 }  // namespace
@@ -932,6 +968,18 @@ int launch_assemble(pa_context *c, int iteration) {
   return launch_elements(c, iteration);
 }
 
+// partial-sum regions: [0, off) per-block partials of the operator kernels, [off, ...) the
+// last-block reductions of the vector kernels
+long long spmv_partials_offset(const pa_context *c) { return ((long long)nblk(c->m.nnode, 256) + 64) & ~63LL; }
+
+// doubles of the partial-sum buffer: operator partials, the vector kernels' reductions (<= 3 per
+// block of at most VGRID_MAX blocks) and the post-increment pass (13 per 256-lane block, from 0)
+size_t partials_size(const pa_context *c) {
+  const size_t vec = (size_t)spmv_partials_offset(c) + 3 * VGRID_MAX + 64;
+  const size_t post = 13 * (size_t)nblk(c->nq, 256) + 64;
+  return std::max(vec, post);
+}
+
 size_t kbytes(const pa_context *c) { return (mixed(c) ? sizeof(float) : sizeof(double)) * 243 * (size_t)c->m.nnode; }
 
 int set_dirichlet(pa_context *c, int inc) {
@@ -983,33 +1031,34 @@ int do_solve(pa_context *c, int it, int *iters) {
   double *X = c->kv[0], *Rr = c->kv[1], *Rl = c->kv[2], *Zr = c->kv[3], *Zl = c->kv[4], *Pr = c->kv[5], *Pl = c->kv[6];
   const unsigned gv = nblk(N, 256), gr = vgrid(N), gn = nblk(c->m.nnode, 256);
   hipLaunchKernelGGL(k_bicg_init, dim3(gr), dim3(256), 0, c->stream, N, c->R, X, Rr, Rl, Zr, Zl, c->dinv, c->kst, c->pb.lin_tol,
-                     c->pb.lin_maxit, c->partials, c->counters + 0);
+                     c->pb.lin_maxit, c->partials + spmv_partials_offset(c), c->counters + 0);
   HIPCHK(hipGetLastError());
   KState hs;
   const int chunk = 16;
+  double *partA = c->partials, *partB = c->partials + spmv_partials_offset(c);
   for (int base = 0;; base += chunk) {
     for (int k = 0; k < chunk; ++k) {
       if (c->method == PA_SOLVER_CG) {
         hipLaunchKernelGGL(k_cg_p, dim3(gv), dim3(256), 0, c->stream, N, Zr, Pr, (const KState *)c->kst, 0);
         if (mixed(c))
           hipLaunchKernelGGL(k_cg_spmv<float>, dim3(gn), dim3(256), 0, c->stream, c->m, (const float *)c->K, (const double *)Pr, Zl,
-                             c->kst, c->partials, c->counters + 1);
+                             (const KState *)c->kst, partA);
         else
           hipLaunchKernelGGL(k_cg_spmv<double>, dim3(gn), dim3(256), 0, c->stream, c->m, (const double *)c->K, (const double *)Pr, Zl,
-                             c->kst, c->partials, c->counters + 1);
+                             (const KState *)c->kst, partA);
         hipLaunchKernelGGL(k_cg_update, dim3(gr), dim3(256), 0, c->stream, N, X, (const double *)Pr, Rr, (const double *)Zl, Zr,
-                           (const double *)c->dinv, c->kst, c->partials, c->counters + 2);
+                           (const double *)c->dinv, c->kst, (const double *)partA, gn, partB, c->counters + 2);
       } else {
         hipLaunchKernelGGL(k_bicg_p, dim3(gv), dim3(256), 0, c->stream, N, (const double *)Zr, (const double *)Zl, Pr, Pl,
                            (const KState *)c->kst);
         if (mixed(c))
           hipLaunchKernelGGL(k_bicg_spmv<float>, dim3(gn), dim3(256), 0, c->stream, c->m, (const float *)c->K, (const double *)Pr,
-                             (const double *)Pl, Zr, Zl, c->kst, c->partials, c->counters + 1);
+                             (const double *)Pl, Zr, Zl, (const KState *)c->kst, partA);
         else
           hipLaunchKernelGGL(k_bicg_spmv<double>, dim3(gn), dim3(256), 0, c->stream, c->m, (const double *)c->K, (const double *)Pr,
-                             (const double *)Pl, Zr, Zl, c->kst, c->partials, c->counters + 1);
+                             (const double *)Pl, Zr, Zl, (const KState *)c->kst, partA);
         hipLaunchKernelGGL(k_bicg_update, dim3(gr), dim3(256), 0, c->stream, N, X, (const double *)Pr, Rr, Rl, Zr, Zl,
-                           (const double *)c->dinv, c->kst, c->partials, c->counters + 2);
+                           (const double *)c->dinv, c->kst, (const double *)partA, gn, partB, c->counters + 2);
       }
     }
     HIPCHK(hipGetLastError());
@@ -1084,7 +1133,7 @@ int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out) {
       (rc = dalloc(c, (double **)&c->K, 243 * nn)) || (rc = dalloc(c, (double **)&c->Dq, 81 * nq)) ||
       (rc = dalloc(c, &c->Pq, 9 * nq)) || (rc = dalloc(c, &c->dinv, N)) || (rc = dalloc(c, &c->con, N)) ||
       (rc = dalloc(c, &c->delu, N)) || (rc = dalloc(c, &c->d_egrain, m.nelem)) || (rc = dalloc(c, &c->kst, 1)) ||
-      (rc = dalloc(c, &c->partials, 16 * (size_t)nblk(std::max(N, nq), 64))) || (rc = dalloc(c, &c->counters, 16)) ||
+      (rc = dalloc(c, &c->partials, partials_size(c))) || (rc = dalloc(c, &c->counters, 16)) ||
       (rc = dalloc(c, &c->d_out, 64))) {
     pa_destroy(c);
     return rc;
@@ -1418,19 +1467,19 @@ int pa_time_kernel(pa_context *c, int which, int reps, double *avg_ms) {
     if (which == 0) {
       if (mixed(c))
         hipLaunchKernelGGL(k_bicg_spmv<float>, dim3(gn), dim3(256), 0, c->stream, c->m, (const float *)c->K, (const double *)c->kv[5],
-                           (const double *)c->kv[6], c->kv[3], c->kv[4], c->kst, c->partials, c->counters + 1);
+                           (const double *)c->kv[6], c->kv[3], c->kv[4], (const KState *)c->kst, c->partials);
       else
         hipLaunchKernelGGL(k_bicg_spmv<double>, dim3(gn), dim3(256), 0, c->stream, c->m, (const double *)c->K, (const double *)c->kv[5],
-                           (const double *)c->kv[6], c->kv[3], c->kv[4], c->kst, c->partials, c->counters + 1);
+                           (const double *)c->kv[6], c->kv[3], c->kv[4], (const KState *)c->kst, c->partials);
     } else if (which == 1) {
       rc = launch_assemble(c, 1);
     } else if (which == 2) {
       if (mixed(c))
         hipLaunchKernelGGL(k_cg_spmv<float>, dim3(gn), dim3(256), 0, c->stream, c->m, (const float *)c->K, (const double *)c->kv[5],
-                           c->kv[4], c->kst, c->partials, c->counters + 1);
+                           c->kv[4], (const KState *)c->kst, c->partials);
       else
         hipLaunchKernelGGL(k_cg_spmv<double>, dim3(gn), dim3(256), 0, c->stream, c->m, (const double *)c->K, (const double *)c->kv[5],
-                           c->kv[4], c->kst, c->partials, c->counters + 1);
+                           c->kv[4], (const KState *)c->kst, c->partials);
     } else if (which == 3) {
       rc = launch_qp_tangent(c);
     } else if (which == 4) {
