@@ -158,6 +158,10 @@ def run_form(args, form, pa, native, synth, torch):
     # Algorithmic bytes per launch: two reads of the nnz matrix values + Pr, Pl read and Zr, Zl
     # written once (8 B per dof each).
     vb = 4 if form == "mixed" else 8
+    # kernel timings at a plastic state: one more Newton iteration (the next increment's first),
+    # then time the kernels on that state (after a commit every qp sits on its yield surface)
+    newton_step()
+    newton_step()
     avg = ctypes.c_double()
     native.check(native.lib().pa_time_kernel(ctx._h, 0, 20, ctypes.byref(avg)))
     spmv_ms = avg.value

@@ -269,12 +269,15 @@ struct QpArgs {
   double *Fe_iter, *Fp_iter, *s_iter, *slip_iter;
   void *D;    // [81][nq] of DT
   double *P;  // [9][nq]
+  unsigned *queue, *qcount;  // deferred qps
 };
 
+// Lanes whose local problem meets a numerically singular A_PA (the pseudo-inverse branch of
+// inactive-slip removal) are deferred: pass 1 runs every qp on the LU path and appends the
+// singular ones to a device queue; pass 2 runs only the queued qps, so the slow branch never
+// holds up a wavefront of regular qps.  Results do not depend on which lane computes a qp.
 template <int NS, typename DT>
-__global__ void __launch_bounds__(256) k_qp_tangent(QpArgs a) {
-  const long long Q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (Q >= a.nq) return;
+__device__ __forceinline__ bool qp_tangent_one(const QpArgs &a, long long Q, bool allow_svd) {
   const long long e = Q >> 3;
   const int q = (int)(Q & 7);
   const int ex = (int)(e % a.m.nx), ey = (int)((e / a.m.nx) % a.m.ny), ez = (int)(e / ((long long)a.m.nx * a.m.ny));
@@ -298,7 +301,9 @@ __global__ void __launch_bounds__(256) k_qp_tangent(QpArgs a) {
   for (int i = 0; i < a.ns; ++i) s[i] = a.s_conv[i * a.nq + Q];
   const double *gt = a.gtab + (long long)a.egrain[e] * a.gstride;
   bool plastic = false;
-  cp::qp_update<NS, DT>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 1, P, T, Fe, Fp, (DT *)a.D + Q, a.nq, &plastic, a.JxW);
+  const int na = cp::qp_update<NS, DT>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 1, P, T, Fe, Fp, (DT *)a.D + Q, a.nq,
+                                       &plastic, a.JxW, allow_svd);
+  if (na < 0) return false;
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
     a.Fe_iter[i * a.nq + Q] = Fe[i];
@@ -309,7 +314,25 @@ __global__ void __launch_bounds__(256) k_qp_tangent(QpArgs a) {
   // slipfraction_iter is written only on the plastic path (calculatePlasticity.cc:454-456)
   if (plastic)
     for (int i = 0; i < a.ns; ++i) a.slip_iter[i * a.nq + Q] = a.slip_conv[i * a.nq + Q] + x[i];
+  return true;
 }
+
+// pass 1: lane = qp, LU path only; singular local problems are queued
+template <int NS, typename DT>
+__global__ void __launch_bounds__(256) k_qp_tangent(QpArgs a) {
+  const long long Q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (Q >= a.nq) return;
+  if (!qp_tangent_one<NS, DT>(a, Q, false)) a.queue[atomicAdd(a.qcount, 1u)] = (unsigned)Q;
+}
+// pass 2: the queued qps, grid-stride, pseudo-inverse allowed
+template <int NS, typename DT>
+__global__ void __launch_bounds__(64) k_qp_tangent_deferred(QpArgs a) {
+  const unsigned n = *a.qcount;
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    qp_tangent_one<NS, DT>(a, a.queue[i], true);
+}
+
+__global__ void k_zero_u32(unsigned *p) { *p = 0u; }
 
 // ---- assembly, stage 2: one colour of the element loop of assemble() (assemble.cc:64-90).
 // Lane = (element, vertex v1) forms 3 rows of K_e and R_e (getElementalValues.cc:144-162) from
@@ -458,6 +481,7 @@ struct PostArgs {
   double *partials;
   unsigned *counter;
   double *out;  // 13: E(6) T(6) slip
+  unsigned *queue, *qcount;  // deferred qps
 };
 
 // symmetric 3x3 eigen-decomposition by cyclic Jacobi rotations
@@ -501,69 +525,89 @@ __device__ void polar_R(const double *Fe, double *R) {
   cp::m3_mul(Fe, Ui, R);
 }
 
-// updateAfterIncrement's qp loop (updateAfterIncrement.cc:77-126) with calculatePlasticity(.,.,0),
-// fused with reorient() (reorient.cc:18-96) and the JxW-weighted sums.
+__device__ __forceinline__ void post_F(const PostArgs &a, long long Q, double F[9]) {
+  const long long e = Q >> 3;
+  const int q = (int)(Q & 7);
+  const int ex = (int)(e % a.m.nx), ey = (int)((e / a.m.nx) % a.m.ny), ez = (int)(e / ((long long)a.m.nx * a.m.ny));
+#pragma unroll
+  for (int i = 0; i < 9; ++i) F[i] = 0.0;
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    const long long nd = (ex + (v & 1)) + (long long)a.m.nnx * ((ey + ((v >> 1) & 1)) + (long long)a.m.nny * (ez + ((v >> 2) & 1)));
+    double g[3];
+    shape_grad(q, v, a.h, g);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double uc = a.u[3 * nd + c];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) F[3 * c + j] += uc * g[j];
+    }
+  }
+  F[0] += 1; F[4] += 1; F[8] += 1;
+}
+
+// updateAfterIncrement's qp loop (updateAfterIncrement.cc:77-126): calculatePlasticity(.,.,0) at
+// the converged displacement; singular local problems deferred as in assembly.
 template <int NS>
-__global__ void __launch_bounds__(256) k_post(PostArgs a) {
+__device__ __forceinline__ bool post_qp_one(const PostArgs &a, long long Q, bool allow_svd) {
+  double F[9];
+  post_F(a, Q, F);
+  double Fp_t[9], s[NS], x[NS], P[9], T[9], Fe[9], Fp[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) Fp_t[i] = a.Fp_conv[i * a.nq + Q];
+  for (int i = 0; i < a.ns; ++i) s[i] = a.s_conv[i * a.nq + Q];
+  const double *gt = a.gtab + (long long)a.egrain[Q >> 3] * a.gstride;
+  bool plastic = false;
+  const int na = cp::qp_update<NS, double>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 0, P, T, Fe, Fp, (double *)nullptr, 0,
+                                           &plastic, 1.0, allow_svd);
+  if (na < 0) return false;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) { a.Fe_iter[i * a.nq + Q] = Fe[i]; a.Fp_iter[i * a.nq + Q] = Fp[i]; a.T[i * a.nq + Q] = T[i]; }
+  for (int i = 0; i < a.ns; ++i) a.s_iter[i * a.nq + Q] = s[i];
+  if (plastic)
+    for (int i = 0; i < a.ns; ++i) a.slip_iter[i * a.nq + Q] = a.slip_conv[i * a.nq + Q] + x[i];
+  return true;
+}
+template <int NS>
+__global__ void __launch_bounds__(256) k_post_qp(PostArgs a) {
+  const long long Q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (Q >= a.nq) return;
+  if (!post_qp_one<NS>(a, Q, false)) a.queue[atomicAdd(a.qcount, 1u)] = (unsigned)Q;
+}
+template <int NS>
+__global__ void __launch_bounds__(64) k_post_qp_deferred(PostArgs a) {
+  const unsigned n = *a.qcount;
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) post_qp_one<NS>(a, a.queue[i], true);
+}
+
+// Green-Lagrange strain, JxW-weighted sums (updateAfterIncrement.cc:107-126, 218-245) and
+// reorient() (reorient.cc:18-96) with Fe_old = Fe_conv, Fe_new = Fe_iter; deterministic sums.
+__global__ void __launch_bounds__(256) k_post_reduce(PostArgs a) {
   __shared__ double lds[13 * 16];
   const long long Q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   double acc[13];
 #pragma unroll
   for (int i = 0; i < 13; ++i) acc[i] = 0.0;
   if (Q < a.nq) {
-    const long long e = Q >> 3;
-    const int q = (int)(Q & 7);
-    const int ex = (int)(e % a.m.nx), ey = (int)((e / a.m.nx) % a.m.ny), ez = (int)(e / ((long long)a.m.nx * a.m.ny));
-    double F[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int v = 0; v < 8; ++v) {
-      const long long nd = (ex + (v & 1)) + (long long)a.m.nnx * ((ey + ((v >> 1) & 1)) + (long long)a.m.nny * (ez + ((v >> 2) & 1)));
-      double g[3];
-      shape_grad(q, v, a.h, g);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const double uc = a.u[3 * nd + c];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) F[3 * c + j] += uc * g[j];
-      }
-    }
-    F[0] += 1; F[4] += 1; F[8] += 1;
-    double Fp_t[9], s[NS], x[NS], P[9], T[9], Fe[9], Fp[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) Fp_t[i] = a.Fp_conv[i * a.nq + Q];
-    for (int i = 0; i < a.ns; ++i) s[i] = a.s_conv[i * a.nq + Q];
-    const double *gt = a.gtab + (long long)a.egrain[e] * a.gstride;
-    bool plastic = false;
-    cp::qp_update<NS, double>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 0, P, T, Fe, Fp, (double *)nullptr, 0, &plastic);
-#pragma unroll
-    for (int i = 0; i < 9; ++i) { a.Fe_iter[i * a.nq + Q] = Fe[i]; a.Fp_iter[i * a.nq + Q] = Fp[i]; a.T[i * a.nq + Q] = T[i]; }
-    for (int i = 0; i < a.ns; ++i) a.s_iter[i * a.nq + Q] = s[i];
-    double slip_sum = 0;
-    if (plastic) {
-      for (int i = 0; i < a.ns; ++i) {
-        const double v = a.slip_conv[i * a.nq + Q] + x[i];
-        a.slip_iter[i * a.nq + Q] = v;
-        slip_sum += v * a.JxW;
-      }
-    } else {
-      for (int i = 0; i < a.ns; ++i) slip_sum += a.slip_iter[i * a.nq + Q] * a.JxW;
-    }
-    // Green-Lagrange strain and JxW-weighted sums (:114-126, :218-221)
-    double C[9];
+    double F[9], C[9], T[9];
+    post_F(a, Q, F);
     cp::m3_tmul(F, F, C);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) T[i] = a.T[i * a.nq + Q];
     const int vi[6] = {0, 4, 8, 5, 2, 1};
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
       acc[c] = 0.5 * (C[vi[c]] - ((vi[c] % 4 == 0) ? 1.0 : 0.0)) * a.JxW;
       acc[6 + c] = T[vi[c]] * a.JxW;
     }
+    double slip_sum = 0;
+    for (int i = 0; i < a.ns; ++i) slip_sum += a.slip_iter[i * a.nq + Q] * a.JxW;
     acc[12] = slip_sum;
-    // reorient (reorient.cc:24-93) with Fe_old = Fe_conv, Fe_new = Fe_iter
-    double Fo[9], Ro[9], Rn[9], Om[9], Om2[9];
+    double Fo[9], Fn[9], Ro[9], Rn[9], Om[9], Om2[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) Fo[i] = a.Fe_conv[i * a.nq + Q];
+    for (int i = 0; i < 9; ++i) { Fo[i] = a.Fe_conv[i * a.nq + Q]; Fn[i] = a.Fe_iter[i * a.nq + Q]; }
     polar_R(Fo, Ro);
-    polar_R(Fe, Rn);
+    polar_R(Fn, Rn);
 #pragma unroll
     for (int i = 0; i < 9; ++i) Om[i] = Rn[i] - Ro[i];
     cp::m3_mult(Om, Rn, Om2);
@@ -887,6 +931,7 @@ struct pa_context {
   void *K = nullptr;   // node-block Jacobian, double (FP64 form) or float (mixed form)
   void *Dq = nullptr;  // [81][nq] JxW dP/dF of the last assembly, double or float
   double *Pq = nullptr;  // [9][nq] JxW P
+  unsigned *queue = nullptr;  // deferred qps of the pseudo-inverse pass
   unsigned char *con = nullptr;
   double *delu = nullptr;
   double *kv[8] = {nullptr};  // Krylov vectors: X Rr Rl Zr Zl Pr Pl spare
@@ -914,6 +959,15 @@ int ns_template(int ns) { return ns <= 12 ? 12 : (ns <= 24 ? 24 : 48); }
 
 bool mixed(const pa_context *c) { return c->precision == PA_PREC_MIXED; }
 
+constexpr unsigned DEFER_GRID = 1024;  // 64-lane blocks of the deferred pass
+
+template <int NS, typename DT>
+void qp_tangent_passes(pa_context *c, const QpArgs &a) {
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, c->stream, a.qcount);
+  hipLaunchKernelGGL((k_qp_tangent<NS, DT>), dim3(nblk(c->nq, 256)), dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL((k_qp_tangent_deferred<NS, DT>), dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
+}
+
 int launch_qp_tangent(pa_context *c) {
   QpArgs a;
   a.m = c->m;
@@ -924,19 +978,12 @@ int launch_qp_tangent(pa_context *c) {
   a.Fp_conv = c->Fp_conv; a.s_conv = c->s_conv; a.slip_conv = c->slip_conv;
   a.Fe_iter = c->Fe_iter; a.Fp_iter = c->Fp_iter; a.s_iter = c->s_iter; a.slip_iter = c->slip_iter;
   a.D = c->Dq; a.P = c->Pq;
-  const unsigned grid = nblk(c->nq, 256);
+  a.queue = c->queue; a.qcount = c->counters + 8;
   const int NS = ns_template(c->ns);
   const bool f = mixed(c);
-  if (NS == 12) {
-    if (f) hipLaunchKernelGGL((k_qp_tangent<12, float>), dim3(grid), dim3(256), 0, c->stream, a);
-    else hipLaunchKernelGGL((k_qp_tangent<12, double>), dim3(grid), dim3(256), 0, c->stream, a);
-  } else if (NS == 24) {
-    if (f) hipLaunchKernelGGL((k_qp_tangent<24, float>), dim3(grid), dim3(256), 0, c->stream, a);
-    else hipLaunchKernelGGL((k_qp_tangent<24, double>), dim3(grid), dim3(256), 0, c->stream, a);
-  } else {
-    if (f) hipLaunchKernelGGL((k_qp_tangent<48, float>), dim3(grid), dim3(256), 0, c->stream, a);
-    else hipLaunchKernelGGL((k_qp_tangent<48, double>), dim3(grid), dim3(256), 0, c->stream, a);
-  }
+  if (NS == 12) { if (f) qp_tangent_passes<12, float>(c, a); else qp_tangent_passes<12, double>(c, a); }
+  else if (NS == 24) { if (f) qp_tangent_passes<24, float>(c, a); else qp_tangent_passes<24, double>(c, a); }
+  else { if (f) qp_tangent_passes<48, float>(c, a); else qp_tangent_passes<48, double>(c, a); }
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1131,7 +1178,7 @@ int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out) {
       (rc = dalloc(c, &c->slip_conv, ns * nq)) || (rc = dalloc(c, &c->slip_iter, ns * nq)) || (rc = dalloc(c, &c->rotnew, 3 * nq)) ||
       (rc = dalloc(c, &c->T, 9 * nq)) || (rc = dalloc(c, &c->u, N)) || (rc = dalloc(c, &c->R, N)) ||
       (rc = dalloc(c, (double **)&c->K, 243 * nn)) || (rc = dalloc(c, (double **)&c->Dq, 81 * nq)) ||
-      (rc = dalloc(c, &c->Pq, 9 * nq)) || (rc = dalloc(c, &c->dinv, N)) || (rc = dalloc(c, &c->con, N)) ||
+      (rc = dalloc(c, &c->Pq, 9 * nq)) || (rc = dalloc(c, &c->queue, nq)) || (rc = dalloc(c, &c->dinv, N)) || (rc = dalloc(c, &c->con, N)) ||
       (rc = dalloc(c, &c->delu, N)) || (rc = dalloc(c, &c->d_egrain, m.nelem)) || (rc = dalloc(c, &c->kst, 1)) ||
       (rc = dalloc(c, &c->partials, partials_size(c))) || (rc = dalloc(c, &c->counters, 16)) ||
       (rc = dalloc(c, &c->d_out, 64))) {
@@ -1292,12 +1339,22 @@ int pa_commit_increment(pa_context *c, double row[15]) {
   a.Fp_conv = c->Fp_conv; a.s_conv = c->s_conv; a.slip_conv = c->slip_conv; a.Fe_conv = c->Fe_conv;
   a.Fe_iter = c->Fe_iter; a.Fp_iter = c->Fp_iter; a.s_iter = c->s_iter; a.slip_iter = c->slip_iter; a.T = c->T; a.rotnew = c->rotnew;
   a.partials = c->partials; a.counter = c->counters + 4; a.out = c->d_out;
+  a.queue = c->queue; a.qcount = c->counters + 9;
   HIPCHK(hipEventRecord(c->ev[0], c->stream));
   const int NS = ns_template(c->ns);
   const unsigned grid = nblk(c->nq, 256);
-  if (NS == 12) hipLaunchKernelGGL(k_post<12>, dim3(grid), dim3(256), 0, c->stream, a);
-  else if (NS == 24) hipLaunchKernelGGL(k_post<24>, dim3(grid), dim3(256), 0, c->stream, a);
-  else hipLaunchKernelGGL(k_post<48>, dim3(grid), dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, c->stream, a.qcount);
+  if (NS == 12) {
+    hipLaunchKernelGGL(k_post_qp<12>, dim3(grid), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_post_qp_deferred<12>, dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
+  } else if (NS == 24) {
+    hipLaunchKernelGGL(k_post_qp<24>, dim3(grid), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_post_qp_deferred<24>, dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
+  } else {
+    hipLaunchKernelGGL(k_post_qp<48>, dim3(grid), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_post_qp_deferred<48>, dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
+  }
+  hipLaunchKernelGGL(k_post_reduce, dim3(grid), dim3(256), 0, c->stream, a);
   HIPCHK(hipGetLastError());
   // _conv <- _iter (updateAfterIncrement.cc:298-305); _iter keeps its values
   const size_t b9 = 9 * c->nq * sizeof(double), bs = (size_t)c->ns * c->nq * sizeof(double);

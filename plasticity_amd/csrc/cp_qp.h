@@ -229,7 +229,8 @@ CP_DEV void dexp_series(const double *X, const double *S, int count3, bool neg, 
   }
 }
 
-// Return: number of systems left active after inactive-slip removal (0 = elastic).
+// Return: number of systems left active after inactive-slip removal (0 = elastic), or -1 when
+// allow_svd is false and a numerically singular A_PA was met (nothing was written then).
 // s holds s_conv on entry and s_tau on exit; x receives x_beta_old (the slip increments).
 // *plastic_path (optional) is set when the potentially-active set was non-empty, the case in
 // which the reference writes slipfraction_iter (calculatePlasticity.cc:454-456).
@@ -239,7 +240,7 @@ template <int NS, typename DT = double>
 CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ gt, int ns, int max_solver_iter,
                      const double *F, const double *Fp_t, double *s, double *x, int flag, double *P, double *T,
                      double *Fe_out, double *Fp_out, DT *D, long long dstride, bool *plastic_path = nullptr,
-                     double dscale = 1.0) {
+                     double dscale = 1.0, bool allow_svd = true) {
   typedef MatLayout<NS> ML;
   typedef GrainLayout<NS> GL;
   const double tol1 = mat[ML::tol];
@@ -325,6 +326,7 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
         double xs[NS];
         for (int i = 0; i < npos; ++i) xs[i] = bp[i];
         if (npos > 0 && !lu_solve<NS>(npos, Ap, xs, 1)) {
+          if (!allow_svd) return -1;  // caller defers this qp to the pseudo-inverse pass
           for (int i = 0; i < npos; ++i)
             for (int j = 0; j < npos; ++j) Ap[i * NS + j] = A[pos[i] * NS + pos[j]];
           svd_pinv_solve<NS>(npos, Ap, bp, xs);
