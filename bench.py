@@ -59,7 +59,7 @@ def main():
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--grains", type=int, default=256)
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
-    ap.add_argument("--solver", choices=["bicg", "cg"], default="bicg")
+    ap.add_argument("--solver", choices=["bicg", "cg", "both"], default="both")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -74,10 +74,14 @@ def main():
     assert torch.cuda.is_available(), "no GPU"
     n = args.n
     forms = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    solvers = ["bicg", "cg"] if args.solver == "both" else [args.solver]
     results = {}
-    for form in forms:
-        results[form] = run_form(args, form, pa, native, synth, torch)
-    head = results[forms[-1]]
+    for solver in solvers:
+        for form in forms:
+            results[(form, solver)] = run_form(args, form, solver, pa, native, synth, torch)
+    # headline: the last form with BASELINE's named solver (Jacobi-PCG) when it was run
+    hs = "cg" if "cg" in solvers else solvers[-1]
+    head = results[(forms[-1], hs)]
     nq = head["nq"]
     out = {
         "metric": METRIC,
@@ -93,38 +97,39 @@ def main():
         "dtype": "f64" if forms[-1] == "fp64" else "f64 (return map, residual, Krylov) + f32 (tangents, matrix)",
         "data": "synthetic (seeded Voronoi grains, random orientations)",
         "config": {"workload": "FCC uniaxial tension, %d^3 Q1 hex (%d qps, %d dofs), %d grains, 5e-4 strain/step, "
-                               "4 Newton iterations/step, Jacobi-%s 1e-10" % (n, nq, head["ndof"], args.grains, args.solver.upper()),
-                   "form": forms[-1], "mesh": [n, n, n], "nnz": head["nnz"], "parallelism": "single GPU"},
+                               "4 Newton iterations/step, Jacobi-%s 1e-10" % (n, nq, head["ndof"], args.grains, "PCG" if hs == "cg" else "BiCG"),
+                   "form": forms[-1], "solver": hs, "mesh": [n, n, n], "nnz": head["nnz"], "parallelism": "single GPU",
+                   "note": "Jacobi-PCG is the solver the workload names; the reference code uses Jacobi-BiCG (both timed)"},
         "roofline": head["roofline"],
     }
-    for form in forms:
-        r = results[form]
-        out["config"][form] = {k: r[k] for k in ("ms_per_step", "qp_updates_per_s", "krylov_iters_per_step", "assembly_ms_per_step",
-                                                 "krylov_ms_per_step", "post_ms_per_increment", "qp_tangent_ms", "element_ms",
-                                                 "spmv_pair_us")}
+    keys = ("ms_per_step", "qp_updates_per_s", "krylov_iters_per_step", "assembly_ms_per_step", "krylov_ms_per_step",
+            "post_ms_per_increment", "qp_tangent_ms", "element_ms", "operator_us")
+    for (form, solver), r in results.items():
+        out["config"]["%s_%s" % (form, solver)] = {k: r[k] for k in keys}
     if len(forms) == 2:
-        a, b = results["fp64"], results["mixed"]
-        out["config"]["speedup_mixed_vs_fp64"] = a["ms_per_step"] / b["ms_per_step"]
-        rows_a, rows_b = np.array(a["rows"]), np.array(b["rows"])
-        err = {}
-        for name, cols in (("avg_stress", slice(6, 12)), ("avg_strain", slice(0, 6)), ("slip_total", slice(14, 15))):
-            err[name] = float(np.abs(rows_b[:, cols] - rows_a[:, cols]).max() / np.abs(rows_a[:, cols]).max())
-        err["qp_cauchy_stress"] = float(np.abs(b["T"] - a["T"]).max() / np.abs(a["T"]).max())
-        err["crss"] = float(np.abs(b["s"] - a["s"]).max() / np.abs(a["s"]).max())
-        err["displacement"] = float(np.abs(b["u"] - a["u"]).max() / np.abs(a["u"]).max())
-        out["config"]["mixed_error_vs_fp64_gpu"] = err
+        for solver in solvers:
+            a, b = results[("fp64", solver)], results[("mixed", solver)]
+            out["config"]["speedup_mixed_vs_fp64_%s" % solver] = a["ms_per_step"] / b["ms_per_step"]
+            rows_a, rows_b = np.array(a["rows"]), np.array(b["rows"])
+            err = {}
+            for name, cols in (("avg_stress", slice(6, 12)), ("avg_strain", slice(0, 6)), ("slip_total", slice(14, 15))):
+                err[name] = float(np.abs(rows_b[:, cols] - rows_a[:, cols]).max() / np.abs(rows_a[:, cols]).max())
+            err["qp_cauchy_stress"] = float(np.abs(b["T"] - a["T"]).max() / np.abs(a["T"]).max())
+            err["crss"] = float(np.abs(b["s"] - a["s"]).max() / np.abs(a["s"]).max())
+            err["displacement"] = float(np.abs(b["u"] - a["u"]).max() / np.abs(a["u"]).max())
+            out["config"]["mixed_vs_fp64_gpu_diff_%s" % solver] = err
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, quick=(n < 32))
     print(json.dumps(out))
 
 
-def run_form(args, form, pa, native, synth, torch):
+def run_form(args, form, solver, pa, native, synth, torch):
     n = args.n
     mat, pb, vox, rod, bcs, _ = synth.fcc_workload(n, args.grains, 20, 5e-4, seed=2024)
     ctx = pa.Context(mat, pb)
     eg = ctx.element_grains(vox)
     ctx.set_grains(eg, rod)
-    ctx.set_solver(native.PA_SOLVER_CG if args.solver == "cg" else native.PA_SOLVER_BICG,
+    ctx.set_solver(native.PA_SOLVER_CG if solver == "cg" else native.PA_SOLVER_BICG,
                    native.PA_PREC_MIXED if form == "mixed" else native.PA_PREC_FP64)
     nmax = pb.max_newton
     state = {"step": 0}
@@ -141,8 +146,8 @@ def run_form(args, form, pa, native, synth, torch):
             rec["post_ms"] = ctx.last_timings()[2]
         state["step"] += 1
         log.append(rec)
-        print("[%s] step %d (increment %d, iteration %d): |R| %.3e, %d Krylov iterations, assembly %.1f ms, Krylov %.1f ms"
-              % (form, s, inc, it, rn, its, asm_ms, kry_ms), file=sys.stderr, flush=True)
+        print("[%s %s] step %d (increment %d, iteration %d): |R| %.3e, %d Krylov iterations, assembly %.1f ms, Krylov %.1f ms"
+              % (form, solver, s, inc, it, rn, its, asm_ms, kry_ms), file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
         newton_step()
@@ -154,18 +159,18 @@ def run_form(args, form, pa, native, synth, torch):
     dt = time.perf_counter() - t0
     timed = log[args.warmup:]
     ms_per_step = 1000.0 * dt / args.steps
-    # roofline of the dominant kernel: the BiCG operator pair (K p and K^T p~), HBM-bound.
-    # Algorithmic bytes per launch: two reads of the nnz matrix values + Pr, Pl read and Zr, Zl
-    # written once (8 B per dof each).
     vb = 4 if form == "mixed" else 8
     # kernel timings at a plastic state: one more Newton iteration (the next increment's first),
     # then time the kernels on that state (after a commit every qp sits on its yield surface)
     newton_step()
     newton_step()
     avg = ctypes.c_double()
-    native.check(native.lib().pa_time_kernel(ctx._h, 0, 20, ctypes.byref(avg)))
+    # the Krylov operator kernel: BiCG's pair (K p and K^T p~: two reads of the matrix values) or
+    # CG's single product; algorithmic bytes = matrix reads + FP64 vectors read and written once
+    which, nmat, nvec = (0, 2, 4) if solver == "bicg" else (2, 1, 2)
+    native.check(native.lib().pa_time_kernel(ctx._h, which, 20, ctypes.byref(avg)))
     spmv_ms = avg.value
-    bytes_spmv = 2 * ctx.nnz * vb + 4 * ctx.ndof * 8
+    bytes_spmv = nmat * ctx.nnz * vb + nvec * ctx.ndof * 8
     achieved = bytes_spmv / (spmv_ms * 1e-3) / 1e9
     native.check(native.lib().pa_time_kernel(ctx._h, 3, 1, ctypes.byref(avg)))
     qp_ms = avg.value
@@ -179,9 +184,11 @@ def run_form(args, form, pa, native, synth, torch):
         "assembly_ms_per_step": float(np.mean([r["asm_ms"] for r in timed])),
         "krylov_ms_per_step": float(np.mean([r["krylov_ms"] for r in timed])),
         "post_ms_per_increment": float(np.mean(posts)) if posts else None,
-        "qp_tangent_ms": qp_ms, "element_ms": el_ms, "spmv_pair_us": spmv_ms * 1e3,
+        "qp_tangent_ms": qp_ms, "element_ms": el_ms, "operator_us": spmv_ms * 1e3,
         "rows": [r["row"] for r in log if "row" in r],
-        "roofline": {"kernel": "k_bicg_spmv<%s> (K p and K^T p~, 27-point node-block)" % ("float" if form == "mixed" else "double"),
+        "roofline": {"kernel": "%s<%s> (27-point node-block %s)" % ("k_bicg_spmv" if solver == "bicg" else "k_cg_spmv",
+                                                                      "float" if form == "mixed" else "double",
+                                                                      "K p and K^T p~" if solver == "bicg" else "K p"),
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": spmv_ms,
                      "algorithmic_bytes": bytes_spmv},

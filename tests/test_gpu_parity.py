@@ -182,3 +182,28 @@ def test_mixed_precision_form_error_vs_oracle():
     for k, v in errs.items():
         assert v < 1e-5, (k, v)
     ctx.close()
+
+
+def test_jacobi_pcg_option_matches_oracle():
+    """The Jacobi-PCG solver option (the solver the benchmark workloads name) against the oracle's
+    Jacobi-PCG, solves driven to 1e-14: same curve and Krylov counts for the O(1) right-hand sides."""
+    case = refcase.load_reference_case("fcc_simpleTension")
+    case.lin_tol = 1e-14
+    n = 4
+    ref = refcase.oracle_simulate(case, n, nthreads=8, want_state=True, solver=1)
+    ctx = _ctx(case)
+    ctx.set_grains(case.elem_grain, case.grain_rod)
+    ctx.set_solver(pa.native.PA_SOLVER_CG, pa.native.PA_PREC_FP64)
+    rows, its = [], []
+    for inc in range(n):
+        r, it, _ = ctx.run_increment(inc)
+        rows.append(r)
+        its.append(it)
+    rows, its = np.array(rows), np.array(its).ravel()
+    print("cg iters gpu", its, "oracle", ref["lin_iters"])
+    first = np.arange(len(its)) % 4 < 2
+    assert np.abs(its - ref["lin_iters"])[first].max() <= 1
+    for cols in (slice(0, 6), slice(6, 12)):
+        assert np.abs(rows[:, cols] - ref["rows"][:, cols]).max() / np.abs(ref["rows"][:, cols]).max() < 1e-9
+    assert _rel(ctx.get_state("T"), ref["state"][:, 0:9]) < 1e-9
+    ctx.close()
