@@ -25,6 +25,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "qp updates/s + Newton-step time, 64^3 FCC: mixed-precision vs plain FP64 GPU"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the Jacobi-PCG operator kernel k_cg_spmv at the 64^3 workload, from two
+# rocprofv3 PMC passes (FETCH_SIZE x 2 + WRITE_SIZE x 1 KiB: the factors that make both counters
+# read the known byte count of the history-commit copies); filled in from the committed profile
+PMC_TRAFFIC_KIB = {}
 
 
 def cpu_baseline(args, quick):
@@ -103,7 +107,7 @@ def main():
         "roofline": head["roofline"],
     }
     keys = ("ms_per_step", "qp_updates_per_s", "krylov_iters_per_step", "assembly_ms_per_step", "krylov_ms_per_step",
-            "post_ms_per_increment", "qp_tangent_ms", "element_ms", "operator_us")
+            "post_ms_per_increment", "qp_tangent_ms", "element_ms", "operator_us", "update_us", "update_GBps", "p_us")
     for (form, solver), r in results.items():
         out["config"]["%s_%s" % (form, solver)] = {k: r[k] for k in keys}
     if len(forms) == 2:
@@ -165,18 +169,27 @@ def run_form(args, form, solver, pa, native, synth, torch):
     newton_step()
     newton_step()
     avg = ctypes.c_double()
-    # the Krylov operator kernel: BiCG's pair (K p and K^T p~: two reads of the matrix values) or
-    # CG's single product; algorithmic bytes = matrix reads + FP64 vectors read and written once
+    # the Krylov operator kernel: BiCG's pair (K p and K^T p~: two reads of the matrix values,
+    # p, p~ read, z, z~ written) or CG's product (p read, q written); algorithmic bytes = matrix
+    # values read once per product + FP64 vectors read or written once
     which, nmat, nvec = (0, 2, 4) if solver == "bicg" else (2, 1, 2)
     native.check(native.lib().pa_time_kernel(ctx._h, which, 20, ctypes.byref(avg)))
     spmv_ms = avg.value
     bytes_spmv = nmat * ctx.nnz * vb + nvec * ctx.ndof * 8
     achieved = bytes_spmv / (spmv_ms * 1e-3) / 1e9
+    upd_us = p_us = None
+    if solver == "cg":  # update kernel: x, p, r, q, D^-1 read; x, r, z written.  p kernel: z, p read, p written
+        native.check(native.lib().pa_time_kernel(ctx._h, 5, 20, ctypes.byref(avg)))
+        upd_us = avg.value * 1e3
+        native.check(native.lib().pa_time_kernel(ctx._h, 6, 20, ctypes.byref(avg)))
+        p_us = avg.value * 1e3
     native.check(native.lib().pa_time_kernel(ctx._h, 3, 1, ctypes.byref(avg)))
     qp_ms = avg.value
     native.check(native.lib().pa_time_kernel(ctx._h, 4, 1, ctypes.byref(avg)))
     el_ms = avg.value
     posts = [r["post_ms"] for r in timed if "post_ms" in r]
+    kib = PMC_TRAFFIC_KIB.get((solver, form, n)) if args.grains == 256 else None
+    traffic = kib * 1024 if kib else None
     res = {
         "nq": ctx.nqp, "ndof": ctx.ndof, "nnz": ctx.nnz,
         "ms_per_step": ms_per_step, "qp_updates_per_s": ctx.nqp / (ms_per_step * 1e-3),
@@ -184,14 +197,17 @@ def run_form(args, form, solver, pa, native, synth, torch):
         "assembly_ms_per_step": float(np.mean([r["asm_ms"] for r in timed])),
         "krylov_ms_per_step": float(np.mean([r["krylov_ms"] for r in timed])),
         "post_ms_per_increment": float(np.mean(posts)) if posts else None,
-        "qp_tangent_ms": qp_ms, "element_ms": el_ms, "operator_us": spmv_ms * 1e3,
+        "qp_tangent_ms": qp_ms, "element_ms": el_ms, "operator_us": spmv_ms * 1e3, "update_us": upd_us, "p_us": p_us,
+        "update_GBps": (8 * ctx.ndof * 8 / (upd_us * 1e-6) / 1e9) if upd_us else None,
         "rows": [r["row"] for r in log if "row" in r],
         "roofline": {"kernel": "%s<%s> (27-point node-block %s)" % ("k_bicg_spmv" if solver == "bicg" else "k_cg_spmv",
                                                                       "float" if form == "mixed" else "double",
                                                                       "K p and K^T p~" if solver == "bicg" else "K p"),
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": spmv_ms,
-                     "algorithmic_bytes": bytes_spmv},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_launch_ms": spmv_ms,
+                     "algorithmic_bytes": bytes_spmv,
+                     "traffic_source": "rocprofv3 FETCH_SIZE x2 + WRITE_SIZE (profiles/k_cg_spmv_mixed_64cube_fcc.md)"
+                                       if traffic else "not measured for this kernel/size"},
     }
     res["T"] = ctx.get_state("T")
     res["s"] = ctx.get_state("s")

@@ -147,8 +147,10 @@ int pa_last_timings(pa_context *ctx, double ms[3]);
  *   2 = Jacobi-PCG operator  Q = K P
  *   3 = the qp-tangent launch alone (qp update + dP/dF of every qp)
  *   4 = the 8 element colour launches alone (K_e, R_e, condensed scatter)
+ *   5 = Jacobi-PCG update  x += a p, r -= a q, z = D^-1 r  (with the z.z, r.z partial sums)
+ *   6 = Jacobi-PCG direction  p = z + b p
  * avg_ms receives the mean time of one launch (of one sweep for which = 1 and 4).  The matrix
- * is not re-zeroed between repetitions (timing only). */
+ * is not re-zeroed between repetitions, and the Krylov vectors are overwritten (timing only). */
 int pa_time_kernel(pa_context *ctx, int which, int reps, double *avg_ms);
 
 #ifdef __cplusplus

@@ -190,13 +190,29 @@ void jacobi_svd(int n, const double *A, std::vector<double> &U, std::vector<doub
   }
 }
 
-// x = pinv(A) b with singular values <= rel*sigma_max dropped (rel = 0 keeps every nonzero one)
+// x = pinv(A) b over the nonzero singular values (rel = 0: compute_inverse_svd(0.0)).  With rel > 0,
+// singular values <= rel*sigma_max are "small".  A single small singular value is inverted, as the
+// reference does, when the result is determined by the data: the step along v_k is
+// (u_k.b)/sigma_k, and the round-off it carries into the other components is about
+// |u_k.b| eps sigma_max / (sigma_k gap), gap = distance to the next singular value; the pair is
+// kept when that is under 1e-2 |b| / sigma_max (symmetric multi-slip: sigma_k ~ 1e-9..1e-16,
+// margin >= 100x).  Otherwise (two or more small singular values, or a round-off one carrying a
+// large u_k.b) the small ones are dropped: the reference's result there depends on round-off.
 void svd_pinv_solve(int n, const double *A, const double *b, double *x, double rel) {
   std::vector<double> U, s, V;
   jacobi_svd(n, A, U, s, V);
   double smax = 0;
   for (int i = 0; i < n; ++i) smax = std::max(smax, s[i]);
-  const double lim = smax * rel;
+  double lim = smax * rel;
+  int nsmall = 0, ks = -1;
+  for (int i = 0; i < n; ++i) if (s[i] <= lim) { nsmall++; ks = i; }
+  if (nsmall == 1 && s[ks] > 0) {
+    double s2 = smax, ub = 0, bb = 0;
+    for (int i = 0; i < n; ++i) if (i != ks) s2 = std::min(s2, s[i]);
+    for (int i = 0; i < n; ++i) { ub += U[i * n + ks] * b[i]; bb += b[i] * b[i]; }
+    const double eps = 2.220446049250313e-16;
+    if (std::fabs(ub) * eps * smax * smax < 1e-2 * std::sqrt(bb) * s[ks] * (s2 - s[ks])) lim = 0.0;
+  }
   for (int i = 0; i < n; ++i) x[i] = 0;
   for (int k = 0; k < n; ++k) {
     if (!(s[k] > lim)) continue;
@@ -207,9 +223,10 @@ void svd_pinv_solve(int n, const double *A, const double *b, double *x, double r
   }
 }
 
-// Relative level below which a pivot (LU) or singular value (SVD fallback) counts as zero.
-// A structurally rank-deficient A_PA shows pivots ~1e-13 max|A|, well under it.
-const double kPinvRtol = 1e-10;
+// Relative level below which a pivot (LU) or singular value (SVD fallback) counts as small.
+// Symmetric multi-slip leaves one singular value at 1e-9..1e-13 sigma_max (inverted, as the
+// reference does); rank-deficient active sets show a null space of dimension >= 2 at round-off.
+double kPinvRtol = 1e-10;
 
 // LU with partial pivoting, multiple right-hand sides (row-major B, n x m, overwritten by X).
 // Returns false if a pivot is <= kPinvRtol * max|A|.
@@ -246,15 +263,43 @@ bool lu_solve(int n, std::vector<double> A, double *B, int m) {
 }
 
 long long g_pinv_calls = 0, g_pinv_singular = 0;  // diagnostics (single-threaded runs only)
+int g_sample_n = 0;                                 // first matrix that took the fallback
+double g_sample_A[OR_MAXT * OR_MAXT], g_sample_b[OR_MAXT];
+int g_diag = 0;                                     // histogram of sigma_min / sigma_max (decades)
+long long g_hist[24];
+std::vector<double> g_records;                      // n, s_min/s_max, s_2nd/s_max, |u_min.b|/|b|
 
 void pinv_solve(int n, const double *A, const double *b, double *x, int mode) {
   if (n == 0) return;
   g_pinv_calls++;
+  if (g_diag) {
+    std::vector<double> U, sg, V;
+    jacobi_svd(n, A, U, sg, V);
+    int kmin = 0, k2 = -1;
+    double hi = 0;
+    for (int k = 0; k < n; ++k) { hi = std::max(hi, sg[k]); if (sg[k] < sg[kmin]) kmin = k; }
+    for (int k = 0; k < n; ++k) if (k != kmin && (k2 < 0 || sg[k] < sg[k2])) k2 = k;
+    const double r = hi > 0 ? sg[kmin] / hi : 0.0;
+    int k = r > 0 ? (int)std::floor(-std::log10(r)) : 23;
+    g_hist[std::min(std::max(k, 0), 23)]++;
+    if (r < 1e-8 && g_records.size() < 4 * 200000) {
+      double ub = 0, bb = 0;
+      for (int i = 0; i < n; ++i) { ub += U[i * n + kmin] * b[i]; bb += b[i] * b[i]; }
+      g_records.push_back(n);
+      g_records.push_back(r);
+      g_records.push_back(k2 >= 0 && hi > 0 ? sg[k2] / hi : 1.0);
+      g_records.push_back(bb > 0 ? std::fabs(ub) / std::sqrt(bb) : 0.0);
+    }
+  }
   if (mode == 1) {
     std::vector<double> Av(A, A + n * n);
     for (int i = 0; i < n; ++i) x[i] = b[i];
     if (lu_solve(n, Av, x, 1)) return;
-    g_pinv_singular++;
+    if (g_pinv_singular++ == 0) {
+      g_sample_n = n;
+      for (int i = 0; i < n * n; ++i) g_sample_A[i] = A[i];
+      for (int i = 0; i < n; ++i) g_sample_b[i] = b[i];
+    }
     svd_pinv_solve(n, A, b, x, kPinvRtol);
     return;
   }
@@ -904,6 +949,26 @@ int or_qp_update(const or_material *mat, const double rot[3], const double F[9],
 void or_pinv_stats(long long *calls, long long *singular) {
   *calls = g_pinv_calls; *singular = g_pinv_singular;
   g_pinv_calls = g_pinv_singular = 0;
+}
+
+void or_set_pinv_rtol(double rtol) { kPinvRtol = rtol; }
+
+int or_pinv_records(double *out, int max_records) {
+  const int n = std::min((int)(g_records.size() / 4), max_records);
+  for (int i = 0; i < 4 * n; ++i) out[i] = g_records[i];
+  g_records.clear();
+  return n;
+}
+
+void or_pinv_histogram(int enable, long long hist[24]) {
+  if (hist) for (int k = 0; k < 24; ++k) { hist[k] = g_hist[k]; g_hist[k] = 0; }
+  g_diag = enable;
+}
+
+int or_pinv_sample(double *A, double *b) {
+  for (int i = 0; i < g_sample_n * g_sample_n; ++i) A[i] = g_sample_A[i];
+  for (int i = 0; i < g_sample_n; ++i) b[i] = g_sample_b[i];
+  return g_sample_n;
 }
 
 void or_odfpoint(const double r[3], double R[9]) { store3(odfpoint(r), R); }

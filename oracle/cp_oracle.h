@@ -23,8 +23,10 @@ extern "C" {
  *       LAPACKFullMatrix::compute_inverse_svd(0.0) semantics,
  *       src/materialModels/crystalPlasticity/inactiveSlipRemoval.cc:26,99)
  *   1 = LU with partial pivoting; when a pivot is <= 1e-10 max|A| falls back to
- *       the SVD pseudo-inverse with singular values <= 1e-10 sigma_max dropped
- *       (the semantics the HIP kernel implements). */
+ *       the SVD pseudo-inverse.  A single singular value <= 1e-10 sigma_max is still
+ *       inverted (the reference's behaviour) when the step along its singular pair is
+ *       determined by the data; otherwise the small ones are dropped (minimum-norm
+ *       solution).  The semantics the HIP kernel implements. */
 typedef struct {
   int n_slip, n_twin, nT;          /* nT = n_slip + n_twin (n_twin=1 dummy when twinning is off) */
   double m[OR_MAXT][3], n[OR_MAXT][3];  /* normalised slip directions / normals */
@@ -104,6 +106,17 @@ int or_krylov(int method, long long n, const long long *rp, const int *ci, const
 /* Counts of inactive-slip-removal solves and of those whose A_PA was numerically
  * singular (LU pivot <= 1e-10 max|A|) since the last call; resets the counters. */
 void or_pinv_stats(long long *calls, long long *singular);
+/* The first A_PA (n x n, row-major) and b_PA that took the fallback since the process started;
+ * returns n (0 if none). */
+int or_pinv_sample(double *A, double *b);
+/* Diagnostics: with enable = 1 every solve adds its sigma_min / sigma_max to a decade histogram
+ * (bin k: [1e-(k+1), 1e-k), bin 23: exactly singular); hist receives and resets the counts. */
+void or_pinv_histogram(int enable, long long hist[24]);
+/* With the histogram enabled, every solve with sigma_min < 1e-8 sigma_max is also recorded as
+ * (n, sigma_min/sigma_max, second smallest/sigma_max, |u_min . b|/|b|); returns and clears them. */
+int or_pinv_records(double *out, int max_records);
+/* Overrides the relative pivot / singular-value level of pinv_mode 1 (experiments only). */
+void or_set_pinv_rtol(double rtol);
 
 /* Single increment helpers exposed for property tests. */
 void or_odfpoint(const double r[3], double R[9]);

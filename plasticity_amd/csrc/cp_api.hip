@@ -652,10 +652,24 @@ __global__ void __launch_bounds__(64) k_qp_batch(long long n, const double *__re
 }
 
 // ------------------------------------------------------------ Krylov (FP64)
+// Solver state on the device.  Reductions never use atomics or fences: a kernel writes one
+// partial sum per block, and the next kernel on the stream (the kernel boundary orders the
+// writes) adds them in every block in the same fixed order, so every block holds the same
+// bitwise total.  rzh[i & 1] holds r.z of iteration i (PETSc's beta), written by block 0 of the
+// kernel that first forms it and read one kernel later.
 struct KState {
-  double beta, betaold, b, alpha, dp, tol, bnorm;
+  double beta, dp, tol, bnorm;
+  double rzh[2];
   int it, done, maxit, breakdown;
 };
+
+constexpr int VB = 256;   // threads per block of the Krylov kernels
+constexpr int VPT = 4;    // vector entries per thread of the update kernels
+
+// Block-uniform test of the "solve ended" flag.  Block 0 of a kernel that detects convergence
+// sets it while other blocks of the same kernel may be starting, so each block takes one
+// decision for all its threads.
+__device__ __forceinline__ bool solve_ended(const KState *st) { return __syncthreads_or(st->done != 0); }
 
 __device__ __forceinline__ long long nb_node(const MeshDims &m, long long n, int i, int j, int k, int s, bool &ok) {
   const int dx = s % 3 - 1, dy = (s / 3) % 3 - 1, dz = s / 9 - 1;
@@ -663,8 +677,25 @@ __device__ __forceinline__ long long nb_node(const MeshDims &m, long long n, int
   return n + dx + (long long)dy * m.nnx + (long long)dz * m.nnx * m.nny;
 }
 
-// Block sums written to partials[blockIdx.x * NV + k]; consumed by the next kernel (kernel
-// boundaries order the writes), which sums them in block order in every block.
+// Neighbour of node n = (i, j, k) in slot s (dx, dy, dz = s % 3 - 1, s / 3 % 3 - 1, s / 9 - 1),
+// without branches: a neighbour outside the mesh is reported as node n with ok = false, and its
+// vector entries are masked to zero by the caller.
+__device__ __forceinline__ long long nb_masked(const MeshDims &m, long long n, int i, int j, int k, int s, bool &ok) {
+  const int dx = s % 3 - 1, dy = (s / 3) % 3 - 1, dz = s / 9 - 1;
+  // one unsigned compare per axis covers both bounds; bitwise & keeps it free of branches
+  ok = ((unsigned)(i + dx) < (unsigned)m.nnx) & ((unsigned)(j + dy) < (unsigned)m.nny) & ((unsigned)(k + dz) < (unsigned)m.nnz);
+  return ok ? n + dx + (long long)dy * m.nnx + (long long)dz * m.nnx * m.nny : n;
+}
+// (i, j, k) of node n in 32-bit arithmetic (node counts stay below 2^31)
+__device__ __forceinline__ void node_ijk32(const MeshDims &m, long long n, int &i, int &j, int &k) {
+  const unsigned u = (unsigned)n, nx = (unsigned)m.nnx, ny = (unsigned)m.nny;
+  const unsigned r = u / nx;
+  i = (int)(u - r * nx);
+  k = (int)(r / ny);
+  j = (int)(r - (unsigned)k * ny);
+}
+
+// Block sums written to partials[blockIdx.x * NV + k] (thread 0).
 template <int NV>
 __device__ __forceinline__ void block_partials(double (&v)[NV], double *partials, double *lds) {
   block_sum<NV>(v, lds);
@@ -673,15 +704,60 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double *partials
     for (int k = 0; k < NV; ++k) partials[blockIdx.x * NV + k] = v[k];
   }
 }
-// Sum of nparts block partials (one value each), identical in every block; returned to all lanes.
-__device__ __forceinline__ double sum_partials(const double *partials, unsigned nparts, double *lds) {
-  double v[1] = {0.0};
-  for (unsigned b = threadIdx.x; b < nparts; b += blockDim.x) v[0] += partials[b];
-  block_sum<1>(v, lds);
-  __shared__ double total;
-  if (threadIdx.x == 0) total = v[0];
+// Totals of nparts blocks' NV partials, identical in every block, returned to every thread.
+// Four independent loads per thread per round keep the latency of the (mostly remote-L2)
+// reads to about one round trip.
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double *partials, unsigned nparts, double (&v)[NV], double *lds) {
+  __shared__ double tot[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = 0.0;
+  for (unsigned b0 = threadIdx.x; b0 < nparts; b0 += 4 * VB) {
+    double t[4][NV];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const unsigned b = b0 + j * VB;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) t[j][k] = (b < nparts) ? partials[(size_t)b * NV + k] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int k = 0; k < NV; ++k) v[k] += t[j][k];
+  }
+  block_sum<NV>(v, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) tot[k] = v[k];
+  }
   __syncthreads();
-  return total;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = tot[k];
+}
+
+// Convergence test and direction coefficient at the start of Krylov iteration it >= 1, from the
+// update kernel's partials (z.z, r.z) of iteration it-1: PETSc's test on the preconditioned norm
+// ||z|| <= tol (KSPConvergedDefault), and b = (r.z)_it / (r.z)_{it-1}.  Returns false when the
+// solve has ended (block 0 then records it, the norm and the breakdown flag).
+__device__ __forceinline__ bool krylov_next(KState *st, const double *upart, unsigned n_upart, int it, double *lds, double &b) {
+  double v[2];
+  sum_partials<2>(upart, n_upart, v, lds);
+  const double dp = sqrt(v[0]);
+  const bool breakdown = (v[1] == 0.0);
+  const bool done = (dp <= st->tol) || it >= st->maxit || breakdown;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->dp = dp;
+    if (done) {
+      st->done = 1;
+      st->it = it;
+      st->breakdown = (breakdown && !(dp <= st->tol)) ? 1 : 0;
+    } else {
+      st->rzh[it & 1] = v[1];
+    }
+  }
+  if (done) return false;
+  b = v[1] / st->rzh[(it - 1) & 1];
+  return true;
 }
 
 // 3x3 block product of one slot: y += B x (B row-major, 9 contiguous values)
@@ -703,27 +779,49 @@ __device__ __forceinline__ void block_tmv(const VT *__restrict__ B, double x0, d
   for (int r = 0; r < 3; ++r) y[r] += (double)b[r] * x0 + (double)b[3 + r] * x1 + (double)b[6 + r] * x2;
 }
 
-// Zr = K Pr, Zl = K^T Pl on the node-block matrix (lane = node); block partials of Zr.Pl (BiCG's dpi)
+// ---- BiCG (PETSc KSPBICG with PCJACOBI, solveLinearSystem.cc:13-16): three kernels per iteration
+
+// Iteration it: convergence test / b from the previous update's partials, then
+// p <- z (it = 0) or p <- z + b p, for both sequences.
+__global__ void __launch_bounds__(VB) k_bicg_p(long long N, const double *__restrict__ Zr, const double *__restrict__ Zl,
+                                               double *__restrict__ Pr, double *__restrict__ Pl, KState *st,
+                                               const double *upart, unsigned n_upart, int it) {
+  __shared__ double lds[32];
+  if (solve_ended(st)) return;
+  double b = 0.0;
+  if (it > 0 && !krylov_next(st, upart, n_upart, it, lds, b)) return;
+  const long long stride = (long long)gridDim.x * VB;
+  for (long long r = blockIdx.x * (long long)VB + threadIdx.x; r < N; r += stride) {
+    if (it == 0) { Pr[r] = Zr[r]; Pl[r] = Zl[r]; }
+    else { Pr[r] = Zr[r] + b * Pr[r]; Pl[r] = Zl[r] + b * Pl[r]; }
+  }
+}
+
+// Zr = K Pr, Zl = K^T Pl on the node-block matrix (lane = node); block partials of Zr.Pl
 template <typename VT>
-__global__ void __launch_bounds__(256) k_bicg_spmv(MeshDims m, const VT *__restrict__ K, const double *__restrict__ Pr,
-                                                   const double *__restrict__ Pl, double *__restrict__ Zr, double *__restrict__ Zl,
-                                                   const KState *st, double *partials) {
+__global__ void __launch_bounds__(VB) k_bicg_spmv(MeshDims m, const VT *__restrict__ K, const double *__restrict__ Pr,
+                                                  const double *__restrict__ Pl, double *__restrict__ Zr, double *__restrict__ Zl,
+                                                  const KState *st, double *partials) {
   __shared__ double lds[16];
-  if (st->done) return;
-  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (solve_ended(st)) return;
+  const long long n = blockIdx.x * (long long)VB + threadIdx.x;
   double acc[1] = {0.0};
   if (n < m.nnode) {
     int i, j, k;
-    node_ijk(m, n, i, j, k);
+    node_ijk32(m, n, i, j, k);
     double y[3] = {0, 0, 0}, yt[3] = {0, 0, 0};
     const long long nn = m.nnode;
+    // a masked neighbour reads the K^T block of node n itself, which multiplies zero
 #pragma unroll 3
     for (int s = 0; s < 27; ++s) {
       bool ok;
-      const long long mn = nb_node(m, n, i, j, k, s, ok);
-      if (!ok) continue;
-      block_mv(K + ((long long)s * nn + n) * 9, Pr[3 * mn], Pr[3 * mn + 1], Pr[3 * mn + 2], y);
-      block_tmv(K + ((long long)(26 - s) * nn + mn) * 9, Pl[3 * mn], Pl[3 * mn + 1], Pl[3 * mn + 2], yt);
+      const long long mn = nb_masked(m, n, i, j, k, s, ok);
+      const double w = ok ? 1.0 : 0.0;
+      double pr[3], pl[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { pr[c] = w * Pr[3 * mn + c]; pl[c] = w * Pl[3 * mn + c]; }
+      block_mv(K + ((long long)s * nn + n) * 9, pr[0], pr[1], pr[2], y);
+      block_tmv(K + ((long long)(26 - s) * nn + mn) * 9, pl[0], pl[1], pl[2], yt);
     }
 #pragma unroll
     for (int r = 0; r < 3; ++r) { Zr[3 * n + r] = y[r]; Zl[3 * n + r] = yt[r]; }
@@ -732,29 +830,20 @@ __global__ void __launch_bounds__(256) k_bicg_spmv(MeshDims m, const VT *__restr
   block_partials<1>(acc, partials, lds);
 }
 
-// p <- z (first iteration) or p <- z + b p
-__global__ void __launch_bounds__(256) k_bicg_p(long long N, const double *__restrict__ Zr, const double *__restrict__ Zl,
-                                                double *__restrict__ Pr, double *__restrict__ Pl, const KState *st) {
-  if (st->done) return;
-  const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (r >= N) return;
-  if (st->it == 0) { Pr[r] = Zr[r]; Pl[r] = Zl[r]; }
-  else { const double b = st->b; Pr[r] = Zr[r] + b * Pr[r]; Pl[r] = Zl[r] + b * Pl[r]; }
-}
-
-// a = beta / dpi (dpi from the operator kernel's partials); x += a p; r -= a z; rl -= a zl;
-// z = D^-1 r; zl = D^-1 rl; ||z|| and z.rl -> convergence test and the next beta
-__global__ void __launch_bounds__(256) k_bicg_update(long long N, double *__restrict__ X, const double *__restrict__ Pr,
-                                                     double *__restrict__ Rr, double *__restrict__ Rl, double *__restrict__ Zr,
-                                                     double *__restrict__ Zl, const double *__restrict__ dinv, KState *st,
-                                                     const double *spmv_partials, unsigned n_spmv_partials, double *partials,
-                                                     unsigned *counter) {
+// a = (r.z)_it / (Zr.Pl); x += a p; r -= a z; rl -= a zl; z = D^-1 r; zl = D^-1 rl;
+// block partials of z.z and z.rl
+__global__ void __launch_bounds__(VB) k_bicg_update(long long N, double *__restrict__ X, const double *__restrict__ Pr,
+                                                    double *__restrict__ Rr, double *__restrict__ Rl, double *__restrict__ Zr,
+                                                    double *__restrict__ Zl, const double *__restrict__ dinv, const KState *st,
+                                                    const double *opart, unsigned n_opart, double *upart, int it) {
   __shared__ double lds[32];
-  if (st->done) return;
-  const double a = st->beta / sum_partials(spmv_partials, n_spmv_partials, lds);
-  const long long stride = (long long)gridDim.x * blockDim.x;
+  if (solve_ended(st)) return;
+  double dpi[1];
+  sum_partials<1>(opart, n_opart, dpi, lds);
+  const double a = st->rzh[it & 1] / dpi[0];
+  const long long stride = (long long)gridDim.x * VB;
   double acc[2] = {0.0, 0.0};
-  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < N; r += stride) {
+  for (long long r = blockIdx.x * (long long)VB + threadIdx.x; r < N; r += stride) {
     const double d = dinv[r];
     X[r] += a * Pr[r];
     const double rr = Rr[r] - a * Zr[r];
@@ -766,29 +855,22 @@ __global__ void __launch_bounds__(256) k_bicg_update(long long N, double *__rest
     acc[1] += zr * rl;
   }
   __syncthreads();
-  if (grid_sum<2>(acc, partials, counter, lds)) {
-    st->alpha = a;
-    st->dp = sqrt(acc[0]);
-    st->it += 1;
-    if (st->dp <= st->tol || st->it >= st->maxit) st->done = 1;
-    st->betaold = st->beta;
-    st->beta = acc[1];
-    st->b = st->beta / st->betaold;
-    if (!st->done && st->beta == 0.0) { st->done = 1; st->breakdown = 1; }
-  }
+  block_partials<2>(acc, upart, lds);
 }
 
-// r = rl = b; z = zl = D^-1 b; tol = lin_tol * ||b||; x = 0
-__global__ void __launch_bounds__(256) k_bicg_init(long long N, const double *__restrict__ B, double *__restrict__ X,
-                                                   double *__restrict__ Rr, double *__restrict__ Rl, double *__restrict__ Zr,
-                                                   double *__restrict__ Zl, const double *__restrict__ dinv, KState *st,
-                                                   double lin_tol, int maxit, double *partials, unsigned *counter) {
+// r = rl = b; z = zl = D^-1 b; x = 0; tol = lin_tol * ||b||, ||z||, r.z (once per solve; the
+// last block to finish reduces the partials)
+__global__ void __launch_bounds__(VB) k_krylov_init(long long N, const double *__restrict__ B, double *__restrict__ X,
+                                                    double *__restrict__ Rr, double *__restrict__ Rl, double *__restrict__ Zr,
+                                                    double *__restrict__ Zl, const double *__restrict__ dinv, KState *st,
+                                                    double lin_tol, int maxit, double *partials, unsigned *counter) {
   __shared__ double lds[48];
-  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long stride = (long long)gridDim.x * VB;
   double acc[3] = {0.0, 0.0, 0.0};
-  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < N; r += stride) {
+  for (long long r = blockIdx.x * (long long)VB + threadIdx.x; r < N; r += stride) {
     const double b = B[r], z = b * dinv[r];
-    X[r] = 0.0; Rr[r] = b; Rl[r] = b; Zr[r] = z; Zl[r] = z;
+    X[r] = 0.0; Rr[r] = b; Zr[r] = z;
+    if (Rl) { Rl[r] = b; Zl[r] = z; }
     acc[0] += b * b; acc[1] += z * z; acc[2] += z * b;
   }
   if (grid_sum<3>(acc, partials, counter, lds)) {
@@ -796,7 +878,8 @@ __global__ void __launch_bounds__(256) k_bicg_init(long long N, const double *__
     st->tol = lin_tol * st->bnorm;
     st->dp = sqrt(acc[1]);
     st->beta = acc[2];
-    st->betaold = 1.0;
+    st->rzh[0] = acc[2];
+    st->rzh[1] = 1.0;
     st->it = 0;
     st->maxit = maxit;
     st->breakdown = 0;
@@ -805,67 +888,89 @@ __global__ void __launch_bounds__(256) k_bicg_init(long long N, const double *__
   }
 }
 
-// Jacobi-PCG (the option BASELINE names): q = K p and block partials of p.q
+// ---- Jacobi-PCG
+
+// Iteration it: convergence test / b from the previous update's partials; p <- z (it = 0) or
+// p <- z + b p (grid-stride over the dofs)
+__global__ void __launch_bounds__(VB) k_cg_p(long long N, const double *__restrict__ Z, double *__restrict__ P, KState *st,
+                                             const double *upart, unsigned n_upart, int it) {
+  __shared__ double lds[32];
+  if (solve_ended(st)) return;
+  double b = 0.0;
+  if (it > 0 && !krylov_next(st, upart, n_upart, it, lds, b)) return;
+  const long long stride = (long long)gridDim.x * VB;
+  for (long long r = blockIdx.x * (long long)VB + threadIdx.x; r < N; r += stride) P[r] = (it == 0) ? Z[r] : Z[r] + b * P[r];
+}
+
+// q = K p on the node-block matrix; block partials of p.q
 template <typename VT>
-__global__ void __launch_bounds__(256) k_cg_spmv(MeshDims m, const VT *__restrict__ K, const double *__restrict__ P,
-                                                 double *__restrict__ Q, const KState *st, double *partials) {
+__global__ void __launch_bounds__(VB) k_cg_spmv(MeshDims m, const VT *__restrict__ K, const double *__restrict__ P,
+                                                double *__restrict__ Q, const KState *st, double *opart) {
   __shared__ double lds[16];
-  if (st->done) return;
-  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (solve_ended(st)) return;
+  const long long n = blockIdx.x * (long long)VB + threadIdx.x;
   double acc[1] = {0.0};
   if (n < m.nnode) {
     int i, j, k;
-    node_ijk(m, n, i, j, k);
+    node_ijk32(m, n, i, j, k);
     double y[3] = {0, 0, 0};
     const long long nn = m.nnode;
-#pragma unroll 3
+    // nine slots in flight per lane for FP32 blocks (three for FP64, whose registers run out first)
+    constexpr int UNR = sizeof(VT) == 4 ? 9 : 3;
+#pragma unroll UNR
     for (int s = 0; s < 27; ++s) {
       bool ok;
-      const long long mn = nb_node(m, n, i, j, k, s, ok);
-      if (!ok) continue;
-      block_mv(K + ((long long)s * nn + n) * 9, P[3 * mn], P[3 * mn + 1], P[3 * mn + 2], y);
+      const long long mn = nb_masked(m, n, i, j, k, s, ok);
+      // unconditional loads (mn is always a valid node), then the mask
+      const double t0 = P[3 * mn], t1 = P[3 * mn + 1], t2 = P[3 * mn + 2];
+      const double w = ok ? 1.0 : 0.0;
+      block_mv(K + ((long long)s * nn + n) * 9, w * t0, w * t1, w * t2, y);
     }
+    const double pc[3] = {P[3 * n], P[3 * n + 1], P[3 * n + 2]};
 #pragma unroll
     for (int r = 0; r < 3; ++r) Q[3 * n + r] = y[r];
-    acc[0] = y[0] * P[3 * n] + y[1] * P[3 * n + 1] + y[2] * P[3 * n + 2];
+    acc[0] = y[0] * pc[0] + y[1] * pc[1] + y[2] * pc[2];
   }
-  block_partials<1>(acc, partials, lds);
+  block_partials<1>(acc, opart, lds);
 }
-__global__ void __launch_bounds__(256) k_cg_update(long long N, double *__restrict__ X, const double *__restrict__ P,
-                                                   double *__restrict__ Rr, const double *__restrict__ Q, double *__restrict__ Z,
-                                                   const double *__restrict__ dinv, KState *st, const double *spmv_partials,
-                                                   unsigned n_spmv_partials, double *partials, unsigned *counter) {
+
+// a = (r.z)_it / (p.q); x += a p; r -= a q; z = D^-1 r; block partials of z.z and r.z.
+// Each thread handles VPT entries, strided by the grid, with all loads issued before use.
+__global__ void __launch_bounds__(VB) k_cg_update(long long N, double *__restrict__ X, const double *__restrict__ P,
+                                                  double *__restrict__ R, const double *__restrict__ Q, double *__restrict__ Z,
+                                                  const double *__restrict__ dinv, const KState *st, const double *opart,
+                                                  unsigned n_opart, double *upart, int it) {
   __shared__ double lds[32];
-  if (st->done) return;
-  const double a = st->beta / sum_partials(spmv_partials, n_spmv_partials, lds);
-  const long long stride = (long long)gridDim.x * blockDim.x;
+  if (solve_ended(st)) return;
+  double pq[1];
+  sum_partials<1>(opart, n_opart, pq, lds);
+  const double a = st->rzh[it & 1] / pq[0];
+  const long long stride = (long long)gridDim.x * VB;
   double acc[2] = {0.0, 0.0};
-  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < N; r += stride) {
-    X[r] += a * P[r];
-    const double rr = Rr[r] - a * Q[r];
-    Rr[r] = rr;
-    const double z = rr * dinv[r];
-    Z[r] = z;
-    acc[0] += z * z;
-    acc[1] += rr * z;
+  for (long long r0 = blockIdx.x * (long long)VB + threadIdx.x; r0 < N; r0 += VPT * stride) {
+    double x[VPT], p[VPT], rr[VPT], q[VPT], d[VPT];
+#pragma unroll
+    for (int j = 0; j < VPT; ++j) {
+      const long long r = r0 + j * stride;
+      const bool ok = r < N;
+      x[j] = ok ? X[r] : 0.0; p[j] = ok ? P[r] : 0.0; rr[j] = ok ? R[r] : 0.0; q[j] = ok ? Q[r] : 0.0; d[j] = ok ? dinv[r] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < VPT; ++j) {
+      const long long r = r0 + j * stride;
+      if (r < N) {
+        const double rn = rr[j] - a * q[j];
+        const double z = rn * d[j];
+        X[r] = x[j] + a * p[j];
+        R[r] = rn;
+        Z[r] = z;
+        acc[0] += z * z;
+        acc[1] += rn * z;
+      }
+    }
   }
   __syncthreads();
-  if (grid_sum<2>(acc, partials, counter, lds)) {
-    st->alpha = a;
-    st->dp = sqrt(acc[0]);
-    st->it += 1;
-    if (st->dp <= st->tol || st->it >= st->maxit) st->done = 1;
-    st->betaold = st->beta;
-    st->beta = acc[1];
-    st->b = st->beta / st->betaold;
-  }
-}
-__global__ void __launch_bounds__(256) k_cg_p(long long N, const double *__restrict__ Z, double *__restrict__ P, const KState *st,
-                                              int first) {
-  if (st->done) return;
-  const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (r >= N) return;
-  P[r] = (first || st->it == 0) ? Z[r] : Z[r] + st->b * P[r];
+  block_partials<2>(acc, upart, lds);
 }
 
 template <typename KT>
@@ -918,6 +1023,8 @@ struct pa_context {
   int method = PA_SOLVER_BICG, precision = PA_PREC_FP64;
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t kev[2] = {nullptr, nullptr};  // Krylov flag copies, one per chunk parity
+  KState *hks = nullptr;                   // pinned host copies of the solver state (2)
   double last_ms[3] = {0, 0, 0};
   bool grains_set = false;
   int n_grain_rows = 0;
@@ -934,7 +1041,7 @@ struct pa_context {
   unsigned *queue = nullptr;  // deferred qps of the pseudo-inverse pass
   unsigned char *con = nullptr;
   double *delu = nullptr;
-  double *kv[8] = {nullptr};  // Krylov vectors: X Rr Rl Zr Zl Pr Pl spare
+  double *kv[7] = {nullptr};  // Krylov vectors: X Rr Rl Zr Zl Pr Pl (CG: X R - Z Q P -)
   KState *kst = nullptr;
   double *partials = nullptr;
   unsigned *counters = nullptr;
@@ -1015,14 +1122,15 @@ int launch_assemble(pa_context *c, int iteration) {
   return launch_elements(c, iteration);
 }
 
-// partial-sum regions: [0, off) per-block partials of the operator kernels, [off, ...) the
-// last-block reductions of the vector kernels
-long long spmv_partials_offset(const pa_context *c) { return ((long long)nblk(c->m.nnode, 256) + 64) & ~63LL; }
+// Partial-sum regions (doubles): [0, off) one per block of the operator kernels; [off, off2) two
+// per block of the update kernels; [off2, ...) the once-per-solve reductions (init, norm; at most
+// 3 per block of VGRID_MAX blocks).  The post-increment pass uses 13 per 256-lane block from 0.
+long long spmv_partials_offset(const pa_context *c) { return ((long long)nblk(c->m.nnode, VB) + 64) & ~63LL; }
+unsigned ugrid(long long n) { return nblk(n, (long long)VB * VPT); }
+long long update_partials_end(const pa_context *c) { return (spmv_partials_offset(c) + 2LL * ugrid(c->ndof) + 64) & ~63LL; }
 
-// doubles of the partial-sum buffer: operator partials, the vector kernels' reductions (<= 3 per
-// block of at most VGRID_MAX blocks) and the post-increment pass (13 per 256-lane block, from 0)
 size_t partials_size(const pa_context *c) {
-  const size_t vec = (size_t)spmv_partials_offset(c) + 3 * VGRID_MAX + 64;
+  const size_t vec = (size_t)update_partials_end(c) + 3 * VGRID_MAX + 64;
   const size_t post = 13 * (size_t)nblk(c->nq, 256) + 64;
   return std::max(vec, post);
 }
@@ -1064,7 +1172,8 @@ int do_assemble(pa_context *c, int inc, int it, double *res_norm) {
   else hipLaunchKernelGGL(k_dinv<double>, dim3(nblk(c->ndof, 256)), dim3(256), 0, c->stream, c->m, (const double *)c->K, c->dinv);
   HIPCHK(hipGetLastError());
   if (res_norm) {
-    hipLaunchKernelGGL(k_norm2, dim3(vgrid(c->ndof)), dim3(256), 0, c->stream, c->ndof, c->R, c->d_out + 16, c->partials,
+    hipLaunchKernelGGL(k_norm2, dim3(vgrid(c->ndof)), dim3(256), 0, c->stream, c->ndof, c->R, c->d_out + 16,
+                       c->partials + update_partials_end(c),
                        c->counters + 3);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(res_norm, c->d_out + 16, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1073,48 +1182,67 @@ int do_assemble(pa_context *c, int inc, int it, double *res_norm) {
   return 0;
 }
 
-int do_solve(pa_context *c, int it, int *iters) {
+// One Krylov launch sequence for iteration k: p update, operator (CG: K p; BiCG: K p and K^T p~), update.
+int launch_krylov_iteration(pa_context *c, int k) {
   const long long N = c->ndof;
   double *X = c->kv[0], *Rr = c->kv[1], *Rl = c->kv[2], *Zr = c->kv[3], *Zl = c->kv[4], *Pr = c->kv[5], *Pl = c->kv[6];
-  const unsigned gv = nblk(N, 256), gr = vgrid(N), gn = nblk(c->m.nnode, 256);
-  hipLaunchKernelGGL(k_bicg_init, dim3(gr), dim3(256), 0, c->stream, N, c->R, X, Rr, Rl, Zr, Zl, c->dinv, c->kst, c->pb.lin_tol,
-                     c->pb.lin_maxit, c->partials + spmv_partials_offset(c), c->counters + 0);
+  const unsigned gu = ugrid(N), gn = nblk(c->m.nnode, VB);
+  double *partA = c->partials, *partU = c->partials + spmv_partials_offset(c);
+  if (c->method == PA_SOLVER_CG) {
+    double *Q = Zl;
+    hipLaunchKernelGGL(k_cg_p, dim3(gu), dim3(VB), 0, c->stream, N, (const double *)Zr, Pr, c->kst, (const double *)partU, gu, k);
+    if (mixed(c))
+      hipLaunchKernelGGL(k_cg_spmv<float>, dim3(gn), dim3(VB), 0, c->stream, c->m, (const float *)c->K, (const double *)Pr, Q,
+                         (const KState *)c->kst, partA);
+    else
+      hipLaunchKernelGGL(k_cg_spmv<double>, dim3(gn), dim3(VB), 0, c->stream, c->m, (const double *)c->K, (const double *)Pr, Q,
+                         (const KState *)c->kst, partA);
+    hipLaunchKernelGGL(k_cg_update, dim3(gu), dim3(VB), 0, c->stream, N, X, (const double *)Pr, Rr, (const double *)Q, Zr,
+                       (const double *)c->dinv, (const KState *)c->kst, (const double *)partA, gn, partU, k);
+  } else {
+    hipLaunchKernelGGL(k_bicg_p, dim3(gu), dim3(VB), 0, c->stream, N, (const double *)Zr, (const double *)Zl, Pr, Pl, c->kst,
+                       (const double *)partU, gu, k);
+    if (mixed(c))
+      hipLaunchKernelGGL(k_bicg_spmv<float>, dim3(gn), dim3(VB), 0, c->stream, c->m, (const float *)c->K, (const double *)Pr,
+                         (const double *)Pl, Zr, Zl, (const KState *)c->kst, partA);
+    else
+      hipLaunchKernelGGL(k_bicg_spmv<double>, dim3(gn), dim3(VB), 0, c->stream, c->m, (const double *)c->K, (const double *)Pr,
+                         (const double *)Pl, Zr, Zl, (const KState *)c->kst, partA);
+    hipLaunchKernelGGL(k_bicg_update, dim3(gu), dim3(VB), 0, c->stream, N, X, (const double *)Pr, Rr, Rl, Zr, Zl,
+                       (const double *)c->dinv, (const KState *)c->kst, (const double *)partA, gn, partU, k);
+  }
   HIPCHK(hipGetLastError());
-  KState hs;
+  return 0;
+}
+
+int do_solve(pa_context *c, int it, int *iters) {
+  const long long N = c->ndof;
+  double *X = c->kv[0], *Rr = c->kv[1], *Rl = c->kv[2], *Zr = c->kv[3], *Zl = c->kv[4];
+  const bool cg = c->method == PA_SOLVER_CG;
+  hipLaunchKernelGGL(k_krylov_init, dim3(vgrid(N)), dim3(VB), 0, c->stream, N, c->R, X, Rr, cg ? nullptr : Rl, Zr, cg ? nullptr : Zl,
+                     c->dinv, c->kst, c->pb.lin_tol, c->pb.lin_maxit, c->partials + update_partials_end(c), c->counters + 0);
+  HIPCHK(hipGetLastError());
+  // The kernels test convergence themselves and return at once after it.  The host reads the
+  // flag every `chunk` iterations, one chunk behind: chunk j+1 is queued before the copy of the
+  // flag after chunk j is waited on, so the stream never idles on the host round trip; at most
+  // one chunk of launches runs after convergence, each returning at its first instruction.
   const int chunk = 16;
-  double *partA = c->partials, *partB = c->partials + spmv_partials_offset(c);
-  for (int base = 0;; base += chunk) {
+  KState hs;
+  for (int j = 0;; ++j) {
     for (int k = 0; k < chunk; ++k) {
-      if (c->method == PA_SOLVER_CG) {
-        hipLaunchKernelGGL(k_cg_p, dim3(gv), dim3(256), 0, c->stream, N, Zr, Pr, (const KState *)c->kst, 0);
-        if (mixed(c))
-          hipLaunchKernelGGL(k_cg_spmv<float>, dim3(gn), dim3(256), 0, c->stream, c->m, (const float *)c->K, (const double *)Pr, Zl,
-                             (const KState *)c->kst, partA);
-        else
-          hipLaunchKernelGGL(k_cg_spmv<double>, dim3(gn), dim3(256), 0, c->stream, c->m, (const double *)c->K, (const double *)Pr, Zl,
-                             (const KState *)c->kst, partA);
-        hipLaunchKernelGGL(k_cg_update, dim3(gr), dim3(256), 0, c->stream, N, X, (const double *)Pr, Rr, (const double *)Zl, Zr,
-                           (const double *)c->dinv, c->kst, (const double *)partA, gn, partB, c->counters + 2);
-      } else {
-        hipLaunchKernelGGL(k_bicg_p, dim3(gv), dim3(256), 0, c->stream, N, (const double *)Zr, (const double *)Zl, Pr, Pl,
-                           (const KState *)c->kst);
-        if (mixed(c))
-          hipLaunchKernelGGL(k_bicg_spmv<float>, dim3(gn), dim3(256), 0, c->stream, c->m, (const float *)c->K, (const double *)Pr,
-                             (const double *)Pl, Zr, Zl, (const KState *)c->kst, partA);
-        else
-          hipLaunchKernelGGL(k_bicg_spmv<double>, dim3(gn), dim3(256), 0, c->stream, c->m, (const double *)c->K, (const double *)Pr,
-                             (const double *)Pl, Zr, Zl, (const KState *)c->kst, partA);
-        hipLaunchKernelGGL(k_bicg_update, dim3(gr), dim3(256), 0, c->stream, N, X, (const double *)Pr, Rr, Rl, Zr, Zl,
-                           (const double *)c->dinv, c->kst, (const double *)partA, gn, partB, c->counters + 2);
-      }
+      const int rc = launch_krylov_iteration(c, j * chunk + k);
+      if (rc) return rc;
     }
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(&hs, c->kst, sizeof(KState), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpyAsync(&c->hks[j & 1], c->kst, sizeof(KState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipEventRecord(c->kev[j & 1], c->stream));
+    if (j == 0) continue;
+    HIPCHK(hipEventSynchronize(c->kev[(j - 1) & 1]));
+    hs = c->hks[(j - 1) & 1];
     if (hs.done) break;
   }
   if (iters) *iters = hs.it;
-  hipLaunchKernelGGL(k_apply_du, dim3(gv), dim3(256), 0, c->stream, N, c->u, (const double *)X, (const unsigned char *)c->con,
+  // the CG iterate x is complete; BiCG likewise (x lives in kv[0] for both)
+  hipLaunchKernelGGL(k_apply_du, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, c->u, (const double *)X, (const unsigned char *)c->con,
                      (const double *)c->delu, (int)(it == 0));
   HIPCHK(hipGetLastError());
   return 0;
@@ -1158,6 +1286,11 @@ int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out) {
   int rc = 0;
   if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return fail(PA_ERR_HIP, "hipStreamCreate failed"); }
   for (int i = 0; i < 4; ++i) hipEventCreate(&c->ev[i]);
+  for (int i = 0; i < 2; ++i) hipEventCreateWithFlags(&c->kev[i], hipEventDisableTiming);
+  if (hipHostMalloc((void **)&c->hks, 2 * sizeof(KState), hipHostMallocDefault) != hipSuccess) {
+    pa_destroy(c);
+    return fail(PA_ERR_HIP, "hipHostMalloc failed");
+  }
   const int NS = ns_template(c->ns);
   // material block
   {
@@ -1222,6 +1355,9 @@ int pa_destroy(pa_context *c) {
   for (void *p : c->allocs) hipFree(p);
   for (int i = 0; i < 4; ++i)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
+  for (int i = 0; i < 2; ++i)
+    if (c->kev[i]) hipEventDestroy(c->kev[i]);
+  if (c->hks) hipHostFree(c->hks);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -1512,35 +1648,52 @@ int pa_qp_update(pa_context *c, long long n, const double *F, const double *Fp_c
 int pa_time_kernel(pa_context *c, int which, int reps, double *avg_ms) {
   if (!c || !avg_ms || reps < 1) return fail(PA_ERR_ARG, "bad argument");
   if (!c->grains_set) return fail(PA_ERR_STATE, "pa_set_grains has not been called");
-  const unsigned gn = nblk(c->m.nnode, 256);
-  // keep the solver state "not done" so the operator kernels run in full
+  const unsigned gn = nblk(c->m.nnode, VB), gu = ugrid(c->ndof);
+  double *partA = c->partials, *partU = c->partials + spmv_partials_offset(c);
+  // solver state that never ends (tol < 0, no iteration limit) with unit r.z history, and update
+  // partials summing to z.z = r.z = 1, so the operator kernels run in full on the iteration >= 1
+  // path (b = 1); operator partials summing to 1 for the update kernel
   KState ks;
   std::memset(&ks, 0, sizeof(ks));
-  ks.beta = 1.0;
+  ks.tol = -1.0;
+  ks.maxit = 0x7fffffff;
+  ks.rzh[0] = ks.rzh[1] = 1.0;
+  static const double ones[2] = {1.0, 1.0};
   HIPCHK(hipMemcpyAsync(c->kst, &ks, sizeof(ks), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(partU, 0, sizeof(double) * 2 * (size_t)gu, c->stream));
+  HIPCHK(hipMemsetAsync(partA, 0, sizeof(double) * (size_t)gn, c->stream));
+  HIPCHK(hipMemcpyAsync(partU, ones, sizeof(ones), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(partA, ones, sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipEventRecord(c->ev[0], c->stream));
   for (int r = 0; r < reps; ++r) {
     int rc = 0;
     if (which == 0) {
       if (mixed(c))
-        hipLaunchKernelGGL(k_bicg_spmv<float>, dim3(gn), dim3(256), 0, c->stream, c->m, (const float *)c->K, (const double *)c->kv[5],
-                           (const double *)c->kv[6], c->kv[3], c->kv[4], (const KState *)c->kst, c->partials);
+        hipLaunchKernelGGL(k_bicg_spmv<float>, dim3(gn), dim3(VB), 0, c->stream, c->m, (const float *)c->K, (const double *)c->kv[5],
+                           (const double *)c->kv[6], c->kv[3], c->kv[4], (const KState *)c->kst, partA);
       else
-        hipLaunchKernelGGL(k_bicg_spmv<double>, dim3(gn), dim3(256), 0, c->stream, c->m, (const double *)c->K, (const double *)c->kv[5],
-                           (const double *)c->kv[6], c->kv[3], c->kv[4], (const KState *)c->kst, c->partials);
+        hipLaunchKernelGGL(k_bicg_spmv<double>, dim3(gn), dim3(VB), 0, c->stream, c->m, (const double *)c->K, (const double *)c->kv[5],
+                           (const double *)c->kv[6], c->kv[3], c->kv[4], (const KState *)c->kst, partA);
     } else if (which == 1) {
       rc = launch_assemble(c, 1);
     } else if (which == 2) {
       if (mixed(c))
-        hipLaunchKernelGGL(k_cg_spmv<float>, dim3(gn), dim3(256), 0, c->stream, c->m, (const float *)c->K, (const double *)c->kv[5],
-                           c->kv[4], (const KState *)c->kst, c->partials);
+        hipLaunchKernelGGL(k_cg_spmv<float>, dim3(gn), dim3(VB), 0, c->stream, c->m, (const float *)c->K, (const double *)c->kv[5],
+                           c->kv[4], (const KState *)c->kst, partA);
       else
-        hipLaunchKernelGGL(k_cg_spmv<double>, dim3(gn), dim3(256), 0, c->stream, c->m, (const double *)c->K, (const double *)c->kv[5],
-                           c->kv[4], (const KState *)c->kst, c->partials);
+        hipLaunchKernelGGL(k_cg_spmv<double>, dim3(gn), dim3(VB), 0, c->stream, c->m, (const double *)c->K, (const double *)c->kv[5],
+                           c->kv[4], (const KState *)c->kst, partA);
     } else if (which == 3) {
       rc = launch_qp_tangent(c);
     } else if (which == 4) {
       rc = launch_elements(c, 1);
+    } else if (which == 6) {
+      hipLaunchKernelGGL(k_cg_p, dim3(gu), dim3(VB), 0, c->stream, c->ndof, (const double *)c->kv[3], c->kv[5], c->kst,
+                         (const double *)partU, gu, 1);
+    } else if (which == 5) {
+      hipLaunchKernelGGL(k_cg_update, dim3(gu), dim3(VB), 0, c->stream, c->ndof, c->kv[0], (const double *)c->kv[5], c->kv[1],
+                         (const double *)c->kv[4], c->kv[3], (const double *)c->dinv, (const KState *)c->kst, (const double *)partA, gn,
+                         partU, 1);
     } else {
       return fail(PA_ERR_ARG, "unknown kernel id");
     }

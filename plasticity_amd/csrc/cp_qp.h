@@ -104,9 +104,11 @@ CP_DEV void m3_expm(const double *A, double *E) {
 }
 
 // ------------------------------------------------------- small dense solvers
-// Relative level below which a pivot (LU) or singular value (SVD fallback) counts as zero.
-// A structurally rank-deficient A_PA (e.g. ten potentially active FCC systems: the latent
-// hardening matrix has rank 4, the elastic coupling rank <= 6) shows pivots ~1e-13 max|A|.
+// Relative level below which a pivot (LU) or singular value (SVD fallback) counts as small.
+// Near-singular A_PA are common: symmetric multi-slip leaves one singular value at 1e-9..1e-13
+// sigma_max (inverted, as the reference does), and ten or more potentially active FCC systems make
+// A_PA rank-deficient (latent-hardening rank 4 + elastic coupling rank <= 6: a null space of
+// dimension >= 2 at round-off, dropped).
 #define CP_PINV_RTOL 1e-10
 
 // LU with partial pivoting on an n x n row-major matrix (ld = NS) with m right-hand
@@ -148,7 +150,8 @@ CP_DEV bool lu_solve(int n, double *A, double *B, int m) {
   return true;
 }
 
-// x = pinv(A) b by one-sided Jacobi SVD, singular values <= CP_PINV_RTOL * sigma_max dropped.
+// x = pinv(A) b by one-sided Jacobi SVD; singular values <= CP_PINV_RTOL * sigma_max are dropped
+// unless there is exactly one and its step is determined by the data (below).
 // Used only when LU finds A_PA numerically singular.
 template <int NS>
 __host__ __device__ __attribute__((noinline)) void svd_pinv_solve(int n, const double *A, const double *b, double *x) {
@@ -187,7 +190,24 @@ __host__ __device__ __attribute__((noinline)) void svd_pinv_solve(int n, const d
       for (int i = 0; i < n; ++i) U[i * NS + j] /= sig[j];
     smax = fmax(smax, sig[j]);
   }
-  const double lim = smax * CP_PINV_RTOL;
+  // A single small singular value is inverted, as compute_inverse_svd(0.0) does
+  // (inactiveSlipRemoval.cc:26,99), when the step it produces is fixed by the data: the step along
+  // v_k is (u_k.b)/sigma_k and the round-off it spreads into the other components is about
+  // |u_k.b| eps sigma_max / (sigma_k gap); the pair is kept when that is under 1e-2 |b| / sigma_max.
+  // Otherwise (two or more small values, or a round-off one carrying a large u_k.b) the small ones
+  // are dropped: the reference's result there depends on round-off.
+  double lim = smax * CP_PINV_RTOL;
+  int nsmall = 0, ks = -1;
+  for (int k = 0; k < n; ++k)
+    if (sig[k] <= lim) { nsmall++; ks = k; }
+  if (nsmall == 1 && sig[ks] > 0) {
+    double s2 = smax, ub = 0, bb = 0;
+    for (int k = 0; k < n; ++k)
+      if (k != ks) s2 = fmin(s2, sig[k]);
+    for (int i = 0; i < n; ++i) { ub += U[i * NS + ks] * b[i]; bb += b[i] * b[i]; }
+    const double eps = 2.220446049250313e-16;
+    if (fabs(ub) * eps * smax * smax < 1e-2 * sqrt(bb) * sig[ks] * (s2 - sig[ks])) lim = 0.0;
+  }
   for (int i = 0; i < n; ++i) x[i] = 0;
   for (int k = 0; k < n; ++k) {
     if (!(sig[k] > lim)) continue;

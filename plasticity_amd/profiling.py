@@ -1,0 +1,56 @@
+"""Summaries of rocprofv3 output databases (rocpd SQLite) as short markdown tables.
+
+  python -m plasticity_amd.profiling trace <results.db> "<command>"
+      per-kernel calls / total / average duration / share of GPU time (--kernel-trace --stats)
+  python -m plasticity_amd.profiling traffic <FETCH_SIZE results.db> <WRITE_SIZE results.db>
+      mean FETCH_SIZE and WRITE_SIZE (KiB) per dispatch of every kernel, grouped by grid size;
+      the device-to-device copies (__amd_rocclr_copyBuffer) move a known number of bytes and
+      calibrate the counters
+"""
+import sqlite3
+import sys
+from collections import defaultdict
+
+
+def short_name(name):
+    return name.replace("(anonymous namespace)::", "").replace("void ", "").split("(")[0]
+
+
+def trace(db, cmd, top=15):
+    c = sqlite3.connect(db)
+    rows = list(c.execute("select name, total_calls, total_duration, average, percentage from top_kernels"))
+    print("Command (run from /tmp with TMPDIR=/tmp, repository at $R):\n")
+    print("```\n%s\n```\n" % cmd)
+    print("| kernel | calls | total ms | avg us | % of GPU time |")
+    print("|---|---:|---:|---:|---:|")
+    for name, n, tot, avg, pct in rows[:top]:  # rocpd top_kernels durations are in microseconds
+        print("| `%s` | %d | %.2f | %.2f | %.1f |" % (short_name(name), n, tot / 1e3, avg, pct))
+
+
+def _counters(db):
+    c = sqlite3.connect(db)
+    out = defaultdict(list)
+    for name, grid, val in c.execute("select kernel_name, grid_size, value from counters_collection"):
+        out[(name, grid)].append(val)
+    return out
+
+
+def traffic(fetch_db, write_db, min_kib=1024):
+    f, w = _counters(fetch_db), _counters(write_db)
+    print("| kernel | grid | dispatches | FETCH_SIZE KiB/dispatch | WRITE_SIZE KiB/dispatch |")
+    print("|---|---:|---:|---:|---:|")
+    for k in sorted(set(f) | set(w), key=lambda k: -sum(f.get(k, [0]))):
+        fv, wv = f.get(k, []), w.get(k, [])
+        if sum(fv) + sum(wv) < min_kib:
+            continue
+        print("| `%s` | %d | %d | %.0f | %.0f |" % (short_name(k[0]), k[1], max(len(fv), len(wv)),
+                                                   sum(fv) / max(len(fv), 1), sum(wv) / max(len(wv), 1)))
+
+
+if __name__ == "__main__":
+    if len(sys.argv) == 4 and sys.argv[1] == "trace":
+        trace(sys.argv[2], sys.argv[3])
+    elif len(sys.argv) == 4 and sys.argv[1] == "traffic":
+        traffic(sys.argv[2], sys.argv[3])
+    else:
+        sys.exit(__doc__)

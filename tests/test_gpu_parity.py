@@ -91,7 +91,9 @@ def test_csr_pattern_bit_exact(nel):
 
 @pytest.mark.parametrize("lin_tol,row_tol,state_tol", [(1e-14, 1e-9, 1e-9), (1e-10, 1e-6, 1e-6)])
 def test_newton_curve_matches_oracle_fcc_tension(lin_tol, row_tol, state_tol):
-    """8^3 mesh with the fcc/simpleTension inputs: 6 increments through yield, 4 Newton iterations each.
+    """8^3 mesh with the fcc/simpleTension inputs: 8 increments through yield, 4 Newton iterations each.
+    Yield (increment 5) brings hundreds of local solves whose A_PA has one singular value near
+    1e-12 sigma_max, which both sides invert (compute_inverse_svd(0.0) semantics).
 
     At the reference's Krylov tolerance (1e-10 on the Jacobi-preconditioned residual,
     solveLinearSystem.cc:13) each Newton correction is only fixed to ~1e-4 and the BiCG stopping
@@ -100,7 +102,7 @@ def test_newton_curve_matches_oracle_fcc_tension(lin_tol, row_tol, state_tol):
     and the curve, the state and the Krylov iteration counts must match."""
     case = refcase.load_reference_case("fcc_simpleTension")
     case.lin_tol = lin_tol
-    n = 6
+    n = 8
     ref = refcase.oracle_simulate(case, n, nthreads=8, want_state=True)
     ctx = _ctx(case)
     ctx.set_grains(case.elem_grain, case.grain_rod)
@@ -114,16 +116,19 @@ def test_newton_curve_matches_oracle_fcc_tension(lin_tol, row_tol, state_tol):
     print("lin iters gpu", its, "oracle", ref["lin_iters"])
     assert len(its) == len(ref["lin_iters"]) == 4 * n  # fixed Newton count (solveNonLinearSystem.cc:14)
     if lin_tol < 1e-12:
-        # the first two solves of each increment carry O(1) right-hand sides and must take the same
-        # number of iterations; late Newton iterations solve for a residual that is itself mostly
-        # round-off of cancelling element sums, so their counts only need to be close
-        first = np.arange(len(its)) % 4 < 2
+        # before yield (increments 0-4) the first two solves of each increment carry O(1)
+        # right-hand sides and must take the same number of iterations; late Newton iterations
+        # solve for a residual that is mostly round-off of cancelling element sums, and plastic
+        # increments inherit 1e-11 state differences, so their counts only need to be close
+        first = (np.arange(len(its)) % 4 < 2) & (np.arange(len(its)) < 20)
         assert np.abs(its - ref["lin_iters"])[first].max() <= 1
         assert np.abs(its - ref["lin_iters"]).max() <= 0.1 * ref["lin_iters"].max()
-    for cols in (slice(0, 6), slice(6, 12), slice(14, 15)):  # strain, stress, SlipTotal
+    # SlipTotal right after yield sums slips of systems that sit at the activation threshold; the
+    # oracle itself moves it by 2e-6 (relative) between Krylov tolerances 1e-14 and 3e-15
+    for cols, tol in ((slice(0, 6), row_tol), (slice(6, 12), row_tol), (slice(14, 15), max(row_tol, 1e-5))):
         scale = np.abs(ref["rows"][:, cols]).max()
         err = np.abs(rows[:, cols] - ref["rows"][:, cols]).max() / scale
-        assert err < row_tol, (cols, err)
+        assert err < tol, (cols, err)
     st = ref["state"]
     nT = case.nT
     assert _rel(ctx.get_state("T"), st[:, 0:9]) < state_tol
@@ -132,6 +137,29 @@ def test_newton_curve_matches_oracle_fcc_tension(lin_tol, row_tol, state_tol):
     assert _rel(ctx.get_state("rotnew"), st[:, 27 + nT:30 + nT]) < state_tol
     assert _rel(ctx.get_state("u"), ref["u"]) < state_tol
     ctx.close()
+
+
+@pytest.mark.parametrize("folder,n", [("fcc_simpleTension", 18), ("fcc_compression", 13), ("bcc_simpleTension", 30),
+                                      ("fcc_FCC_RandomOrientationBlock", 12)])
+def test_reference_golden_curve(folder, n):
+    """The reference's own inputs (mesh, grain map, orientations, slip systems, BCs, 1e-10 BiCG) run
+    through the GPU path reproduce its golden stress-strain rows (tests/compile_tests/*/
+    correct_stressstrain.txt, 6 significant digits) to print precision, over the increments before
+    rank-deficient active sets make the reference's own curve irreproducible."""
+    case = refcase.load_reference_case(folder)
+    ctx = _ctx(case)
+    ctx.set_grains(case.elem_grain, case.grain_rod)
+    rows = np.array([ctx.run_increment(inc)[0] for inc in range(n)])
+    ctx.close()
+    gold = case.golden[:n]
+    for cols in (slice(0, 6), slice(6, 12)):
+        scale = np.abs(gold[:, cols]).max(axis=1)
+        err = (np.abs(rows[:, cols] - gold[:, cols]).max(axis=1) / scale).max()
+        assert err < 1.2e-5, (folder, cols, err)
+    slip = gold[:, 14] > 1e-7
+    if slip.any():
+        assert (np.abs(rows[slip, 14] - gold[slip, 14]) / gold[slip, 14]).max() < 5e-5
+    assert (rows[~slip, 14] < 1e-6).all()
 
 
 def test_first_newton_iteration_stagewise():
