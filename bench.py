@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the Jacobi-PCG operator kernel k_cg_spmv at the 64^3 workload, from two
 # rocprofv3 PMC passes (FETCH_SIZE x 2 + WRITE_SIZE x 1 KiB: the factors that make both counters
 # read the known byte count of the history-commit copies); filled in from the committed profile
-PMC_TRAFFIC_KIB = {}
+PMC_TRAFFIC_KIB = {("cg", "mixed", 64): 142996 * 2 + 6373}
 
 
 def cpu_baseline(args, quick):
@@ -107,7 +107,8 @@ def main():
         "roofline": head["roofline"],
     }
     keys = ("ms_per_step", "qp_updates_per_s", "krylov_iters_per_step", "assembly_ms_per_step", "krylov_ms_per_step",
-            "post_ms_per_increment", "qp_tangent_ms", "element_ms", "operator_us", "update_us", "update_GBps", "p_us")
+            "post_ms_per_increment", "qp_tangent_ms", "element_ms", "operator_us", "update_us", "update_GBps", "p_us",
+            "singular_local_solves_per_step")
     for (form, solver), r in results.items():
         out["config"]["%s_%s" % (form, solver)] = {k: r[k] for k in keys}
     if len(forms) == 2:
@@ -144,14 +145,16 @@ def run_form(args, form, solver, pa, native, synth, torch):
         inc, it = divmod(s, nmax)
         rn, its = ctx.newton_iteration(inc, it)
         asm_ms, kry_ms, _ = ctx.last_timings()
-        rec = {"inc": inc, "it": it, "res": rn, "lin_iters": its, "asm_ms": asm_ms, "krylov_ms": kry_ms}
+        rec = {"inc": inc, "it": it, "res": rn, "lin_iters": its, "asm_ms": asm_ms, "krylov_ms": kry_ms,
+               "deferred": int(ctx.last_counts()[0])}
         if it == nmax - 1:
             rec["row"] = ctx.commit_increment().tolist()
             rec["post_ms"] = ctx.last_timings()[2]
+            rec["post_deferred"] = int(ctx.last_counts()[1])
         state["step"] += 1
         log.append(rec)
-        print("[%s %s] step %d (increment %d, iteration %d): |R| %.3e, %d Krylov iterations, assembly %.1f ms, Krylov %.1f ms"
-              % (form, solver, s, inc, it, rn, its, asm_ms, kry_ms), file=sys.stderr, flush=True)
+        print("[%s %s] step %d (increment %d, iteration %d): |R| %.3e, %d Krylov iterations, assembly %.1f ms, Krylov %.1f ms, "
+              "%d singular local solves" % (form, solver, s, inc, it, rn, its, asm_ms, kry_ms, rec["deferred"]), file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
         newton_step()
@@ -197,6 +200,7 @@ def run_form(args, form, solver, pa, native, synth, torch):
         "assembly_ms_per_step": float(np.mean([r["asm_ms"] for r in timed])),
         "krylov_ms_per_step": float(np.mean([r["krylov_ms"] for r in timed])),
         "post_ms_per_increment": float(np.mean(posts)) if posts else None,
+        "singular_local_solves_per_step": float(np.mean([r["deferred"] for r in timed])),
         "qp_tangent_ms": qp_ms, "element_ms": el_ms, "operator_us": spmv_ms * 1e3, "update_us": upd_us, "p_us": p_us,
         "update_GBps": (8 * ctx.ndof * 8 / (upd_us * 1e-6) / 1e9) if upd_us else None,
         "rows": [r["row"] for r in log if "row" in r],

@@ -140,6 +140,11 @@ int pa_qp_update(pa_context *ctx, long long nqp, const double *F, const double *
  * [0] assembly (qp + tangent + K_e + scatter), [1] Krylov, [2] post-increment pass. */
 int pa_last_timings(pa_context *ctx, double ms[3]);
 
+/* Instrumentation (no counterpart in the reference): quadrature points whose local active-set
+ * matrix was numerically singular and went through the pseudo-inverse pass, in the last assembly
+ * (counts[0]) and the last post-increment pass (counts[1]). */
+int pa_last_counts(pa_context *ctx, long long counts[2]);
+
 /* Instrumentation (no counterpart in the reference): times `reps` launches of one kernel
  * between HIP events on the context stream, on the context's current data.  which:
  *   0 = BiCG operator pair  Zr = K Pr, Zl = K^T Pl  (the Krylov hot kernel)

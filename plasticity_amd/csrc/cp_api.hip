@@ -1541,6 +1541,16 @@ int pa_last_timings(pa_context *c, double ms[3]) {
   return 0;
 }
 
+int pa_last_counts(pa_context *c, long long counts[2]) {
+  if (!c || !counts) return fail(PA_ERR_ARG, "null argument");
+  unsigned h[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(h, c->counters + 8, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  counts[0] = h[0];
+  counts[1] = h[1];
+  return 0;
+}
+
 int pa_get_state(pa_context *c, int which, double *host, long long n) {
   if (!c || !host) return fail(PA_ERR_ARG, "null argument");
   const long long nq = c->nq;

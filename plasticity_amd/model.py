@@ -250,6 +250,13 @@ class Context:
         check(native.lib().pa_last_timings(self._h, ptr(ms)))
         return ms
 
+    def last_counts(self):
+        """Quadrature points that went through the pseudo-inverse pass in the last assembly and
+        the last post-increment pass."""
+        c = np.zeros(2, np.int64)
+        check(native.lib().pa_last_counts(self._h, ptr(c)))
+        return c
+
     def qp_update(self, F, Fp, s, grain, flag=1):
         F = f64(F).reshape(-1, 9)
         n = len(F)
