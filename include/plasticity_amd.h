@@ -51,8 +51,10 @@ typedef struct {
   double stress_tol;                     /* Stress Tolerance */
   int max_slip_search;                   /* Max Slip Search Iterations (must be 1) */
   int max_solver_iter;                   /* Max Solver Iterations */
-  int kinematic_hardening;               /* Enable Kinematic Hardening (must be 0) */
+  int kinematic_hardening;               /* Enable Kinematic Hardening */
   int twinning;                          /* Twinning enabled (must be 0) */
+  double C1[PA_MAX_SLIP];                /* C_1 Slip Kinematic Hardening (used when kinematic_hardening) */
+  double C2[PA_MAX_SLIP];                /* C_2 Slip Kinematic Hardening */
 } pa_material;
 
 /* Mesh, loading and solver inputs of ellipticBVP<dim> (userInputParameters.cc:17-235). */
@@ -118,7 +120,8 @@ int pa_commit_increment(pa_context *ctx, double row[15]);
 int pa_run_increment(pa_context *ctx, int increment, double row[15], int *lin_iters, double *res_norms);
 
 /* State readback for parity.  which: 0 T (9/qp), 1 Fp_conv (9), 2 Fe_conv (9), 3 s_conv (n_slip),
- * 4 rotnew (3), 5 slip_conv (n_slip), 6 u (3/node), 7 R (3/node), 8 Fp_iter, 9 Fe_iter, 10 s_iter.
+ * 4 rotnew (3), 5 slip_conv (n_slip), 6 u (3/node), 7 R (3/node), 8 Fp_iter, 9 Fe_iter, 10 s_iter,
+ * 11 W_kh_conv (n_slip, backstress; kinematic hardening only), 12 W_kh_iter.
  * Output is qp-major (or node-major).  n = number of doubles the caller provides. */
 int pa_get_state(pa_context *ctx, int which, double *host, long long n);
 
@@ -131,10 +134,12 @@ int pa_sizes(pa_context *ctx, long long *nelem, long long *nqp, long long *ndof,
 /* calculatePlasticity(cellID, q, flag) over a batch of quadrature points (calculatePlasticity.cc:13-959),
  * host buffers in/out, with grain[i] selecting the orientation row set by pa_set_grains.
  * F, Fp_conv: 9/qp; s_conv: n_slip/qp.  Outputs P, T, Fe, Fp: 9/qp; dPdF: 81/qp (flag 1, may be
- * NULL for flag 0); s_out, slip_inc (x_beta_old): n_slip/qp; n_active: 1/qp. */
+ * NULL for flag 0); s_out, slip_inc (x_beta_old): n_slip/qp; n_active: 1/qp.  With kinematic
+ * hardening, W_conv (n_slip/qp, NULL = zero) is the backstress W_kh_conv and W_out (may be NULL)
+ * receives W_kh_tau; both are ignored otherwise. */
 int pa_qp_update(pa_context *ctx, long long nqp, const double *F, const double *Fp_conv, const double *s_conv,
                  const unsigned *grain, int flag, double *P, double *T, double *dPdF, double *Fe, double *Fp,
-                 double *s_out, double *slip_inc, int *n_active);
+                 double *s_out, double *slip_inc, int *n_active, const double *W_conv, double *W_out);
 
 /* Timing of the last pa_newton_iteration / pa_run_increment phases on the context stream (ms):
  * [0] assembly (qp + tangent + K_e + scatter), [1] Krylov, [2] post-increment pass. */

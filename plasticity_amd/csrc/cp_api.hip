@@ -265,8 +265,8 @@ struct QpArgs {
   const double *mat;
   int ns, max_solver_iter;
   long long nq;
-  const double *Fp_conv, *s_conv, *slip_conv;
-  double *Fe_iter, *Fp_iter, *s_iter, *slip_iter;
+  const double *Fp_conv, *s_conv, *slip_conv, *W_conv;
+  double *Fe_iter, *Fp_iter, *s_iter, *slip_iter, *W_iter;  // W_*: backstress (kinematic hardening only)
   void *D;    // [81][nq] of DT
   double *P;  // [9][nq]
   unsigned *queue, *qcount;  // deferred qps
@@ -276,7 +276,7 @@ struct QpArgs {
 // inactive-slip removal) are deferred: pass 1 runs every qp on the LU path and appends the
 // singular ones to a device queue; pass 2 runs only the queued qps, so the slow branch never
 // holds up a wavefront of regular qps.  Results do not depend on which lane computes a qp.
-template <int NS, typename DT>
+template <int NS, typename DT, bool KIN>
 __device__ __forceinline__ bool qp_tangent_one(const QpArgs &a, long long Q, bool allow_svd) {
   const long long e = Q >> 3;
   const int q = (int)(Q & 7);
@@ -295,14 +295,16 @@ __device__ __forceinline__ bool qp_tangent_one(const QpArgs &a, long long Q, boo
     }
   }
   F[0] += 1; F[4] += 1; F[8] += 1;
-  double Fp_t[9], s[NS], x[NS], P[9], T[9], Fe[9], Fp[9];
+  double Fp_t[9], s[NS], x[NS], P[9], T[9], Fe[9], Fp[9], W[KIN ? NS : 1];
 #pragma unroll
   for (int i = 0; i < 9; ++i) Fp_t[i] = a.Fp_conv[i * a.nq + Q];
   for (int i = 0; i < a.ns; ++i) s[i] = a.s_conv[i * a.nq + Q];
+  if constexpr (KIN)
+    for (int i = 0; i < a.ns; ++i) W[i] = a.W_conv[i * a.nq + Q];
   const double *gt = a.gtab + (long long)a.egrain[e] * a.gstride;
   bool plastic = false;
-  const int na = cp::qp_update<NS, DT>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 1, P, T, Fe, Fp, (DT *)a.D + Q, a.nq,
-                                       &plastic, a.JxW, allow_svd);
+  const int na = cp::qp_update<NS, DT, KIN>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 1, P, T, Fe, Fp, (DT *)a.D + Q,
+                                            a.nq, &plastic, a.JxW, allow_svd, W);
   if (na < 0) return false;
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
@@ -311,6 +313,8 @@ __device__ __forceinline__ bool qp_tangent_one(const QpArgs &a, long long Q, boo
     a.P[i * a.nq + Q] = a.JxW * P[i];
   }
   for (int i = 0; i < a.ns; ++i) a.s_iter[i * a.nq + Q] = s[i];
+  if constexpr (KIN)
+    for (int i = 0; i < a.ns; ++i) a.W_iter[i * a.nq + Q] = W[i];
   // slipfraction_iter is written only on the plastic path (calculatePlasticity.cc:454-456)
   if (plastic)
     for (int i = 0; i < a.ns; ++i) a.slip_iter[i * a.nq + Q] = a.slip_conv[i * a.nq + Q] + x[i];
@@ -318,18 +322,18 @@ __device__ __forceinline__ bool qp_tangent_one(const QpArgs &a, long long Q, boo
 }
 
 // pass 1: lane = qp, LU path only; singular local problems are queued
-template <int NS, typename DT>
+template <int NS, typename DT, bool KIN>
 __global__ void __launch_bounds__(256) k_qp_tangent(QpArgs a) {
   const long long Q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (Q >= a.nq) return;
-  if (!qp_tangent_one<NS, DT>(a, Q, false)) a.queue[atomicAdd(a.qcount, 1u)] = (unsigned)Q;
+  if (!qp_tangent_one<NS, DT, KIN>(a, Q, false)) a.queue[atomicAdd(a.qcount, 1u)] = (unsigned)Q;
 }
 // pass 2: the queued qps, grid-stride, pseudo-inverse allowed
-template <int NS, typename DT>
+template <int NS, typename DT, bool KIN>
 __global__ void __launch_bounds__(64) k_qp_tangent_deferred(QpArgs a) {
   const unsigned n = *a.qcount;
   for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    qp_tangent_one<NS, DT>(a, a.queue[i], true);
+    qp_tangent_one<NS, DT, KIN>(a, a.queue[i], true);
 }
 
 __global__ void k_zero_u32(unsigned *p) { *p = 0u; }
@@ -476,8 +480,8 @@ struct PostArgs {
   const double *mat;
   int ns, max_solver_iter;
   long long nq;
-  const double *Fp_conv, *s_conv, *slip_conv, *Fe_conv;
-  double *Fe_iter, *Fp_iter, *s_iter, *slip_iter, *T, *rotnew;
+  const double *Fp_conv, *s_conv, *slip_conv, *Fe_conv, *W_conv;
+  double *Fe_iter, *Fp_iter, *s_iter, *slip_iter, *T, *rotnew, *W_iter;
   double *partials;
   unsigned *counter;
   double *out;  // 13: E(6) T(6) slip
@@ -548,36 +552,40 @@ __device__ __forceinline__ void post_F(const PostArgs &a, long long Q, double F[
 
 // updateAfterIncrement's qp loop (updateAfterIncrement.cc:77-126): calculatePlasticity(.,.,0) at
 // the converged displacement; singular local problems deferred as in assembly.
-template <int NS>
+template <int NS, bool KIN>
 __device__ __forceinline__ bool post_qp_one(const PostArgs &a, long long Q, bool allow_svd) {
   double F[9];
   post_F(a, Q, F);
-  double Fp_t[9], s[NS], x[NS], P[9], T[9], Fe[9], Fp[9];
+  double Fp_t[9], s[NS], x[NS], P[9], T[9], Fe[9], Fp[9], W[KIN ? NS : 1];
 #pragma unroll
   for (int i = 0; i < 9; ++i) Fp_t[i] = a.Fp_conv[i * a.nq + Q];
   for (int i = 0; i < a.ns; ++i) s[i] = a.s_conv[i * a.nq + Q];
+  if constexpr (KIN)
+    for (int i = 0; i < a.ns; ++i) W[i] = a.W_conv[i * a.nq + Q];
   const double *gt = a.gtab + (long long)a.egrain[Q >> 3] * a.gstride;
   bool plastic = false;
-  const int na = cp::qp_update<NS, double>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 0, P, T, Fe, Fp, (double *)nullptr, 0,
-                                           &plastic, 1.0, allow_svd);
+  const int na = cp::qp_update<NS, double, KIN>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 0, P, T, Fe, Fp, (double *)nullptr,
+                                                0, &plastic, 1.0, allow_svd, W);
   if (na < 0) return false;
 #pragma unroll
   for (int i = 0; i < 9; ++i) { a.Fe_iter[i * a.nq + Q] = Fe[i]; a.Fp_iter[i * a.nq + Q] = Fp[i]; a.T[i * a.nq + Q] = T[i]; }
   for (int i = 0; i < a.ns; ++i) a.s_iter[i * a.nq + Q] = s[i];
+  if constexpr (KIN)
+    for (int i = 0; i < a.ns; ++i) a.W_iter[i * a.nq + Q] = W[i];
   if (plastic)
     for (int i = 0; i < a.ns; ++i) a.slip_iter[i * a.nq + Q] = a.slip_conv[i * a.nq + Q] + x[i];
   return true;
 }
-template <int NS>
+template <int NS, bool KIN>
 __global__ void __launch_bounds__(256) k_post_qp(PostArgs a) {
   const long long Q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (Q >= a.nq) return;
-  if (!post_qp_one<NS>(a, Q, false)) a.queue[atomicAdd(a.qcount, 1u)] = (unsigned)Q;
+  if (!post_qp_one<NS, KIN>(a, Q, false)) a.queue[atomicAdd(a.qcount, 1u)] = (unsigned)Q;
 }
-template <int NS>
+template <int NS, bool KIN>
 __global__ void __launch_bounds__(64) k_post_qp_deferred(PostArgs a) {
   const unsigned n = *a.qcount;
-  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) post_qp_one<NS>(a, a.queue[i], true);
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) post_qp_one<NS, KIN>(a, a.queue[i], true);
 }
 
 // Green-Lagrange strain, JxW-weighted sums (updateAfterIncrement.cc:107-126, 218-245) and
@@ -632,23 +640,41 @@ __global__ void __launch_bounds__(256) k_post_reduce(PostArgs a) {
   }
 }
 
-// batch of calculatePlasticity calls with host-provided inputs (parity entry point)
-template <int NS>
+// batch of calculatePlasticity calls with host-provided inputs (parity entry point); W (KIN):
+// backstress in (W_kh_conv) and out (W_kh_tau), n_slip per qp
+template <int NS, bool KIN>
 __global__ void __launch_bounds__(64) k_qp_batch(long long n, const double *__restrict__ F, const double *__restrict__ Fpc,
                                                  const double *__restrict__ sc, const unsigned *__restrict__ grain,
                                                  const double *gtab, int gstride, const double *mat, int ns, int msi, int flag,
                                                  double *P, double *T, double *D, double *Fe, double *Fp, double *so, double *xo,
-                                                 int *na) {
+                                                 int *na, double *Wio) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double f[9], fp[9], s[NS], x[NS], p[9], t[9], fe[9], fpo[9];
+  double f[9], fp[9], s[NS], x[NS], p[9], t[9], fe[9], fpo[9], w[KIN ? NS : 1];
   for (int k = 0; k < 9; ++k) { f[k] = F[9 * i + k]; fp[k] = Fpc[9 * i + k]; }
   for (int k = 0; k < ns; ++k) s[k] = sc[(long long)ns * i + k];
+  if constexpr (KIN)
+    for (int k = 0; k < ns; ++k) w[k] = Wio[(long long)ns * i + k];
   const double *gt = gtab + (long long)grain[i] * gstride;
-  const int r = cp::qp_update<NS>(mat, gt, ns, msi, f, fp, s, x, flag, p, t, fe, fpo, flag ? D + 81 * i : nullptr, 1);
+  const int r = cp::qp_update<NS, double, KIN>(mat, gt, ns, msi, f, fp, s, x, flag, p, t, fe, fpo, flag ? D + 81 * i : nullptr, 1,
+                                               nullptr, 1.0, true, w);
   for (int k = 0; k < 9; ++k) { P[9 * i + k] = p[k]; T[9 * i + k] = t[k]; Fe[9 * i + k] = fe[k]; Fp[9 * i + k] = fpo[k]; }
   for (int k = 0; k < ns; ++k) { so[(long long)ns * i + k] = s[k]; xo[(long long)ns * i + k] = x[k]; }
+  if constexpr (KIN)
+    for (int k = 0; k < ns; ++k) Wio[(long long)ns * i + k] = w[k];
   na[i] = r;
+}
+
+template <int NS>
+void launch_qp_batch(hipStream_t st, bool kin, unsigned grid, long long n, const double *F, const double *Fpc, const double *sc,
+                     const unsigned *grain, const double *gtab, int gstride, const double *mat, int ns, int msi, int flag, double *P,
+                     double *T, double *D, double *Fe, double *Fp, double *so, double *xo, int *na, double *Wio) {
+  if (kin)
+    hipLaunchKernelGGL((k_qp_batch<NS, true>), dim3(grid), dim3(64), 0, st, n, F, Fpc, sc, grain, gtab, gstride, mat, ns, msi, flag, P,
+                       T, D, Fe, Fp, so, xo, na, Wio);
+  else
+    hipLaunchKernelGGL((k_qp_batch<NS, false>), dim3(grid), dim3(64), 0, st, n, F, Fpc, sc, grain, gtab, gstride, mat, ns, msi, flag, P,
+                       T, D, Fe, Fp, so, xo, na, Wio);
 }
 
 // ------------------------------------------------------------ Krylov (FP64)
@@ -1034,6 +1060,8 @@ struct pa_context {
   unsigned *d_egrain = nullptr;
   double *Fp_conv = nullptr, *Fp_iter = nullptr, *Fe_conv = nullptr, *Fe_iter = nullptr, *s_conv = nullptr, *s_iter = nullptr;
   double *slip_conv = nullptr, *slip_iter = nullptr, *rotnew = nullptr, *T = nullptr;
+  bool kin = false;                                // kinematic hardening: backstress state below
+  double *W_conv = nullptr, *W_iter = nullptr;     // [ns][nq], allocated only with kinematic hardening
   double *u = nullptr, *R = nullptr, *dinv = nullptr;
   void *K = nullptr;   // node-block Jacobian, double (FP64 form) or float (mixed form)
   void *Dq = nullptr;  // [81][nq] JxW dP/dF of the last assembly, double or float
@@ -1071,8 +1099,13 @@ constexpr unsigned DEFER_GRID = 1024;  // 64-lane blocks of the deferred pass
 template <int NS, typename DT>
 void qp_tangent_passes(pa_context *c, const QpArgs &a) {
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, c->stream, a.qcount);
-  hipLaunchKernelGGL((k_qp_tangent<NS, DT>), dim3(nblk(c->nq, 256)), dim3(256), 0, c->stream, a);
-  hipLaunchKernelGGL((k_qp_tangent_deferred<NS, DT>), dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
+  if (c->kin) {
+    hipLaunchKernelGGL((k_qp_tangent<NS, DT, true>), dim3(nblk(c->nq, 256)), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL((k_qp_tangent_deferred<NS, DT, true>), dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
+  } else {
+    hipLaunchKernelGGL((k_qp_tangent<NS, DT, false>), dim3(nblk(c->nq, 256)), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL((k_qp_tangent_deferred<NS, DT, false>), dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
+  }
 }
 
 int launch_qp_tangent(pa_context *c) {
@@ -1082,8 +1115,8 @@ int launch_qp_tangent(pa_context *c) {
   a.JxW = c->JxW;
   a.u = c->u; a.egrain = c->d_egrain; a.gtab = c->d_gtab; a.gstride = c->gstride; a.mat = c->d_mat;
   a.ns = c->ns; a.max_solver_iter = c->mat.max_solver_iter; a.nq = c->nq;
-  a.Fp_conv = c->Fp_conv; a.s_conv = c->s_conv; a.slip_conv = c->slip_conv;
-  a.Fe_iter = c->Fe_iter; a.Fp_iter = c->Fp_iter; a.s_iter = c->s_iter; a.slip_iter = c->slip_iter;
+  a.Fp_conv = c->Fp_conv; a.s_conv = c->s_conv; a.slip_conv = c->slip_conv; a.W_conv = c->W_conv;
+  a.Fe_iter = c->Fe_iter; a.Fp_iter = c->Fp_iter; a.s_iter = c->s_iter; a.slip_iter = c->slip_iter; a.W_iter = c->W_iter;
   a.D = c->Dq; a.P = c->Pq;
   a.queue = c->queue; a.qcount = c->counters + 8;
   const int NS = ns_template(c->ns);
@@ -1093,6 +1126,17 @@ int launch_qp_tangent(pa_context *c) {
   else { if (f) qp_tangent_passes<48, float>(c, a); else qp_tangent_passes<48, double>(c, a); }
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+template <int NS>
+void post_passes(pa_context *c, const PostArgs &a, unsigned grid) {
+  if (c->kin) {
+    hipLaunchKernelGGL((k_post_qp<NS, true>), dim3(grid), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL((k_post_qp_deferred<NS, true>), dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
+  } else {
+    hipLaunchKernelGGL((k_post_qp<NS, false>), dim3(grid), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL((k_post_qp_deferred<NS, false>), dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
+  }
 }
 
 int launch_elements(pa_context *c, int iteration) {
@@ -1260,7 +1304,6 @@ int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out) {
   if (mat->n_slip < 1 || mat->n_slip > PA_MAX_SLIP) return fail(PA_ERR_ARG, "n_slip out of range");
   if (mat->max_slip_search != 1) return fail(PA_ERR_UNSUPPORTED, "Max Slip Search Iterations must be 1");
   if (mat->max_solver_iter < 1) return fail(PA_ERR_ARG, "Max Solver Iterations must be >= 1");
-  if (mat->kinematic_hardening) return fail(PA_ERR_UNSUPPORTED, "kinematic hardening is not implemented");
   if (mat->twinning) return fail(PA_ERR_UNSUPPORTED, "twinning is not implemented");
   if (pb->nx < 1 || pb->ny < 1 || pb->nz < 1) return fail(PA_ERR_ARG, "mesh size must be >= 1");
   if (pb->total_increments < 1 || pb->max_newton < 1 || pb->lin_maxit < 1) return fail(PA_ERR_ARG, "bad increment/solver counts");
@@ -1269,6 +1312,7 @@ int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out) {
   pa_context *c = new pa_context();
   c->mat = *mat;
   c->pb = *pb;
+  c->kin = mat->kinematic_hardening != 0;
   c->ns = mat->n_slip;
   MeshDims &m = c->m;
   m.nx = pb->nx; m.ny = pb->ny; m.nz = pb->nz;
@@ -1294,12 +1338,15 @@ int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out) {
   const int NS = ns_template(c->ns);
   // material block
   {
-    std::vector<double> mb(3 * NS + NS * NS + 1, 0.0);
+    // layout of cp::MatLayout<NS>: h0 ss pw q tol C_1 C_2
+    const int tol = 3 * NS + NS * NS, c1 = tol + 1, c2 = c1 + NS;
+    std::vector<double> mb(c2 + NS, 0.0);
     for (int a = 0; a < c->ns; ++a) {
       mb[a] = mat->h0[a]; mb[NS + a] = mat->ss[a]; mb[2 * NS + a] = mat->pw[a];
       for (int b = 0; b < c->ns; ++b) mb[3 * NS + a * NS + b] = mat->q[a][b];
+      if (c->kin) { mb[c1 + a] = mat->C1[a]; mb[c2 + a] = mat->C2[a]; }
     }
-    mb[3 * NS + NS * NS] = mat->stress_tol;
+    mb[tol] = mat->stress_tol;
     if ((rc = dalloc(c, &c->d_mat, mb.size()))) { pa_destroy(c); return rc; }
     if (hipMemcpy(c->d_mat, mb.data(), mb.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) { pa_destroy(c); return fail(PA_ERR_HIP, "copy material"); }
   }
@@ -1337,6 +1384,13 @@ int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out) {
         hipMemcpy(c->s_iter, sb.data(), sb.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
       pa_destroy(c);
       return fail(PA_ERR_HIP, "state init copy");
+    }
+  }
+  if (c->kin) {  // backstress W_kh starts at zero (init.cc:302-330)
+    if ((rc = dalloc(c, &c->W_conv, ns * nq)) || (rc = dalloc(c, &c->W_iter, ns * nq))) { pa_destroy(c); return rc; }
+    if (hipMemset(c->W_conv, 0, sizeof(double) * ns * nq) != hipSuccess || hipMemset(c->W_iter, 0, sizeof(double) * ns * nq) != hipSuccess) {
+      pa_destroy(c);
+      return fail(PA_ERR_HIP, "state init");
     }
   }
   if (hipMemset(c->slip_conv, 0, sizeof(double) * ns * nq) != hipSuccess || hipMemset(c->slip_iter, 0, sizeof(double) * ns * nq) != hipSuccess ||
@@ -1472,24 +1526,18 @@ int pa_commit_increment(pa_context *c, double row[15]) {
   a.JxW = c->JxW;
   a.u = c->u; a.egrain = c->d_egrain; a.gtab = c->d_gtab; a.gstride = c->gstride; a.mat = c->d_mat;
   a.ns = c->ns; a.max_solver_iter = c->mat.max_solver_iter; a.nq = c->nq;
-  a.Fp_conv = c->Fp_conv; a.s_conv = c->s_conv; a.slip_conv = c->slip_conv; a.Fe_conv = c->Fe_conv;
+  a.Fp_conv = c->Fp_conv; a.s_conv = c->s_conv; a.slip_conv = c->slip_conv; a.Fe_conv = c->Fe_conv; a.W_conv = c->W_conv;
   a.Fe_iter = c->Fe_iter; a.Fp_iter = c->Fp_iter; a.s_iter = c->s_iter; a.slip_iter = c->slip_iter; a.T = c->T; a.rotnew = c->rotnew;
+  a.W_iter = c->W_iter;
   a.partials = c->partials; a.counter = c->counters + 4; a.out = c->d_out;
   a.queue = c->queue; a.qcount = c->counters + 9;
   HIPCHK(hipEventRecord(c->ev[0], c->stream));
   const int NS = ns_template(c->ns);
   const unsigned grid = nblk(c->nq, 256);
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, c->stream, a.qcount);
-  if (NS == 12) {
-    hipLaunchKernelGGL(k_post_qp<12>, dim3(grid), dim3(256), 0, c->stream, a);
-    hipLaunchKernelGGL(k_post_qp_deferred<12>, dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
-  } else if (NS == 24) {
-    hipLaunchKernelGGL(k_post_qp<24>, dim3(grid), dim3(256), 0, c->stream, a);
-    hipLaunchKernelGGL(k_post_qp_deferred<24>, dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
-  } else {
-    hipLaunchKernelGGL(k_post_qp<48>, dim3(grid), dim3(256), 0, c->stream, a);
-    hipLaunchKernelGGL(k_post_qp_deferred<48>, dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
-  }
+  if (NS == 12) post_passes<12>(c, a, grid);
+  else if (NS == 24) post_passes<24>(c, a, grid);
+  else post_passes<48>(c, a, grid);
   hipLaunchKernelGGL(k_post_reduce, dim3(grid), dim3(256), 0, c->stream, a);
   HIPCHK(hipGetLastError());
   // _conv <- _iter (updateAfterIncrement.cc:298-305); _iter keeps its values
@@ -1498,6 +1546,7 @@ int pa_commit_increment(pa_context *c, double row[15]) {
   HIPCHK(hipMemcpyAsync(c->Fp_conv, c->Fp_iter, b9, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->s_conv, c->s_iter, bs, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->slip_conv, c->slip_iter, bs, hipMemcpyDeviceToDevice, c->stream));
+  if (c->kin) HIPCHK(hipMemcpyAsync(c->W_conv, c->W_iter, bs, hipMemcpyDeviceToDevice, c->stream));
   double h[13];
   HIPCHK(hipMemcpyAsync(h, c->d_out, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipEventRecord(c->ev[1], c->stream));
@@ -1569,7 +1618,13 @@ int pa_get_state(pa_context *c, int which, double *host, long long n) {
     case 8: src = c->Fp_iter; comps = 9; break;
     case 9: src = c->Fe_iter; comps = 9; break;
     case 10: src = c->s_iter; comps = c->ns; break;
+    case 11: src = c->W_conv; comps = c->ns; break;
+    case 12: src = c->W_iter; comps = c->ns; break;
     default: return fail(PA_ERR_ARG, "unknown state id");
+  }
+  if (!src) {
+    if (which == 11 || which == 12) return fail(PA_ERR_STATE, "no backstress state: kinematic hardening is off");
+    return fail(PA_ERR_STATE, "state not allocated");
   }
   const long long cnt = node ? c->ndof : comps * nq;
   if (n < cnt) return fail(PA_ERR_ARG, "host buffer too small");
@@ -1614,7 +1669,8 @@ int pa_get_csr(pa_context *c, long long *row_ptr, int *col_idx, double *values) 
 }
 
 int pa_qp_update(pa_context *c, long long n, const double *F, const double *Fp_conv, const double *s_conv, const unsigned *grain,
-                 int flag, double *P, double *T, double *dPdF, double *Fe, double *Fp, double *s_out, double *slip_inc, int *n_active) {
+                 int flag, double *P, double *T, double *dPdF, double *Fe, double *Fp, double *s_out, double *slip_inc, int *n_active,
+                 const double *W_conv, double *W_out) {
   if (!c || n < 0 || !F || !Fp_conv || !s_conv || !grain || !P || !T || !Fe || !Fp || !s_out || !slip_inc || !n_active)
     return fail(PA_ERR_ARG, "null argument");
   if (flag && !dPdF) return fail(PA_ERR_ARG, "dPdF required for flag 1");
@@ -1624,32 +1680,38 @@ int pa_qp_update(pa_context *c, long long n, const double *F, const double *Fp_c
   if (n == 0) return 0;
   const int ns = c->ns;
   const size_t b9 = 9 * n * sizeof(double), bs = (size_t)ns * n * sizeof(double);
-  double *dF, *dFp, *ds, *dP, *dT, *dD = nullptr, *dFe, *dFpo, *dso, *dx;
+  double *dF, *dFp, *ds, *dP, *dT, *dD = nullptr, *dFe, *dFpo, *dso, *dx, *dW = nullptr;
   unsigned *dg;
   int *dna;
   std::vector<void *> tmp;
   auto al = [&](void **p, size_t b) { if (hipMalloc(p, b) != hipSuccess) return false; tmp.push_back(*p); return true; };
   bool ok = al((void **)&dF, b9) && al((void **)&dFp, b9) && al((void **)&ds, bs) && al((void **)&dP, b9) && al((void **)&dT, b9) &&
             al((void **)&dFe, b9) && al((void **)&dFpo, b9) && al((void **)&dso, bs) && al((void **)&dx, bs) &&
-            al((void **)&dg, n * sizeof(unsigned)) && al((void **)&dna, n * sizeof(int)) && (!flag || al((void **)&dD, 81 * n * sizeof(double)));
+            al((void **)&dg, n * sizeof(unsigned)) && al((void **)&dna, n * sizeof(int)) && (!flag || al((void **)&dD, 81 * n * sizeof(double))) &&
+            (!c->kin || al((void **)&dW, bs));
   int rc = 0;
   if (!ok) rc = fail(PA_ERR_HIP, "hipMalloc");
   if (!rc && (hipMemcpy(dF, F, b9, hipMemcpyHostToDevice) != hipSuccess || hipMemcpy(dFp, Fp_conv, b9, hipMemcpyHostToDevice) != hipSuccess ||
               hipMemcpy(ds, s_conv, bs, hipMemcpyHostToDevice) != hipSuccess || hipMemcpy(dg, grain, n * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess))
     rc = fail(PA_ERR_HIP, "copy inputs");
+  if (!rc && c->kin &&
+      (W_conv ? hipMemcpy(dW, W_conv, bs, hipMemcpyHostToDevice) : hipMemset(dW, 0, bs)) != hipSuccess)
+    rc = fail(PA_ERR_HIP, "copy inputs");
   if (!rc) {
     const int NS = ns_template(ns);
     const unsigned grid = nblk(n, 64);
-    if (NS == 12) hipLaunchKernelGGL(k_qp_batch<12>, dim3(grid), dim3(64), 0, c->stream, n, dF, dFp, ds, dg, c->d_gtab, c->gstride, c->d_mat, ns, c->mat.max_solver_iter, flag, dP, dT, dD, dFe, dFpo, dso, dx, dna);
-    else if (NS == 24) hipLaunchKernelGGL(k_qp_batch<24>, dim3(grid), dim3(64), 0, c->stream, n, dF, dFp, ds, dg, c->d_gtab, c->gstride, c->d_mat, ns, c->mat.max_solver_iter, flag, dP, dT, dD, dFe, dFpo, dso, dx, dna);
-    else hipLaunchKernelGGL(k_qp_batch<48>, dim3(grid), dim3(64), 0, c->stream, n, dF, dFp, ds, dg, c->d_gtab, c->gstride, c->d_mat, ns, c->mat.max_solver_iter, flag, dP, dT, dD, dFe, dFpo, dso, dx, dna);
+    const int msi = c->mat.max_solver_iter;
+    if (NS == 12) launch_qp_batch<12>(c->stream, c->kin, grid, n, dF, dFp, ds, dg, c->d_gtab, c->gstride, c->d_mat, ns, msi, flag, dP, dT, dD, dFe, dFpo, dso, dx, dna, dW);
+    else if (NS == 24) launch_qp_batch<24>(c->stream, c->kin, grid, n, dF, dFp, ds, dg, c->d_gtab, c->gstride, c->d_mat, ns, msi, flag, dP, dT, dD, dFe, dFpo, dso, dx, dna, dW);
+    else launch_qp_batch<48>(c->stream, c->kin, grid, n, dF, dFp, ds, dg, c->d_gtab, c->gstride, c->d_mat, ns, msi, flag, dP, dT, dD, dFe, dFpo, dso, dx, dna, dW);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(PA_ERR_HIP, "qp kernel");
   }
   if (!rc && (hipMemcpy(P, dP, b9, hipMemcpyDeviceToHost) != hipSuccess || hipMemcpy(T, dT, b9, hipMemcpyDeviceToHost) != hipSuccess ||
               hipMemcpy(Fe, dFe, b9, hipMemcpyDeviceToHost) != hipSuccess || hipMemcpy(Fp, dFpo, b9, hipMemcpyDeviceToHost) != hipSuccess ||
               hipMemcpy(s_out, dso, bs, hipMemcpyDeviceToHost) != hipSuccess || hipMemcpy(slip_inc, dx, bs, hipMemcpyDeviceToHost) != hipSuccess ||
               hipMemcpy(n_active, dna, n * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
-              (flag && hipMemcpy(dPdF, dD, 81 * n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)))
+              (flag && hipMemcpy(dPdF, dD, 81 * n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) ||
+              (c->kin && W_out && hipMemcpy(W_out, dW, bs, hipMemcpyDeviceToHost) != hipSuccess)))
     rc = fail(PA_ERR_HIP, "copy outputs");
   for (void *p : tmp) hipFree(p);
   return rc;

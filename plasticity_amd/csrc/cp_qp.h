@@ -221,7 +221,8 @@ __host__ __device__ __attribute__((noinline)) void svd_pinv_solve(int n, const d
 // ------------------------------------------------------------ table layouts
 // material block (doubles): h0[NS] ss[NS] pw[NS] q[NS*NS] tol
 template <int NS> struct MatLayout {
-  static constexpr int h0 = 0, ss = NS, pw = 2 * NS, q = 3 * NS, tol = 3 * NS + NS * NS, size = tol + 1;
+  static constexpr int h0 = 0, ss = NS, pw = 2 * NS, q = 3 * NS, tol = 3 * NS + NS * NS, c1 = tol + 1, c2 = c1 + NS,
+                       size = c2 + NS;
 };
 // grain block (doubles): S[NS][9] Dmat[36] TM[81]
 template <int NS> struct GrainLayout {
@@ -256,11 +257,14 @@ CP_DEV void dexp_series(const double *X, const double *S, int count3, bool neg, 
 // which the reference writes slipfraction_iter (calculatePlasticity.cc:454-456).
 // D (flag = 1) receives dscale * dP/dF with element stride `dstride`:
 // D[((3i+k)*9 + 3j+l) * dstride] = dscale * dP_ik / dF_jl, stored as DT.
-template <int NS, typename DT = double>
+// KIN: kinematic hardening (calculatePlasticity.cc:294-305, 431-436, with C_1, C_2 of
+// multiphaseInit.cc:15-32): W holds the backstress W_kh_conv on entry and W_kh_tau on exit, and
+// every sign test on the resolved shear uses tau - W.  Without it the backstress is zero.
+template <int NS, typename DT = double, bool KIN = false>
 CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ gt, int ns, int max_solver_iter,
                      const double *F, const double *Fp_t, double *s, double *x, int flag, double *P, double *T,
                      double *Fe_out, double *Fp_out, DT *D, long long dstride, bool *plastic_path = nullptr,
-                     double dscale = 1.0, bool allow_svd = true) {
+                     double dscale = 1.0, bool allow_svd = true, double *W = nullptr) {
   typedef MatLayout<NS> ML;
   typedef GrainLayout<NS> GL;
   const double tol1 = mat[ML::tol];
@@ -280,13 +284,19 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
   for (int i = 0; i < 9; ++i) { Fp[i] = Fp_t[i]; Ce[i] = CE[i]; TstTau[i] = Tst[i]; }
 
   double tau_tr[NS], b[NS];
+  // backstress at the start of the increment (W_kh_t) and its running value (W_kh_tau)
+  double Wt[KIN ? NS : 1];
+  auto wtau = [&](int a) -> double { if constexpr (KIN) return W[a]; else return 0.0; };
+  auto wt = [&](int a) -> double { if constexpr (KIN) return Wt[a]; else return 0.0; };
+  if constexpr (KIN)
+    for (int a = 0; a < ns; ++a) Wt[a] = W[a];
   int PA[NS];
   int nPA = 0;
   m3_mul(CE, Tst, tmp);                                  // :203-240
   for (int a = 0; a < ns; ++a) {
     const double r = m3_ddot(tmp, Sg + 9 * a);
     tau_tr[a] = r;
-    b[a] = fabs(r) - s[a];
+    b[a] = fabs(r - wtau(a)) - s[a];
     x[a] = 0.0;
     if (b[a] >= tol1) PA[nPA++] = a;
   }
@@ -317,8 +327,13 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
       for (int ii = 0; ii < n0; ++ii) {
         const int i = PA0[ii];
         const double d = m3_ddot(Sg + 9 * i, t1);
-        const bool opp = (tau_tr[i] * tau_tr[j]) < 0.0;
-        A[ii * NS + jj] = mat[ML::q + i * NS + j] * h[j] + (opp ? -d : d);
+        const bool opp = ((tau_tr[i] - wtau(i)) * (tau_tr[j] - wtau(j))) < 0.0;
+        double v = mat[ML::q + i * NS + j] * h[j] + (opp ? -d : d);
+        if constexpr (KIN) {  // :302-305, added to every column of row i
+          const double k = mat[ML::c1 + i] - mat[ML::c2 + i] * W[i];
+          v += ((tau_tr[i] - W[i]) < 0.0) ? -k : k;
+        }
+        A[ii * NS + jj] = v;
       }
     }
     // PA as positions into PA0 from here on
@@ -365,7 +380,7 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
       for (int a = 0; a < ns; ++a) x[a] += xb[a];        // :119
       for (int i = 0; i < 9; ++i) XdelFp[i] = 0.0;      // :338-357
       for (int a = 0; a < ns; ++a) {
-        const double g = (tau_tr[a] > 0) ? x[a] : -x[a];
+        const double g = ((tau_tr[a] - wtau(a)) > 0) ? x[a] : -x[a];
 #pragma unroll
         for (int k = 0; k < 9; ++k) XdelFp[k] += g * Sg[9 * a + k];
       }
@@ -385,7 +400,11 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
         double sa = s[a] + h1;
         if (sa > mat[ML::ss + a]) sa = mat[ML::ss + a];
         s[a] = sa;
-        b[a] = fabs(m3_ddot(tmp, Sg + 9 * a)) - sa;
+        if constexpr (KIN) {  // :431-436
+          const double c1 = mat[ML::c1 + a], c2 = mat[ML::c2 + a];
+          W[a] += (((tau_tr[a] - W[a]) > 0) ? (c1 - c2 * W[a]) : (-c1 - c2 * W[a])) * xb[a];
+        }
+        b[a] = fabs(m3_ddot(tmp, Sg + 9 * a) - wtau(a)) - sa;
       }
     }
     n_active = npos;
@@ -465,7 +484,7 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
       for (int kk = 0; kk < nPA; ++kk) {
         const int a = PA[kk];
         const double v = m3_ddot(dtau, Sg + 9 * a);
-        rhs[kk * 9 + kl] = (tau_tr[a] < 0) ? -v : v;
+        rhs[kk * 9 + kl] = ((tau_tr[a] - wt(a)) < 0) ? -v : v;
       }
     }
     // A2 (:622-709)
@@ -486,12 +505,17 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
       m3_mul(t2, TstTau, t3);
 #pragma unroll
       for (int k = 0; k < 9; ++k) tm[k] += 2.0 * t3[k];
-      const bool nj = tau_tr[j] < 0.0;
+      const bool nj = (tau_tr[j] - wt(j)) < 0.0;
       for (int ii = 0; ii < nPA; ++ii) {
         const int i = PA[ii];
         const double d = m3_ddot(Sg + 9 * i, tm);
-        const bool ni = tau_tr[i] < 0.0;
-        A2[ii * NS + jj] = mat[ML::q + i * NS + j] * h[j] + ((ni ^ nj) ? -d : d);
+        const bool ni = (tau_tr[i] - wt(i)) < 0.0;
+        double v = mat[ML::q + i * NS + j] * h[j] + ((ni ^ nj) ? -d : d);
+        if constexpr (KIN) {  // :700-703: sign from W_kh_t, value from W_kh_tau
+          const double k = mat[ML::c1 + i] - mat[ML::c2 + i] * W[i];
+          v += ni ? -k : k;
+        }
+        A2[ii * NS + jj] = v;
       }
     }
     // delgamma = A2^-1 (delb - dels_PA) with dels = 0 (:712-722)
@@ -504,7 +528,7 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
       double diff[9], t1[9];
       dexp_series(XdelFp, Sg + 9 * j, count3, false, diff);
       m3_mul(diff, Fp_t, t1);
-      const double sg = (tau_tr[j] < 0) ? -1.0 : 1.0;
+      const double sg = ((tau_tr[j] - wt(j)) < 0) ? -1.0 : 1.0;
 #pragma unroll
       for (int r = 0; r < 9; ++r) {
         const double sv = sg * t1[r];

@@ -69,7 +69,9 @@ class userInputParameters:
         self.saturationStress1 = fl(g("Saturation Stress", ""))
         self.slipDirectionsFile1 = g("Slip Directions File", "")
         self.slipNormalsFile1 = g("Slip Normals File", "")
-        self.enableKinematicHardening1 = b("Enable Kinematic Hardening", "false")
+        self.enableKinematicHardening1 = b("Enable Kinematic Hardening", "false")  # userInputParameters.cc:274-279
+        self.C_1_slip1 = fl(g("C_1 Slip Kinematic Hardening", "")) if self.enableKinematicHardening1 else []
+        self.C_2_slip1 = fl(g("C_2 Slip Kinematic Hardening", "")) if self.enableKinematicHardening1 else []
         self.enableTwinning1 = b("Twinning enabled", "false")
         self.enableMultiphase = b("Enable Multiphase", "false")
         self.modelStressTolerance = float(g("Stress Tolerance", "-1"))
@@ -226,7 +228,8 @@ class Context:
         return row, its, res
 
     STATE = {"T": (0, 9), "Fp": (1, 9), "Fe": (2, 9), "s": (3, None), "rotnew": (4, 3), "slip": (5, None),
-             "u": (6, None), "R": (7, None), "Fp_iter": (8, 9), "Fe_iter": (9, 9), "s_iter": (10, None)}
+             "u": (6, None), "R": (7, None), "Fp_iter": (8, 9), "Fe_iter": (9, 9), "s_iter": (10, None),
+             "W": (11, None), "W_iter": (12, None)}
 
     def get_state(self, name):
         which, comps = self.STATE[name]
@@ -257,7 +260,8 @@ class Context:
         check(native.lib().pa_last_counts(self._h, ptr(c)))
         return c
 
-    def qp_update(self, F, Fp, s, grain, flag=1):
+    def qp_update(self, F, Fp, s, grain, flag=1, W=None):
+        """calculatePlasticity over a batch; W: backstress W_kh_conv (kinematic hardening only)."""
         F = f64(F).reshape(-1, 9)
         n = len(F)
         Fp = f64(Fp).reshape(n, 9)
@@ -268,13 +272,17 @@ class Context:
         out["s"] = np.zeros((n, self.n_slip))
         out["slip_inc"] = np.zeros((n, self.n_slip))
         out["n_active"] = np.zeros(n, dtype=np.int32)
+        out["W"] = np.zeros((n, self.n_slip))
+        Win = None if W is None else f64(W).reshape(n, self.n_slip)
         check(native.lib().pa_qp_update(self._h, ctypes.c_longlong(n), ptr(F), ptr(Fp), ptr(s), ptr(grain), ctypes.c_int(flag),
                                         ptr(out["P"]), ptr(out["T"]), ptr(out["dPdF"]), ptr(out["Fe"]), ptr(out["Fp"]),
-                                        ptr(out["s"]), ptr(out["slip_inc"]), ptr(out["n_active"])))
+                                        ptr(out["s"]), ptr(out["slip_inc"]), ptr(out["n_active"]), ptr(Win), ptr(out["W"])))
         return out
 
 
-def make_material(m, n, q, C, s0, h0, ss, pw, tol, max_slip_search=1, max_solver_iter=1):
+def make_material(m, n, q, C, s0, h0, ss, pw, tol, max_slip_search=1, max_solver_iter=1, C1=None, C2=None):
+    """pa_material from the prm inputs; C1, C2: kinematic hardening (C_1 / C_2 Slip Kinematic
+    Hardening), off when None."""
     mat = PaMaterial()
     ns = len(m)
     if ns > native.MAX_SLIP:
@@ -293,6 +301,10 @@ def make_material(m, n, q, C, s0, h0, ss, pw, tol, max_slip_search=1, max_solver
     mat.stress_tol = tol
     mat.max_slip_search = max_slip_search
     mat.max_solver_iter = max_solver_iter
+    if C1 is not None:
+        mat.kinematic_hardening = 1
+        for a in range(ns):
+            mat.C1[a], mat.C2[a] = C1[a], C2[a]
     return mat
 
 
@@ -321,10 +333,11 @@ class crystalPlasticity:
         m = read_vectors(ui.file(ui.slipDirectionsFile1), ns)
         n = read_vectors(ui.file(ui.slipNormalsFile1), ns)
         q = read_table(ui.file(ui.latentHardeningRatioFileName1), ns)
+        kin = ui.enableKinematicHardening1
         self.material = make_material(m, n, q, ui.elasticStiffness1, ui.initialSlipResistance1, ui.initialHardeningModulus1,
                                       ui.saturationStress1, ui.powerLawExponent1, ui.modelStressTolerance,
-                                      ui.modelMaxSlipSearchIterations, ui.modelMaxSolverIterations)
-        self.material.kinematic_hardening = int(ui.enableKinematicHardening1)
+                                      ui.modelMaxSlipSearchIterations, ui.modelMaxSolverIterations,
+                                      ui.C_1_slip1 if kin else None, ui.C_2_slip1 if kin else None)
         self.material.twinning = int(ui.enableTwinning1)
         nel = [s * 2 ** ui.meshRefineFactor for s in ui.subdivisions]
         self.totalIncrements = int(round(ui.totalTime / ui.delT))  # ellipticBVP.cc:40

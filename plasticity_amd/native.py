@@ -33,6 +33,7 @@ class PaMaterial(ctypes.Structure):
         ("ss", ctypes.c_double * MAX_SLIP), ("pw", ctypes.c_double * MAX_SLIP),
         ("stress_tol", ctypes.c_double), ("max_slip_search", ctypes.c_int), ("max_solver_iter", ctypes.c_int),
         ("kinematic_hardening", ctypes.c_int), ("twinning", ctypes.c_int),
+        ("C1", ctypes.c_double * MAX_SLIP), ("C2", ctypes.c_double * MAX_SLIP),
     ]
 
 

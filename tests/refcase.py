@@ -149,6 +149,7 @@ class Case:
         self.elem_grain = None if elem_grain is None else np.asarray(elem_grain, np.uint32)
         self.grain_rod = None if grain_rod is None else np.asarray(grain_rod, float)
         self.cyclic = (0, 1, 1, 1.0)
+        self.C1 = self.C2 = None  # kinematic hardening off
 
     # --- reference defaults for the inert twin system (userInputParameters.cc:300-311)
     def material(self, pinv_mode=1):
@@ -169,6 +170,9 @@ class Case:
             for j in range(6):
                 mat.C[i][j] = self.C[i, j]
         mat.tol, mat.max_slip_search, mat.max_solver_iter = self.tol, self.max_slip_search, self.max_solver_iter
+        if self.C1 is not None:
+            for a in range(ns):
+                mat.C1[a], mat.C2[a] = self.C1[a], self.C2[a]
         mat.pinv_mode = pinv_mode
         return mat
 
@@ -219,6 +223,11 @@ def load_reference_case(folder):
                 max_newton=int(prm["Maximum non linear iterations"]),
                 lin_tol=float(prm["Relative linear solver tolerance"]),
                 lin_maxit=int(prm["Maximum linear solver iterations"]))
+    if prm.get("Enable Kinematic Hardening", "false").strip().lower() == "true":  # userInputParameters.cc:274-279
+        case.C1 = np.array(_floats(prm["C_1 Slip Kinematic Hardening"]))
+        case.C2 = np.array(_floats(prm["C_2 Slip Kinematic Hardening"]))
+    if prm.get("Enable cyclic loading", "false").strip().lower() == "true":
+        case.cyclic = (1, int(prm["Cyclic loading face"]), int(prm["Cyclic loading direction"]), float(prm["Quarter cycle time"]))
     case.vox, case.nv = vox, nv
     case.grain_rod = rod
     case.elem_grain = oracle_element_grains(case)
@@ -287,13 +296,13 @@ def oracle_simulate(case, n_increments, pinv_mode=1, nthreads=1, want_state=Fals
     return out
 
 
-def oracle_qp(case, rot, F, Fp=None, s=None, flag=1, pinv_mode=1, slip=None):
+def oracle_qp(case, rot, F, Fp=None, s=None, flag=1, pinv_mode=1, slip=None, W=None):
     lib = oracle()
     mat = case.material(pinv_mode)
     nT, ns = case.nT, case.n_slip
     Fp = np.eye(3) if Fp is None else np.asarray(Fp, float)
     s = np.array(list(case.s0) + [10e14]) if s is None else np.asarray(s, float)
-    W = np.zeros(nT)
+    W = np.zeros(nT) if W is None else np.append(np.asarray(W, float)[:ns], np.zeros(nT - ns))
     slip_c = np.zeros(ns) if slip is None else np.asarray(slip, float)
     P, T, D, Fe, Fpo = np.zeros(9), np.zeros(9), np.zeros(81), np.zeros(9), np.zeros(9)
     so, Wo, sl = np.zeros(nT), np.zeros(nT), slip_c.copy()
@@ -303,7 +312,7 @@ def oracle_qp(case, rot, F, Fp=None, s=None, flag=1, pinv_mode=1, slip=None):
     na = lib.or_qp_update(ctypes.byref(mat), ptr(rot), ptr(F), ptr(Fp), ptr(s), ptr(W), ptr(slip_c), ctypes.c_int(flag),
                           ptr(P), ptr(T), ptr(D), ptr(Fe), ptr(Fpo), ptr(so), ptr(Wo), ptr(sl))
     return {"P": P.reshape(3, 3), "T": T.reshape(3, 3), "dPdF": D.reshape(9, 9), "Fe": Fe.reshape(3, 3),
-            "Fp": Fpo.reshape(3, 3), "s": so, "slip": sl, "n_active": na}
+            "Fp": Fpo.reshape(3, 3), "s": so, "slip": sl, "n_active": na, "W": Wo}
 
 
 def fcc_case(**kw):
