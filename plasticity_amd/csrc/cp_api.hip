@@ -817,9 +817,23 @@ __global__ void __launch_bounds__(VB) k_bicg_p(long long N, const double *__rest
   double b = 0.0;
   if (it > 0 && !krylov_next(st, upart, n_upart, it, lds, b)) return;
   const long long stride = (long long)gridDim.x * VB;
-  for (long long r = blockIdx.x * (long long)VB + threadIdx.x; r < N; r += stride) {
-    if (it == 0) { Pr[r] = Zr[r]; Pl[r] = Zl[r]; }
-    else { Pr[r] = Zr[r] + b * Pr[r]; Pl[r] = Zl[r] + b * Pl[r]; }
+  for (long long r0 = blockIdx.x * (long long)VB + threadIdx.x; r0 < N; r0 += VPT * stride) {
+    double zr[VPT], zl[VPT], pr[VPT], pl[VPT];
+#pragma unroll
+    for (int j = 0; j < VPT; ++j) {
+      const long long r = r0 + j * stride;
+      const bool ok = r < N, old = ok && it > 0;
+      zr[j] = ok ? Zr[r] : 0.0; zl[j] = ok ? Zl[r] : 0.0;
+      pr[j] = old ? Pr[r] : 0.0; pl[j] = old ? Pl[r] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < VPT; ++j) {
+      const long long r = r0 + j * stride;
+      if (r < N) {
+        Pr[r] = (it == 0) ? zr[j] : zr[j] + b * pr[j];
+        Pl[r] = (it == 0) ? zl[j] : zl[j] + b * pl[j];
+      }
+    }
   }
 }
 
@@ -869,16 +883,29 @@ __global__ void __launch_bounds__(VB) k_bicg_update(long long N, double *__restr
   const double a = st->rzh[it & 1] / dpi[0];
   const long long stride = (long long)gridDim.x * VB;
   double acc[2] = {0.0, 0.0};
-  for (long long r = blockIdx.x * (long long)VB + threadIdx.x; r < N; r += stride) {
-    const double d = dinv[r];
-    X[r] += a * Pr[r];
-    const double rr = Rr[r] - a * Zr[r];
-    const double rl = Rl[r] - a * Zl[r];
-    Rr[r] = rr; Rl[r] = rl;
-    const double zr = rr * d, zl = rl * d;
-    Zr[r] = zr; Zl[r] = zl;
-    acc[0] += zr * zr;
-    acc[1] += zr * rl;
+  for (long long r0 = blockIdx.x * (long long)VB + threadIdx.x; r0 < N; r0 += 2 * stride) {
+    // two entries per thread, all loads issued before use
+    double d[2], x[2], p[2], rr[2], rl[2], zr[2], zl[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const long long r = r0 + j * stride;
+      const bool ok = r < N;
+      d[j] = ok ? dinv[r] : 0.0; x[j] = ok ? X[r] : 0.0; p[j] = ok ? Pr[r] : 0.0;
+      rr[j] = ok ? Rr[r] : 0.0; rl[j] = ok ? Rl[r] : 0.0; zr[j] = ok ? Zr[r] : 0.0; zl[j] = ok ? Zl[r] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const long long r = r0 + j * stride;
+      if (r < N) {
+        X[r] = x[j] + a * p[j];
+        const double rrn = rr[j] - a * zr[j], rln = rl[j] - a * zl[j];
+        Rr[r] = rrn; Rl[r] = rln;
+        const double zrn = rrn * d[j], zln = rln * d[j];
+        Zr[r] = zrn; Zl[r] = zln;
+        acc[0] += zrn * zrn;
+        acc[1] += zrn * rln;
+      }
+    }
   }
   __syncthreads();
   block_partials<2>(acc, upart, lds);
@@ -924,8 +951,23 @@ __global__ void __launch_bounds__(VB) k_cg_p(long long N, const double *__restri
   if (solve_ended(st)) return;
   double b = 0.0;
   if (it > 0 && !krylov_next(st, upart, n_upart, it, lds, b)) return;
+  // VPT entries per thread strided by the grid, all loads issued before the stores
   const long long stride = (long long)gridDim.x * VB;
-  for (long long r = blockIdx.x * (long long)VB + threadIdx.x; r < N; r += stride) P[r] = (it == 0) ? Z[r] : Z[r] + b * P[r];
+  for (long long r0 = blockIdx.x * (long long)VB + threadIdx.x; r0 < N; r0 += VPT * stride) {
+    double z[VPT], p[VPT];
+#pragma unroll
+    for (int j = 0; j < VPT; ++j) {
+      const long long r = r0 + j * stride;
+      const bool ok = r < N;
+      z[j] = ok ? Z[r] : 0.0;
+      p[j] = (ok && it > 0) ? P[r] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < VPT; ++j) {
+      const long long r = r0 + j * stride;
+      if (r < N) P[r] = (it == 0) ? z[j] : z[j] + b * p[j];
+    }
+  }
 }
 
 // q = K p on the node-block matrix; block partials of p.q
