@@ -90,6 +90,65 @@ def test_qp_update_with_kinematic_hardening_matches_oracle(strain):
     ctx.close()
 
 
+def _hcp_case():
+    """hcp/simpleTension's 18 slip systems (basal, prismatic, pyramidal <a>, pyramidal <c+a>),
+    hexagonal elastic constants and hardening, twinning off, loaded 4x faster to reach plasticity
+    in 16 increments."""
+    case = refcase.load_reference_case("hcp_simpleTension")
+    case.bcs = [(5, 1, 0.0), (5, 2, 0.0), (5, 3, 0.0), (6, 3, 0.12)]
+    return case
+
+
+def test_qp_update_hcp_slip_matches_oracle():
+    """18 HCP slip systems (the 24-system kernel variant) at 1e-10 against the oracle."""
+    case = _hcp_case()
+    rng = np.random.default_rng(3)
+    rod = rng.uniform(-0.5, 0.5, size=(8, 3))
+    n = 192
+    F = np.eye(3) + rng.normal(scale=3e-3, size=(n, 3, 3))
+    Fp = np.eye(3) + rng.normal(scale=6e-4, size=(n, 3, 3))
+    Fp /= np.cbrt(np.linalg.det(Fp))[:, None, None]
+    s = np.array([rng.uniform(1.0, 1.5) * case.s0 for _ in range(n)])
+    grain = rng.integers(0, 8, size=n).astype(np.uint32)
+    ctx = _ctx(case, nel=(1, 1, 1))
+    ctx.set_grains(np.zeros(1, np.uint32), rod)
+    for flag in (0, 1):
+        g = ctx.qp_update(F.reshape(n, 9), Fp.reshape(n, 9), s, grain, flag)
+        nplastic = 0
+        for i in range(n):
+            o = refcase.oracle_qp(case, rod[grain[i]], F[i], Fp[i], np.append(s[i], 10e14), flag)
+            assert g["n_active"][i] == o["n_active"], (i, g["n_active"][i], o["n_active"])
+            nplastic += o["n_active"] > 0
+            for k in ("P", "T", "Fe", "Fp"):
+                assert _rel(g[k][i], o[k].ravel()) < 1e-10, (k, i)
+            assert _rel(g["s"][i], o["s"][:18]) < 1e-10
+            if flag:
+                assert _rel(g["dPdF"][i], o["dPdF"].ravel()) < 1e-10, (i, _rel(g["dPdF"][i], o["dPdF"].ravel()))
+        assert nplastic > n // 4
+    ctx.close()
+
+
+def test_hcp_slip_curve_matches_oracle():
+    """8^3 hcp/simpleTension mesh, grains and orientations, 18 slip systems, through yield (16
+    increments, Krylov solves to 1e-14): curve to 1e-9, per-qp stress and CRSS to 1e-9."""
+    case = _hcp_case()
+    case.lin_tol = 1e-14
+    n = 16
+    ref = refcase.oracle_simulate(case, n, nthreads=8, want_state=True)
+    ctx = _ctx(case)
+    ctx.set_grains(case.elem_grain, case.grain_rod)
+    rows = np.array([ctx.run_increment(inc)[0] for inc in range(n)])
+    assert ref["rows"][-1, 14] > 1e-4  # plastic by the end
+    for cols in (slice(0, 6), slice(6, 12)):
+        err = np.abs(rows[:, cols] - ref["rows"][:, cols]).max() / np.abs(ref["rows"][:, cols]).max()
+        assert err < 1e-9, (cols, err)
+    assert abs(rows[-1, 14] - ref["rows"][-1, 14]) / ref["rows"][-1, 14] < 1e-6
+    st = ref["state"]
+    assert _rel(ctx.get_state("T"), st[:, 0:9]) < 1e-9
+    assert _rel(ctx.get_state("s"), st[:, 27:27 + 18]) < 1e-9
+    ctx.close()
+
+
 def test_element_grains_bit_exact():
     case = refcase.load_reference_case("fcc_simpleTension")
     ctx = _ctx(case)
