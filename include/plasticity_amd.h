@@ -23,6 +23,7 @@ extern "C" {
 #endif
 
 #define PA_MAX_SLIP 48
+#define PA_MAX_TABLE 32
 
 enum {
   PA_OK = 0,
@@ -71,6 +72,10 @@ typedef struct {
   int cyclic;                            /* Enable cyclic loading */
   int cyclic_face, cyclic_dof;           /* Cyclic loading face / direction (1-based) */
   double quarter_cycle_time;             /* Quarter cycle time */
+  int tabular;                           /* Use Tabular BCs: the table drives the constrained (face,dof) rules */
+  int n_table;                           /* Number of time data for Tabular BCs (2..PA_MAX_TABLE) */
+  double table_time[PA_MAX_TABLE];       /* Tabular Time Table */
+  double table_u[6][3][PA_MAX_TABLE];    /* Tabular BC file: displacement of each rule at the table times */
 } pa_problem;
 
 typedef struct pa_context pa_context;

@@ -752,6 +752,18 @@ CSR make_csr(const Mesh &m) {
   return A;
 }
 
+// Table segment of increment inc (boundaryConditions.cc:162-170): timeCounter starts at 1 and
+// moves on while the increment's end time delT*(inc+1) lies past the table time it points at; the
+// reference re-tests it at every boundary node, which is a while loop over one increment.
+int tabular_segment(const or_problem *pb, int inc) {
+  int tc = 1;
+  for (int k = 0; k <= inc; ++k) {
+    const double t = pb->delT * (k + 1);
+    while (tc < pb->n_table - 1 && t > pb->table_time[tc]) ++tc;
+  }
+  return tc;
+}
+
 // setBoundaryValues + applyDirichletBCs (BVP boundaryConditions.cc:6-114, 452-491): constrained
 // flag and the per-increment value deluConstraint (BVP init.cc:327-348) of every dof.
 void dirichlet(const Mesh &m, const or_problem *pb, int inc, std::vector<char> &con, std::vector<double> &delu) {
@@ -759,6 +771,11 @@ void dirichlet(const Mesh &m, const or_problem *pb, int inc, std::vector<char> &
   delu.assign(m.ndof, 0.0);
   double dU[6][3];
   for (int f = 0; f < 6; ++f) for (int d = 0; d < 3; ++d) dU[f][d] = pb->face_total_u[f][d] / pb->total_increments;
+  if (pb->tabular) {  // boundaryConditions.cc:153-267: slope of the current table segment times delT
+    const int tc = tabular_segment(pb, inc);
+    for (int f = 0; f < 6; ++f) for (int d = 0; d < 3; ++d)
+      dU[f][d] = (pb->table_u[f][d][tc] - pb->table_u[f][d][tc - 1]) / (pb->table_time[tc] - pb->table_time[tc - 1]) * pb->delT;
+  }
   double cyc_sign = 1.0;
   if (pb->cyclic) {
     const int cf = pb->cyclic_face - 1, cd = pb->cyclic_dof - 1;

@@ -17,6 +17,7 @@ extern "C" {
 #endif
 
 #define OR_MAXT 49 /* BCC-48 + the inert twin system */
+#define OR_MAX_TABLE 32 /* time points of tabular boundary conditions */
 
 /* pinv_mode: how x = pinv(A_PA) b_PA is formed inside inactive_slip_removal
  *   0 = Jacobi-SVD pseudo-inverse with threshold 0 (deal.II
@@ -55,6 +56,10 @@ typedef struct {
   int cyclic_face, cyclic_dof;      /* 1-based as in prm */
   double quarter_cycle_time;
   int nthreads;                     /* OpenMP threads for the CPU baseline (<=0: all) */
+  int tabular;                      /* Use Tabular BCs: face_total_u is ignored, the table below drives */
+  int n_table;                      /* Number of time data for Tabular BCs (<= OR_MAX_TABLE) */
+  double table_time[OR_MAX_TABLE];  /* Tabular Time Table */
+  double table_u[6][3][OR_MAX_TABLE];  /* displacement of each (face,dof) rule at the table times */
 } or_problem;
 
 /* One calculatePlasticity(cellID,q,flag) call.  In/out arrays follow the

@@ -1232,6 +1232,17 @@ int set_dirichlet(pa_context *c, int inc) {
       bc.fdc[f][d] = c->pb.face_dof_constrained[f][d];
       bc.dU[f][d] = c->pb.face_total_u[f][d] / c->pb.total_increments;  // deluConstraint, init.cc:341
     }
+  if (c->pb.tabular) {  // boundaryConditions.cc:153-267: slope of the current table segment times delT
+    int tc = 1;          // timeCounter (:162-170), re-tested at every boundary node of every increment
+    for (int k = 0; k <= inc; ++k) {
+      const double t = c->pb.delT * (k + 1);
+      while (tc < c->pb.n_table - 1 && t > c->pb.table_time[tc]) ++tc;
+    }
+    for (int f = 0; f < 6; ++f)
+      for (int d = 0; d < 3; ++d)
+        bc.dU[f][d] = (c->pb.table_u[f][d][tc] - c->pb.table_u[f][d][tc - 1]) / (c->pb.table_time[tc] - c->pb.table_time[tc - 1]) *
+                      c->pb.delT;
+  }
   bc.cyclic = c->pb.cyclic;
   bc.csign = 1.0;
   if (c->pb.cyclic) {
@@ -1349,6 +1360,8 @@ int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out) {
   if (mat->twinning) return fail(PA_ERR_UNSUPPORTED, "twinning is not implemented");
   if (pb->nx < 1 || pb->ny < 1 || pb->nz < 1) return fail(PA_ERR_ARG, "mesh size must be >= 1");
   if (pb->total_increments < 1 || pb->max_newton < 1 || pb->lin_maxit < 1) return fail(PA_ERR_ARG, "bad increment/solver counts");
+  if (pb->tabular && (pb->n_table < 2 || pb->n_table > PA_MAX_TABLE)) return fail(PA_ERR_ARG, "tabular BCs need 2..PA_MAX_TABLE time points");
+  if (pb->tabular && pb->cyclic) return fail(PA_ERR_UNSUPPORTED, "tabular and cyclic boundary conditions together");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(PA_ERR_HIP, "no HIP device available");
   pa_context *c = new pa_context();

@@ -85,7 +85,14 @@ class userInputParameters:
         self.cyclicLoadingFace = int(g("Cyclic loading face", "1"))
         self.cyclicLoadingDOF = int(g("Cyclic loading direction", "1"))
         self.quarterCycleTime = float(g("Quarter cycle time", "1"))
-        for key in ("Use Periodic BCs", "Use Tabular BCs", "Use velocity gradient BC", "Use Neumann BCs",
+        self.enableTabularBCs = b("Use Tabular BCs", "false")  # userInputParameters.cc:84-88
+        self.Tabular_BCfilename = g("Tabular Boundary condition filename", "")
+        self.tabularBCs_InputStepNumber = int(g("Number of time data for Tabular BCs", "0"))
+        self.tabularNumberofBCs = int(g("Number of tabular boundary conditions", "0"))
+        self.tabularTimeInput = fl(g("Tabular Time Table", ""))
+        if self.enableTabularBCs and (self.enableSimpleBCs or self.enableCyclicLoading):
+            raise native.PlasticityError("tabular boundary conditions together with simple or cyclic ones are not implemented")
+        for key in ("Use Periodic BCs", "Use velocity gradient BC", "Use Neumann BCs",
                     "Use Indentation BCs", "Use Torsion BCs", "Flag To Use Taylor Model", "Use Element Deletion"):
             if b(key, "false"):
                 raise native.PlasticityError("'%s' is outside the implemented path" % key)
@@ -308,7 +315,9 @@ def make_material(m, n, q, C, s0, h0, ss, pw, tol, max_slip_search=1, max_solver
     return mat
 
 
-def make_problem(nel, span, total_increments, delT, bcs, max_newton=4, lin_tol=1e-10, lin_maxit=50000, cyclic=None):
+def make_problem(nel, span, total_increments, delT, bcs, max_newton=4, lin_tol=1e-10, lin_maxit=50000, cyclic=None, tabular=None):
+    """pa_problem; bcs: (face, dof, total displacement) rules; cyclic: (face, dof, quarter cycle time);
+    tabular: (times, [(face, dof, displacements at the times)])."""
     pb = PaProblem()
     pb.nx, pb.ny, pb.nz = nel
     for d in range(3):
@@ -321,6 +330,17 @@ def make_problem(nel, span, total_increments, delT, bcs, max_newton=4, lin_tol=1
     if cyclic:
         pb.cyclic = 1
         pb.cyclic_face, pb.cyclic_dof, pb.quarter_cycle_time = cyclic
+    if tabular:
+        times, rules = tabular
+        if not 2 <= len(times) <= native.MAX_TABLE:
+            raise native.PlasticityError("tabular boundary conditions need 2..%d time points" % native.MAX_TABLE)
+        pb.tabular, pb.n_table = 1, len(times)
+        for i, t in enumerate(times):
+            pb.table_time[i] = t
+        for f, d, us in rules:
+            pb.face_dof_constrained[f - 1][d - 1] = 1
+            for i, u in enumerate(us[:len(times)]):
+                pb.table_u[f - 1][d - 1][i] = u
     return pb
 
 
@@ -349,8 +369,17 @@ class crystalPlasticity:
                 v = line.split()
                 bcs.append((int(v[0]), int(v[1]), float(v[2])))
         cyc = (ui.cyclicLoadingFace, ui.cyclicLoadingDOF, ui.quarterCycleTime) if ui.enableCyclicLoading else None
+        tab = None
+        if ui.enableTabularBCs:  # init.cc:389-415: no header lines; face dof u(t_0) .. u(t_n-1) per line
+            nt, rules = ui.tabularBCs_InputStepNumber, []
+            with open(ui.file(ui.Tabular_BCfilename)) as f:
+                for line in f:
+                    v = line.split()
+                    if len(v) >= 2 + nt and len(rules) < ui.tabularNumberofBCs:
+                        rules.append((int(v[0]), int(v[1]), [float(x) for x in v[2:2 + nt]]))
+            tab = (ui.tabularTimeInput[:nt], rules)
         self.problem = make_problem(nel, ui.span, self.totalIncrements, ui.delT, bcs, ui.maxNonLinearIterations,
-                                    ui.relLinearSolverTolerance, ui.maxLinearSolverIterations, cyc)
+                                    ui.relLinearSolverTolerance, ui.maxLinearSolverIterations, cyc, tab)
         self.orientations = crystalOrientationsIO()
         self.ctx = None
 

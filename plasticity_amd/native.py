@@ -11,6 +11,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libplasticity_amd.so")
 MAX_SLIP = 48
+MAX_TABLE = 32
 
 PA_SOLVER_BICG, PA_SOLVER_CG = 0, 1
 PA_PREC_FP64, PA_PREC_MIXED = 0, 1
@@ -45,6 +46,8 @@ class PaProblem(ctypes.Structure):
         ("face_dof_constrained", ctypes.c_int * 3 * 6), ("face_total_u", ctypes.c_double * 3 * 6),
         ("cyclic", ctypes.c_int), ("cyclic_face", ctypes.c_int), ("cyclic_dof", ctypes.c_int),
         ("quarter_cycle_time", ctypes.c_double),
+        ("tabular", ctypes.c_int), ("n_table", ctypes.c_int), ("table_time", ctypes.c_double * MAX_TABLE),
+        ("table_u", ctypes.c_double * MAX_TABLE * 3 * 6),
     ]
 
 

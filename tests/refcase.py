@@ -123,6 +123,8 @@ class Problem(ctypes.Structure):
         ("face_total_u", ctypes.c_double * 3 * 6),
         ("cyclic", ctypes.c_int), ("cyclic_face", ctypes.c_int), ("cyclic_dof", ctypes.c_int),
         ("quarter_cycle_time", ctypes.c_double), ("nthreads", ctypes.c_int),
+        ("tabular", ctypes.c_int), ("n_table", ctypes.c_int), ("table_time", ctypes.c_double * 32),
+        ("table_u", ctypes.c_double * 32 * 3 * 6),
     ]
 
 
@@ -150,6 +152,7 @@ class Case:
         self.grain_rod = None if grain_rod is None else np.asarray(grain_rod, float)
         self.cyclic = (0, 1, 1, 1.0)
         self.C1 = self.C2 = None  # kinematic hardening off
+        self.tabular = None       # (times, [(face, dof, displacements)]) for tabular BCs
 
     # --- reference defaults for the inert twin system (userInputParameters.cc:300-311)
     def material(self, pinv_mode=1):
@@ -190,6 +193,15 @@ class Case:
             pb.face_total_u[f - 1][d - 1] = u
         pb.cyclic, pb.cyclic_face, pb.cyclic_dof, pb.quarter_cycle_time = self.cyclic
         pb.nthreads = nthreads
+        if self.tabular is not None:
+            times, rules = self.tabular
+            pb.tabular, pb.n_table = 1, len(times)
+            for i, t in enumerate(times):
+                pb.table_time[i] = t
+            for f, d, us in rules:
+                pb.face_dof_constrained[f - 1][d - 1] = 1
+                for i, u in enumerate(us):
+                    pb.table_u[f - 1][d - 1][i] = u
         return pb
 
     @property
@@ -213,8 +225,20 @@ def load_reference_case(folder):
     nv = [int(float(prm["Voxels in %s direction" % a])) for a in "XYZ"]
     vox = read_voxels(os.path.join(d, prm["Grain ID file name"]), int(prm.get("Header Lines GrainID File", "0")), nv)
     rod = read_orientations(os.path.join(d, prm["Orientations file name"]))
-    bcs = read_bcs(os.path.join(d, prm["Boundary condition filename"]), int(prm["BC file number of header lines"]),
-                   int(prm["Number of boundary conditions"]))
+    tabular = None
+    if prm.get("Use Tabular BCs", "false").strip().lower() == "true":  # init.cc:389-415
+        nt = int(prm["Number of time data for Tabular BCs"])
+        rules = []
+        with open(os.path.join(d, prm["Tabular Boundary condition filename"])) as f:
+            for line in f:
+                v = line.split()
+                if len(v) >= 2 + nt and len(rules) < int(prm["Number of tabular boundary conditions"]):
+                    rules.append((int(v[0]), int(v[1]), [float(x) for x in v[2:2 + nt]]))
+        tabular = (_floats(prm["Tabular Time Table"])[:nt], rules)
+        bcs = []
+    else:
+        bcs = read_bcs(os.path.join(d, prm["Boundary condition filename"]), int(prm["BC file number of header lines"]),
+                       int(prm["Number of boundary conditions"]))
     case = Case(m, n, q, C, _floats(prm["Initial Slip Resistance"]), _floats(prm["Initial Hardening Modulus"]),
                 _floats(prm["Saturation Stress"]), _floats(prm["Power Law Exponent"]),
                 tol=float(prm["Stress Tolerance"]), max_slip_search=int(prm["Max Slip Search Iterations"]),
@@ -228,6 +252,7 @@ def load_reference_case(folder):
         case.C2 = np.array(_floats(prm["C_2 Slip Kinematic Hardening"]))
     if prm.get("Enable cyclic loading", "false").strip().lower() == "true":
         case.cyclic = (1, int(prm["Cyclic loading face"]), int(prm["Cyclic loading direction"]), float(prm["Quarter cycle time"]))
+    case.tabular = tabular
     case.vox, case.nv = vox, nv
     case.grain_rod = rod
     case.elem_grain = oracle_element_grains(case)

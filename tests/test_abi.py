@@ -33,16 +33,27 @@ def test_error_path_without_gpu_is_reported():
     assert b"n_slip" in nat.lib().pa_last_error()
 
 
-def test_struct_layout_matches_header():
-    """ctypes mirrors of pa_material / pa_problem have the header's sizes (field order = header)."""
+def test_struct_layout_matches_header(tmp_path):
+    """ctypes mirrors of pa_material / pa_problem have the sizes and field offsets the C compiler
+    gives the header's structs."""
     import ctypes
-    src = open(os.path.join(ROOT, "include", "plasticity_amd.h")).read()
-    assert "double C1[PA_MAX_SLIP]" in src and "double C2[PA_MAX_SLIP]" in src
-    ms, d = nat.MAX_SLIP, ctypes.sizeof(ctypes.c_double)
-    # n_slip, m, n, q, C, s0 h0 ss pw, stress_tol, 4 ints, C1, C2 (with the int -> double padding)
-    body = 4 + 4 + (3 * ms * 2 + ms * ms + 36 + 4 * ms + 1) * d + 4 * 4
-    body += (-body) % d
-    assert ctypes.sizeof(nat.PaMaterial) == body + 2 * ms * d
+    import shutil
+    import subprocess
+    import pytest
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "plasticity_amd.h"\n'
+                   'int main(void) { printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(pa_material), sizeof(pa_problem),'
+                   ' offsetof(pa_material, C1), offsetof(pa_material, C2), offsetof(pa_problem, table_time),'
+                   ' offsetof(pa_problem, table_u)); return 0; }\n')
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    want = [ctypes.sizeof(nat.PaMaterial), ctypes.sizeof(nat.PaProblem), nat.PaMaterial.C1.offset, nat.PaMaterial.C2.offset,
+            nat.PaProblem.table_time.offset, nat.PaProblem.table_u.offset]
+    assert got == want
 
 
 def test_prm_with_kinematic_hardening_and_cyclic_loading_maps_onto_the_abi():
@@ -57,3 +68,14 @@ def test_prm_with_kinematic_hardening_and_cyclic_loading_maps_onto_the_abi():
     assert (pb.cyclic, pb.cyclic_face, pb.cyclic_dof, pb.quarter_cycle_time) == (1, 6, 3, 5.0)
     assert cp.totalIncrements == 4000
     assert pb.face_dof_constrained[5][2] == 1 and pb.face_total_u[5][2] == -0.05
+
+
+def test_prm_with_tabular_bcs_maps_onto_the_abi():
+    """fcc/tabularSimpleCompression's prm.prm: the table reaches pa_problem (init.cc:389-415)."""
+    from plasticity_amd import model
+    ui = model.userInputParameters(os.path.join(ROOT, "tests", "golden", "fcc_tabularSimpleCompression", "prm.prm"))
+    pb = model.crystalPlasticity(ui).problem
+    assert (pb.tabular, pb.n_table) == (1, 3)
+    assert list(pb.table_time[:3]) == [0.0, 20.0, 100.0]
+    assert list(pb.table_u[1][0][:3]) == [0.0, -0.2, -0.2]
+    assert [pb.face_dof_constrained[0][d] for d in range(3)] == [1, 1, 1] and pb.face_dof_constrained[1][0] == 1

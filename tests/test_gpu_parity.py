@@ -16,7 +16,8 @@ def _ctx(case, nel=None, n_grain=None):
     mat = pa.make_material(case.m, case.n, case.q, case.C, case.s0, case.h0, case.ss, case.pw, case.tol,
                            case.max_slip_search, case.max_solver_iter, case.C1, case.C2)
     pb = pa.make_problem(nel or (case.nx, case.ny, case.nz), case.span, case.total_increments, case.delT, case.bcs,
-                         case.max_newton, case.lin_tol, case.lin_maxit, case.cyclic[1:] if case.cyclic[0] else None)
+                         case.max_newton, case.lin_tol, case.lin_maxit, case.cyclic[1:] if case.cyclic[0] else None,
+                         case.tabular)
     return pa.Context(mat, pb)
 
 
@@ -229,7 +230,8 @@ def test_newton_curve_matches_oracle_fcc_tension(lin_tol, row_tol, state_tol):
 
 
 @pytest.mark.parametrize("folder,n", [("fcc_simpleTension", 18), ("fcc_compression", 13), ("bcc_simpleTension", 30),
-                                      ("fcc_FCC_RandomOrientationBlock", 12), ("fcc_CyclicLoading", 13)])
+                                      ("fcc_FCC_RandomOrientationBlock", 12), ("fcc_CyclicLoading", 13),
+                                      ("fcc_tabularSimpleCompression", 16)])
 def test_reference_golden_curve(folder, n):
     """The reference's own inputs (mesh, grain map, orientations, slip systems, BCs, 1e-10 BiCG) run
     through the GPU path reproduce its golden stress-strain rows (tests/compile_tests/*/
