@@ -76,6 +76,8 @@ typedef struct {
   int n_table;                           /* Number of time data for Tabular BCs (2..PA_MAX_TABLE) */
   double table_time[PA_MAX_TABLE];       /* Tabular Time Table */
   double table_u[6][3][PA_MAX_TABLE];    /* Tabular BC file: displacement of each rule at the table times */
+  int velgrad;                           /* Use velocity gradient BC: every boundary dof follows F */
+  double vel_grad[3][3];                 /* Velocity gradient rows 1-3 (L) */
 } pa_problem;
 
 typedef struct pa_context pa_context;

@@ -776,6 +776,21 @@ void dirichlet(const Mesh &m, const or_problem *pb, int inc, std::vector<char> &
     for (int f = 0; f < 6; ++f) for (int d = 0; d < 3; ++d)
       dU[f][d] = (pb->table_u[f][d][tc] - pb->table_u[f][d][tc - 1]) / (pb->table_time[tc] - pb->table_time[tc - 1]) * pb->delT;
   }
+  // velocity-gradient BC (updateBeforeIncrement.cc:10-36, boundaryConditions.cc:269-275): F_(k+1) =
+  // (I - L dt)^-1 F_k from F_0 = I, and every boundary dof moves by (F_(inc+1) - F_inc) X at
+  // iteration 0
+  M3 dF = zero3();
+  if (pb->velgrad) {
+    M3 IL = eye3();
+    for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) IL[a][b] -= pb->vel_grad[a][b] * pb->delT;
+    const M3 Minv = inv3(IL);
+    M3 Fk = eye3();
+    for (int k = 0; k <= inc; ++k) {
+      const M3 Fn = mul(Minv, Fk);
+      for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) dF[a][b] = Fn[a][b] - Fk[a][b];
+      Fk = Fn;
+    }
+  }
   double cyc_sign = 1.0;
   if (pb->cyclic) {
     const int cf = pb->cyclic_face - 1, cd = pb->cyclic_dof - 1;
@@ -795,6 +810,11 @@ void dirichlet(const Mesh &m, const or_problem *pb, int inc, std::vector<char> &
       } else {
         for (int f = 0; f < 6; ++f)
           if (pb->face_dof_constrained[f][d] && onb[f]) { flag = true; val = dU[f][d]; break; }
+      }
+      if (!flag && pb->velgrad) {  // the last rule of setBoundaryValues: every boundary dof
+        const double X[3] = {i * m.h[0], j * m.h[1], k * m.h[2]};
+        flag = true;
+        for (int c = 0; c < 3; ++c) val += dF[d][c] * X[c];
       }
       if (flag) { con[3 * n + d] = 1; delu[3 * n + d] = val; }
     }

@@ -60,6 +60,8 @@ typedef struct {
   int n_table;                      /* Number of time data for Tabular BCs (<= OR_MAX_TABLE) */
   double table_time[OR_MAX_TABLE];  /* Tabular Time Table */
   double table_u[6][3][OR_MAX_TABLE];  /* displacement of each (face,dof) rule at the table times */
+  int velgrad;                      /* Use velocity gradient BC */
+  double vel_grad[3][3];            /* L */
 } or_problem;
 
 /* One calculatePlasticity(cellID,q,flag) call.  In/out arrays follow the

@@ -32,6 +32,25 @@ int fail(int code, const std::string &msg) {
   return code;
 }
 
+// Inverse of a 3x3 matrix by cofactors; false when the determinant is zero.
+bool host_inv3(const double A[3][3], double R[3][3]) {
+  const double c00 = A[1][1] * A[2][2] - A[1][2] * A[2][1], c01 = A[1][2] * A[2][0] - A[1][0] * A[2][2],
+               c02 = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+  const double det = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
+  if (det == 0.0) return false;
+  const double id = 1.0 / det;
+  R[0][0] = c00 * id;
+  R[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) * id;
+  R[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) * id;
+  R[1][0] = c01 * id;
+  R[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) * id;
+  R[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) * id;
+  R[2][0] = c02 * id;
+  R[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) * id;
+  R[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) * id;
+  return true;
+}
+
 #define HIPCHK(x)                                                                          \
   do {                                                                                     \
     hipError_t e_ = (x);                                                                   \
@@ -220,13 +239,16 @@ __global__ void k_csr_cols(MeshDims m, const long long *__restrict__ rp, int *__
 }
 
 // ------------------------------------------------------------ Dirichlet lines
-// setBoundaryValues + applyDirichletBCs (boundaryConditions.cc:6-114, 452-491): the first
-// matching face rule wins; cyclic rule first when enabled.
+// setBoundaryValues + applyDirichletBCs (boundaryConditions.cc:6-114, 269-275, 452-491): the first
+// matching face rule wins; cyclic rule first when enabled; the velocity-gradient rule, when
+// enabled, takes every boundary dof no face rule took, with value dF X at the node's position X.
 struct BCArgs {
   int fdc[6][3];
   double dU[6][3];
   int cyclic, cface, cdof;
   double csign;
+  int velgrad;
+  double dF[3][3], h[3];
 };
 __global__ void k_dirichlet(MeshDims m, BCArgs bc, unsigned char *__restrict__ con, double *__restrict__ delu) {
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -245,6 +267,10 @@ __global__ void k_dirichlet(MeshDims m, BCArgs bc, unsigned char *__restrict__ c
       } else {
         for (int f = 0; f < 6; ++f)
           if (bc.fdc[f][d] && onb[f]) { flag = true; val = bc.dU[f][d]; break; }
+      }
+      if (!flag && bc.velgrad) {
+        flag = true;
+        val = bc.dF[d][0] * (i * bc.h[0]) + bc.dF[d][1] * (j * bc.h[1]) + bc.dF[d][2] * (k * bc.h[2]);
       }
     }
     con[3 * n + d] = flag ? 1 : 0;
@@ -1252,6 +1278,21 @@ int set_dirichlet(pa_context *c, int inc) {
     const double qt = c->pb.quarter_cycle_time, t = std::fmod(inc * c->pb.delT, 4 * qt);
     bc.csign = (t < qt) ? 1.0 : (t < 3 * qt ? -1.0 : 1.0);
   }
+  bc.velgrad = c->pb.velgrad;
+  if (c->pb.velgrad) {  // updateBeforeIncrement.cc:10-32: F_(k+1) = (I - L delT)^-1 F_k from F_0 = I
+    double A[3][3], Minv[3][3], F[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) A[a][b] = (a == b ? 1.0 : 0.0) - c->pb.vel_grad[a][b] * c->pb.delT;
+    if (!host_inv3(A, Minv)) return fail(PA_ERR_ARG, "I - L*delT is singular");
+    for (int k = 0; k <= inc; ++k) {
+      double Fn[3][3];
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) Fn[a][b] = Minv[a][0] * F[0][b] + Minv[a][1] * F[1][b] + Minv[a][2] * F[2][b];
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) { bc.dF[a][b] = Fn[a][b] - F[a][b]; F[a][b] = Fn[a][b]; }
+    }
+    for (int d = 0; d < 3; ++d) bc.h[d] = c->h[d];
+  }
   hipLaunchKernelGGL(k_dirichlet, dim3(nblk(c->m.nnode, 256)), dim3(256), 0, c->stream, c->m, bc, c->con, c->delu);
   HIPCHK(hipGetLastError());
   c->bc_inc = inc;
@@ -1362,6 +1403,12 @@ int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out) {
   if (pb->total_increments < 1 || pb->max_newton < 1 || pb->lin_maxit < 1) return fail(PA_ERR_ARG, "bad increment/solver counts");
   if (pb->tabular && (pb->n_table < 2 || pb->n_table > PA_MAX_TABLE)) return fail(PA_ERR_ARG, "tabular BCs need 2..PA_MAX_TABLE time points");
   if (pb->tabular && pb->cyclic) return fail(PA_ERR_UNSUPPORTED, "tabular and cyclic boundary conditions together");
+  if (pb->velgrad) {
+    double A[3][3], R[3][3];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) A[a][b] = (a == b ? 1.0 : 0.0) - pb->vel_grad[a][b] * pb->delT;
+    if (!host_inv3(A, R)) return fail(PA_ERR_ARG, "velocity gradient BC: I - L*delT is singular");
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(PA_ERR_HIP, "no HIP device available");
   pa_context *c = new pa_context();

@@ -92,7 +92,11 @@ class userInputParameters:
         self.tabularTimeInput = fl(g("Tabular Time Table", ""))
         if self.enableTabularBCs and (self.enableSimpleBCs or self.enableCyclicLoading):
             raise native.PlasticityError("tabular boundary conditions together with simple or cyclic ones are not implemented")
-        for key in ("Use Periodic BCs", "Use velocity gradient BC", "Use Neumann BCs",
+        self.useVelocityGrad = b("Use velocity gradient BC", "false")  # userInputParameters.cc:78-83
+        self.targetVelGrad = [fl(g("Velocity gradient row %d" % (i + 1), "0, 0, 0")) for i in range(3)]
+        if self.useVelocityGrad and any(len(r) != 3 for r in self.targetVelGrad):
+            raise native.PlasticityError("'Velocity gradient row' needs three values")
+        for key in ("Use Periodic BCs", "Use Neumann BCs",
                     "Use Indentation BCs", "Use Torsion BCs", "Flag To Use Taylor Model", "Use Element Deletion"):
             if b(key, "false"):
                 raise native.PlasticityError("'%s' is outside the implemented path" % key)
@@ -315,9 +319,11 @@ def make_material(m, n, q, C, s0, h0, ss, pw, tol, max_slip_search=1, max_solver
     return mat
 
 
-def make_problem(nel, span, total_increments, delT, bcs, max_newton=4, lin_tol=1e-10, lin_maxit=50000, cyclic=None, tabular=None):
+def make_problem(nel, span, total_increments, delT, bcs, max_newton=4, lin_tol=1e-10, lin_maxit=50000, cyclic=None, tabular=None,
+                 vel_grad=None):
     """pa_problem; bcs: (face, dof, total displacement) rules; cyclic: (face, dof, quarter cycle time);
-    tabular: (times, [(face, dof, displacements at the times)])."""
+    tabular: (times, [(face, dof, displacements at the times)]); vel_grad: 3x3 velocity gradient L,
+    which moves every boundary dof no face rule constrains by (F_(n+1) - F_n) X."""
     pb = PaProblem()
     pb.nx, pb.ny, pb.nz = nel
     for d in range(3):
@@ -341,6 +347,11 @@ def make_problem(nel, span, total_increments, delT, bcs, max_newton=4, lin_tol=1
             pb.face_dof_constrained[f - 1][d - 1] = 1
             for i, u in enumerate(us[:len(times)]):
                 pb.table_u[f - 1][d - 1][i] = u
+    if vel_grad is not None:
+        pb.velgrad = 1
+        for a in range(3):
+            for b in range(3):
+                pb.vel_grad[a][b] = vel_grad[a][b]
     return pb
 
 
@@ -379,7 +390,8 @@ class crystalPlasticity:
                         rules.append((int(v[0]), int(v[1]), [float(x) for x in v[2:2 + nt]]))
             tab = (ui.tabularTimeInput[:nt], rules)
         self.problem = make_problem(nel, ui.span, self.totalIncrements, ui.delT, bcs, ui.maxNonLinearIterations,
-                                    ui.relLinearSolverTolerance, ui.maxLinearSolverIterations, cyc, tab)
+                                    ui.relLinearSolverTolerance, ui.maxLinearSolverIterations, cyc, tab,
+                                    ui.targetVelGrad if ui.useVelocityGrad else None)
         self.orientations = crystalOrientationsIO()
         self.ctx = None
 

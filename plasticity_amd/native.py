@@ -48,6 +48,7 @@ class PaProblem(ctypes.Structure):
         ("quarter_cycle_time", ctypes.c_double),
         ("tabular", ctypes.c_int), ("n_table", ctypes.c_int), ("table_time", ctypes.c_double * MAX_TABLE),
         ("table_u", ctypes.c_double * MAX_TABLE * 3 * 6),
+        ("velgrad", ctypes.c_int), ("vel_grad", ctypes.c_double * 3 * 3),
     ]
 
 

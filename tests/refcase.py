@@ -125,6 +125,7 @@ class Problem(ctypes.Structure):
         ("quarter_cycle_time", ctypes.c_double), ("nthreads", ctypes.c_int),
         ("tabular", ctypes.c_int), ("n_table", ctypes.c_int), ("table_time", ctypes.c_double * 32),
         ("table_u", ctypes.c_double * 32 * 3 * 6),
+        ("velgrad", ctypes.c_int), ("vel_grad", ctypes.c_double * 3 * 3),
     ]
 
 
@@ -153,6 +154,7 @@ class Case:
         self.cyclic = (0, 1, 1, 1.0)
         self.C1 = self.C2 = None  # kinematic hardening off
         self.tabular = None       # (times, [(face, dof, displacements)]) for tabular BCs
+        self.vel_grad = None      # 3x3 velocity gradient L for velocity-gradient BCs
 
     # --- reference defaults for the inert twin system (userInputParameters.cc:300-311)
     def material(self, pinv_mode=1):
@@ -202,11 +204,20 @@ class Case:
                 pb.face_dof_constrained[f - 1][d - 1] = 1
                 for i, u in enumerate(us):
                     pb.table_u[f - 1][d - 1][i] = u
+        if self.vel_grad is not None:
+            pb.velgrad = 1
+            for a in range(3):
+                for b in range(3):
+                    pb.vel_grad[a][b] = self.vel_grad[a][b]
         return pb
 
     @property
     def nT(self):
         return self.n_slip + 1
+
+
+def _true(prm, key):
+    return prm.get(key, "false").strip().lower() == "true"
 
 
 def load_reference_case(folder):
@@ -236,6 +247,8 @@ def load_reference_case(folder):
                     rules.append((int(v[0]), int(v[1]), [float(x) for x in v[2:2 + nt]]))
         tabular = (_floats(prm["Tabular Time Table"])[:nt], rules)
         bcs = []
+    elif _true(prm, "Use velocity gradient BC"):  # userInputParameters.cc:78
+        bcs = []
     else:
         bcs = read_bcs(os.path.join(d, prm["Boundary condition filename"]), int(prm["BC file number of header lines"]),
                        int(prm["Number of boundary conditions"]))
@@ -253,6 +266,8 @@ def load_reference_case(folder):
     if prm.get("Enable cyclic loading", "false").strip().lower() == "true":
         case.cyclic = (1, int(prm["Cyclic loading face"]), int(prm["Cyclic loading direction"]), float(prm["Quarter cycle time"]))
     case.tabular = tabular
+    if _true(prm, "Use velocity gradient BC"):
+        case.vel_grad = np.array([_floats(prm["Velocity gradient row %d" % (i + 1)]) for i in range(3)])
     case.vox, case.nv = vox, nv
     case.grain_rod = rod
     case.elem_grain = oracle_element_grains(case)
@@ -353,3 +368,25 @@ def fcc_case(**kw):
     args = dict(s0=[16.0] * 12, h0=[180.0] * 12, ss=[148.0] * 12, pw=[2.25] * 12)
     args.update(kw)
     return Case(m, n, q, C, **args)
+
+
+def velgrad_case():
+    """A single rotated FCC grain (elastic: s0 = 1e6) on a 4x3x5 mesh of span (1, 0.8, 1.2) driven by a
+    non-symmetric, non-nilpotent velocity gradient.  Under all-boundary affine BCs a homogeneous body
+    deforms affinely, so after n increments every node sits at u = (F_n - I) X."""
+    case = fcc_case(s0=[1e6] * 12, nx=4, ny=3, nz=5, span=(1.0, 0.8, 1.2), delT=1.0, total_time=10.0,
+                    bcs=(), lin_tol=1e-14)
+    case.vel_grad = np.array([[0.012, 0.02, -0.004], [0.003, -0.006, 0.008], [-0.005, 0.001, 0.009]])
+    case.grain_rod = np.array([[0.21, -0.13, 0.34]])
+    case.elem_grain = np.zeros(case.nx * case.ny * case.nz, np.uint32)
+    return case
+
+
+def velgrad_affine_u(case, n):
+    """(F_n - I) X at every node, F_(k+1) = (I - L delT)^-1 F_k from F_0 = I (updateBeforeIncrement.cc:14-31)."""
+    M = np.linalg.inv(np.eye(3) - case.vel_grad * case.delT)
+    F = np.linalg.matrix_power(M, n)
+    h = np.array(case.span) / np.array([case.nx, case.ny, case.nz])
+    k, j, i = np.meshgrid(np.arange(case.nz + 1), np.arange(case.ny + 1), np.arange(case.nx + 1), indexing="ij")
+    X = np.stack([i.ravel() * h[0], j.ravel() * h[1], k.ravel() * h[2]], axis=1)  # node = i + nnx (j + nny k)
+    return (X @ (F - np.eye(3)).T).ravel()

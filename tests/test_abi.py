@@ -47,12 +47,14 @@ def test_struct_layout_matches_header(tmp_path):
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "plasticity_amd.h"\n'
                    'int main(void) { printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(pa_material), sizeof(pa_problem),'
                    ' offsetof(pa_material, C1), offsetof(pa_material, C2), offsetof(pa_problem, table_time),'
-                   ' offsetof(pa_problem, table_u)); return 0; }\n')
+                   ' offsetof(pa_problem, table_u)); printf("%zu %zu\\n", offsetof(pa_problem, velgrad),'
+                   ' offsetof(pa_problem, vel_grad)); return 0; }\n')
     exe = tmp_path / "layout"
     subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
     want = [ctypes.sizeof(nat.PaMaterial), ctypes.sizeof(nat.PaProblem), nat.PaMaterial.C1.offset, nat.PaMaterial.C2.offset,
-            nat.PaProblem.table_time.offset, nat.PaProblem.table_u.offset]
+            nat.PaProblem.table_time.offset, nat.PaProblem.table_u.offset, nat.PaProblem.velgrad.offset,
+            nat.PaProblem.vel_grad.offset]
     assert got == want
 
 
@@ -79,3 +81,16 @@ def test_prm_with_tabular_bcs_maps_onto_the_abi():
     assert list(pb.table_time[:3]) == [0.0, 20.0, 100.0]
     assert list(pb.table_u[1][0][:3]) == [0.0, -0.2, -0.2]
     assert [pb.face_dof_constrained[0][d] for d in range(3)] == [1, 1, 1] and pb.face_dof_constrained[1][0] == 1
+
+
+def test_prm_with_velocity_gradient_bc_maps_onto_the_abi():
+    """fcc/DeformationGraidientBCs's prm.prm: L reaches pa_problem (userInputParameters.cc:78-83) and no
+    face rule is set (Use Simple BCs = false)."""
+    from plasticity_amd import model
+    ui = model.userInputParameters(os.path.join(ROOT, "tests", "golden", "fcc_DeformationGraidientBCs", "prm.prm"))
+    cp = model.crystalPlasticity(ui)
+    pb = cp.problem
+    assert pb.velgrad == 1
+    assert [[pb.vel_grad[a][b] for b in range(3)] for a in range(3)] == [[-0.005, 0, 0], [0, -0.005, 0], [0, 0, 0.01]]
+    assert all(pb.face_dof_constrained[f][d] == 0 for f in range(6) for d in range(3))
+    assert cp.totalIncrements == 1000 and (pb.nx, pb.ny, pb.nz) == (8, 8, 8)

@@ -17,7 +17,7 @@ def _ctx(case, nel=None, n_grain=None):
                            case.max_slip_search, case.max_solver_iter, case.C1, case.C2)
     pb = pa.make_problem(nel or (case.nx, case.ny, case.nz), case.span, case.total_increments, case.delT, case.bcs,
                          case.max_newton, case.lin_tol, case.lin_maxit, case.cyclic[1:] if case.cyclic[0] else None,
-                         case.tabular)
+                         case.tabular, case.vel_grad)
     return pa.Context(mat, pb)
 
 
@@ -231,7 +231,7 @@ def test_newton_curve_matches_oracle_fcc_tension(lin_tol, row_tol, state_tol):
 
 @pytest.mark.parametrize("folder,n", [("fcc_simpleTension", 18), ("fcc_compression", 13), ("bcc_simpleTension", 30),
                                       ("fcc_FCC_RandomOrientationBlock", 12), ("fcc_CyclicLoading", 13),
-                                      ("fcc_tabularSimpleCompression", 16)])
+                                      ("fcc_tabularSimpleCompression", 16), ("fcc_DeformationGraidientBCs", 17)])
 def test_reference_golden_curve(folder, n):
     """The reference's own inputs (mesh, grain map, orientations, slip systems, BCs, 1e-10 BiCG) run
     through the GPU path reproduce its golden stress-strain rows (tests/compile_tests/*/
@@ -251,6 +251,22 @@ def test_reference_golden_curve(folder, n):
     if slip.any():
         assert (np.abs(rows[slip, 14] - gold[slip, 14]) / gold[slip, 14]).max() < 5e-5
     assert (rows[~slip, 14] < 1e-6).all()
+
+
+def test_velocity_gradient_bc_affine_field_and_oracle():
+    """Non-symmetric velocity gradient on a homogeneous grain: the GPU field is the affine (F_n - I) X at
+    every node and matches the oracle's field and rows."""
+    case = refcase.velgrad_case()
+    n = 3
+    ref = refcase.oracle_simulate(case, n, nthreads=4, want_state=True)
+    ctx = _ctx(case)
+    ctx.set_grains(case.elem_grain, case.grain_rod)
+    rows = np.array([ctx.run_increment(inc)[0] for inc in range(n)])
+    u = ctx.get_state("u")
+    ctx.close()
+    assert _rel(u, refcase.velgrad_affine_u(case, n)) < 1e-9
+    assert _rel(u, ref["u"]) < 1e-10
+    assert _rel(rows[:, :12], ref["rows"][:, :12]) < 1e-9
 
 
 def test_first_newton_iteration_stagewise():

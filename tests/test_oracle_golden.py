@@ -29,7 +29,7 @@ def _vm_stress(a):
 # increments before the first rank-deficient active set perturbs the reference's own curve
 @pytest.mark.parametrize("folder,n", [("fcc_simpleTension", 18), ("fcc_compression", 13), ("bcc_simpleTension", 30),
                                       ("fcc_FCC_RandomOrientationBlock", 12), ("fcc_CyclicLoading", 13),
-                                      ("fcc_tabularSimpleCompression", 16)])
+                                      ("fcc_tabularSimpleCompression", 16), ("fcc_DeformationGraidientBCs", 17)])
 def test_oracle_matches_golden_to_print_precision(folder, n):
     case = refcase.load_reference_case(folder)
     out = refcase.oracle_simulate(case, n, nthreads=8)
