@@ -63,7 +63,7 @@ def main():
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--grains", type=int, default=256)
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
-    ap.add_argument("--solver", choices=["bicg", "cg", "both"], default="both")
+    ap.add_argument("--solver", choices=["bicg", "cg", "mgcg", "all"], default="all")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -78,7 +78,7 @@ def main():
     assert torch.cuda.is_available(), "no GPU"
     n = args.n
     forms = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
-    solvers = ["bicg", "cg"] if args.solver == "both" else [args.solver]
+    solvers = ["bicg", "cg", "mgcg"] if args.solver == "all" else [args.solver]
     results = {}
     for solver in solvers:
         for form in forms:
@@ -108,7 +108,7 @@ def main():
     }
     keys = ("ms_per_step", "qp_updates_per_s", "krylov_iters_per_step", "assembly_ms_per_step", "krylov_ms_per_step",
             "post_ms_per_increment", "qp_tangent_ms", "element_ms", "operator_us", "update_us", "update_GBps", "p_us",
-            "singular_local_solves_per_step")
+            "singular_local_solves_per_step", "rerun_lane_solves")
     for (form, solver), r in results.items():
         out["config"]["%s_%s" % (form, solver)] = {k: r[k] for k in keys}
     if len(forms) == 2:
@@ -134,7 +134,7 @@ def run_form(args, form, solver, pa, native, synth, torch):
     ctx = pa.Context(mat, pb)
     eg = ctx.element_grains(vox)
     ctx.set_grains(eg, rod)
-    ctx.set_solver(native.PA_SOLVER_CG if solver == "cg" else native.PA_SOLVER_BICG,
+    ctx.set_solver({"bicg": native.PA_SOLVER_BICG, "cg": native.PA_SOLVER_CG, "mgcg": native.PA_SOLVER_MGCG}[solver],
                    native.PA_PREC_MIXED if form == "mixed" else native.PA_PREC_FP64)
     nmax = pb.max_newton
     state = {"step": 0}
@@ -213,6 +213,7 @@ def run_form(args, form, solver, pa, native, synth, torch):
                      "traffic_source": "rocprofv3 FETCH_SIZE x2 + WRITE_SIZE (profiles/k_cg_spmv_mixed_64cube_fcc.md)"
                                        if traffic else "not measured for this kernel/size"},
     }
+    res["rerun_lane_solves"] = int(ctx.last_counts()[2])
     res["T"] = ctx.get_state("T")
     res["s"] = ctx.get_state("s")
     res["u"] = ctx.get_state("u")

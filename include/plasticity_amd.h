@@ -34,7 +34,7 @@ enum {
   PA_ERR_STATE = 5        /* call out of order (e.g. no grains set) */
 };
 
-enum { PA_SOLVER_BICG = 0, PA_SOLVER_CG = 1 };
+enum { PA_SOLVER_BICG = 0, PA_SOLVER_CG = 1, PA_SOLVER_MGCG = 2 };
 enum { PA_PREC_FP64 = 0, PA_PREC_MIXED = 1 };
 
 /* Single-phase material tables, as crystalPlasticity<dim>::init() builds them from the prm
@@ -154,8 +154,10 @@ int pa_last_timings(pa_context *ctx, double ms[3]);
 
 /* Instrumentation (no counterpart in the reference): quadrature points whose local active-set
  * matrix was numerically singular and went through the pseudo-inverse pass, in the last assembly
- * (counts[0]) and the last post-increment pass (counts[1]). */
-int pa_last_counts(pa_context *ctx, long long counts[2]);
+ * (counts[0]) and the last post-increment pass (counts[1]); counts[2]: reruns, since the context
+ * was created, that met a singular system other than the one solved for them and solved it in
+ * their own lane. */
+int pa_last_counts(pa_context *ctx, long long counts[3]);
 
 /* Instrumentation (no counterpart in the reference): times `reps` launches of one kernel
  * between HIP events on the context stream, on the context's current data.  which:

@@ -278,6 +278,216 @@ __global__ void k_dirichlet(MeshDims m, BCArgs bc, unsigned char *__restrict__ c
   }
 }
 
+// ------------------------------------------------------------ singular local systems
+// The pseudo-inverse branch of inactive-slip removal (inactiveSlipRemoval.cc:26,99), for the
+// qps whose A_PA LU finds numerically singular.  Pass 1 runs every qp on the LU path; such a qp
+// claims entry e of the deferred queue and, below the store's capacity, leaves its system
+// (A_PA, b_PA, n) in the store.  Then SING_ROUNDS rounds of: k_sing_svd solves every pending
+// system with a group of lanes per system (lane = row, U and V in LDS) and appends x to the
+// entry's solutions; a replay kernel reruns each pending qp, feeding the stored solutions to its
+// singular solves in order, and either finishes it or leaves its next singular system.  Entries
+// still pending after the rounds, or above capacity, are finished with the pseudo-inverse in
+// their own lane.  Every path computes the same x (same Jacobi sweep order and rules).
+constexpr int SING_ROUNDS = 4;
+struct SingStore {
+  unsigned cap;  // entries with storage
+  double *A;     // [cap][NS*NS] A_PA, leading dimension NS
+  double *b;     // [cap][NS]
+  int *n;        // [cap]
+  double *X;     // [cap][SING_ROUNDS][NS] solutions in the order the qp meets its singular systems
+  unsigned long long *key;  // [cap][SING_ROUNDS] hash of the bits of each solved system
+  int *have;     // [cap] solutions stored
+  int *done;     // [cap] 1 once the qp is finished
+  unsigned *miss;  // replays whose system differed from the stored one (solved in their lane)
+};
+// FNV-1a over the bit patterns of n, A_PA and b_PA
+template <int NS>
+__device__ __forceinline__ unsigned long long sing_key(int n, const double *A, const double *b) {
+  unsigned long long h = 1469598103934665603ull ^ (unsigned long long)n;
+  auto mix = [&](double v) { h = (h ^ (unsigned long long)__double_as_longlong(v)) * 1099511628211ull; };
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < n; ++j) mix(A[i * NS + j]);
+    mix(b[i]);
+  }
+  return h;
+}
+template <int NS>
+__device__ __forceinline__ void sing_write(const SingStore &st, unsigned e, int round, int n, const double *A, const double *b) {
+  double *dA = st.A + (size_t)e * NS * NS;
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < n; ++j) dA[i * NS + j] = A[i * NS + j];
+    st.b[(size_t)e * NS + i] = b[i];
+  }
+  st.n[e] = n;
+  if (round < SING_ROUNDS) st.key[(size_t)e * SING_ROUNDS + round] = sing_key<NS>(n, A, b);
+}
+// What a qp does at a numerically singular A_PA, by pass: queue it (pass 1), replay the stored
+// solutions (rounds), or solve in its own lane (the last pass).  One functor type for all passes
+// means one compiled qp_update, so a rerun meets bit-identical systems in the same order; a
+// stored solution is still used only for the system it was computed from (checked by hash).
+enum { SING_CLAIM = 0, SING_REPLAY = 1, SING_LANE = 2 };
+template <int NS>
+struct SingAny {
+  int mode;
+  SingStore st;
+  unsigned *count, *queue;
+  unsigned Q, e;
+  int have, k;
+  __device__ bool operator()(int n, const double *A, const double *b, double *x) {
+    if (mode == SING_LANE) {
+      cp::svd_pinv_solve<NS>(n, A, b, x);
+      return true;
+    }
+    if (mode == SING_CLAIM) {
+      const unsigned ee = atomicAdd(count, 1u);
+      queue[ee] = Q;
+      if (ee < st.cap) {
+        sing_write<NS>(st, ee, 0, n, A, b);
+        st.have[ee] = 0;
+        st.done[ee] = 0;
+      }
+      return false;
+    }
+    if (k < have) {
+      if (sing_key<NS>(n, A, b) == st.key[(size_t)e * SING_ROUNDS + k]) {
+        const double *src = st.X + ((size_t)e * SING_ROUNDS + k) * NS;
+        for (int i = 0; i < n; ++i) x[i] = src[i];
+      } else {
+        atomicAdd(st.miss, 1u);
+        cp::svd_pinv_solve<NS>(n, A, b, x);
+      }
+      ++k;
+      return true;
+    }
+    sing_write<NS>(st, e, k, n, A, b);
+    return false;
+  }
+};
+
+template <int NS>
+struct SingRef {
+  SingAny<NS> *f;
+  __device__ bool operator()(int n, const double *A, const double *b, double *x) const { return (*f)(n, A, b, x); }
+};
+
+template <int NS>
+constexpr int sing_group() { return NS <= 16 ? 16 : (NS <= 32 ? 32 : 64); }
+// systems per block of k_sing_svd: U and V take 2 NS^2 doubles of LDS each
+template <int NS>
+constexpr int sing_per_block() { return NS <= 24 ? 4 : 1; }
+
+// Orders this lane's LDS writes before the group's next reads: the lanes of a group are in one
+// wavefront, which executes its LDS instructions in order; the fences keep the compiler from
+// moving accesses across this point.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// x = pinv(A_PA) b_PA for every pending entry: svd_pinv_solve (cp_qp.h) with the rows spread
+// over a group of lanes (lane i owns row i of U and V).  Every sum runs in the in-lane order
+// (each lane reads the whole column from LDS), so the result is the one the in-lane routine
+// gives.  Control flow is uniform within a group; a group sits inside one wavefront.
+template <int NS>
+__global__ void __launch_bounds__(sing_per_block<NS>() * sing_group<NS>()) k_sing_svd(SingStore st, const unsigned *count) {
+  constexpr int G = sing_group<NS>(), SPB = sing_per_block<NS>();
+  __shared__ double lds[SPB][2 * NS * NS + NS];
+  const int grp = threadIdx.x / G, i = threadIdx.x % G;
+  double *U = lds[grp], *V = lds[grp] + NS * NS, *bv = lds[grp] + 2 * NS * NS;  // column j, row r at j * NS + r
+  const unsigned ne = min(*count, st.cap);
+  for (unsigned e = blockIdx.x * SPB + grp; e < ne; e += gridDim.x * SPB) {
+    const int have = st.have[e];
+    if (st.done[e] || have >= SING_ROUNDS) continue;
+    const int n = st.n[e];
+    const bool row = i < n;
+    const double *A = st.A + (size_t)e * NS * NS;
+    if (row) {
+      for (int j = 0; j < n; ++j) {
+        U[j * NS + i] = A[i * NS + j];
+        V[j * NS + i] = (i == j) ? 1.0 : 0.0;
+      }
+      bv[i] = st.b[(size_t)e * NS + i];
+    }
+    wave_lds_sync();
+    for (int sweep = 0; sweep < 80; ++sweep) {
+      bool rotated = false;
+      for (int p = 0; p < n - 1; ++p)
+        for (int q = p + 1; q < n; ++q) {
+          double al = 0, be = 0, ga = 0;
+          for (int k = 0; k < n; ++k) {
+            const double up = U[p * NS + k], uq = U[q * NS + k];
+            al += up * up; be += uq * uq; ga += up * uq;
+          }
+          if (ga == 0.0 || fabs(ga) <= 1e-15 * sqrt(al * be)) continue;
+          rotated = true;
+          const double zeta = (be - al) / (2.0 * ga);
+          const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
+          wave_lds_sync();  // every lane has read column p and q
+          if (row) {
+            const double up = U[p * NS + i], uq = U[q * NS + i];
+            U[p * NS + i] = c * up - sn * uq;
+            U[q * NS + i] = sn * up + c * uq;
+            const double vp = V[p * NS + i], vq = V[q * NS + i];
+            V[p * NS + i] = c * vp - sn * vq;
+            V[q * NS + i] = sn * vp + c * vq;
+          }
+          wave_lds_sync();
+        }
+      if (!rotated) break;
+    }
+    double sig[NS], smax = 0.0;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      sig[j] = 0.0;
+      if (j < n) {
+        double ss = 0.0;
+        for (int k = 0; k < n; ++k) ss += U[j * NS + k] * U[j * NS + k];
+        sig[j] = sqrt(ss);
+        smax = fmax(smax, sig[j]);
+      }
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int j = 0; j < NS; ++j)
+      if (j < n && row && sig[j] > 0) U[j * NS + i] /= sig[j];
+    wave_lds_sync();
+    // a single small singular value is kept when the data fix its step (svd_pinv_solve)
+    double lim = smax * CP_PINV_RTOL;
+    int nsmall = 0, ks = -1;
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+      if (k < n && sig[k] <= lim) { nsmall++; ks = k; }
+    if (nsmall == 1) {
+      double sks = 0.0, s2 = smax;
+#pragma unroll
+      for (int k = 0; k < NS; ++k) {
+        if (k == ks) sks = sig[k];
+        else if (k < n) s2 = fmin(s2, sig[k]);
+      }
+      if (sks > 0) {
+        double ub = 0, bb = 0;
+        for (int r = 0; r < n; ++r) { ub += U[ks * NS + r] * bv[r]; bb += bv[r] * bv[r]; }
+        const double eps = 2.220446049250313e-16;
+        if (fabs(ub) * eps * smax * smax < 1e-2 * sqrt(bb) * sks * (s2 - sks)) lim = 0.0;
+      }
+    }
+    double xi = 0.0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      if (k >= n || !(sig[k] > lim)) continue;
+      double ub = 0;
+      for (int r = 0; r < n; ++r) ub += U[k * NS + r] * bv[r];
+      ub /= sig[k];
+      if (row) xi += V[k * NS + i] * ub;
+    }
+    if (row) st.X[((size_t)e * SING_ROUNDS + have) * NS + i] = xi;
+    if (i == 0) st.have[e] = have + 1;
+    wave_lds_sync();  // the group's next system overwrites U, V, b
+  }
+}
+
 // ------------------------------------------------------------ assembly
 // ---- assembly, stage 1: one lane per qp.  F from u, calculatePlasticity(.,.,1)
 // (getElementalValues.cc:129-141); writes the iter history and JxW * dP/dF (as DT) and JxW * P.
@@ -296,6 +506,7 @@ struct QpArgs {
   void *D;    // [81][nq] of DT
   double *P;  // [9][nq]
   unsigned *queue, *qcount;  // deferred qps
+  SingStore sing;            // their singular systems
 };
 
 // Lanes whose local problem meets a numerically singular A_PA (the pseudo-inverse branch of
@@ -303,7 +514,7 @@ struct QpArgs {
 // singular ones to a device queue; pass 2 runs only the queued qps, so the slow branch never
 // holds up a wavefront of regular qps.  Results do not depend on which lane computes a qp.
 template <int NS, typename DT, bool KIN>
-__device__ __forceinline__ bool qp_tangent_one(const QpArgs &a, long long Q, bool allow_svd) {
+__device__ __attribute__((noinline)) bool qp_tangent_one(const QpArgs &a, long long Q, SingAny<NS> &sing) {
   const long long e = Q >> 3;
   const int q = (int)(Q & 7);
   const int ex = (int)(e % a.m.nx), ey = (int)((e / a.m.nx) % a.m.ny), ez = (int)(e / ((long long)a.m.nx * a.m.ny));
@@ -330,7 +541,7 @@ __device__ __forceinline__ bool qp_tangent_one(const QpArgs &a, long long Q, boo
   const double *gt = a.gtab + (long long)a.egrain[e] * a.gstride;
   bool plastic = false;
   const int na = cp::qp_update<NS, DT, KIN>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 1, P, T, Fe, Fp, (DT *)a.D + Q,
-                                            a.nq, &plastic, a.JxW, allow_svd, W);
+                                            a.nq, &plastic, a.JxW, SingRef<NS>{&sing}, W);
   if (na < 0) return false;
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
@@ -352,14 +563,28 @@ template <int NS, typename DT, bool KIN>
 __global__ void __launch_bounds__(256) k_qp_tangent(QpArgs a) {
   const long long Q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (Q >= a.nq) return;
-  if (!qp_tangent_one<NS, DT, KIN>(a, Q, false)) a.queue[atomicAdd(a.qcount, 1u)] = (unsigned)Q;
+  SingAny<NS> f{SING_CLAIM, a.sing, a.qcount, a.queue, (unsigned)Q, 0u, 0, 0};
+  qp_tangent_one<NS, DT, KIN>(a, Q, f);  // singular: queued
+}
+// rounds: the pending entries with their stored solutions
+template <int NS, typename DT, bool KIN>
+__global__ void __launch_bounds__(64) k_qp_tangent_replay(QpArgs a) {
+  const unsigned n = min(*a.qcount, a.sing.cap);
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    if (a.sing.done[e]) continue;
+    SingAny<NS> f{SING_REPLAY, a.sing, a.qcount, a.queue, 0u, e, a.sing.have[e], 0};
+    if (qp_tangent_one<NS, DT, KIN>(a, a.queue[e], f)) a.sing.done[e] = 1;
+  }
 }
 // pass 2: the queued qps, grid-stride, pseudo-inverse allowed
 template <int NS, typename DT, bool KIN>
 __global__ void __launch_bounds__(64) k_qp_tangent_deferred(QpArgs a) {
   const unsigned n = *a.qcount;
   for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    qp_tangent_one<NS, DT, KIN>(a, a.queue[i], true);
+    if (i >= a.sing.cap || !a.sing.done[i]) {
+      SingAny<NS> f{SING_LANE, a.sing, a.qcount, a.queue, 0u, i, 0, 0};
+      qp_tangent_one<NS, DT, KIN>(a, a.queue[i], f);
+    }
 }
 
 __global__ void k_zero_u32(unsigned *p) { *p = 0u; }
@@ -512,6 +737,7 @@ struct PostArgs {
   unsigned *counter;
   double *out;  // 13: E(6) T(6) slip
   unsigned *queue, *qcount;  // deferred qps
+  SingStore sing;            // their singular systems
 };
 
 // symmetric 3x3 eigen-decomposition by cyclic Jacobi rotations
@@ -579,7 +805,7 @@ __device__ __forceinline__ void post_F(const PostArgs &a, long long Q, double F[
 // updateAfterIncrement's qp loop (updateAfterIncrement.cc:77-126): calculatePlasticity(.,.,0) at
 // the converged displacement; singular local problems deferred as in assembly.
 template <int NS, bool KIN>
-__device__ __forceinline__ bool post_qp_one(const PostArgs &a, long long Q, bool allow_svd) {
+__device__ __attribute__((noinline)) bool post_qp_one(const PostArgs &a, long long Q, SingAny<NS> &sing) {
   double F[9];
   post_F(a, Q, F);
   double Fp_t[9], s[NS], x[NS], P[9], T[9], Fe[9], Fp[9], W[KIN ? NS : 1];
@@ -591,7 +817,7 @@ __device__ __forceinline__ bool post_qp_one(const PostArgs &a, long long Q, bool
   const double *gt = a.gtab + (long long)a.egrain[Q >> 3] * a.gstride;
   bool plastic = false;
   const int na = cp::qp_update<NS, double, KIN>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 0, P, T, Fe, Fp, (double *)nullptr,
-                                                0, &plastic, 1.0, allow_svd, W);
+                                                0, &plastic, 1.0, SingRef<NS>{&sing}, W);
   if (na < 0) return false;
 #pragma unroll
   for (int i = 0; i < 9; ++i) { a.Fe_iter[i * a.nq + Q] = Fe[i]; a.Fp_iter[i * a.nq + Q] = Fp[i]; a.T[i * a.nq + Q] = T[i]; }
@@ -606,12 +832,26 @@ template <int NS, bool KIN>
 __global__ void __launch_bounds__(256) k_post_qp(PostArgs a) {
   const long long Q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (Q >= a.nq) return;
-  if (!post_qp_one<NS, KIN>(a, Q, false)) a.queue[atomicAdd(a.qcount, 1u)] = (unsigned)Q;
+  SingAny<NS> f{SING_CLAIM, a.sing, a.qcount, a.queue, (unsigned)Q, 0u, 0, 0};
+  post_qp_one<NS, KIN>(a, Q, f);
+}
+template <int NS, bool KIN>
+__global__ void __launch_bounds__(64) k_post_qp_replay(PostArgs a) {
+  const unsigned n = min(*a.qcount, a.sing.cap);
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    if (a.sing.done[e]) continue;
+    SingAny<NS> f{SING_REPLAY, a.sing, a.qcount, a.queue, 0u, e, a.sing.have[e], 0};
+    if (post_qp_one<NS, KIN>(a, a.queue[e], f)) a.sing.done[e] = 1;
+  }
 }
 template <int NS, bool KIN>
 __global__ void __launch_bounds__(64) k_post_qp_deferred(PostArgs a) {
   const unsigned n = *a.qcount;
-  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) post_qp_one<NS, KIN>(a, a.queue[i], true);
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (i >= a.sing.cap || !a.sing.done[i]) {
+      SingAny<NS> f{SING_LANE, a.sing, a.qcount, a.queue, 0u, i, 0, 0};
+      post_qp_one<NS, KIN>(a, a.queue[i], f);
+    }
 }
 
 // Green-Lagrange strain, JxW-weighted sums (updateAfterIncrement.cc:107-126, 218-245) and
@@ -683,7 +923,7 @@ __global__ void __launch_bounds__(64) k_qp_batch(long long n, const double *__re
     for (int k = 0; k < ns; ++k) w[k] = Wio[(long long)ns * i + k];
   const double *gt = gtab + (long long)grain[i] * gstride;
   const int r = cp::qp_update<NS, double, KIN>(mat, gt, ns, msi, f, fp, s, x, flag, p, t, fe, fpo, flag ? D + 81 * i : nullptr, 1,
-                                               nullptr, 1.0, true, w);
+                                               nullptr, 1.0, cp::SvdInLane<NS>(), w);
   for (int k = 0; k < 9; ++k) { P[9 * i + k] = p[k]; T[9 * i + k] = t[k]; Fe[9 * i + k] = fe[k]; Fp[9 * i + k] = fpo[k]; }
   for (int k = 0; k < ns; ++k) { so[(long long)ns * i + k] = s[k]; xo[(long long)ns * i + k] = x[k]; }
   if constexpr (KIN)
@@ -1098,6 +1338,8 @@ __global__ void k_zero(long long N, double *__restrict__ v) {
   if (r < N) v[r] = 0.0;
 }
 
+#include "cp_mg.h"
+
 // ------------------------------------------------------------ host side
 inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 // grid of the reducing vector kernels: grid-stride loops over at most this many 256-lane blocks,
@@ -1135,6 +1377,7 @@ struct pa_context {
   void *Dq = nullptr;  // [81][nq] JxW dP/dF of the last assembly, double or float
   double *Pq = nullptr;  // [9][nq] JxW P
   unsigned *queue = nullptr;  // deferred qps of the pseudo-inverse pass
+  SingStore sing = {};        // their singular systems and solutions
   unsigned char *con = nullptr;
   double *delu = nullptr;
   double *kv[7] = {nullptr};  // Krylov vectors: X Rr Rl Zr Zl Pr Pl (CG: X R - Z Q P -)
@@ -1144,6 +1387,14 @@ struct pa_context {
   double *d_out = nullptr;
   long long *d_rp = nullptr;
   int bc_inc = -1;
+  // multigrid preconditioner (PA_SOLVER_MGCG): levels 0..mg_levels, mg_levels the coarsest;
+  // level 0 is the Jacobian with the Krylov residual as right-hand side and z as solution
+  int mg_levels = 0;
+  MeshDims mg_m[MG_MAX_LEVELS];
+  void *mg_A[MG_MAX_LEVELS] = {nullptr};
+  double *mg_dinv[MG_MAX_LEVELS] = {nullptr}, *mg_b[MG_MAX_LEVELS] = {nullptr}, *mg_x[MG_MAX_LEVELS] = {nullptr};
+  double *mg_r[MG_MAX_LEVELS] = {nullptr}, *mg_d[MG_MAX_LEVELS] = {nullptr};
+  double *mg_lam = nullptr, *mg_minv = nullptr, *mg_part = nullptr;
   std::vector<void *> allocs;
 };
 
@@ -1164,16 +1415,27 @@ bool mixed(const pa_context *c) { return c->precision == PA_PREC_MIXED; }
 
 constexpr unsigned DEFER_GRID = 1024;  // 64-lane blocks of the deferred pass
 
+constexpr unsigned SVD_GRID = 8192;    // blocks of k_sing_svd (grid-stride over the entries)
+
+template <int NS>
+void sing_svd_round(pa_context *c, const unsigned *count) {
+  hipLaunchKernelGGL(k_sing_svd<NS>, dim3(SVD_GRID), dim3(sing_per_block<NS>() * sing_group<NS>()), 0, c->stream, c->sing, count);
+}
+
+template <int NS, typename DT, bool KIN>
+void qp_tangent_passes_k(pa_context *c, const QpArgs &a) {
+  hipLaunchKernelGGL((k_qp_tangent<NS, DT, KIN>), dim3(nblk(c->nq, 256)), dim3(256), 0, c->stream, a);
+  for (int r = 0; r < SING_ROUNDS; ++r) {
+    sing_svd_round<NS>(c, a.qcount);
+    hipLaunchKernelGGL((k_qp_tangent_replay<NS, DT, KIN>), dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
+  }
+  hipLaunchKernelGGL((k_qp_tangent_deferred<NS, DT, KIN>), dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
+}
 template <int NS, typename DT>
 void qp_tangent_passes(pa_context *c, const QpArgs &a) {
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, c->stream, a.qcount);
-  if (c->kin) {
-    hipLaunchKernelGGL((k_qp_tangent<NS, DT, true>), dim3(nblk(c->nq, 256)), dim3(256), 0, c->stream, a);
-    hipLaunchKernelGGL((k_qp_tangent_deferred<NS, DT, true>), dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
-  } else {
-    hipLaunchKernelGGL((k_qp_tangent<NS, DT, false>), dim3(nblk(c->nq, 256)), dim3(256), 0, c->stream, a);
-    hipLaunchKernelGGL((k_qp_tangent_deferred<NS, DT, false>), dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
-  }
+  if (c->kin) qp_tangent_passes_k<NS, DT, true>(c, a);
+  else qp_tangent_passes_k<NS, DT, false>(c, a);
 }
 
 int launch_qp_tangent(pa_context *c) {
@@ -1186,7 +1448,7 @@ int launch_qp_tangent(pa_context *c) {
   a.Fp_conv = c->Fp_conv; a.s_conv = c->s_conv; a.slip_conv = c->slip_conv; a.W_conv = c->W_conv;
   a.Fe_iter = c->Fe_iter; a.Fp_iter = c->Fp_iter; a.s_iter = c->s_iter; a.slip_iter = c->slip_iter; a.W_iter = c->W_iter;
   a.D = c->Dq; a.P = c->Pq;
-  a.queue = c->queue; a.qcount = c->counters + 8;
+  a.queue = c->queue; a.qcount = c->counters + 8; a.sing = c->sing;
   const int NS = ns_template(c->ns);
   const bool f = mixed(c);
   if (NS == 12) { if (f) qp_tangent_passes<12, float>(c, a); else qp_tangent_passes<12, double>(c, a); }
@@ -1196,15 +1458,19 @@ int launch_qp_tangent(pa_context *c) {
   return 0;
 }
 
+template <int NS, bool KIN>
+void post_passes_k(pa_context *c, const PostArgs &a, unsigned grid) {
+  hipLaunchKernelGGL((k_post_qp<NS, KIN>), dim3(grid), dim3(256), 0, c->stream, a);
+  for (int r = 0; r < SING_ROUNDS; ++r) {
+    sing_svd_round<NS>(c, a.qcount);
+    hipLaunchKernelGGL((k_post_qp_replay<NS, KIN>), dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
+  }
+  hipLaunchKernelGGL((k_post_qp_deferred<NS, KIN>), dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
+}
 template <int NS>
 void post_passes(pa_context *c, const PostArgs &a, unsigned grid) {
-  if (c->kin) {
-    hipLaunchKernelGGL((k_post_qp<NS, true>), dim3(grid), dim3(256), 0, c->stream, a);
-    hipLaunchKernelGGL((k_post_qp_deferred<NS, true>), dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
-  } else {
-    hipLaunchKernelGGL((k_post_qp<NS, false>), dim3(grid), dim3(256), 0, c->stream, a);
-    hipLaunchKernelGGL((k_post_qp_deferred<NS, false>), dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
-  }
+  if (c->kin) post_passes_k<NS, true>(c, a, grid);
+  else post_passes_k<NS, false>(c, a, grid);
 }
 
 int launch_elements(pa_context *c, int iteration) {
@@ -1320,13 +1586,154 @@ int do_assemble(pa_context *c, int inc, int it, double *res_norm) {
   return 0;
 }
 
+// ---- multigrid preconditioner, host side
+
+// Levels: halve every axis while the level has more than MG_COARSE_NODES nodes.
+int mg_plan(pa_context *c) {
+  MeshDims m = c->m;
+  int L = 0;
+  c->mg_m[0] = m;
+  while (m.nnode > MG_COARSE_NODES) {
+    if ((m.nx & 1) || (m.ny & 1) || (m.nz & 1) || L + 1 >= MG_MAX_LEVELS)
+      return fail(PA_ERR_UNSUPPORTED, "multigrid needs element counts that halve down to at most 2x2x2 elements");
+    m.nx /= 2; m.ny /= 2; m.nz /= 2;
+    m.nnx = m.nx + 1; m.nny = m.ny + 1; m.nnz = m.nz + 1;
+    m.nnode = (long long)m.nnx * m.nny * m.nnz;
+    m.nelem = (long long)m.nx * m.ny * m.nz;
+    c->mg_m[++L] = m;
+  }
+  if (L == 0) return fail(PA_ERR_UNSUPPORTED, "multigrid needs a mesh with more than 2x2x2 elements");
+  c->mg_levels = L;
+  return 0;
+}
+
+// Level storage (once per context): operators sized for FP64 so that either form fits.
+int mg_alloc(pa_context *c) {
+  if (c->mg_lam) return 0;
+  int rc = mg_plan(c);
+  if (rc) return rc;
+  const int L = c->mg_levels;
+  c->mg_A[0] = c->K;
+  c->mg_dinv[0] = c->dinv;
+  c->mg_b[0] = c->kv[1];  // the Krylov residual r
+  c->mg_x[0] = c->kv[3];  // z = M r
+  c->mg_r[0] = c->kv[2];  // vectors the CG does not use
+  c->mg_d[0] = c->kv[6];
+  for (int l = 1; l <= L; ++l) {
+    const long long nn = c->mg_m[l].nnode;
+    if ((rc = dalloc(c, (double **)&c->mg_A[l], 243 * nn)) || (rc = dalloc(c, &c->mg_dinv[l], 3 * nn)) ||
+        (rc = dalloc(c, &c->mg_b[l], 3 * nn)) || (rc = dalloc(c, &c->mg_x[l], 3 * nn)) || (rc = dalloc(c, &c->mg_r[l], 3 * nn)) ||
+        (rc = dalloc(c, &c->mg_d[l], 3 * nn)))
+      return rc;
+  }
+  if ((rc = dalloc(c, &c->mg_minv, (size_t)MG_COARSE_DOFS * MG_COARSE_DOFS)) || (rc = dalloc(c, &c->mg_part, nblk(c->m.nnode, 256) + 64)) ||
+      (rc = dalloc(c, &c->mg_lam, MG_MAX_LEVELS)))
+    return rc;
+  return 0;
+}
+
+// Galerkin operators, diagonals, Gershgorin bounds and the coarsest inverse for the current Jacobian.
+template <typename VT>
+int mg_setup_t(pa_context *c) {
+  const int L = c->mg_levels;
+  c->mg_A[0] = c->K;
+  for (int l = 1; l <= L; ++l)
+    hipLaunchKernelGGL(k_mg_galerkin<VT>, dim3(nblk(27 * c->mg_m[l].nnode, 256)), dim3(256), 0, c->stream, c->mg_m[l - 1],
+                       (const VT *)c->mg_A[l - 1], l == 1 ? (const unsigned char *)c->con : nullptr, c->mg_m[l], (VT *)c->mg_A[l]);
+  for (int l = 0; l < L; ++l) {
+    const unsigned g = nblk(c->mg_m[l].nnode, 256);
+    hipLaunchKernelGGL(k_mg_diag<VT>, dim3(g), dim3(256), 0, c->stream, c->mg_m[l], (const VT *)c->mg_A[l],
+                       l == 0 ? nullptr : c->mg_dinv[l], c->mg_part);
+    hipLaunchKernelGGL(k_mg_lam, dim3(1), dim3(256), 0, c->stream, (const double *)c->mg_part, g, c->mg_lam + l);
+  }
+  hipLaunchKernelGGL(k_mg_factor<VT>, dim3(1), dim3(256), 0, c->stream, c->mg_m[L], (const VT *)c->mg_A[L], c->mg_minv);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+int mg_setup(pa_context *c) { return mixed(c) ? mg_setup_t<float>(c) : mg_setup_t<double>(c); }
+
+// z = M r: one V-cycle; the last kernel leaves the block partials of r.z at the front of partials.
+// Levels above lt run one kernel per step; levels lt..L run in the one-block k_mg_tail.
+template <typename VT>
+int mg_vcycle_t(pa_context *c) {
+  const int L = c->mg_levels;
+  const KState *st = c->kst;
+  auto mask = [&](int l) { return l == 0 ? (const unsigned char *)c->con : nullptr; };
+  int lt = 1;
+  while (lt < L && c->mg_m[lt].nnode > MG_TAIL_NODES) ++lt;
+  for (int l = 0; l < lt; ++l) {
+    const MeshDims &m = c->mg_m[l];
+    const unsigned g = nblk(m.nnode, VB);
+    const VT *A = (const VT *)c->mg_A[l];
+    hipLaunchKernelGGL(k_mg_pre<VT>, dim3(g), dim3(VB), 0, c->stream, m, A, (const double *)c->mg_dinv[l], (const double *)(c->mg_lam + l),
+                       (const double *)c->mg_b[l], c->mg_x[l], c->mg_d[l], c->mg_r[l], st);
+    hipLaunchKernelGGL(k_mg_resid<VT>, dim3(g), dim3(VB), 0, c->stream, m, A, (const double *)c->mg_d[l], c->mg_r[l], st);
+    if (l + 1 < lt)
+      hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(c->mg_m[l + 1].nnode, VB)), dim3(VB), 0, c->stream, m, (const double *)c->mg_r[l],
+                         mask(l), c->mg_m[l + 1], c->mg_b[l + 1], st);
+  }
+  MgTail t;
+  std::memset(&t, 0, sizeof(t));
+  t.lt = lt;
+  t.L = L;
+  t.mf = c->mg_m[lt - 1];
+  t.rf = c->mg_r[lt - 1];
+  t.con = mask(lt - 1);
+  for (int l = lt; l <= L; ++l) {
+    t.m[l] = c->mg_m[l];
+    t.A[l] = c->mg_A[l];
+    t.dinv[l] = c->mg_dinv[l];
+    t.b[l] = c->mg_b[l];
+    t.x[l] = c->mg_x[l];
+    t.r[l] = c->mg_r[l];
+    t.d[l] = c->mg_d[l];
+  }
+  t.lam = c->mg_lam;
+  t.minv = c->mg_minv;
+  hipLaunchKernelGGL(k_mg_tail<VT>, dim3(1), dim3(MG_TAIL_THREADS), 0, c->stream, t, st);
+  for (int l = lt - 1; l >= 0; --l) {
+    const MeshDims &m = c->mg_m[l];
+    const unsigned g = nblk(m.nnode, VB);
+    const VT *A = (const VT *)c->mg_A[l];
+    hipLaunchKernelGGL(k_mg_prolong, dim3(g), dim3(VB), 0, c->stream, m, c->mg_x[l], mask(l), c->mg_m[l + 1],
+                       (const double *)c->mg_x[l + 1], st);
+    hipLaunchKernelGGL(k_mg_post1<VT>, dim3(g), dim3(VB), 0, c->stream, m, A, (const double *)c->mg_dinv[l],
+                       (const double *)(c->mg_lam + l), (const double *)c->mg_b[l], (const double *)c->mg_x[l], c->mg_d[l], c->mg_r[l], st);
+    if (l == 0)
+      hipLaunchKernelGGL((k_mg_post2<VT, true>), dim3(g), dim3(VB), 0, c->stream, m, A, (const double *)c->mg_dinv[l],
+                         (const double *)(c->mg_lam + l), (const double *)c->mg_b[l], c->mg_x[l], (const double *)c->mg_d[l],
+                         (const double *)c->mg_r[l], st, c->partials);
+    else
+      hipLaunchKernelGGL((k_mg_post2<VT, false>), dim3(g), dim3(VB), 0, c->stream, m, A, (const double *)c->mg_dinv[l],
+                         (const double *)(c->mg_lam + l), (const double *)c->mg_b[l], c->mg_x[l], (const double *)c->mg_d[l],
+                         (const double *)c->mg_r[l], st, (double *)nullptr);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+int mg_vcycle(pa_context *c) { return mixed(c) ? mg_vcycle_t<float>(c) : mg_vcycle_t<double>(c); }
+
 // One Krylov launch sequence for iteration k: p update, operator (CG: K p; BiCG: K p and K^T p~), update.
 int launch_krylov_iteration(pa_context *c, int k) {
   const long long N = c->ndof;
   double *X = c->kv[0], *Rr = c->kv[1], *Rl = c->kv[2], *Zr = c->kv[3], *Zl = c->kv[4], *Pr = c->kv[5], *Pl = c->kv[6];
   const unsigned gu = ugrid(N), gn = nblk(c->m.nnode, VB);
   double *partA = c->partials, *partU = c->partials + spmv_partials_offset(c);
-  if (c->method == PA_SOLVER_CG) {
+  if (c->method == PA_SOLVER_MGCG) {
+    double *Q = Zl;
+    hipLaunchKernelGGL(k_mgcg_p, dim3(gu), dim3(VB), 0, c->stream, N, (const double *)Zr, Pr, c->kst, (const double *)partU, gu,
+                       (const double *)partA, gn, k);
+    if (mixed(c))
+      hipLaunchKernelGGL(k_cg_spmv<float>, dim3(gn), dim3(VB), 0, c->stream, c->m, (const float *)c->K, (const double *)Pr, Q,
+                         (const KState *)c->kst, partA);
+    else
+      hipLaunchKernelGGL(k_cg_spmv<double>, dim3(gn), dim3(VB), 0, c->stream, c->m, (const double *)c->K, (const double *)Pr, Q,
+                         (const KState *)c->kst, partA);
+    hipLaunchKernelGGL(k_mgcg_update, dim3(gu), dim3(VB), 0, c->stream, N, X, (const double *)Pr, Rr, (const double *)Q,
+                       (const double *)c->dinv, (const KState *)c->kst, (const double *)partA, gn, partU, k);
+    HIPCHK(hipGetLastError());
+    return mg_vcycle(c);
+  } else if (c->method == PA_SOLVER_CG) {
     double *Q = Zl;
     hipLaunchKernelGGL(k_cg_p, dim3(gu), dim3(VB), 0, c->stream, N, (const double *)Zr, Pr, c->kst, (const double *)partU, gu, k);
     if (mixed(c))
@@ -1356,15 +1763,26 @@ int launch_krylov_iteration(pa_context *c, int k) {
 int do_solve(pa_context *c, int it, int *iters) {
   const long long N = c->ndof;
   double *X = c->kv[0], *Rr = c->kv[1], *Rl = c->kv[2], *Zr = c->kv[3], *Zl = c->kv[4];
-  const bool cg = c->method == PA_SOLVER_CG;
+  const bool cg = c->method != PA_SOLVER_BICG;
+  const bool mg = c->method == PA_SOLVER_MGCG;
+  if (mg) {
+    const int rc = mg_setup(c);
+    if (rc) return rc;
+  }
   hipLaunchKernelGGL(k_krylov_init, dim3(vgrid(N)), dim3(VB), 0, c->stream, N, c->R, X, Rr, cg ? nullptr : Rl, Zr, cg ? nullptr : Zl,
                      c->dinv, c->kst, c->pb.lin_tol, c->pb.lin_maxit, c->partials + update_partials_end(c), c->counters + 0);
   HIPCHK(hipGetLastError());
+  if (mg) {
+    const int rc = mg_vcycle(c);  // z_0 = M r_0 and the partials of r_0.z_0 for iteration 0
+    if (rc) return rc;
+  }
   // The kernels test convergence themselves and return at once after it.  The host reads the
   // flag every `chunk` iterations, one chunk behind: chunk j+1 is queued before the copy of the
   // flag after chunk j is waited on, so the stream never idles on the host round trip; at most
   // one chunk of launches runs after convergence, each returning at its first instruction.
-  const int chunk = 16;
+  // (multigrid: a few dozen iterations of ~30 launches each, so a short chunk bounds the
+  // launches that run after convergence)
+  const int chunk = mg ? 4 : 16;
   KState hs;
   for (int j = 0;; ++j) {
     for (int k = 0; k < chunk; ++k) {
@@ -1469,6 +1887,19 @@ int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out) {
   }
   for (int i = 0; i < 7; ++i)
     if ((rc = dalloc(c, &c->kv[i], N))) { pa_destroy(c); return rc; }
+  {  // singular-system store: as many entries as fit 768 MiB (at most one per qp)
+    const size_t ns_t = (size_t)ns_template(c->ns);
+    const size_t per = sizeof(double) * (ns_t * ns_t + ns_t + SING_ROUNDS * ns_t) + SING_ROUNDS * 8 + 3 * sizeof(int);
+    const size_t cap = std::min<size_t>((size_t)nq, std::max<size_t>(64, ((size_t)768 << 20) / per));
+    c->sing.cap = (unsigned)cap;
+    c->sing.miss = c->counters + 10;
+    if ((rc = dalloc(c, &c->sing.A, cap * ns_t * ns_t)) || (rc = dalloc(c, &c->sing.b, cap * ns_t)) || (rc = dalloc(c, &c->sing.n, cap)) ||
+        (rc = dalloc(c, &c->sing.X, cap * SING_ROUNDS * ns_t)) || (rc = dalloc(c, &c->sing.have, cap)) ||
+        (rc = dalloc(c, &c->sing.done, cap)) || (rc = dalloc(c, &c->sing.key, cap * SING_ROUNDS))) {
+      pa_destroy(c);
+      return rc;
+    }
+  }
   // initial state (crystalPlasticity::init, init.cc:302-330): Fp = Fe = I, s = s0, slip = 0
   {
     std::vector<double> hbuf(9 * nq, 0.0);
@@ -1582,8 +2013,12 @@ int pa_set_grains(pa_context *c, const unsigned *elem_grain, const double *grain
 
 int pa_set_solver(pa_context *c, int method, int precision) {
   if (!c) return fail(PA_ERR_ARG, "null context");
-  if (method != PA_SOLVER_BICG && method != PA_SOLVER_CG) return fail(PA_ERR_ARG, "unknown solver");
+  if (method != PA_SOLVER_BICG && method != PA_SOLVER_CG && method != PA_SOLVER_MGCG) return fail(PA_ERR_ARG, "unknown solver");
   if (precision != PA_PREC_FP64 && precision != PA_PREC_MIXED) return fail(PA_ERR_ARG, "unknown precision");
+  if (method == PA_SOLVER_MGCG) {
+    const int rc = mg_alloc(c);
+    if (rc) return rc;
+  }
   c->method = method;
   c->precision = precision;
   return 0;
@@ -1632,7 +2067,7 @@ int pa_commit_increment(pa_context *c, double row[15]) {
   a.Fe_iter = c->Fe_iter; a.Fp_iter = c->Fp_iter; a.s_iter = c->s_iter; a.slip_iter = c->slip_iter; a.T = c->T; a.rotnew = c->rotnew;
   a.W_iter = c->W_iter;
   a.partials = c->partials; a.counter = c->counters + 4; a.out = c->d_out;
-  a.queue = c->queue; a.qcount = c->counters + 9;
+  a.queue = c->queue; a.qcount = c->counters + 9; a.sing = c->sing;
   HIPCHK(hipEventRecord(c->ev[0], c->stream));
   const int NS = ns_template(c->ns);
   const unsigned grid = nblk(c->nq, 256);
@@ -1692,13 +2127,12 @@ int pa_last_timings(pa_context *c, double ms[3]) {
   return 0;
 }
 
-int pa_last_counts(pa_context *c, long long counts[2]) {
+int pa_last_counts(pa_context *c, long long counts[3]) {
   if (!c || !counts) return fail(PA_ERR_ARG, "null argument");
-  unsigned h[2] = {0, 0};
+  unsigned h[3] = {0, 0, 0};
   HIPCHK(hipMemcpyAsync(h, c->counters + 8, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  counts[0] = h[0];
-  counts[1] = h[1];
+  for (int i = 0; i < 3; ++i) counts[i] = h[i];
   return 0;
 }
 

@@ -218,6 +218,20 @@ __host__ __device__ __attribute__((noinline)) void svd_pinv_solve(int n, const d
   }
 }
 
+// What qp_update does when LU finds A_PA numerically singular: the functor receives A_PA
+// (n x n, leading dimension NS, intact) and b_PA and either writes x = pinv(A_PA) b_PA and
+// returns true, or returns false, which makes qp_update return -1 at once.
+template <int NS>
+struct SvdInLane {  // the pseudo-inverse in this lane
+  CP_DEV bool operator()(int n, const double *A, const double *b, double *x) const {
+    svd_pinv_solve<NS>(n, A, b, x);
+    return true;
+  }
+};
+struct LuOnly {  // give up: the caller handles the qp elsewhere
+  CP_DEV bool operator()(int, const double *, const double *, double *) const { return false; }
+};
+
 // ------------------------------------------------------------ table layouts
 // material block (doubles): h0[NS] ss[NS] pw[NS] q[NS*NS] tol
 template <int NS> struct MatLayout {
@@ -251,7 +265,7 @@ CP_DEV void dexp_series(const double *X, const double *S, int count3, bool neg, 
 }
 
 // Return: number of systems left active after inactive-slip removal (0 = elastic), or -1 when
-// allow_svd is false and a numerically singular A_PA was met (nothing was written then).
+// the singular-solve functor declined a numerically singular A_PA (nothing was written then).
 // s holds s_conv on entry and s_tau on exit; x receives x_beta_old (the slip increments).
 // *plastic_path (optional) is set when the potentially-active set was non-empty, the case in
 // which the reference writes slipfraction_iter (calculatePlasticity.cc:454-456).
@@ -260,11 +274,11 @@ CP_DEV void dexp_series(const double *X, const double *S, int count3, bool neg, 
 // KIN: kinematic hardening (calculatePlasticity.cc:294-305, 431-436, with C_1, C_2 of
 // multiphaseInit.cc:15-32): W holds the backstress W_kh_conv on entry and W_kh_tau on exit, and
 // every sign test on the resolved shear uses tau - W.  Without it the backstress is zero.
-template <int NS, typename DT = double, bool KIN = false>
+template <int NS, typename DT = double, bool KIN = false, typename SING = SvdInLane<NS>>
 CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ gt, int ns, int max_solver_iter,
                      const double *F, const double *Fp_t, double *s, double *x, int flag, double *P, double *T,
                      double *Fe_out, double *Fp_out, DT *D, long long dstride, bool *plastic_path = nullptr,
-                     double dscale = 1.0, bool allow_svd = true, double *W = nullptr) {
+                     double dscale = 1.0, SING sing = SING(), double *W = nullptr) {
   typedef MatLayout<NS> ML;
   typedef GrainLayout<NS> GL;
   const double tol1 = mat[ML::tol];
@@ -361,10 +375,9 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
         double xs[NS];
         for (int i = 0; i < npos; ++i) xs[i] = bp[i];
         if (npos > 0 && !lu_solve<NS>(npos, Ap, xs, 1)) {
-          if (!allow_svd) return -1;  // caller defers this qp to the pseudo-inverse pass
           for (int i = 0; i < npos; ++i)
             for (int j = 0; j < npos; ++j) Ap[i * NS + j] = A[pos[i] * NS + pos[j]];
-          svd_pinv_solve<NS>(npos, Ap, bp, xs);
+          if (!sing(npos, Ap, bp, xs)) return -1;  // the caller solves this system elsewhere
         }
         for (int a = 0; a < ns; ++a) xb[a] = 0.0;
         for (int i = 0; i < npos; ++i) xb[PA0[pos[i]]] = xs[i];

@@ -266,8 +266,9 @@ class Context:
 
     def last_counts(self):
         """Quadrature points that went through the pseudo-inverse pass in the last assembly and
-        the last post-increment pass."""
-        c = np.zeros(2, np.int64)
+        the last post-increment pass, and the reruns (since creation) that met a singular system
+        other than the one solved for them."""
+        c = np.zeros(3, np.int64)
         check(native.lib().pa_last_counts(self._h, ptr(c)))
         return c
 

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "libplasticity_amd.so")
 MAX_SLIP = 48
 MAX_TABLE = 32
 
-PA_SOLVER_BICG, PA_SOLVER_CG = 0, 1
+PA_SOLVER_BICG, PA_SOLVER_CG, PA_SOLVER_MGCG = 0, 1, 2
 PA_PREC_FP64, PA_PREC_MIXED = 0, 1
 
 # every symbol include/plasticity_amd.h declares
