@@ -156,15 +156,29 @@ void elasticmoduli(const M3 &R, const double C[6][6], double Ar[6][6]) {
 }
 
 // ------------------------------------------------------ small dense solvers
-// one-sided (Hestenes) Jacobi SVD of an n x n row-major matrix: A = U diag(sig) V^T
+// Pair k of round r of the circle (round-robin) schedule on m = n + (n odd) columns: column m-1
+// stays, the others turn; every pair meets once in the m-1 rounds of a sweep, and the m/2 pairs of
+// a round share no column.  Returns p < q; q = n is the padding column of odd n (no rotation).
+void rr_pair(int m, int r, int k, int &p, int &q) {
+  const int a = (k == 0) ? r : (r + k) % (m - 1), b = (k == 0) ? m - 1 : (r - k + m - 1) % (m - 1);
+  p = std::min(a, b);
+  q = std::max(a, b);
+}
+
+// one-sided (Hestenes) Jacobi SVD of an n x n row-major matrix: A = U diag(sig) V^T, sweeps in
+// round-robin order
 void jacobi_svd(int n, const double *A, std::vector<double> &U, std::vector<double> &sig, std::vector<double> &V) {
   U.assign(A, A + n * n);
   V.assign(n * n, 0.0);
   for (int i = 0; i < n; ++i) V[i * n + i] = 1.0;
+  const int m = n + (n & 1);
   for (int sweep = 0; sweep < 80; ++sweep) {
     bool rotated = false;
-    for (int p = 0; p < n - 1; ++p)
-      for (int q = p + 1; q < n; ++q) {
+    for (int r = 0; r < m - 1; ++r)
+      for (int k = 0; k < m / 2; ++k) {
+        int p, q;
+        rr_pair(m, r, k, p, q);
+        if (q >= n) continue;
         double al = 0, be = 0, ga = 0;
         for (int i = 0; i < n; ++i) { const double up = U[i * n + p], uq = U[i * n + q]; al += up * up; be += uq * uq; ga += up * uq; }
         if (ga == 0.0 || std::fabs(ga) <= 1e-15 * std::sqrt(al * be)) continue;

