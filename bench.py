@@ -24,6 +24,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "qp updates/s + Newton-step time, 64^3 FCC: mixed-precision vs plain FP64 GPU"
+SOLVER_NAMES = {"bicg": "Jacobi-BiCG", "cg": "Jacobi-PCG", "mgcg": "multigrid-PCG"}
+FP64_PEAK_TFLOPS = 78.6
+# FP64 operations issued by one launch of the first qp pass (k_qp_tangent, every qp on the LU path)
+# at the state bench.py times it on, from rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64
+# (wave instructions x 64 lanes, FMA = 2), mean over the timed launches; filled in from the
+# committed profile.  Masked-off lanes are counted, so this is an upper bound of the useful work.
+PMC_QP_FLOPS = {("mixed", 64): (57443100 + 108127000 + 2 * 375451000 + 4212740) * 64}
+# HBM bytes (KiB) per launch of the same pass: FETCH_SIZE x 2 + WRITE_SIZE, as for the Krylov operator
+PMC_QP_TRAFFIC_KIB = {("mixed", 64): 9114060 * 2 + 17337100}
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the Jacobi-PCG operator kernel k_cg_spmv at the 64^3 workload, from two
 # rocprofv3 PMC passes (FETCH_SIZE x 2 + WRITE_SIZE x 1 KiB: the factors that make both counters
@@ -33,11 +42,12 @@ PMC_TRAFFIC_KIB = {("cg", "mixed", 64): 142996 * 2 + 6373}
 
 def cpu_baseline(args, quick):
     """Oracle (CPU FP64 restatement, test infrastructure) on a bounded sample of the same workload:
-    a 16^3 mesh with the same material/grain statistics, 4 Newton steps of one increment past yield."""
+    a 24^3 mesh with the same material and grain statistics, 2 increments x 4 Newton steps
+    (about 15-20 s on 16 host cores)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import refcase
     from plasticity_amd import synth
-    n = 8 if quick else 16
+    n = 8 if quick else 24
     mat, pb, vox, rod, bcs, t = synth.fcc_workload(n, 16, 20, 5e-4, seed=2024)
     case = refcase.Case(t["m"] / np.linalg.norm(t["m"], axis=1)[:, None], t["n"] / np.linalg.norm(t["n"], axis=1)[:, None],
                         t["q"], t["C"], t["s0"], t["h0"], t["ss"], t["pw"], tol=t["tol"], nx=n, ny=n, nz=n, delT=1.0,
@@ -83,8 +93,9 @@ def main():
     for solver in solvers:
         for form in forms:
             results[(form, solver)] = run_form(args, form, solver, pa, native, synth, torch)
-    # headline: the last form with BASELINE's named solver (Jacobi-PCG) when it was run
-    hs = "cg" if "cg" in solvers else solvers[-1]
+    # headline: the last form (mixed when both ran) with the multigrid-preconditioned CG when it ran,
+    # else Jacobi-PCG (the solver the FP64 workload names), else BiCG
+    hs = "mgcg" if "mgcg" in solvers else ("cg" if "cg" in solvers else solvers[-1])
     head = results[(forms[-1], hs)]
     nq = head["nq"]
     out = {
@@ -101,20 +112,25 @@ def main():
         "dtype": "f64" if forms[-1] == "fp64" else "f64 (return map, residual, Krylov) + f32 (tangents, matrix)",
         "data": "synthetic (seeded Voronoi grains, random orientations)",
         "config": {"workload": "FCC uniaxial tension, %d^3 Q1 hex (%d qps, %d dofs), %d grains, 5e-4 strain/step, "
-                               "4 Newton iterations/step, Jacobi-%s 1e-10" % (n, nq, head["ndof"], args.grains, "PCG" if hs == "cg" else "BiCG"),
+                               "4 Newton iterations/step, %s 1e-10" % (n, nq, head["ndof"], args.grains, SOLVER_NAMES[hs]),
                    "form": forms[-1], "solver": hs, "mesh": [n, n, n], "nnz": head["nnz"], "parallelism": "single GPU",
-                   "note": "Jacobi-PCG is the solver the workload names; the reference code uses Jacobi-BiCG (both timed)"},
+                   "note": "Jacobi-PCG is the solver the FP64 workload names and Jacobi-BiCG the reference code's; "
+                           "multigrid-PCG reaches the same stopping test (||D^-1 r|| <= 1e-10 ||b||) in ~30 instead of "
+                           "~1,450 iterations (all three timed)"},
         "roofline": head["roofline"],
     }
     keys = ("ms_per_step", "qp_updates_per_s", "krylov_iters_per_step", "assembly_ms_per_step", "krylov_ms_per_step",
             "post_ms_per_increment", "qp_tangent_ms", "element_ms", "operator_us", "update_us", "update_GBps", "p_us",
-            "singular_local_solves_per_step", "rerun_lane_solves")
+            "singular_local_solves_per_step", "rerun_lane_solves", "qp_main_pass_ms")
+    out["config"]["roofline_krylov_operator"] = head["roofline_krylov"]
     for (form, solver), r in results.items():
         out["config"]["%s_%s" % (form, solver)] = {k: r[k] for k in keys}
     if len(forms) == 2:
         for solver in solvers:
             a, b = results[("fp64", solver)], results[("mixed", solver)]
             out["config"]["speedup_mixed_vs_fp64_%s" % solver] = a["ms_per_step"] / b["ms_per_step"]
+            if solver != "cg" and ("fp64", "cg") in results:
+                out["config"]["speedup_mixed_%s_vs_fp64_cg" % solver] = results[("fp64", "cg")]["ms_per_step"] / b["ms_per_step"]
             rows_a, rows_b = np.array(a["rows"]), np.array(b["rows"])
             err = {}
             for name, cols in (("avg_stress", slice(6, 12)), ("avg_strain", slice(0, 6)), ("slip_total", slice(14, 15))):
@@ -190,9 +206,15 @@ def run_form(args, form, solver, pa, native, synth, torch):
     qp_ms = avg.value
     native.check(native.lib().pa_time_kernel(ctx._h, 4, 1, ctypes.byref(avg)))
     el_ms = avg.value
+    # the first qp pass alone (the dominant kernel): return map + dP/dF of every qp
+    native.check(native.lib().pa_time_kernel(ctx._h, 7, 3, ctypes.byref(avg)))
+    qp_main_ms = avg.value
     posts = [r["post_ms"] for r in timed if "post_ms" in r]
-    kib = PMC_TRAFFIC_KIB.get((solver, form, n)) if args.grains == 256 else None
+    kib = PMC_TRAFFIC_KIB.get(("cg" if solver == "mgcg" else solver, form, n)) if args.grains == 256 else None
     traffic = kib * 1024 if kib else None
+    qp_flops = PMC_QP_FLOPS.get((form, n)) if args.grains == 256 else None
+    qp_kib = PMC_QP_TRAFFIC_KIB.get((form, n)) if args.grains == 256 else None
+    qp_tflops = qp_flops / (qp_main_ms * 1e-3) / 1e12 if qp_flops else None
     res = {
         "nq": ctx.nqp, "ndof": ctx.ndof, "nnz": ctx.nnz,
         "ms_per_step": ms_per_step, "qp_updates_per_s": ctx.nqp / (ms_per_step * 1e-3),
@@ -203,8 +225,18 @@ def run_form(args, form, solver, pa, native, synth, torch):
         "singular_local_solves_per_step": float(np.mean([r["deferred"] for r in timed])),
         "qp_tangent_ms": qp_ms, "element_ms": el_ms, "operator_us": spmv_ms * 1e3, "update_us": upd_us, "p_us": p_us,
         "update_GBps": (8 * ctx.ndof * 8 / (upd_us * 1e-6) / 1e9) if upd_us else None,
+        "qp_main_pass_ms": qp_main_ms,
         "rows": [r["row"] for r in log if "row" in r],
-        "roofline": {"kernel": "%s<%s> (27-point node-block %s)" % ("k_bicg_spmv" if solver == "bicg" else "k_cg_spmv",
+        "roofline": {"kernel": "k_qp_tangent<%d, %s> (first qp pass: return map + dP/dF of all %d qps, singular ones queued)"
+                               % (12, "float" if form == "mixed" else "double", ctx.nqp),
+                     "bound": "mfma", "achieved": qp_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": qp_tflops / FP64_PEAK_TFLOPS if qp_tflops else None,
+                     "traffic": qp_kib * 1024 if qp_kib else None, "avg_launch_ms": qp_main_ms, "flops_per_launch": qp_flops,
+                     "flops_source": "rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 x 64 lanes, FMA = 2 (upper bound: "
+                                     "masked lanes count) and FETCH_SIZE x2 + WRITE_SIZE (profiles/k_qp_tangent_mixed_64cube_fcc.md); "
+                                     "FP64 VALU, whose MI355X peak equals the FP64 matrix figure"
+                                     if qp_flops else "not measured for this form/size"},
+        "roofline_krylov": {"kernel": "%s<%s> (27-point node-block %s)" % ("k_bicg_spmv" if solver == "bicg" else "k_cg_spmv",
                                                                       "float" if form == "mixed" else "double",
                                                                       "K p and K^T p~" if solver == "bicg" else "K p"),
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -168,6 +168,7 @@ int pa_last_counts(pa_context *ctx, long long counts[3]);
  *   4 = the 8 element colour launches alone (K_e, R_e, condensed scatter)
  *   5 = Jacobi-PCG update  x += a p, r -= a q, z = D^-1 r  (with the z.z, r.z partial sums)
  *   6 = Jacobi-PCG direction  p = z + b p
+ *   7 = the first qp pass alone (every qp on the LU path; singular local systems only queued)
  * avg_ms receives the mean time of one launch (of one sweep for which = 1 and 4).  The matrix
  * is not re-zeroed between repetitions, and the Krylov vectors are overwritten (timing only). */
 int pa_time_kernel(pa_context *ctx, int which, int reps, double *avg_ms);

@@ -6,6 +6,9 @@
       mean FETCH_SIZE and WRITE_SIZE (KiB) per dispatch of every kernel, grouped by grid size;
       the device-to-device copies (__amd_rocclr_copyBuffer) move a known number of bytes and
       calibrate the counters
+  python -m plasticity_amd.profiling counters <results.db> <kernel name part> <grid> <n>
+      every counter of a --pmc run, summed over its per-unit rows, averaged over the last n
+      dispatches of the kernel with that grid size
 """
 import sqlite3
 import sys
@@ -47,10 +50,25 @@ def traffic(fetch_db, write_db, min_kib=1024):
                                                    sum(fv) / max(len(fv), 1), sum(wv) / max(len(wv), 1)))
 
 
+def counters(db, part, grid, n):
+    c = sqlite3.connect(db)
+    per = defaultdict(lambda: defaultdict(float))
+    for name, g, cname, val, did in c.execute(
+            "select kernel_name, grid_size, counter_name, value, dispatch_id from counters_collection"):
+        if part in name and g == grid:
+            per[did][cname] += val
+    last = sorted(per)[-n:]
+    print("%d dispatches of `%s` (grid %d); mean over the last %d:" % (len(per), part, grid, len(last)))
+    for cname in sorted({k for d in last for k in per[d]}):
+        print("  %s %.6g" % (cname, sum(per[d][cname] for d in last) / len(last)))
+
+
 if __name__ == "__main__":
     if len(sys.argv) == 4 and sys.argv[1] == "trace":
         trace(sys.argv[2], sys.argv[3])
     elif len(sys.argv) == 4 and sys.argv[1] == "traffic":
         traffic(sys.argv[2], sys.argv[3])
+    elif len(sys.argv) == 6 and sys.argv[1] == "counters":
+        counters(sys.argv[2], sys.argv[3], int(sys.argv[4]), int(sys.argv[5]))
     else:
         sys.exit(__doc__)
