@@ -285,10 +285,11 @@ __global__ void k_dirichlet(MeshDims m, BCArgs bc, unsigned char *__restrict__ c
 // (A_PA, b_PA, n) in the store.  Then SING_ROUNDS rounds of: k_sing_svd solves every pending
 // system with a group of lanes per system (lane = row, U and V in LDS) and appends x to the
 // entry's solutions; a replay kernel reruns each pending qp, feeding the stored solutions to its
-// singular solves in order, and either finishes it or leaves its next singular system.  Entries
-// still pending after the rounds, or above capacity, are finished with the pseudo-inverse in
-// their own lane.  Every path computes the same x (same Jacobi sweep order and rules).
-constexpr int SING_ROUNDS = 4;
+// singular solves in order; in the last round a further singular solve runs in the lane itself
+// (few qps meet a second one, and a round costs the latency of a whole qp), otherwise the qp
+// leaves its next system.  Entries above capacity are finished with the pseudo-inverse in their
+// own lane.  Every path computes the same x (same Jacobi sweep order and rules).
+constexpr int SING_ROUNDS = 1;
 struct SingStore {
   unsigned cap;  // entries with storage
   double *A;     // [cap][NS*NS] A_PA, leading dimension NS
@@ -356,6 +357,11 @@ struct SingAny {
         atomicAdd(st.miss, 1u);
         cp::svd_pinv_solve<NS>(n, A, b, x);
       }
+      ++k;
+      return true;
+    }
+    if (have >= SING_ROUNDS) {  // last round: solve here
+      cp::svd_pinv_solve<NS>(n, A, b, x);
       ++k;
       return true;
     }
@@ -1448,8 +1454,9 @@ void sing_svd_round(pa_context *c, const unsigned *count) {
 }
 
 template <int NS, typename DT, bool KIN>
-void qp_tangent_passes_k(pa_context *c, const QpArgs &a) {
+void qp_tangent_passes_k(pa_context *c, const QpArgs &a, bool main_only) {
   hipLaunchKernelGGL((k_qp_tangent<NS, DT, KIN>), dim3(nblk(c->nq, 256)), dim3(256), 0, c->stream, a);
+  if (main_only) return;
   for (int r = 0; r < SING_ROUNDS; ++r) {
     sing_svd_round<NS>(c, a.qcount);
     hipLaunchKernelGGL((k_qp_tangent_replay<NS, DT, KIN>), dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
@@ -1457,13 +1464,14 @@ void qp_tangent_passes_k(pa_context *c, const QpArgs &a) {
   hipLaunchKernelGGL((k_qp_tangent_deferred<NS, DT, KIN>), dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
 }
 template <int NS, typename DT>
-void qp_tangent_passes(pa_context *c, const QpArgs &a) {
+void qp_tangent_passes(pa_context *c, const QpArgs &a, bool main_only) {
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, c->stream, a.qcount);
-  if (c->kin) qp_tangent_passes_k<NS, DT, true>(c, a);
-  else qp_tangent_passes_k<NS, DT, false>(c, a);
+  if (c->kin) qp_tangent_passes_k<NS, DT, true>(c, a, main_only);
+  else qp_tangent_passes_k<NS, DT, false>(c, a, main_only);
 }
 
-int launch_qp_tangent(pa_context *c) {
+// main_only: pass 1 alone (every qp on the LU path; the singular ones are only queued)
+int launch_qp_tangent(pa_context *c, bool main_only = false) {
   QpArgs a;
   a.m = c->m;
   for (int d = 0; d < 3; ++d) a.h[d] = c->h[d];
@@ -1476,9 +1484,9 @@ int launch_qp_tangent(pa_context *c) {
   a.queue = c->queue; a.qcount = c->counters + 8; a.sing = c->sing;
   const int NS = ns_template(c->ns);
   const bool f = mixed(c);
-  if (NS == 12) { if (f) qp_tangent_passes<12, float>(c, a); else qp_tangent_passes<12, double>(c, a); }
-  else if (NS == 24) { if (f) qp_tangent_passes<24, float>(c, a); else qp_tangent_passes<24, double>(c, a); }
-  else { if (f) qp_tangent_passes<48, float>(c, a); else qp_tangent_passes<48, double>(c, a); }
+  if (NS == 12) { if (f) qp_tangent_passes<12, float>(c, a, main_only); else qp_tangent_passes<12, double>(c, a, main_only); }
+  else if (NS == 24) { if (f) qp_tangent_passes<24, float>(c, a, main_only); else qp_tangent_passes<24, double>(c, a, main_only); }
+  else { if (f) qp_tangent_passes<48, float>(c, a, main_only); else qp_tangent_passes<48, double>(c, a, main_only); }
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -2323,6 +2331,8 @@ int pa_time_kernel(pa_context *c, int which, int reps, double *avg_ms) {
     } else if (which == 6) {
       hipLaunchKernelGGL(k_cg_p, dim3(gu), dim3(VB), 0, c->stream, c->ndof, (const double *)c->kv[3], c->kv[5], c->kst,
                          (const double *)partU, gu, 1);
+    } else if (which == 7) {
+      rc = launch_qp_tangent(c, true);
     } else if (which == 5) {
       hipLaunchKernelGGL(k_cg_update, dim3(gu), dim3(VB), 0, c->stream, c->ndof, c->kv[0], (const double *)c->kv[5], c->kv[1],
                          (const double *)c->kv[4], c->kv[3], (const double *)c->dinv, (const KState *)c->kst, (const double *)partA, gn,

@@ -392,6 +392,25 @@ def test_multigrid_pcg_matches_oracle(precision):
     assert its[pa.native.PA_SOLVER_MGCG].sum() * 3 < its[pa.native.PA_SOLVER_CG].sum()
 
 
+def test_time_kernel_ids_used_by_bench():
+    """Every kernel id bench.py times exists and runs on an assembled context (ids 0..7)."""
+    import ctypes
+    case = refcase.load_reference_case("fcc_simpleTension")
+    ctx = _ctx(case)
+    ctx.set_grains(case.elem_grain, case.grain_rod)
+    ctx.set_solver(pa.native.PA_SOLVER_MGCG, pa.native.PA_PREC_MIXED)
+    for inc in range(6):
+        ctx.run_increment(inc)
+    ctx.newton_iteration(6, 0)
+    avg = ctypes.c_double()
+    for which in range(8):
+        pa.native.check(pa.native.lib().pa_time_kernel(ctx._h, which, 1, ctypes.byref(avg)))
+        assert avg.value > 0.0, which
+    with pytest.raises(pa.native.PlasticityError):
+        pa.native.check(pa.native.lib().pa_time_kernel(ctx._h, 8, 1, ctypes.byref(avg)))
+    ctx.close()
+
+
 def test_multigrid_rejects_meshes_that_do_not_halve():
     case = refcase.fcc_case(nx=4, ny=3, nz=5)
     ctx = _ctx(case)
