@@ -589,17 +589,23 @@ __device__ __attribute__((noinline)) bool qp_tangent_one(const QpArgs &a, long l
   return true;
 }
 
-// pass 1: lane = qp, LU path only; singular local problems are queued
+// The qp passes keep their active-set and tangent matrices in per-lane private memory and are
+// bound by its latency: two waves per SIMD (256 VGPRs, more spilled) run the first pass 7 % faster
+// than one wave with 512 registers; three or four waves are no faster than one.  The cap is set
+// on every kernel that calls the per-qp function, which is compiled once for all of them.
+#define QP_WAVES __attribute__((amdgpu_waves_per_eu(2)))
+
+// pass 1: lane = qp (grid-stride), LU path only; singular local problems are queued
 template <int NS, typename DT, bool KIN>
-__global__ void __launch_bounds__(256) k_qp_tangent(QpArgs a) {
-  const long long Q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (Q >= a.nq) return;
-  SingAny<NS> f{SING_CLAIM, a.sing, a.qcount, a.queue, (unsigned)Q, 0u, 0, 0};
-  qp_tangent_one<NS, DT, KIN>(a, Q, f);  // singular: queued
+__global__ void __launch_bounds__(256) QP_WAVES k_qp_tangent(QpArgs a) {
+  for (long long Q = blockIdx.x * (long long)blockDim.x + threadIdx.x; Q < a.nq; Q += (long long)gridDim.x * blockDim.x) {
+    SingAny<NS> f{SING_CLAIM, a.sing, a.qcount, a.queue, (unsigned)Q, 0u, 0, 0};
+    qp_tangent_one<NS, DT, KIN>(a, Q, f);  // singular: queued
+  }
 }
 // rounds: the pending entries with their stored solutions
 template <int NS, typename DT, bool KIN>
-__global__ void __launch_bounds__(64) k_qp_tangent_replay(QpArgs a) {
+__global__ void __launch_bounds__(64) QP_WAVES k_qp_tangent_replay(QpArgs a) {
   const unsigned n = min(*a.qcount, a.sing.cap);
   for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
     if (a.sing.done[e]) continue;
@@ -609,7 +615,7 @@ __global__ void __launch_bounds__(64) k_qp_tangent_replay(QpArgs a) {
 }
 // pass 2: the queued qps, grid-stride, pseudo-inverse allowed
 template <int NS, typename DT, bool KIN>
-__global__ void __launch_bounds__(64) k_qp_tangent_deferred(QpArgs a) {
+__global__ void __launch_bounds__(64) QP_WAVES k_qp_tangent_deferred(QpArgs a) {
   const unsigned n = *a.qcount;
   for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     if (i >= a.sing.cap || !a.sing.done[i]) {
@@ -1455,7 +1461,8 @@ void sing_svd_round(pa_context *c, const unsigned *count) {
 
 template <int NS, typename DT, bool KIN>
 void qp_tangent_passes_k(pa_context *c, const QpArgs &a, bool main_only) {
-  hipLaunchKernelGGL((k_qp_tangent<NS, DT, KIN>), dim3(nblk(c->nq, 256)), dim3(256), 0, c->stream, a);
+  // one-wave blocks, a grid of 2048 (eight per CU) striding over the qps
+  hipLaunchKernelGGL((k_qp_tangent<NS, DT, KIN>), dim3(std::min(2048u, nblk(c->nq, 64))), dim3(64), 0, c->stream, a);
   if (main_only) return;
   for (int r = 0; r < SING_ROUNDS; ++r) {
     sing_svd_round<NS>(c, a.qcount);
