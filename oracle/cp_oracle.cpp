@@ -160,7 +160,11 @@ void elasticmoduli(const M3 &R, const double C[6][6], double Ar[6][6]) {
 // stays, the others turn; every pair meets once in the m-1 rounds of a sweep, and the m/2 pairs of
 // a round share no column.  Returns p < q; q = n is the padding column of odd n (no rotation).
 void rr_pair(int m, int r, int k, int &p, int &q) {
-  const int a = (k == 0) ? r : (r + k) % (m - 1), b = (k == 0) ? m - 1 : (r - k + m - 1) % (m - 1);
+  // (r + k) mod (m - 1) and (r - k) mod (m - 1); both sums lie in [0, 2 (m - 1))
+  int a = r + k, b = r - k + m - 1;
+  if (a >= m - 1) a -= m - 1;
+  if (b >= m - 1) b -= m - 1;
+  if (k == 0) { a = r; b = m - 1; }
   p = std::min(a, b);
   q = std::max(a, b);
 }

@@ -1450,7 +1450,7 @@ int ns_template(int ns) { return ns <= 12 ? 12 : (ns <= 24 ? 24 : 48); }
 
 bool mixed(const pa_context *c) { return c->precision == PA_PREC_MIXED; }
 
-constexpr unsigned DEFER_GRID = 1024;  // 64-lane blocks of the deferred pass
+constexpr unsigned DEFER_GRID = 2048;  // 64-lane blocks of the replay / deferred passes (two waves per SIMD)
 
 constexpr unsigned SVD_GRID = 8192;    // blocks of k_sing_svd (grid-stride over the entries)
 

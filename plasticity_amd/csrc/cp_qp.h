@@ -155,7 +155,11 @@ CP_DEV bool lu_solve(int n, double *A, double *B, int m) {
 // a round share no column, so they may rotate in any order or at once with the same result.
 // Returns p < q; q = n is the padding column of odd n (no rotation).
 CP_DEV void rr_pair(int m, int r, int k, int &p, int &q) {
-  const int a = (k == 0) ? r : (r + k) % (m - 1), b = (k == 0) ? m - 1 : (r - k + m - 1) % (m - 1);
+  // (r + k) mod (m - 1) and (r - k) mod (m - 1); both sums lie in [0, 2 (m - 1))
+  int a = r + k, b = r - k + m - 1;
+  if (a >= m - 1) a -= m - 1;
+  if (b >= m - 1) b -= m - 1;
+  if (k == 0) { a = r; b = m - 1; }
   p = a < b ? a : b;
   q = a < b ? b : a;
 }
