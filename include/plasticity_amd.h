@@ -148,6 +148,15 @@ int pa_qp_update(pa_context *ctx, long long nqp, const double *F, const double *
                  const unsigned *grain, int flag, double *P, double *T, double *dPdF, double *Fe, double *Fp,
                  double *s_out, double *slip_inc, int *n_active, const double *W_conv, double *W_out);
 
+/* x = pinv(A) b for `count` dense n x n systems (row-major A: n*n per system; b, x: n per
+ * system; n <= 48, count <= 2^20), host buffers, on the current device: the pseudo-inverse step
+ * that inactiveSlipRemoval.cc:26,99 takes with FullMatrix::compute_inverse_svd(0.0) when the
+ * active-set matrix A_PA is singular, with the same small-singular-value rule as the quadrature-
+ * point kernels.  path 0 solves each system in one lane (the routine a quadrature point runs in
+ * its own lane), path 1 with the cooperative kernel that solves queued singular systems (a group
+ * of lanes per system); both give bit-identical x.  No context needed. */
+int pa_pinv_solve(int path, int n, long long count, const double *A, const double *b, double *x);
+
 /* Timing of the last pa_newton_iteration / pa_run_increment phases on the context stream (ms):
  * [0] assembly (qp + tangent + K_e + scatter), [1] Krylov, [2] post-increment pass. */
 int pa_last_timings(pa_context *ctx, double ms[3]);

@@ -519,6 +519,16 @@ __global__ void __launch_bounds__(sing_per_block<NS>() * sing_group<NS>()) k_sin
   }
 }
 
+// svd_pinv_solve in one lane per system, for pa_pinv_solve (A: [count][NS*NS], b and x: [count][NS])
+template <int NS>
+__global__ void __launch_bounds__(64) k_pinv_lane(unsigned count, int n, const double *A, const double *b, double *x) {
+  const unsigned e = blockIdx.x * 64 + threadIdx.x;
+  if (e >= count) return;
+  double xl[NS];
+  cp::svd_pinv_solve<NS>(n, A + (size_t)e * NS * NS, b + (size_t)e * NS, xl);
+  for (int i = 0; i < n; ++i) x[(size_t)e * NS + i] = xl[i];
+}
+
 // ------------------------------------------------------------ assembly
 // ---- assembly, stage 1: one lane per qp.  F from u, calculatePlasticity(.,.,1)
 // (getElementalValues.cc:129-141); writes the iter history and JxW * dP/dF (as DT) and JxW * P.
@@ -2289,6 +2299,56 @@ int pa_qp_update(pa_context *c, long long n, const double *F, const double *Fp_c
               (flag && hipMemcpy(dPdF, dD, 81 * n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) ||
               (c->kin && W_out && hipMemcpy(W_out, dW, bs, hipMemcpyDeviceToHost) != hipSuccess)))
     rc = fail(PA_ERR_HIP, "copy outputs");
+  for (void *p : tmp) hipFree(p);
+  return rc;
+}
+
+int pa_pinv_solve(int path, int n, long long count, const double *A, const double *b, double *x) {
+  if ((path != 0 && path != 1) || n < 1 || n > 48 || count < 0 || count > (1ll << 20) || (count && (!A || !b || !x)))
+    return fail(PA_ERR_ARG, "bad argument");
+  if (count == 0) return 0;
+  const int NS = ns_template(n);
+  const unsigned cnt = (unsigned)count;
+  std::vector<double> hA((size_t)cnt * NS * NS, 0.0), hb((size_t)cnt * NS, 0.0), hx((size_t)cnt * NS, 0.0);
+  for (size_t e = 0; e < cnt; ++e)
+    for (int i = 0; i < n; ++i) {
+      for (int j = 0; j < n; ++j) hA[(e * NS + i) * NS + j] = A[(e * n + i) * n + j];
+      hb[e * NS + i] = b[e * n + i];
+    }
+  std::vector<void *> tmp;
+  auto al = [&](void **p, size_t bytes) { if (hipMalloc(p, bytes) != hipSuccess) return false; tmp.push_back(*p); return true; };
+  SingStore st{};
+  unsigned *dcount = nullptr;
+  int rc = 0;
+  if (!(al((void **)&st.A, hA.size() * sizeof(double)) && al((void **)&st.b, hb.size() * sizeof(double)) &&
+        al((void **)&st.X, hx.size() * sizeof(double)) && al((void **)&st.n, cnt * sizeof(int)) &&
+        al((void **)&st.have, cnt * sizeof(int)) && al((void **)&st.done, cnt * sizeof(int)) && al((void **)&dcount, sizeof(unsigned))))
+    rc = fail(PA_ERR_HIP, "hipMalloc");
+  st.cap = cnt;
+  const std::vector<int> hn(cnt, n);
+  if (!rc && (hipMemcpy(st.A, hA.data(), hA.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy(st.b, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy(st.X, hx.data(), hx.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy(st.n, hn.data(), cnt * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy(dcount, &cnt, sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemset(st.have, 0, cnt * sizeof(int)) != hipSuccess || hipMemset(st.done, 0, cnt * sizeof(int)) != hipSuccess))
+    rc = fail(PA_ERR_HIP, "copy inputs");
+  if (!rc) {
+    if (path == 0) {
+      if (NS == 12) hipLaunchKernelGGL(k_pinv_lane<12>, dim3(nblk(cnt, 64)), dim3(64), 0, 0, cnt, n, st.A, st.b, st.X);
+      else if (NS == 24) hipLaunchKernelGGL(k_pinv_lane<24>, dim3(nblk(cnt, 64)), dim3(64), 0, 0, cnt, n, st.A, st.b, st.X);
+      else hipLaunchKernelGGL(k_pinv_lane<48>, dim3(nblk(cnt, 64)), dim3(64), 0, 0, cnt, n, st.A, st.b, st.X);
+    } else {
+      if (NS == 12) hipLaunchKernelGGL(k_sing_svd<12>, dim3(SVD_GRID), dim3(sing_per_block<12>() * sing_group<12>()), 0, 0, st, (const unsigned *)dcount);
+      else if (NS == 24) hipLaunchKernelGGL(k_sing_svd<24>, dim3(SVD_GRID), dim3(sing_per_block<24>() * sing_group<24>()), 0, 0, st, (const unsigned *)dcount);
+      else hipLaunchKernelGGL(k_sing_svd<48>, dim3(SVD_GRID), dim3(sing_per_block<48>() * sing_group<48>()), 0, 0, st, (const unsigned *)dcount);
+    }
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = fail(PA_ERR_HIP, "pinv kernel");
+  }
+  if (!rc && hipMemcpy(hx.data(), st.X, hx.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) rc = fail(PA_ERR_HIP, "copy outputs");
+  if (!rc)
+    for (size_t e = 0; e < cnt; ++e)
+      for (int i = 0; i < n; ++i) x[e * n + i] = hx[e * NS + i];
   for (void *p : tmp) hipFree(p);
   return rc;
 }

@@ -466,10 +466,11 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
   if (!flag) return n_active;
 
   // ---- tangent: in-loop block (:473-828) with delFp_delF_prev = 0 and dels_delF = 0
-  double dFp[81];
-#pragma unroll
-  for (int i = 0; i < 81; ++i) dFp[i] = 0.0;
-  if (plastic && nPA > 0) {
+  // delFp_delF = sum_j SV_j (x) delgamma_j (rank nPA): SV[j] (9) and delgamma (rhs, 9 per j) are
+  // kept and each column kl of it is formed where the final tangent needs it
+  double rhs[NS * 9], SV[NS * 9];
+  const int nsv = (plastic && nPA > 0) ? nPA : 0;
+  if (nsv > 0) {
     // count3 (:607-619): number of series terms until ||X^k / k!||_F <= max(tol/1e4, 1e-12)
     int count3 = 0;
     {
@@ -488,7 +489,6 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
       }
     }
     // delb (:581-604): delb(k, kl) = +- S_k : deltau_delF(:, kl), streamed per column kl
-    double rhs[NS * 9];
     for (int kl = 0; kl < 9; ++kl) {
       const int k = kl / 3, l = kl % 3;
       double dE[9], dT[9], dtau[9];
@@ -561,11 +561,7 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
       m3_mul(diff, Fp_t, t1);
       const double sg = ((tau_tr[j] - wt(j)) < 0) ? -1.0 : 1.0;
 #pragma unroll
-      for (int r = 0; r < 9; ++r) {
-        const double sv = sg * t1[r];
-#pragma unroll
-        for (int c = 0; c < 9; ++c) dFp[9 * r + c] += sv * rhs[jj * 9 + c];
-      }
+      for (int r = 0; r < 9; ++r) SV[jj * 9 + r] = sg * t1[r];
     }
   }
 
@@ -577,7 +573,15 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
   m3_mult(q3, F, q5);
   for (int kl = 0; kl < 9; ++kl) {
     const int k = kl / 3, l = kl % 3;
-    double dFe[9], W[9], dTs[9];
+    double dFe[9], W[9], dTs[9], dFp[9];
+    // column kl of delFp_delF, summed over the active systems in order
+#pragma unroll
+    for (int r = 0; r < 9; ++r) dFp[r] = 0.0;
+    for (int jj = 0; jj < nsv; ++jj) {
+      const double g = rhs[jj * 9 + kl];
+#pragma unroll
+      for (int r = 0; r < 9; ++r) dFp[r] += SV[jj * 9 + r] * g;
+    }
     // dFe(ij,kl) = d_ik Fpinv(l,j) - sum_ab FE(i,a) dFp(ab,kl) Fpinv(b,j)
 #pragma unroll
     for (int a = 0; a < 3; ++a)
@@ -585,7 +589,7 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
       for (int j = 0; j < 3; ++j) {
         double sacc = 0;
 #pragma unroll
-        for (int bb = 0; bb < 3; ++bb) sacc += dFp[9 * (3 * a + bb) + kl] * Fpinv[3 * bb + j];
+        for (int bb = 0; bb < 3; ++bb) sacc += dFp[3 * a + bb] * Fpinv[3 * bb + j];
         W[3 * a + j] = sacc;  // (dFp . Fpinv)(a, j)
       }
 #pragma unroll

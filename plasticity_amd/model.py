@@ -292,6 +292,18 @@ class Context:
         return out
 
 
+def pinv_solve(A, b, path=0):
+    """x = pinv(A) b for a batch of n x n systems (A: (count, n, n), b: (count, n)) on the GPU:
+    the pseudo-inverse step of inactiveSlipRemoval.cc:26,99; path 0 = one lane per system,
+    1 = the cooperative kernel (a group of lanes per system)."""
+    A = f64(A)
+    count, n = A.shape[0], A.shape[1]
+    b = f64(b).reshape(count, n)
+    x = np.zeros((count, n))
+    check(native.lib().pa_pinv_solve(ctypes.c_int(path), ctypes.c_int(n), ctypes.c_longlong(count), ptr(A), ptr(b), ptr(x)))
+    return x
+
+
 def make_material(m, n, q, C, s0, h0, ss, pw, tol, max_slip_search=1, max_solver_iter=1, C1=None, C2=None):
     """pa_material from the prm inputs; C1, C2: kinematic hardening (C_1 / C_2 Slip Kinematic
     Hardening), off when None."""

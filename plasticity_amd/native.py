@@ -21,7 +21,7 @@ EXPORTS = [
     "pa_last_error", "pa_create", "pa_destroy", "pa_element_grains", "pa_set_grains", "pa_set_solver",
     "pa_newton_iteration", "pa_assemble", "pa_linear_solve", "pa_commit_increment", "pa_run_increment",
     "pa_get_state", "pa_get_csr", "pa_sizes", "pa_qp_update", "pa_last_timings", "pa_time_kernel",
-    "pa_last_counts",
+    "pa_last_counts", "pa_pinv_solve",
 ]
 
 

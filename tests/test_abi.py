@@ -33,6 +33,16 @@ def test_error_path_without_gpu_is_reported():
     assert b"n_slip" in nat.lib().pa_last_error()
 
 
+def test_pinv_solve_rejects_bad_arguments_before_any_device_call():
+    import numpy as np
+    import pytest
+    import plasticity_amd as pa
+    with pytest.raises(pa.PlasticityError):
+        pa.model.pinv_solve(np.eye(3)[None], np.ones((1, 3)), path=2)
+    with pytest.raises(pa.PlasticityError):
+        pa.model.pinv_solve(np.zeros((1, 49, 49)), np.zeros((1, 49)))
+    assert pa.model.pinv_solve(np.zeros((0, 4, 4)), np.zeros((0, 4))).shape == (0, 4)
+
 def test_struct_layout_matches_header(tmp_path):
     """ctypes mirrors of pa_material / pa_problem have the sizes and field offsets the C compiler
     gives the header's structs."""
