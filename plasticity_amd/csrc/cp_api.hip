@@ -1711,15 +1711,18 @@ int mg_vcycle_t(pa_context *c) {
   auto mask = [&](int l) { return l == 0 ? (const unsigned char *)c->con : nullptr; };
   int lt = 1;
   while (lt < L && c->mg_m[lt].nnode > MG_TAIL_NODES) ++lt;
+  // one-wave blocks below level 0, so that the coarser levels spread over many CUs (level 0 keeps
+  // VB-lane blocks: its block partials feed the Krylov kernels)
+  auto bs = [&](int l) { return l == 0 ? (unsigned)VB : 64u; };
   for (int l = 0; l < lt; ++l) {
     const MeshDims &m = c->mg_m[l];
-    const unsigned g = nblk(m.nnode, VB);
+    const unsigned g = nblk(m.nnode, bs(l));
     const VT *A = (const VT *)c->mg_A[l];
-    hipLaunchKernelGGL(k_mg_pre<VT>, dim3(g), dim3(VB), 0, c->stream, m, A, (const double *)c->mg_dinv[l], (const double *)(c->mg_lam + l),
+    hipLaunchKernelGGL(k_mg_pre<VT>, dim3(g), dim3(bs(l)), 0, c->stream, m, A, (const double *)c->mg_dinv[l], (const double *)(c->mg_lam + l),
                        (const double *)c->mg_b[l], c->mg_x[l], c->mg_d[l], c->mg_r[l], st);
-    hipLaunchKernelGGL(k_mg_resid<VT>, dim3(g), dim3(VB), 0, c->stream, m, A, (const double *)c->mg_d[l], c->mg_r[l], st);
+    hipLaunchKernelGGL(k_mg_resid<VT>, dim3(g), dim3(bs(l)), 0, c->stream, m, A, (const double *)c->mg_d[l], c->mg_r[l], st);
     if (l + 1 < lt)
-      hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(c->mg_m[l + 1].nnode, VB)), dim3(VB), 0, c->stream, m, (const double *)c->mg_r[l],
+      hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(c->mg_m[l + 1].nnode, 64)), dim3(64), 0, c->stream, m, (const double *)c->mg_r[l],
                          mask(l), c->mg_m[l + 1], c->mg_b[l + 1], st);
   }
   MgTail t;
@@ -1743,18 +1746,18 @@ int mg_vcycle_t(pa_context *c) {
   hipLaunchKernelGGL(k_mg_tail<VT>, dim3(1), dim3(MG_TAIL_THREADS), 0, c->stream, t, st);
   for (int l = lt - 1; l >= 0; --l) {
     const MeshDims &m = c->mg_m[l];
-    const unsigned g = nblk(m.nnode, VB);
+    const unsigned g = nblk(m.nnode, bs(l));
     const VT *A = (const VT *)c->mg_A[l];
-    hipLaunchKernelGGL(k_mg_prolong, dim3(g), dim3(VB), 0, c->stream, m, c->mg_x[l], mask(l), c->mg_m[l + 1],
+    hipLaunchKernelGGL(k_mg_prolong, dim3(g), dim3(bs(l)), 0, c->stream, m, c->mg_x[l], mask(l), c->mg_m[l + 1],
                        (const double *)c->mg_x[l + 1], st);
-    hipLaunchKernelGGL(k_mg_post1<VT>, dim3(g), dim3(VB), 0, c->stream, m, A, (const double *)c->mg_dinv[l],
+    hipLaunchKernelGGL(k_mg_post1<VT>, dim3(g), dim3(bs(l)), 0, c->stream, m, A, (const double *)c->mg_dinv[l],
                        (const double *)(c->mg_lam + l), (const double *)c->mg_b[l], (const double *)c->mg_x[l], c->mg_d[l], c->mg_r[l], st);
     if (l == 0)
       hipLaunchKernelGGL((k_mg_post2<VT, true>), dim3(g), dim3(VB), 0, c->stream, m, A, (const double *)c->mg_dinv[l],
                          (const double *)(c->mg_lam + l), (const double *)c->mg_b[l], c->mg_x[l], (const double *)c->mg_d[l],
                          (const double *)c->mg_r[l], st, c->partials);
     else
-      hipLaunchKernelGGL((k_mg_post2<VT, false>), dim3(g), dim3(VB), 0, c->stream, m, A, (const double *)c->mg_dinv[l],
+      hipLaunchKernelGGL((k_mg_post2<VT, false>), dim3(g), dim3(bs(l)), 0, c->stream, m, A, (const double *)c->mg_dinv[l],
                          (const double *)(c->mg_lam + l), (const double *)c->mg_b[l], c->mg_x[l], (const double *)c->mg_d[l],
                          (const double *)c->mg_r[l], st, (double *)nullptr);
   }

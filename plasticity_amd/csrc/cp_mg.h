@@ -354,27 +354,28 @@ __device__ __forceinline__ void prolong_node(const MeshDims &mf, double *__restr
     if (!(con && con[3 * n + r])) xf[3 * n + r] += o[r];
 }
 
-// ---- per-level kernels (lane = node)
+// ---- per-level kernels (lane = node; blocks of at most VB lanes, exactly VB for the block partials of
+// level 0)
 
 template <typename VT>
 __global__ void __launch_bounds__(VB) k_mg_pre(MeshDims m, const VT *__restrict__ A, const double *__restrict__ dinv,
                                               const double *__restrict__ lam, const double *__restrict__ b, double *__restrict__ x,
                                               double *__restrict__ d, double *__restrict__ res, const KState *st) {
   if (solve_done(st)) return;
-  const long long n = blockIdx.x * (long long)VB + threadIdx.x;
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n < m.nnode) mg_pre_node<mg_unroll<VT>()>(m, A, dinv, cheb2(*lam), b, x, d, res, n);
 }
 template <typename VT>
 __global__ void __launch_bounds__(VB) k_mg_resid(MeshDims m, const VT *__restrict__ A, const double *__restrict__ d,
                                                 double *__restrict__ res, const KState *st) {
   if (solve_done(st)) return;
-  const long long n = blockIdx.x * (long long)VB + threadIdx.x;
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n < m.nnode) mg_resid_node<mg_unroll<VT>()>(m, A, d, res, n);
 }
 __global__ void __launch_bounds__(VB) k_mg_restrict(MeshDims mf, const double *__restrict__ rf, const unsigned char *__restrict__ con,
                                                    MeshDims mc, double *__restrict__ bc, const KState *st) {
   if (solve_done(st)) return;
-  const long long I = blockIdx.x * (long long)VB + threadIdx.x;
+  const long long I = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (I >= mc.nnode) return;
   int ix, iy, iz;
   node_ijk32(mc, I, ix, iy, iz);
@@ -386,7 +387,7 @@ __global__ void __launch_bounds__(VB) k_mg_restrict(MeshDims mf, const double *_
 __global__ void __launch_bounds__(VB) k_mg_prolong(MeshDims mf, double *__restrict__ xf, const unsigned char *__restrict__ con,
                                                   MeshDims mc, const double *__restrict__ xc, const KState *st) {
   if (solve_done(st)) return;
-  const long long n = blockIdx.x * (long long)VB + threadIdx.x;
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n < mf.nnode) prolong_node(mf, xf, con, mc, xc, n);
 }
 template <typename VT>
@@ -395,7 +396,7 @@ __global__ void __launch_bounds__(VB) k_mg_post1(MeshDims m, const VT *__restric
                                                 const double *__restrict__ x, double *__restrict__ d, double *__restrict__ res,
                                                 const KState *st) {
   if (solve_done(st)) return;
-  const long long n = blockIdx.x * (long long)VB + threadIdx.x;
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n < m.nnode) mg_post1_node<mg_unroll<VT>()>(m, A, dinv, cheb2(*lam), b, x, d, res, n);
 }
 // on level 0 (ROOT) the block partials of b.x (= r.z of the Krylov solve) follow
@@ -406,7 +407,7 @@ __global__ void __launch_bounds__(VB) k_mg_post2(MeshDims m, const VT *__restric
                                                 double *partials) {
   __shared__ double lds[16];
   if (solve_done(st)) return;
-  const long long n = blockIdx.x * (long long)VB + threadIdx.x;
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   double acc[1] = {0.0};
   if (n < m.nnode) acc[0] = mg_post2_node<mg_unroll<VT>()>(m, A, dinv, cheb2(*lam), b, x, d, res, n);
   if (ROOT) block_partials<1>(acc, partials, lds);
@@ -418,8 +419,8 @@ __global__ void __launch_bounds__(VB) k_mg_post2(MeshDims m, const VT *__restric
 // by one block of MG_TAIL_THREADS lanes, phases separated by block barriers: restriction of the
 // residual of level lt-1, the downward leg, the dense solve, the upward leg.  The result is x of
 // level lt; its prolongation to level lt-1 is a per-level kernel again.
-constexpr int MG_TAIL_NODES = 1024;
-constexpr int MG_TAIL_THREADS = 1024;
+constexpr int MG_TAIL_NODES = 256;
+constexpr int MG_TAIL_THREADS = 256;
 struct MgTail {
   int lt, L;
   MeshDims mf;                    // level lt-1
@@ -433,7 +434,7 @@ struct MgTail {
 };
 template <typename VT>
 __global__ void __launch_bounds__(MG_TAIL_THREADS) k_mg_tail(MgTail t, const KState *st) {
-  constexpr int TU = sizeof(VT) == 4 ? 3 : 2;  // 1024 lanes leave 128 VGPRs each
+  constexpr int TU = sizeof(VT) == 4 ? 3 : 2;
   __shared__ double bc[MG_COARSE_DOFS];
   if (solve_done(st)) return;
   const int L = t.L;
@@ -527,7 +528,7 @@ __global__ void __launch_bounds__(VB) k_mgcg_p(long long N, const double *__rest
     b = rz[0] / st->rzh[(it - 1) & 1];
   }
   const long long stride = (long long)gridDim.x * VB;
-  for (long long r0 = blockIdx.x * (long long)VB + threadIdx.x; r0 < N; r0 += VPT * stride) {
+  for (long long r0 = blockIdx.x * (long long)blockDim.x + threadIdx.x; r0 < N; r0 += VPT * stride) {
     double z[VPT], p[VPT];
 #pragma unroll
     for (int j = 0; j < VPT; ++j) {
@@ -556,7 +557,7 @@ __global__ void __launch_bounds__(VB) k_mgcg_update(long long N, double *__restr
   const double a = st->rzh[it & 1] / pq[0];
   const long long stride = (long long)gridDim.x * VB;
   double acc[1] = {0.0};
-  for (long long r0 = blockIdx.x * (long long)VB + threadIdx.x; r0 < N; r0 += VPT * stride) {
+  for (long long r0 = blockIdx.x * (long long)blockDim.x + threadIdx.x; r0 < N; r0 += VPT * stride) {
     double x[VPT], p[VPT], rr[VPT], q[VPT], d[VPT];
 #pragma unroll
     for (int j = 0; j < VPT; ++j) {
