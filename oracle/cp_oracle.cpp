@@ -930,13 +930,17 @@ int pcg(const CSR &A, const std::vector<double> &b, std::vector<double> &x, doub
 }
 
 // symmetric 3x3 eigen-decomposition by cyclic Jacobi rotations (the reference calls
-// LAPACK dsyevx through compute_eigenvalues_symmetric, reorient.cc:35,48)
+// LAPACK dsyevx through compute_eigenvalues_symmetric, reorient.cc:35,48).  The sweeps end when
+// the off-diagonal is zero or a sweep leaves the eigenvalues and V unchanged (later rotations are
+// smaller still, down to the identity once th * th overflows).
 void sym_eig3(const M3 &Cin, double lam[3], M3 &V) {
   M3 C = Cin;
   V = eye3();
   for (int sweep = 0; sweep < 50; ++sweep) {
     const double off = C[0][1] * C[0][1] + C[0][2] * C[0][2] + C[1][2] * C[1][2];
     if (off == 0.0) break;
+    const M3 V0 = V;
+    const double d0[3] = {C[0][0], C[1][1], C[2][2]};
     for (int p = 0; p < 2; ++p)
       for (int q = p + 1; q < 3; ++q) {
         if (C[p][q] == 0.0) continue;
@@ -956,6 +960,10 @@ void sym_eig3(const M3 &Cin, double lam[3], M3 &V) {
           V[k][p] = c * vkp - s * vkq; V[k][q] = s * vkp + c * vkq;
         }
       }
+    bool same = d0[0] == C[0][0] && d0[1] == C[1][1] && d0[2] == C[2][2];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) same = same && V0[i][j] == V[i][j];
+    if (same) break;
   }
   for (int i = 0; i < 3; ++i) lam[i] = C[i][i];
 }

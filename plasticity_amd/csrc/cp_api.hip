@@ -787,32 +787,49 @@ struct PostArgs {
   SingStore sing;            // their singular systems
 };
 
-// symmetric 3x3 eigen-decomposition by cyclic Jacobi rotations
+// symmetric 3x3 eigen-decomposition by cyclic Jacobi rotations.  The sweeps end when the
+// off-diagonal is zero or a sweep leaves the eigenvalues (diagonal of C) and V unchanged: the
+// rotations only get smaller from there (an off-diagonal below ~1e-154 of the diagonal even gives
+// the identity rotation, since th * th overflows), so further sweeps return the same result.
 __device__ void sym_eig3(const double *Cin, double lam[3], double V[9]) {
   double C[9];
+#pragma unroll
   for (int i = 0; i < 9; ++i) { C[i] = Cin[i]; V[i] = (i % 4 == 0) ? 1.0 : 0.0; }
   for (int sweep = 0; sweep < 50; ++sweep) {
     const double off = C[1] * C[1] + C[2] * C[2] + C[5] * C[5];
     if (off == 0.0) break;
+    double prev[12];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) prev[i] = V[i];
+    prev[9] = C[0]; prev[10] = C[4]; prev[11] = C[8];
+#pragma unroll
     for (int p = 0; p < 2; ++p)
+#pragma unroll
       for (int q = p + 1; q < 3; ++q) {
         if (C[3 * p + q] == 0.0) continue;
         const double th = (C[3 * q + q] - C[3 * p + p]) / (2.0 * C[3 * p + q]);
         const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
         const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+#pragma unroll
         for (int k = 0; k < 3; ++k) {
           const double ckp = C[3 * k + p], ckq = C[3 * k + q];
           C[3 * k + p] = c * ckp - s * ckq; C[3 * k + q] = s * ckp + c * ckq;
         }
+#pragma unroll
         for (int k = 0; k < 3; ++k) {
           const double cpk = C[3 * p + k], cqk = C[3 * q + k];
           C[3 * p + k] = c * cpk - s * cqk; C[3 * q + k] = s * cpk + c * cqk;
         }
+#pragma unroll
         for (int k = 0; k < 3; ++k) {
           const double vkp = V[3 * k + p], vkq = V[3 * k + q];
           V[3 * k + p] = c * vkp - s * vkq; V[3 * k + q] = s * vkp + c * vkq;
         }
       }
+    bool same = prev[9] == C[0] && prev[10] == C[4] && prev[11] == C[8];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) same = same && prev[i] == V[i];
+    if (same) break;
   }
   lam[0] = C[0]; lam[1] = C[4]; lam[2] = C[8];
 }
