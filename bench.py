@@ -32,7 +32,7 @@ FP64_PEAK_TFLOPS = 78.6
 # committed profile.  Masked-off lanes are counted, so this is an upper bound of the useful work.
 PMC_QP_FLOPS = {("mixed", 64): (57443100 + 108127000 + 2 * 375451000 + 4212740) * 64}
 # HBM bytes (KiB) per launch of the same pass: FETCH_SIZE x 2 + WRITE_SIZE, as for the Krylov operator
-PMC_QP_TRAFFIC_KIB = {("mixed", 64): 9114060 * 2 + 17337100}
+PMC_QP_TRAFFIC_KIB = {("mixed", 64): 26750200 * 2 + 18846400}
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the Jacobi-PCG operator kernel k_cg_spmv at the 64^3 workload, from two
 # rocprofv3 PMC passes (FETCH_SIZE x 2 + WRITE_SIZE x 1 KiB: the factors that make both counters
