@@ -182,6 +182,46 @@ int pa_last_counts(pa_context *ctx, long long counts[3]);
  * is not re-zeroed between repetitions, and the Krylov vectors are overwritten (timing only). */
 int pa_time_kernel(pa_context *ctx, int which, int reps, double *avg_ms);
 
+/* ---- Learned warm start of the local return map (no counterpart in the reference: its local
+ * loop, calculatePlasticity.cc:311-441, always starts from zero slip).  An MLP 18 -> 64 -> 64 -> 64
+ * -> 12 (SiLU; BF16 weights and activations on the matrix cores, FP32 accumulation and master
+ * weights) predicts the 12 slip increments of an FCC quadrature point from its 12 trial ratios
+ * (tau_a - W_a)/s_a and the lattice-frame increment of its elastic Green-Lagrange strain (6, Voigt
+ * 11,22,33,23,13,12).  The plain local loop then corrects the prediction and judges convergence by
+ * the reference's test (||b_PA||_inf <= Stress Tolerance), so the network changes how many local
+ * solves a qp takes, not what it converges to.
+ *
+ * Parameter blob (PA_MLP_NBLOB floats): per layer l = 0..3, W^T[N_l][K_l] row-major then b[N_l],
+ * padded sizes (K, N) = (32, 64), (64, 64), (64, 64), (64, 16) (padding zero); then in_mean[18],
+ * in_scale[18], out_scale[12].  Network input = (x - in_mean) * in_scale, prediction = y * out_scale. */
+#define PA_MLP_NIN 18
+#define PA_MLP_NOUT 12
+#define PA_MLP_NPARAM 11472
+#define PA_MLP_NBLOB 11520
+typedef struct pa_mlp pa_mlp;
+int pa_mlp_create(pa_mlp **out);
+int pa_mlp_destroy(pa_mlp *net);
+int pa_mlp_set_params(pa_mlp *net, const float *blob);   /* resets the Adam moments */
+int pa_mlp_get_params(pa_mlp *net, float *blob);
+/* Y[n][12] = prediction for X[n][18] (host buffers). */
+int pa_mlp_predict(pa_mlp *net, long long n, const float *X, float *Y);
+/* Gradient of the loss mean_{samples, 12 outputs} (y - Y/out_scale)^2 w.r.t. the PA_MLP_NPARAM
+ * weights and biases, and the loss, over n <= 65536 samples as one batch. */
+int pa_mlp_grad(pa_mlp *net, long long n, const float *X, const float *Y, float *grad, double *loss);
+/* Adam (beta 0.9/0.999, eps 1e-8) on that loss, minibatches of `batch` samples in a seeded
+ * per-epoch permutation, learning rate cosine-decayed from lr to lr/20; epoch_loss (optional,
+ * `epochs` entries) receives the mean minibatch loss of each epoch.  Runs on the device. */
+int pa_mlp_train(pa_mlp *net, long long n, const float *X, const float *Y, int epochs, int batch, float lr,
+                 unsigned long long seed, double *epoch_loss);
+/* pa_qp_update for FCC without kinematic hardening, with the local loop started from the network's
+ * prediction (net may be NULL: the reference's start from zero).  Fe_conv (9/qp) enters the strain
+ * increment input.  iters: local solves per qp; feat (optional, 18/qp): the network inputs;
+ * x0 (optional, 12/qp, net only): the predictions.  Material Max Solver Iterations caps the loop. */
+int pa_qp_update_warm(pa_context *ctx, pa_mlp *net, long long nqp, const double *F, const double *Fp_conv,
+                      const double *Fe_conv, const double *s_conv, const unsigned *grain, int flag, double *P,
+                      double *T, double *dPdF, double *Fe, double *Fp, double *s_out, double *slip_inc,
+                      int *n_active, int *iters, float *feat, float *x0);
+
 #ifdef __cplusplus
 }
 #endif

@@ -401,9 +401,11 @@ M3 dexp_series(const M3 &X, const M3 &Sj, int count3, double sign) {
 // calculatePlasticity(cellID, q, flag), CP calculatePlasticity.cc:13-959 (twinning branch :962-1019 off)
 int qp_update(const or_material *mat, const Grain &g, const double *Fin, const double *Fp_conv, const double *s_conv,
               const double *W_conv, const double *slip_conv, int flag, double *Pout, double *Tout, double *dPdF,
-              double *Fe_out, double *Fp_out, double *s_out, double *W_out, double *slip_iter) {
+              double *Fe_out, double *Fp_out, double *s_out, double *W_out, double *slip_iter,
+              const double *x0 = nullptr, int *iters = nullptr) {
   const int nT = mat->nT, ns = mat->n_slip, ntw = mat->n_twin;
   const double tol1 = mat->tol;
+  if (iters) *iters = 0;
   const M3 F = load3(Fin);
   const M3 FP_t = load3(Fp_conv);
   std::vector<double> s_tau(s_conv, s_conv + nT), W_tau(W_conv, W_conv + nT), W_t(W_conv, W_conv + nT);
@@ -474,13 +476,9 @@ int qp_update(const or_material *mat, const Grain &g, const double *Fin, const d
       }
 
     std::fill(x_old.begin(), x_old.end(), 0.0);           // :311
-    unsigned count1 = 0;
     auto linf_bPA = [&]() { double m = 0; for (int p : PA) m = std::max(m, std::fabs(b[p])); return m; };
-    while (linf_bPA() > tol1) {                           // :322
-      count1++;
-      if (count1 > (unsigned)mat->max_solver_iter) break;
-      std::fill(x_beta.begin(), x_beta.end(), 0.0);
-      inactive_slip_removal(nT, x_old, x_beta, PA, b, A, mat->pinv_mode);  // :336
+    // one update of the loop body from x_beta (increment) and x_old (total) (:338-440)
+    auto update = [&]() {
       del_FP = zero3();                                   // :338-357
       for (int i = 0; i < nT; ++i)
         for (int j = 0; j < 3; ++j) for (int k = 0; k < 3; ++k) {
@@ -511,8 +509,79 @@ int qp_update(const or_material *mat, const Grain &g, const double *Fin, const d
         else W_tau[i] = W_tau[i] + (-mat->C1[i] - mat->C2[i] * W_tau[i]) * x_beta[i];
       }
       for (int i = 0; i < nT; ++i) b[i] = std::fabs(tau[i] - W_tau[i]) - s_tau[i];  // :438-440
+    };
+    // the local loop (:322-441), `done` solves already counted against Max Solver Iterations;
+    // `stop_on_drop`: stop as soon as inactive-slip removal drops a system.  Returns the solves.
+    auto local_loop = [&](bool, bool stop_on_drop, unsigned done = 0) {
+      unsigned count1 = done;
+      while (linf_bPA() > tol1) {                         // :322
+        count1++;
+        if (count1 > (unsigned)mat->max_solver_iter) break;
+        std::fill(x_beta.begin(), x_beta.end(), 0.0);
+        const size_t n_before = PA.size();
+        inactive_slip_removal(nT, x_old, x_beta, PA, b, A, mat->pinv_mode);  // :336
+        if (stop_on_drop && PA.size() < n_before) { update(); break; }
+        update();
+      }
+      return (int)(std::min(count1, (unsigned)mat->max_solver_iter) - done);
+    };
+    // Warm start (no counterpart in the reference, which always starts from x_beta_old = 0, :311).
+    // The reference's loop is path dependent: its first solve on the trial set, with inactive-slip
+    // removal, fixes the set S1 the later iterations work in (they only drop systems), and its
+    // converged states routinely leave dropped systems above the yield surface, so a converged
+    // start on another set is another answer.  The first solve therefore always runs.  If the
+    // predicted set {a in PA : x0[a] > 0} equals S1, its increment is replaced by x0 (applied as one
+    // pass of the loop body) and the loop continues on S1; the start stands if the loop converges
+    // without dropping a system, otherwise the loop restarts from zero.  If the sets differ the
+    // loop simply continues from the first solve, as without a prediction.  iters counts every
+    // solve, the restarted ones included.
+    int n_solves = 0;
+    std::vector<int> PAw;
+    if (x0)
+      for (int p : PA) if (p < ns && x0[p] > 0) PAw.push_back(p);
+    if (!PAw.empty()) {
+      const std::vector<int> PA_trial = PA;
+      const std::vector<double> s_start = s_tau, W_start = W_tau, b_start = b;
+      const M3 FE_start = FE_tau;
+      bool warm_used = false;
+      if (linf_bPA() > tol1 && mat->max_solver_iter >= 1) {
+        std::fill(x_beta.begin(), x_beta.end(), 0.0);
+        inactive_slip_removal(nT, x_old, x_beta, PA, b, A, mat->pinv_mode);  // the first solve (:336)
+        n_solves = 1;
+        std::vector<int> S1 = PA, P2 = PAw;
+        std::sort(S1.begin(), S1.end());
+        if (S1 == P2) {
+          warm_used = true;
+          std::fill(x_beta.begin(), x_beta.end(), 0.0);
+          for (int p : PAw) x_beta[p] = x0[p];
+          x_old = x_beta;
+        }
+        update();
+        if (warm_used) {
+          const size_t n_set = PA.size();
+          const int c = local_loop(false, true);
+          n_solves += c;
+          bool ok = PA.size() == n_set && c < mat->max_solver_iter;
+          for (int p : PA) if (!(std::fabs(b[p]) <= tol1)) ok = false;
+          if (!ok) {
+            PA = PA_trial;
+            s_tau = s_start; W_tau = W_start; b = b_start;
+            FP_tau = FP_t2; FE_tau = FE_start; Ce_tau = zero3(); Tstar_tau = Tstar_tr; tau = tau_tr;
+            std::fill(x_old.begin(), x_old.end(), 0.0);
+            std::fill(x_beta.begin(), x_beta.end(), 0.0);
+            n_solves += local_loop(false, false);
+          }
+        } else {
+          n_solves += local_loop(false, false, 1);
+        }
+      } else {
+        n_solves = local_loop(false, false);
+      }
+    } else {
+      n_solves = local_loop(false, false);
     }
 
+    if (iters) *iters = n_solves;
     for (int i = 0; i < ns; ++i) slip_iter[i] = slip_conv[i] + x_old[i];  // :454-456
 
     Fpn_inv = inv3(FP_tau);                               // :458-468
@@ -1007,6 +1076,16 @@ int or_qp_update(const or_material *mat, const double rot[3], const double F[9],
   Grain g;
   grain_setup(mat, rot, g);
   return qp_update(mat, g, F, Fp_conv, s_conv, W_conv, slip_conv, flag, P, T, dPdF, Fe_iter, Fp_iter, s_iter, W_iter, slip_iter);
+}
+
+int or_qp_update_warm(const or_material *mat, const double rot[3], const double F[9], const double Fp_conv[9],
+                      const double *s_conv, const double *W_conv, const double *slip_conv, int flag, const double *x0,
+                      double P[9], double T[9], double dPdF[81], double Fe_iter[9], double Fp_iter[9], double *s_iter,
+                      double *W_iter, double *slip_iter, int *iters) {
+  Grain g;
+  grain_setup(mat, rot, g);
+  return qp_update(mat, g, F, Fp_conv, s_conv, W_conv, slip_conv, flag, P, T, dPdF, Fe_iter, Fp_iter, s_iter, W_iter,
+                   slip_iter, x0, iters);
 }
 
 void or_pinv_stats(long long *calls, long long *singular) {

@@ -75,6 +75,16 @@ int or_qp_update(const or_material *mat, const double rot[3], const double F[9],
                  double P[9], double T[9], double dPdF[81], double Fe_iter[9],
                  double Fp_iter[9], double *s_iter, double *W_iter, double *slip_iter);
 
+/* or_qp_update with a warm start (no counterpart in the reference): x0 (n_slip values, may be
+ * NULL = the reference's start from zero) holds predicted slip increments; those of the
+ * potentially-active systems that are positive are applied once before the first solve of the
+ * local loop (calculatePlasticity.cc:322-441), which then runs unchanged.  iters receives the
+ * number of local solves the loop took (0 on the elastic path). */
+int or_qp_update_warm(const or_material *mat, const double rot[3], const double F[9], const double Fp_conv[9],
+                      const double *s_conv, const double *W_conv, const double *slip_conv, int flag, const double *x0,
+                      double P[9], double T[9], double dPdF[81], double Fe_iter[9], double Fp_iter[9], double *s_iter,
+                      double *W_iter, double *slip_iter, int *iters);
+
 /* Nearest-voxel grain lookup of a point (crystalOrientationsIO.cc:115-194).
  * vox is [nvx][nvy][nvz] (z fastest) as read from the grain-ID file. */
 unsigned or_material_id(const unsigned *vox, const int nv[3], const double span[3], const double p[3]);

@@ -22,6 +22,7 @@
 #include "../../include/plasticity_amd.h"
 #include "cp_qp.h"
 #include "cp_tables.h"
+#include "cp_mlp.h"
 
 namespace {
 
@@ -1007,6 +1008,44 @@ void launch_qp_batch(hipStream_t st, bool kin, unsigned grid, long long n, const
                        T, D, Fe, Fp, so, xo, na, Wio);
 }
 
+// warm-start inputs of a batch of qps (FCC: 12 ratios + 6 strain components per qp, row-major)
+template <int NS>
+__global__ void __launch_bounds__(64) k_qp_features(long long n, const double *__restrict__ F, const double *__restrict__ Fpc,
+                                                    const double *__restrict__ Fec, const double *__restrict__ sc,
+                                                    const double *__restrict__ Wc, const unsigned *__restrict__ grain,
+                                                    const double *gtab, int gstride, int ns, float *feat) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double f[9], fp[9], fe[9], s[NS], w[NS];
+  for (int k = 0; k < 9; ++k) { f[k] = F[9 * i + k]; fp[k] = Fpc[9 * i + k]; fe[k] = Fec[9 * i + k]; }
+  for (int k = 0; k < ns; ++k) { s[k] = sc[(long long)ns * i + k]; w[k] = Wc ? Wc[(long long)ns * i + k] : 0.0; }
+  cp::qp_features<NS>(gtab + (long long)grain[i] * gstride, mlp::NOUT, f, fp, fe, s, w, feat + (long long)mlp::NIN * i, 1);
+}
+
+// batch of calculatePlasticity calls whose local loop starts from x0 (12 per qp, may be null = the
+// reference's start from zero); iters receives the local solves of each qp
+template <int NS>
+__global__ void __launch_bounds__(64) k_qp_batch_warm(long long n, const double *__restrict__ F, const double *__restrict__ Fpc,
+                                                      const double *__restrict__ sc, const unsigned *__restrict__ grain,
+                                                      const double *gtab, int gstride, const double *mat, int ns, int msi,
+                                                      int flag, double *P, double *T, double *D, double *Fe, double *Fp,
+                                                      double *so, double *xo, int *na, const float *x0, int *iters) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double f[9], fp[9], s[NS], x[NS], p[9], t[9], fe[9], fpo[9];
+  for (int k = 0; k < 9; ++k) { f[k] = F[9 * i + k]; fp[k] = Fpc[9 * i + k]; }
+  for (int k = 0; k < ns; ++k) s[k] = sc[(long long)ns * i + k];
+  const double *gt = gtab + (long long)grain[i] * gstride;
+  int it = 0;
+  const int r = cp::qp_update<NS, double, false, cp::SvdInLane<NS>, true>(
+      mat, gt, ns, msi, f, fp, s, x, flag, p, t, fe, fpo, flag ? D + 81 * i : nullptr, 1, nullptr, 1.0, cp::SvdInLane<NS>(),
+      nullptr, x0 ? x0 + (long long)mlp::NOUT * i : nullptr, 1, &it);
+  for (int k = 0; k < 9; ++k) { P[9 * i + k] = p[k]; T[9 * i + k] = t[k]; Fe[9 * i + k] = fe[k]; Fp[9 * i + k] = fpo[k]; }
+  for (int k = 0; k < ns; ++k) { so[(long long)ns * i + k] = s[k]; xo[(long long)ns * i + k] = x[k]; }
+  na[i] = r;
+  iters[i] = it;
+}
+
 // ------------------------------------------------------------ Krylov (FP64)
 // Solver state on the device.  Reductions never use atomics or fences: a kernel writes one
 // partial sum per block, and the next kernel on the stream (the kernel boundary orders the
@@ -1940,7 +1979,7 @@ int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out) {
     if ((rc = dalloc(c, &c->d_mat, mb.size()))) { pa_destroy(c); return rc; }
     if (hipMemcpy(c->d_mat, mb.data(), mb.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) { pa_destroy(c); return fail(PA_ERR_HIP, "copy material"); }
   }
-  c->gstride = 9 * NS + 36 + 81;
+  c->gstride = 9 * NS + 36 + 81 + 9;
   const long long nq = c->nq, nn = m.nnode, N = c->ndof;
   const int ns = c->ns;
   if ((rc = dalloc(c, &c->Fp_conv, 9 * nq)) || (rc = dalloc(c, &c->Fp_iter, 9 * nq)) || (rc = dalloc(c, &c->Fe_conv, 9 * nq)) ||
@@ -2321,6 +2360,249 @@ int pa_qp_update(pa_context *c, long long n, const double *F, const double *Fp_c
     rc = fail(PA_ERR_HIP, "copy outputs");
   for (void *p : tmp) hipFree(p);
   return rc;
+}
+
+// ------------------------------------------------------------ warm-start network
+struct pa_mlp {
+  float *blob = nullptr, *m1 = nullptr, *m2 = nullptr, *part = nullptr, *lpart = nullptr, *grad = nullptr;
+  unsigned short *wt = nullptr, *w = nullptr;
+  double *lacc = nullptr;
+  long long part_waves = 0;
+  long long adam_steps = 0;
+  hipStream_t stream = nullptr;
+};
+
+namespace {
+mlp::Weights mlp_weights(const pa_mlp *m) { return mlp::Weights{m->wt, m->w, m->blob}; }
+
+int mlp_part(pa_mlp *m, long long waves) {
+  if (waves <= m->part_waves) return 0;
+  if (m->part) hipFree(m->part);
+  if (m->lpart) hipFree(m->lpart);
+  m->part = nullptr; m->lpart = nullptr; m->part_waves = 0;
+  HIPCHK(hipMalloc(&m->part, (size_t)waves * mlp::NPARAM * sizeof(float)));
+  HIPCHK(hipMalloc(&m->lpart, (size_t)waves * sizeof(float)));
+  m->part_waves = waves;
+  return 0;
+}
+
+// inference on device buffers, on stream st
+int mlp_predict_dev(const pa_mlp *m, hipStream_t st, long long n, const float *X, float *Y) {
+  if (n == 0) return 0;
+  const long long tiles = (n + 15) / 16;
+  const unsigned grid = (unsigned)std::min<long long>((tiles + 3) / 4, 8192);
+  hipLaunchKernelGGL(mlp::k_mlp_predict, dim3(grid), dim3(256), 0, st, mlp_weights(m), n, X, Y);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// gradient of one batch of `count` samples (indices (mul*(base+k)+add) mod n); mode 0: into m->grad,
+// mode 1: one Adam step at lr_t
+int mlp_step(pa_mlp *m, long long n, const float *X, const float *Y, long long base, long long count, long long mul,
+             long long add, int mode, float lr_t) {
+  const long long blocks = (count + 63) / 64;
+  if (int rc = mlp_part(m, blocks * 4)) return rc;
+  hipLaunchKernelGGL(mlp::k_mlp_grad, dim3((unsigned)blocks), dim3(256), 0, m->stream, mlp_weights(m), n, X, Y, base, count, mul, add,
+                     1.0f / (float)(count * mlp::NOUT), m->part, m->lpart);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(mlp::k_mlp_reduce, dim3(nblk(mlp::NPARAM, 256)), dim3(256), 0, m->stream, mode, (int)(blocks * 4),
+                     (const float *)m->part, (const float *)m->lpart, m->grad, m->blob, m->m1, m->m2, m->wt, m->w, lr_t, 0.9f,
+                     0.999f, 1e-8f, m->lacc);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+}  // namespace
+
+int pa_mlp_create(pa_mlp **out) {
+  if (!out) return fail(PA_ERR_ARG, "null argument");
+  *out = nullptr;
+  pa_mlp *m = new pa_mlp();
+  auto bad = [&](const char *what) { pa_mlp_destroy(m); return fail(PA_ERR_HIP, what); };
+  if (hipStreamCreate(&m->stream) != hipSuccess) return bad("hipStreamCreate");
+  if (hipMalloc(&m->blob, mlp::NBLOB * sizeof(float)) != hipSuccess || hipMalloc(&m->m1, mlp::NPARAM * sizeof(float)) != hipSuccess ||
+      hipMalloc(&m->m2, mlp::NPARAM * sizeof(float)) != hipSuccess || hipMalloc(&m->grad, mlp::NPARAM * sizeof(float)) != hipSuccess ||
+      hipMalloc(&m->wt, mlp::NWBF * sizeof(unsigned short)) != hipSuccess ||
+      hipMalloc(&m->w, mlp::NWBF * sizeof(unsigned short)) != hipSuccess || hipMalloc(&m->lacc, sizeof(double)) != hipSuccess)
+    return bad("hipMalloc");
+  if (hipMemset(m->blob, 0, mlp::NBLOB * sizeof(float)) != hipSuccess || hipMemset(m->m1, 0, mlp::NPARAM * sizeof(float)) != hipSuccess ||
+      hipMemset(m->m2, 0, mlp::NPARAM * sizeof(float)) != hipSuccess || hipMemset(m->wt, 0, mlp::NWBF * 2) != hipSuccess ||
+      hipMemset(m->w, 0, mlp::NWBF * 2) != hipSuccess)
+    return bad("hipMemset");
+  *out = m;
+  return 0;
+}
+
+int pa_mlp_destroy(pa_mlp *m) {
+  if (!m) return 0;
+  for (void *p : {(void *)m->blob, (void *)m->m1, (void *)m->m2, (void *)m->part, (void *)m->lpart, (void *)m->grad, (void *)m->wt,
+                  (void *)m->w, (void *)m->lacc})
+    if (p) hipFree(p);
+  if (m->stream) hipStreamDestroy(m->stream);
+  delete m;
+  return 0;
+}
+
+int pa_mlp_set_params(pa_mlp *m, const float *blob) {
+  if (!m || !blob) return fail(PA_ERR_ARG, "null argument");
+  for (int i = 0; i < mlp::NBLOB; ++i)
+    if (!std::isfinite(blob[i])) return fail(PA_ERR_ARG, "non-finite parameter");
+  for (int i = 0; i < mlp::NOUT; ++i)
+    if (blob[mlp::OFF_OSCALE + i] == 0.0f) return fail(PA_ERR_ARG, "zero output scale");
+  HIPCHK(hipMemcpy(m->blob, blob, mlp::NBLOB * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemsetAsync(m->m1, 0, mlp::NPARAM * sizeof(float), m->stream));
+  HIPCHK(hipMemsetAsync(m->m2, 0, mlp::NPARAM * sizeof(float), m->stream));
+  m->adam_steps = 0;
+  hipLaunchKernelGGL(mlp::k_mlp_cast, dim3(nblk(mlp::NPARAM, 256)), dim3(256), 0, m->stream, (const float *)m->blob, m->wt, m->w);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(m->stream));
+  return 0;
+}
+
+int pa_mlp_get_params(pa_mlp *m, float *blob) {
+  if (!m || !blob) return fail(PA_ERR_ARG, "null argument");
+  HIPCHK(hipStreamSynchronize(m->stream));
+  HIPCHK(hipMemcpy(blob, m->blob, mlp::NBLOB * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C++" {
+namespace {
+struct DevBuf {
+  std::vector<void *> p;
+  ~DevBuf() { for (void *q : p) (void)hipFree(q); }
+  template <typename T> bool get(T **q, size_t bytes) {
+    if (hipMalloc((void **)q, bytes ? bytes : 16) != hipSuccess) return false;
+    p.push_back(*q);
+    return true;
+  }
+};
+}  // namespace
+}  // extern "C++"
+
+int pa_mlp_predict(pa_mlp *m, long long n, const float *X, float *Y) {
+  if (!m || n < 0 || (n && (!X || !Y))) return fail(PA_ERR_ARG, "bad argument");
+  if (n == 0) return 0;
+  DevBuf b;
+  float *dX, *dY;
+  if (!b.get(&dX, n * mlp::NIN * sizeof(float)) || !b.get(&dY, n * mlp::NOUT * sizeof(float))) return fail(PA_ERR_HIP, "hipMalloc");
+  HIPCHK(hipMemcpy(dX, X, n * mlp::NIN * sizeof(float), hipMemcpyHostToDevice));
+  if (int rc = mlp_predict_dev(m, m->stream, n, dX, dY)) return rc;
+  HIPCHK(hipStreamSynchronize(m->stream));
+  HIPCHK(hipMemcpy(Y, dY, n * mlp::NOUT * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pa_mlp_grad(pa_mlp *m, long long n, const float *X, const float *Y, float *grad, double *loss) {
+  if (!m || n < 1 || n > 65536 || !X || !Y || !grad) return fail(PA_ERR_ARG, "bad argument (1 <= n <= 65536)");
+  DevBuf b;
+  float *dX, *dY;
+  if (!b.get(&dX, n * mlp::NIN * sizeof(float)) || !b.get(&dY, n * mlp::NOUT * sizeof(float))) return fail(PA_ERR_HIP, "hipMalloc");
+  HIPCHK(hipMemcpy(dX, X, n * mlp::NIN * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dY, Y, n * mlp::NOUT * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemsetAsync(m->lacc, 0, sizeof(double), m->stream));
+  if (int rc = mlp_step(m, n, dX, dY, 0, n, 1, 0, 0, 0.0f)) return rc;
+  double l = 0;
+  HIPCHK(hipMemcpyAsync(grad, m->grad, mlp::NPARAM * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+  HIPCHK(hipMemcpyAsync(&l, m->lacc, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  if (loss) *loss = l;
+  return 0;
+}
+
+int pa_mlp_train(pa_mlp *m, long long n, const float *X, const float *Y, int epochs, int batch, float lr,
+                 unsigned long long seed, double *epoch_loss) {
+  if (!m || n < 1 || !X || !Y || epochs < 0 || batch < 1 || batch > 65536 || !(lr > 0.0f))
+    return fail(PA_ERR_ARG, "bad argument (n >= 1, 1 <= batch <= 65536, lr > 0)");
+  DevBuf b;
+  float *dX, *dY;
+  if (!b.get(&dX, n * mlp::NIN * sizeof(float)) || !b.get(&dY, n * mlp::NOUT * sizeof(float))) return fail(PA_ERR_HIP, "hipMalloc");
+  HIPCHK(hipMemcpy(dX, X, n * mlp::NIN * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dY, Y, n * mlp::NOUT * sizeof(float), hipMemcpyHostToDevice));
+  const long long steps_per_epoch = (n + batch - 1) / batch;
+  const double total = (double)steps_per_epoch * epochs;
+  unsigned long long st = seed * 0x9E3779B97F4A7C15ull + 1;
+  auto rnd = [&]() { st ^= st << 13; st ^= st >> 7; st ^= st << 17; return st; };
+  auto gcd = [](long long a, long long c) { while (c) { const long long t = a % c; a = c; c = t; } return a; };
+  long long k = 0;
+  for (int e = 0; e < epochs; ++e) {
+    // the epoch visits the samples in the order (mul * i + add) mod n, gcd(mul, n) = 1
+    long long mul = n > 1 ? (long long)(rnd() % (unsigned long long)n) : 1, add = n > 1 ? (long long)(rnd() % (unsigned long long)n) : 0;
+    while (n > 1 && (mul == 0 || gcd(mul, n) != 1)) mul = (mul + 1) % n;
+    if (n == 1) mul = 1;
+    HIPCHK(hipMemsetAsync(m->lacc, 0, sizeof(double), m->stream));
+    for (long long s = 0; s < steps_per_epoch; ++s, ++k) {
+      const long long base = s * batch, count = std::min<long long>(batch, n - base);
+      // cosine decay from lr to lr/20 over the run, Adam bias corrections folded in
+      const double lr_k = lr * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(M_PI * k / std::max(1.0, total))));
+      m->adam_steps++;
+      const double t = (double)m->adam_steps;
+      const float lr_t = (float)(lr_k * std::sqrt(1.0 - std::pow(0.999, t)) / (1.0 - std::pow(0.9, t)));
+      if (int rc = mlp_step(m, n, dX, dY, base, count, mul, add, 1, lr_t)) return rc;
+    }
+    double l = 0;
+    HIPCHK(hipMemcpyAsync(&l, m->lacc, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    if (epoch_loss) epoch_loss[e] = l / (double)steps_per_epoch;
+  }
+  HIPCHK(hipStreamSynchronize(m->stream));
+  return 0;
+}
+
+int pa_qp_update_warm(pa_context *c, pa_mlp *net, long long n, const double *F, const double *Fp_conv, const double *Fe_conv,
+                      const double *s_conv, const unsigned *grain, int flag, double *P, double *T, double *dPdF, double *Fe,
+                      double *Fp, double *s_out, double *slip_inc, int *n_active, int *iters, float *feat, float *x0) {
+  if (!c || n < 0 || !F || !Fp_conv || !Fe_conv || !s_conv || !grain || !P || !T || !Fe || !Fp || !s_out || !slip_inc || !n_active || !iters)
+    return fail(PA_ERR_ARG, "null argument");
+  if (flag && !dPdF) return fail(PA_ERR_ARG, "dPdF required for flag 1");
+  if (c->ns != mlp::NOUT) return fail(PA_ERR_UNSUPPORTED, "the warm start covers 12 slip systems (FCC)");
+  if (c->kin) return fail(PA_ERR_UNSUPPORTED, "the warm start does not cover kinematic hardening");
+  if (!c->grains_set) return fail(PA_ERR_STATE, "pa_set_grains has not been called");
+  for (long long i = 0; i < n; ++i)
+    if ((int)grain[i] >= c->n_grain_rows) return fail(PA_ERR_ARG, "grain index out of range");
+  if (n == 0) return 0;
+  const int ns = c->ns;
+  const size_t b9 = 9 * n * sizeof(double), bs = (size_t)ns * n * sizeof(double);
+  DevBuf b;
+  double *dF, *dFp, *dFec, *ds, *dP, *dT, *dD = nullptr, *dFe, *dFpo, *dso, *dx;
+  unsigned *dg;
+  int *dna, *dit;
+  float *dfeat, *dx0 = nullptr;
+  if (!(b.get(&dF, b9) && b.get(&dFp, b9) && b.get(&dFec, b9) && b.get(&ds, bs) && b.get(&dP, b9) && b.get(&dT, b9) &&
+        b.get(&dFe, b9) && b.get(&dFpo, b9) && b.get(&dso, bs) && b.get(&dx, bs) && b.get(&dg, n * sizeof(unsigned)) &&
+        b.get(&dna, n * sizeof(int)) && b.get(&dit, n * sizeof(int)) && b.get(&dfeat, n * mlp::NIN * sizeof(float)) &&
+        (!flag || b.get(&dD, 81 * n * sizeof(double))) && (!net || b.get(&dx0, n * mlp::NOUT * sizeof(float)))))
+    return fail(PA_ERR_HIP, "hipMalloc");
+  HIPCHK(hipMemcpy(dF, F, b9, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dFp, Fp_conv, b9, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dFec, Fe_conv, b9, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ds, s_conv, bs, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dg, grain, n * sizeof(unsigned), hipMemcpyHostToDevice));
+  const unsigned grid = nblk(n, 64);
+  hipLaunchKernelGGL(k_qp_features<12>, dim3(grid), dim3(64), 0, c->stream, n, (const double *)dF, (const double *)dFp,
+                     (const double *)dFec, (const double *)ds, (const double *)nullptr, (const unsigned *)dg, (const double *)c->d_gtab,
+                     c->gstride, ns, dfeat);
+  HIPCHK(hipGetLastError());
+  if (net) {
+    HIPCHK(hipStreamSynchronize(net->stream));  // weights settled
+    if (int rc = mlp_predict_dev(net, c->stream, n, dfeat, dx0)) return rc;
+  }
+  hipLaunchKernelGGL(k_qp_batch_warm<12>, dim3(grid), dim3(64), 0, c->stream, n, (const double *)dF, (const double *)dFp,
+                     (const double *)ds, (const unsigned *)dg, (const double *)c->d_gtab, c->gstride, (const double *)c->d_mat, ns,
+                     c->mat.max_solver_iter, flag, dP, dT, dD, dFe, dFpo, dso, dx, dna, (const float *)dx0, dit);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(P, dP, b9, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(T, dT, b9, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(Fe, dFe, b9, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(Fp, dFpo, b9, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(s_out, dso, bs, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(slip_inc, dx, bs, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(n_active, dna, n * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(iters, dit, n * sizeof(int), hipMemcpyDeviceToHost));
+  if (flag) HIPCHK(hipMemcpy(dPdF, dD, 81 * n * sizeof(double), hipMemcpyDeviceToHost));
+  if (feat) HIPCHK(hipMemcpy(feat, dfeat, n * mlp::NIN * sizeof(float), hipMemcpyDeviceToHost));
+  if (x0 && net) HIPCHK(hipMemcpy(x0, dx0, n * mlp::NOUT * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
 }
 
 int pa_pinv_solve(int path, int n, long long count, const double *A, const double *b, double *x) {

@@ -258,8 +258,40 @@ template <int NS> struct MatLayout {
 };
 // grain block (doubles): S[NS][9] Dmat[36] TM[81]
 template <int NS> struct GrainLayout {
-  static constexpr int S = 0, D = 9 * NS, TM = 9 * NS + 36, size = 9 * NS + 36 + 81;
+  static constexpr int S = 0, D = 9 * NS, TM = 9 * NS + 36, R = 9 * NS + 36 + 81, size = 9 * NS + 36 + 81 + 9;
 };
+
+// Warm-start inputs of one qp (no counterpart in the reference): feat[0..nf) = (tau_a - W_a)/s_a,
+// the trial resolved shear of :142-240 relative to the slip resistance, and feat[nf..nf+6) = the
+// increment of the elastic Green-Lagrange strain E(F Fp_t^-1) - E(Fe_conv) rotated to the lattice
+// frame (R^T dE R), Voigt order (11,22,33,23,13,12).  Written with stride fstride.
+template <int NS>
+CP_DEV void qp_features(const double *__restrict__ gt, int nf, const double *F, const double *Fp_t, const double *Fe_c,
+                        const double *s, const double *W, float *feat, long long fstride) {
+  typedef GrainLayout<NS> GL;
+  double Fpinv[9], FE[9], CE[9], E[9], Tst[9], tmp[9];
+  m3_inv(Fp_t, Fpinv);
+  m3_mul(F, Fpinv, FE);
+  m3_tmul(FE, FE, CE);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) E[i] = 0.5 * (CE[i] - ((i % 4 == 0) ? 1.0 : 0.0));
+  dmat_apply(gt + GL::D, E, Tst);
+  m3_mul(CE, Tst, tmp);
+  for (int a = 0; a < nf; ++a) {
+    const double t = m3_ddot(tmp, gt + GL::S + 9 * a) - (W ? W[a] : 0.0);
+    feat[a * fstride] = (float)(t / s[a]);
+  }
+  double dE[9], C0[9];
+  m3_tmul(Fe_c, Fe_c, C0);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) dE[i] = E[i] - 0.5 * (C0[i] - ((i % 4 == 0) ? 1.0 : 0.0));
+  const double *R = gt + GL::R;
+  m3_tmul(R, dE, tmp);   // R^T dE
+  m3_mul(tmp, R, CE);    // R^T dE R
+  const int vi[6] = {0, 4, 8, 5, 2, 1};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) feat[(nf + k) * fstride] = (float)CE[vi[k]];
+}
 
 // dexp series of calculatePlasticity.cc:645-676 (sign = -1) and :748-779 (sign = +1):
 // out = S + sum_{k=1..count3} c_k W_k, W_k = sum_{l=0..k} X^l S X^(k-l), c_k = (+-1)^k / (k+1)!.
@@ -292,11 +324,19 @@ CP_DEV void dexp_series(const double *X, const double *S, int count3, bool neg, 
 // KIN: kinematic hardening (calculatePlasticity.cc:294-305, 431-436, with C_1, C_2 of
 // multiphaseInit.cc:15-32): W holds the backstress W_kh_conv on entry and W_kh_tau on exit, and
 // every sign test on the resolved shear uses tau - W.  Without it the backstress is zero.
-template <int NS, typename DT = double, bool KIN = false, typename SING = SvdInLane<NS>>
+// WARM (no counterpart in the reference, which starts from x_beta_old = 0, :311): x0 (ns predicted
+// slip increments, stride x0stride; may be null) starts the local loop on the potentially-active
+// systems it predicts positive, applied as one pass of the loop body before the first solve.  The
+// start stands only if the loop converges on that set without dropping a system; otherwise the
+// loop restarts from zero on the trial set.  iters (may be null) receives the local solves of both
+// attempts.  Not combined with kinematic hardening.
+template <int NS, typename DT = double, bool KIN = false, typename SING = SvdInLane<NS>, bool WARM = false>
 CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ gt, int ns, int max_solver_iter,
                      const double *F, const double *Fp_t, double *s, double *x, int flag, double *P, double *T,
                      double *Fe_out, double *Fp_out, DT *D, long long dstride, bool *plastic_path = nullptr,
-                     double dscale = 1.0, SING sing = SING(), double *W = nullptr) {
+                     double dscale = 1.0, SING sing = SING(), double *W = nullptr, const float *x0 = nullptr,
+                     long long x0stride = 1, int *iters = nullptr) {
+  static_assert(!(WARM && KIN), "the warm start does not cover kinematic hardening");
   typedef MatLayout<NS> ML;
   typedef GrainLayout<NS> GL;
   const double tol1 = mat[ML::tol];
@@ -333,6 +373,7 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
     if (b[a] >= tol1) PA[nPA++] = a;
   }
   const bool plastic = nPA > 0;
+  if constexpr (WARM) if (iters) *iters = 0;
   if (plastic_path) *plastic_path = plastic;
   double detFE;
   int n_active = 0;
@@ -377,12 +418,12 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
       for (int i = 0; i < npos; ++i) m = fmax(m, fabs(b[PA0[pos[i]]]));
       return m;
     };
-    int count1 = 0;
-    while (linf_b() > tol1) {                            // :322
-      count1++;
-      if (count1 > max_solver_iter) break;
+    int count1 = 0, n_solves = 0;
+    double xb[NS];
+    // one solve of inactive_slip_removal on the current set; false: a singular system that the
+    // caller's SING policy defers
+    auto solve_step = [&]() -> bool {
       // inactive_slip_removal (inactiveSlipRemoval.cc:4-124)
-      double xb[NS];
       for (int a = 0; a < ns; ++a) xb[a] = 0.0;
       while (true) {
         double Ap[NS * NS], bp[NS];
@@ -395,7 +436,7 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
         if (npos > 0 && !lu_solve<NS>(npos, Ap, xs, 1)) {
           for (int i = 0; i < npos; ++i)
             for (int j = 0; j < npos; ++j) Ap[i * NS + j] = A[pos[i] * NS + pos[j]];
-          if (!sing(npos, Ap, bp, xs)) return -1;  // the caller solves this system elsewhere
+          if (!sing(npos, Ap, bp, xs)) return false;  // the caller solves this system elsewhere
         }
         for (int a = 0; a < ns; ++a) xb[a] = 0.0;
         for (int i = 0; i < npos; ++i) xb[PA0[pos[i]]] = xs[i];
@@ -409,6 +450,10 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
         npos = keep;
       }
       for (int a = 0; a < ns; ++a) x[a] += xb[a];        // :119
+      return true;
+    };
+    // one pass of the loop body from xb (increment) and x (total)
+    auto update_step = [&]() {
       for (int i = 0; i < 9; ++i) XdelFp[i] = 0.0;      // :338-357
       for (int a = 0; a < ns; ++a) {
         const double g = ((tau_tr[a] - wtau(a)) > 0) ? x[a] : -x[a];
@@ -437,7 +482,82 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
         }
         b[a] = fabs(m3_ddot(tmp, Sg + 9 * a) - wtau(a)) - sa;
       }
+    };
+    // the reference's loop from `done` solves already counted; stop_on_drop: stop as soon as
+    // inactive-slip removal drops a system.  Returns false when a solve is deferred.
+    auto local_loop = [&](int done, bool stop_on_drop) -> bool {
+      count1 = done;
+      while (linf_b() > tol1) {                          // :322
+        count1++;
+        if (count1 > max_solver_iter) break;
+        const int n_before = npos;
+        if (!solve_step()) return false;
+        update_step();
+        if (stop_on_drop && npos < n_before) break;
+      }
+      return true;
+    };
+    auto solves = [&](int done) { return (count1 < max_solver_iter ? count1 : max_solver_iter) - done; };
+    bool plain = true;
+    if constexpr (WARM) if (x0) {
+      // Warm start.  The reference's loop is path dependent: its first solve on the trial set,
+      // with inactive-slip removal, fixes the set S1 that later iterations only shrink, and its
+      // converged states leave dropped systems above the yield surface, so a converged start on
+      // another set is another answer.  The first solve always runs; if the predicted set
+      // {a : x0[a] > 0} equals S1 its increment is replaced by x0 and the loop continues, the start
+      // standing if the loop converges on S1 without a drop (otherwise it restarts from zero); if
+      // the sets differ the loop continues from the first solve as without a prediction.
+      bool pred[NS];
+      int npred = 0;
+      for (int i = 0; i < n0; ++i) {
+        pred[i] = (double)x0[PA0[i] * x0stride] > 0.0;
+        npred += pred[i];
+      }
+      if (npred > 0) {
+        plain = false;
+        if (linf_b() > tol1 && max_solver_iter >= 1) {
+          double s_t[NS], b_t[NS];
+          for (int a = 0; a < ns; ++a) { s_t[a] = s[a]; b_t[a] = b[a]; }
+          if (!solve_step()) return -1;
+          n_solves = 1;
+          bool same = npos == npred;
+          for (int i = 0; i < npos; ++i) same = same && pred[pos[i]];
+          if (same) {
+            for (int a = 0; a < ns; ++a) xb[a] = 0.0;
+            for (int i = 0; i < npos; ++i) { const int a = PA0[pos[i]]; xb[a] = (double)x0[a * x0stride]; }
+            for (int a = 0; a < ns; ++a) x[a] = xb[a];
+          }
+          update_step();
+          if (same) {
+            const int n_set = npos;
+            if (!local_loop(0, true)) return -1;
+            const int c = solves(0);
+            n_solves += c;
+            bool ok = npos == n_set && c < max_solver_iter;
+            for (int i = 0; i < npos; ++i)
+              if (!(fabs(b[PA0[pos[i]]]) <= tol1)) ok = false;
+            if (!ok) {  // restart from zero on the trial set
+              for (int i = 0; i < n0; ++i) pos[i] = i;
+              npos = n0;
+              for (int a = 0; a < ns; ++a) { s[a] = s_t[a]; b[a] = b_t[a]; x[a] = 0.0; }
+#pragma unroll
+              for (int i = 0; i < 9; ++i) { Fp[i] = Fp_t[i]; Ce[i] = CE[i]; TstTau[i] = Tst[i]; }
+              if (!local_loop(0, false)) return -1;
+              n_solves += solves(0);
+            }
+          } else {
+            if (!local_loop(1, false)) return -1;
+            n_solves += solves(1);
+          }
+        }
+      }
     }
+    if (plain) {
+      if (!local_loop(0, false)) return -1;
+      n_solves = solves(0);
+    }
+    if constexpr (WARM)
+      if (iters) *iters = n_solves;
     n_active = npos;
     // final active set as system indices
     for (int i = 0; i < npos; ++i) PA[i] = PA0[pos[i]];

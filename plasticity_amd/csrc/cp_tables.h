@@ -4,7 +4,7 @@
 
 namespace {
 
-// per-grain constants: rotated Schmid tensors, Dmat (shear columns doubled), TM
+// per-grain constants: rotated Schmid tensors, Dmat (shear columns doubled), TM, R
 // (calculatePlasticity.cc:39-112, rotationOperations.cc:4-51, matrixOperations.cc:189-234)
 inline void grain_table(const pa_material &mat, const double mv[][3], const double nv[][3], const double r[3], int ns, int NS,
                  double *out) {
@@ -51,6 +51,9 @@ inline void grain_table(const pa_material &mat, const double mv[][3], const doub
   const int v2[9] = {0, 5, 4, 5, 1, 3, 4, 3, 2};
   for (int i = 0; i < 9; ++i)
     for (int j = 0; j < 9; ++j) TM[9 * i + j] = D2[v2[i]][v2[j]];
+  double *Rout = TM + 81;  // the rotation itself (lattice -> sample), row-major
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) Rout[3 * i + j] = R[i][j];
 }
 
 }  // namespace

@@ -355,6 +355,25 @@ def oracle_qp(case, rot, F, Fp=None, s=None, flag=1, pinv_mode=1, slip=None, W=N
             "Fp": Fpo.reshape(3, 3), "s": so, "slip": sl, "n_active": na, "W": Wo}
 
 
+def oracle_qp_warm(case, rot, F, Fp, s, x0=None, flag=0, pinv_mode=1):
+    """or_qp_update_warm: the local loop started from x0 (n_slip values, None = from zero)."""
+    lib = oracle()
+    mat = case.material(pinv_mode)
+    nT, ns = case.nT, case.n_slip
+    s = np.append(np.asarray(s, float)[:ns], [10e14] * (nT - ns))
+    W = np.zeros(nT)
+    P, T, D, Fe, Fpo = np.zeros(9), np.zeros(9), np.zeros(81), np.zeros(9), np.zeros(9)
+    so, Wo, sl = np.zeros(nT), np.zeros(nT), np.zeros(ns)
+    it = ctypes.c_int(0)
+    x0 = None if x0 is None else np.ascontiguousarray(x0, float)
+    na = lib.or_qp_update_warm(ctypes.byref(mat), ptr(np.ascontiguousarray(rot, float)),
+                               ptr(np.ascontiguousarray(F, float).reshape(9)), ptr(np.ascontiguousarray(Fp, float).reshape(9)),
+                               ptr(s), ptr(W), ptr(np.zeros(ns)), ctypes.c_int(flag), ptr(x0) if x0 is not None else None,
+                               ptr(P), ptr(T), ptr(D), ptr(Fe), ptr(Fpo), ptr(so), ptr(Wo), ptr(sl), ctypes.byref(it))
+    return {"P": P.reshape(3, 3), "T": T.reshape(3, 3), "dPdF": D.reshape(9, 9), "Fe": Fe.reshape(3, 3),
+            "Fp": Fpo.reshape(3, 3), "s": so[:ns], "slip": sl, "n_active": na, "iters": it.value}
+
+
 def fcc_case(**kw):
     """The FCC material of the 8^3 tension workload (cubic 170/124/75 GPa, s0 16, h0 180, s_s 148, a 2.25, q 1.4)."""
     d = os.path.join(GOLDEN, "fcc_simpleTension")

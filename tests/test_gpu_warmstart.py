@@ -1,0 +1,225 @@
+"""GPU tests of the learned warm start (through the C ABI): the network's BF16 matrix-core forward
+and backward kernels against the CPU oracle, the warm-started local loop against the oracle's, and
+what the trained network gains over the plain return map.
+
+Tolerances: network outputs 2e-3 and gradients 2e-3 (norm) relative to the bf16-mirroring oracle
+(products are exact in FP32, sums run in FP32 on the GPU and in double in the oracle, so a bf16
+rounding of an intermediate can flip); qp results 1e-10 relative; local-solve counts equal except
+where the stopping test |b| <= tol meets a |b| within rounding of tol (at most 1 in 100 plain, 1 in
+50 warm-started qps); network inputs 1e-5 (they are FP32).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import plasticity_amd as pa
+import refcase
+from plasticity_amd import warmstart as ws
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import mlp_oracle as mo  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, float).ravel(), np.asarray(b, float).ravel()
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _net_blob(rng, n=512):
+    X = rng.normal(size=(n, mo.NIN)) * rng.uniform(0.5, 2.0, mo.NIN) + rng.normal(size=mo.NIN)
+    Y = np.abs(rng.normal(size=(n, mo.NOUT))) * 1e-3
+    blob = ws.init_blob(X, Y, seed=int(rng.integers(1 << 30)))
+    for l in range(4):
+        nout = mo.NOUT if l == 3 else 64
+        blob[mo.OFF_B[l]:mo.OFF_B[l] + nout] = rng.normal(scale=0.1, size=nout)
+    return blob, X.astype(np.float32), Y.astype(np.float32)
+
+
+def test_mlp_predict_matches_oracle():
+    rng = np.random.default_rng(5)
+    blob, X, _ = _net_blob(rng, 1000)  # not a multiple of 16: a ragged last tile
+    net = ws.WarmStartNet(blob)
+    y = net.predict(X)
+    ref = mo.predict(blob, X)
+    print("predict: max rel err %.2e" % _rel(y, ref))
+    assert _rel(y, ref) < 2e-3
+    assert np.allclose(net.get_params(), blob)
+    net.close()
+
+
+def test_mlp_gradient_matches_oracle():
+    rng = np.random.default_rng(6)
+    for n in (16, 200, 4096):
+        blob, X, Y = _net_blob(rng, n)
+        net = ws.WarmStartNet(blob)
+        loss, g = net.grad(X, Y)
+        loss_r, g_r = mo.loss_grad(blob, X, Y, quant=True)
+        errs = []
+        for l in range(4):
+            sl = slice(mo.OFF_W[l], mo.OFF_B[l] + mo.LN[l])
+            errs.append(np.linalg.norm(g[sl] - g_r[sl]) / np.linalg.norm(g_r[sl]))
+        print("n=%d loss %.6e vs %.6e, per-layer gradient rel. err %s" % (n, loss, loss_r, ["%.1e" % e for e in errs]))
+        assert abs(loss - loss_r) <= 1e-4 * loss_r
+        assert max(errs) < 2e-3
+        # padding carries no gradient
+        WT0 = g[:mo.OFF_B[0]].reshape(64, 32)
+        assert not WT0[:, mo.NIN:].any()
+        net.close()
+
+
+def _ctx(case, nel=(2, 2, 2)):
+    mat = pa.make_material(case.m, case.n, case.q, case.C, case.s0, case.h0, case.ss, case.pw, case.tol,
+                           case.max_slip_search, case.max_solver_iter)
+    pb = pa.make_problem(nel, case.span, case.total_increments, case.delT, case.bcs, case.max_newton, case.lin_tol)
+    return pa.Context(mat, pb)
+
+
+def _states(rng, n):
+    F = np.eye(3) + rng.normal(scale=1.5e-3, size=(n, 3, 3))
+    Fp = np.eye(3) + rng.normal(scale=3e-4, size=(n, 3, 3))
+    Fp /= np.cbrt(np.linalg.det(Fp))[:, None, None]
+    Fe = np.eye(3) + rng.normal(scale=5e-4, size=(n, 3, 3))
+    s = rng.uniform(16.0, 40.0, size=(n, 12))
+    return F, Fp, Fe, s
+
+
+def _features_np(case, rot, F, Fp, Fe_c, s):
+    """The network inputs restated in the lattice frame: tau_a = (Ce T*) : (m_a x n_a) with
+    T* = C : E (engineering shear strains), and R^T dE R."""
+    R = np.zeros(9)
+    refcase.oracle().or_odfpoint(refcase.ptr(np.asarray(rot, float)), refcase.ptr(R))
+    R = R.reshape(3, 3)
+    Fe = F @ np.linalg.inv(Fp)
+    E = 0.5 * (Fe.T @ Fe - np.eye(3))
+    Ec = R.T @ E @ R
+    eps = np.array([Ec[0, 0], Ec[1, 1], Ec[2, 2], 2 * Ec[1, 2], 2 * Ec[0, 2], 2 * Ec[0, 1]])
+    sg = case.C @ eps
+    Tc = np.array([[sg[0], sg[5], sg[4]], [sg[5], sg[1], sg[3]], [sg[4], sg[3], sg[2]]])
+    M = (np.eye(3) + 2 * Ec) @ Tc
+    m = case.m / np.linalg.norm(case.m, axis=1)[:, None]
+    nn = case.n / np.linalg.norm(case.n, axis=1)[:, None]
+    tau = np.einsum("ai,ij,aj->a", m, M, nn)
+    dE = E - 0.5 * (Fe_c.T @ Fe_c - np.eye(3))
+    dEc = R.T @ dE @ R
+    return np.concatenate([tau / s, [dEc[0, 0], dEc[1, 1], dEc[2, 2], dEc[1, 2], dEc[0, 2], dEc[0, 1]]])
+
+
+def test_warm_qp_without_network_matches_oracle():
+    """net = None is the reference's start from zero: results and local-solve counts equal the
+    oracle's, and the network inputs equal a lattice-frame restatement."""
+    case = refcase.fcc_case(max_solver_iter=50)
+    rng = np.random.default_rng(21)
+    rods = rng.uniform(-0.6, 0.6, size=(5, 3))
+    ctx = _ctx(case)
+    ctx.set_grains(np.zeros(8, np.uint32), rods)
+    n = 400
+    F, Fp, Fe, s = _states(rng, n)
+    grain = rng.integers(0, 5, size=n).astype(np.uint32)
+    out = ws.qp_update_warm(ctx, F, Fp, Fe, s, grain, None, flag=1)
+    nplastic = differ = 0
+    for i in range(n):
+        o = refcase.oracle_qp_warm(case, rods[grain[i]], F[i], Fp[i], s[i], None, flag=1)
+        for k, kk in (("P", "P"), ("T", "T"), ("Fp", "Fp"), ("dPdF", "dPdF")):
+            assert _rel(out[k][i], o[kk]) < 1e-10, (i, k)
+        assert _rel(out["s"][i], o["s"]) < 1e-12
+        assert np.abs(out["slip_inc"][i] - o["slip"]).max() <= 1e-10 * max(np.abs(o["slip"]).max(), 1e-6), i
+        # the stopping test |b| <= tol can fall either way for a |b| within rounding of tol
+        assert abs(int(out["iters"][i]) - o["iters"]) <= 1, i
+        differ += out["iters"][i] != o["iters"]
+        nplastic += o["n_active"] > 0
+        f = _features_np(case, rods[grain[i]], F[i], Fp[i], Fe[i], s[i])
+        assert np.abs(out["feat"][i] - f).max() <= 1e-5 * np.abs(f).max(), i
+    print("plastic qps %d of %d; local-solve counts that differ from the oracle's: %d" % (nplastic, n, differ))
+    assert nplastic > n // 4
+    assert differ <= n // 100
+    ctx.close()
+
+
+def test_warm_qp_with_network_matches_oracle():
+    """A network's predictions start the loop on the GPU and in the oracle alike: the same results
+    and the same local-solve counts, whether the start stands or the loop restarts from zero."""
+    case = refcase.fcc_case(max_solver_iter=50)
+    rng = np.random.default_rng(22)
+    rods = rng.uniform(-0.6, 0.6, size=(5, 3))
+    ctx = _ctx(case)
+    ctx.set_grains(np.zeros(8, np.uint32), rods)
+    n = 400
+    F, Fp, Fe, s = _states(rng, n)
+    grain = rng.integers(0, 5, size=n).astype(np.uint32)
+    plain = ws.qp_update_warm(ctx, F, Fp, Fe, s, grain, None)
+    # a network whose output is the converged slip scaled by a random factor near 1 cannot be built
+    # directly; a network fitted for a few epochs gives starts near and far from it
+    blob = ws.init_blob(plain["feat"], plain["slip_inc"], seed=1)
+    net = ws.WarmStartNet(blob)
+    net.train(plain["feat"], plain["slip_inc"], epochs=30, batch=64, lr=3e-3, seed=2)
+    warm = ws.qp_update_warm(ctx, F, Fp, Fe, s, grain, net)
+    stood = differ = 0
+    for i in range(n):
+        o = refcase.oracle_qp_warm(case, rods[grain[i]], F[i], Fp[i], s[i], warm["x0"][i].astype(np.float64))
+        assert _rel(warm["T"][i], o["T"]) < 1e-10, i
+        assert _rel(warm["Fp"][i], o["Fp"]) < 1e-10, i
+        assert np.abs(warm["slip_inc"][i] - o["slip"]).max() <= 1e-10 * max(np.abs(o["slip"]).max(), 1e-6), i
+        differ += warm["iters"][i] != o["iters"]
+        # every result, whether the start stood or not, is the plain loop's converged stress
+        assert _rel(warm["T"][i], plain["T"][i]) < 1e-8, i
+        stood += warm["iters"][i] < plain["iters"][i]
+    print("warm starts that saved solves: %d of %d; local-solve counts that differ from the oracle's: %d" % (stood, n, differ))
+    assert differ <= n // 50
+    net.close()
+    ctx.close()
+
+
+def _run_states(case, seed, increments):
+    rng = np.random.default_rng(seed)
+    nel = (case.nx, case.ny, case.nz)
+    ctx = _ctx(case, nel)
+    ng = 8
+    eg = rng.integers(0, ng, size=int(np.prod(nel))).astype(np.uint32)
+    rods = rng.uniform(-0.6, 0.6, size=(ng, 3))
+    ctx.set_grains(eg, rods)
+    st = ws.collect_trial_states(ctx, nel, case.span, eg, increments, case.max_newton)
+    return ctx, st
+
+
+def test_learned_warm_start_gain():
+    """Train the network on the GPU on (trial state -> converged slip) pairs of one 8^3 tension run,
+    then run the local loop with and without it on the trial states of another run (other grains
+    and orientations).  Every result equals the plain loop's (1e-8); the local solves with and
+    without the network are reported.  The start saves solves only where the network predicts the
+    converged slip far closer than the loop's own first solve does; the loop's cost with a start
+    is bounded by the plain cost plus the solves of a rejected start."""
+    case = refcase.fcc_case(max_solver_iter=50, nx=8, ny=8, nz=8)
+    ctx, st = _run_states(case, 31, 5)
+    plain = ws.qp_update_warm(ctx, st["F"], st["Fp"], st["Fe"], st["s"], st["grain"], None)
+    act = plain["iters"] > 0
+    X, Y = plain["feat"][act], plain["slip_inc"][act]
+    net = ws.WarmStartNet(ws.init_blob(X, Y, seed=0))
+    loss0 = net.grad(X[:65536], Y[:65536])[0]
+    losses = net.train(X, Y, epochs=60, batch=256, lr=2e-3, seed=1)
+    ctx.close()
+    print("training pairs %d, loss untrained %.3e, first/last epoch %.3e / %.3e" % (len(X), loss0, losses[0], losses[-1]))
+    assert losses[-1] < 0.3 * loss0 and losses[-1] < losses[0]
+
+    ctx2, st2 = _run_states(case, 32, 4)
+    p2 = ws.qp_update_warm(ctx2, st2["F"], st2["Fp"], st2["Fe"], st2["s"], st2["grain"], None)
+    w2 = ws.qp_update_warm(ctx2, st2["F"], st2["Fp"], st2["Fe"], st2["s"], st2["grain"], net)
+    act2 = p2["iters"] > 0
+    Yt = p2["slip_inc"][act2]
+    pred = np.maximum(w2["x0"][act2], 0)
+    err = np.linalg.norm(pred - Yt, axis=1) / np.maximum(np.linalg.norm(Yt, axis=1), 1e-12)
+    same_set = ((pred > 0) == (Yt > 0)).all(axis=1).mean()
+    ip, iw = p2["iters"][act2].mean(), w2["iters"][act2].mean()
+    saved = (w2["iters"][act2] < p2["iters"][act2]).mean()
+    dT = np.abs(w2["T"] - p2["T"]).max(axis=1) / np.abs(p2["T"]).max(axis=1)
+    print("held-out plastic qps %d: mean local solves plain %.3f, with the network %.3f (%+.1f%%); qps with fewer solves "
+          "%.1f%%; prediction rel. err median %.3f, p90 %.3f; predicted slip set = converged set %.1f%%; max stress diff %.2e"
+          % (act2.sum(), ip, iw, 100 * (iw / ip - 1), 100 * saved, np.median(err), np.percentile(err, 90), 100 * same_set,
+             dT.max()))
+    assert dT.max() < 1e-8
+    assert iw <= ip + 1.0
+    net.close()
+    ctx2.close()
