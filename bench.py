@@ -75,6 +75,7 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
     ap.add_argument("--solver", choices=["bicg", "cg", "mgcg", "all"], default="all")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-warm", action="store_true", help="skip the learned warm-start measurement")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md for how it would shard)",
@@ -139,9 +140,56 @@ def main():
             err["crss"] = float(np.abs(b["s"] - a["s"]).max() / np.abs(a["s"]).max())
             err["displacement"] = float(np.abs(b["u"] - a["u"]).max() / np.abs(a["u"]).max())
             out["config"]["mixed_vs_fp64_gpu_diff_%s" % solver] = err
+    if not args.no_warm:
+        out["config"]["learned_warm_start"] = warm_start_leg(pa, native)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, quick=(n < 32))
     print(json.dumps(out))
+
+
+def warm_start_leg(pa, native):
+    """The learned warm start of the converged local loop (Max Solver Iterations 50): the network is
+    trained on the GPU on the trial states of every Newton iteration of an 8^3 tension run (8
+    grains, 5 increments of 1e-3) and judged on the trial states of a 16^3 run with other grains:
+    mean local solves with and without it, and the largest stress difference between the two."""
+    from plasticity_amd import synth, warmstart as ws
+
+    def run(nel, grains, incs, seed):
+        mat, pb, vox, rod, _, _ = synth.fcc_workload(nel, grains, 10, 1e-3, seed=seed)
+        mat.max_solver_iter = 50
+        ctx = pa.Context(mat, pb)
+        eg = ctx.element_grains(vox)
+        ctx.set_grains(eg, rod)
+        st = ws.collect_trial_states(ctx, (nel, nel, nel), (1.0, 1.0, 1.0), eg, incs, pb.max_newton)
+        return ctx, st
+
+    t0 = time.perf_counter()
+    ctx, st = run(8, 8, 5, 7)
+    plain = ws.qp_update_warm(ctx, st["F"], st["Fp"], st["Fe"], st["s"], st["grain"], None)
+    ctx.close()
+    act = plain["iters"] > 0
+    X, Y = plain["feat"][act], plain["slip_inc"][act]
+    net = ws.WarmStartNet(ws.init_blob(X, Y, seed=0))
+    epochs = 60
+    t1 = time.perf_counter()
+    losses = net.train(X, Y, epochs=epochs, batch=256, lr=2e-3, seed=1)
+    train_s = time.perf_counter() - t1
+    ctx, st = run(16, 16, 3, 8)
+    p = ws.qp_update_warm(ctx, st["F"], st["Fp"], st["Fe"], st["s"], st["grain"], None)
+    w = ws.qp_update_warm(ctx, st["F"], st["Fp"], st["Fe"], st["s"], st["grain"], net)
+    ctx.close()
+    a = p["iters"] > 0
+    pred, Yt = np.maximum(w["x0"][a], 0), p["slip_inc"][a]
+    err = np.linalg.norm(pred - Yt, axis=1) / np.maximum(np.linalg.norm(Yt, axis=1), 1e-12)
+    dT = np.abs(w["T"] - p["T"]).max(axis=1) / np.abs(p["T"]).max(axis=1)
+    net.close()
+    return {"train_pairs": int(len(X)), "epochs": epochs, "train_s": train_s,
+            "train_pairs_per_s": len(X) * epochs / train_s, "loss_first_last_epoch": [float(losses[0]), float(losses[-1])],
+            "heldout_plastic_qps": int(a.sum()), "local_solves_plain": float(p["iters"][a].mean()),
+            "local_solves_warm": float(w["iters"][a].mean()),
+            "qps_with_fewer_solves": float((w["iters"][a] < p["iters"][a]).mean()),
+            "prediction_rel_err_median": float(np.median(err)), "max_stress_diff_warm_vs_plain": float(dT.max()),
+            "leg_s": time.perf_counter() - t0}
 
 
 def run_form(args, form, solver, pa, native, synth, torch):
