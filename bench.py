@@ -30,9 +30,9 @@ FP64_PEAK_TFLOPS = 78.6
 # at the state bench.py times it on, from rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64
 # (wave instructions x 64 lanes, FMA = 2), mean over the timed launches; filled in from the
 # committed profile.  Masked-off lanes are counted, so this is an upper bound of the useful work.
-PMC_QP_FLOPS = {("mixed", 64): (57443100 + 108127000 + 2 * 375451000 + 4212740) * 64}
+PMC_QP_FLOPS = {("mixed", 64): (57438700 + 108138000 + 2 * 375480000 + 4213720) * 64}
 # HBM bytes (KiB) per launch of the same pass: FETCH_SIZE x 2 + WRITE_SIZE, as for the Krylov operator
-PMC_QP_TRAFFIC_KIB = {("mixed", 64): 26750200 * 2 + 18846400}
+PMC_QP_TRAFFIC_KIB = {("mixed", 64): 23369400 * 2 + 18264900}
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the Jacobi-PCG operator kernel k_cg_spmv at the 64^3 workload, from two
 # rocprofv3 PMC passes (FETCH_SIZE x 2 + WRITE_SIZE x 1 KiB: the factors that make both counters
