@@ -132,6 +132,13 @@ int pa_run_increment(pa_context *ctx, int increment, double row[15], int *lin_it
  * Output is qp-major (or node-major).  n = number of doubles the caller provides. */
 int pa_get_state(pa_context *ctx, int which, double *host, long long n);
 
+/* Grain-averaged output of updateAfterIncrement (src/materialModels/crystalPlasticity/
+ * updateAfterIncrement.cc:611-707) over the Cauchy stress of the last pa_commit_increment: row r
+ * (grain id r + 1 of the element grain map, the reference's cellOrientationMap - 1) holds
+ * [volume, Txx, Tyy, Tzz, Txy, Txz, Tyz], the stresses volume-averaged when the volume exceeds
+ * 1e-15.  out: n_grains x 7 doubles.  Deterministic (fixed summation order) on the device. */
+int pa_grain_averages(pa_context *ctx, int n_grains, double *out);
+
 /* Sparsity of the Jacobian (DoFTools::make_sparsity_pattern over the full 27-node stencil,
  * src/ellipticBVP/init.cc:183-199), built on the GPU: row_ptr (ndof+1), col_idx (nnz); values
  * (optional, nnz) = the last assembled condensed Jacobian.  Sizes: pa_sizes. */

@@ -1008,6 +1008,38 @@ void launch_qp_batch(hipStream_t st, bool kin, unsigned grid, long long n, const
                        T, D, Fe, Fp, so, xo, na, Wio);
 }
 
+// Per-grain sums of JxW and JxW * Cauchy stress (updateAfterIncrement.cc:611-668): one block per
+// grain id g = blockIdx.x + 1, its 256 lanes visiting the elements in index order (stride 256), then
+// a fixed-order tree reduction, so the sums are deterministic.  out[(g-1)*7 + c]: c = 0 sum of JxW,
+// 1..6 sums of JxW * (Txx, Tyy, Tzz, Txy, Txz, Tyz).
+__global__ void __launch_bounds__(256) k_grain_sums(long long nelem, long long nq, const unsigned *__restrict__ egrain,
+                                                    const double *__restrict__ T, double jxw, double *__restrict__ out) {
+  __shared__ double red[7][256];
+  const unsigned g = blockIdx.x + 1;
+  const int t = threadIdx.x;
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  const int comp[6] = {0, 4, 8, 1, 2, 5};
+  for (long long e = t; e < nelem; e += 256) {
+    if (egrain[e] != g) continue;
+    for (int q = 0; q < 8; ++q) {
+      const long long Q = 8 * e + q;
+      acc[0] += jxw;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) acc[1 + c] += T[comp[c] * nq + Q] * jxw;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 7; ++c) red[c][t] = acc[c];
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if (t < w)
+#pragma unroll
+      for (int c = 0; c < 7; ++c) red[c][t] += red[c][t + w];
+    __syncthreads();
+  }
+  if (t < 7) out[(long long)blockIdx.x * 7 + t] = red[t][0];
+}
+
 // warm-start inputs of a batch of qps (FCC: 12 ratios + 6 strain components per qp, row-major)
 template <int NS>
 __global__ void __launch_bounds__(64) k_qp_features(long long n, const double *__restrict__ F, const double *__restrict__ Fpc,
@@ -2602,6 +2634,23 @@ int pa_qp_update_warm(pa_context *c, pa_mlp *net, long long n, const double *F, 
   if (flag) HIPCHK(hipMemcpy(dPdF, dD, 81 * n * sizeof(double), hipMemcpyDeviceToHost));
   if (feat) HIPCHK(hipMemcpy(feat, dfeat, n * mlp::NIN * sizeof(float), hipMemcpyDeviceToHost));
   if (x0 && net) HIPCHK(hipMemcpy(x0, dx0, n * mlp::NOUT * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pa_grain_averages(pa_context *c, int n_grains, double *out) {
+  if (!c || n_grains < 1 || n_grains > (1 << 20) || !out) return fail(PA_ERR_ARG, "bad argument (1 <= n_grains <= 2^20)");
+  if (!c->grains_set) return fail(PA_ERR_STATE, "pa_set_grains has not been called");
+  DevBuf b;
+  double *d;
+  if (!b.get(&d, (size_t)n_grains * 7 * sizeof(double))) return fail(PA_ERR_HIP, "hipMalloc");
+  hipLaunchKernelGGL(k_grain_sums, dim3((unsigned)n_grains), dim3(256), 0, c->stream, (long long)c->m.nelem, (long long)c->nq,
+                     (const unsigned *)c->d_egrain, (const double *)c->T, c->JxW, d);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(out, d, (size_t)n_grains * 7 * sizeof(double), hipMemcpyDeviceToHost));
+  for (int g = 0; g < n_grains; ++g)  // updateAfterIncrement.cc:676-682
+    if (std::fabs(out[7 * g]) > 1e-15)
+      for (int j = 1; j < 7; ++j) out[7 * g + j] = out[7 * g + j] / out[7 * g];
   return 0;
 }
 

@@ -92,6 +92,14 @@ class userInputParameters:
         self.tabularTimeInput = fl(g("Tabular Time Table", ""))
         if self.enableTabularBCs and (self.enableSimpleBCs or self.enableCyclicLoading):
             raise native.PlasticityError("tabular boundary conditions together with simple or cyclic ones are not implemented")
+        # grain-averaged output (userInputParameters.cc:52-60, 663-671)
+        self.tabularOutput = b("Tabular Output", "false")
+        self.tabularTimeOutput = fl(g("Tabular Time Output Table", ""))
+        self.writeGrainAveragedOutput = b("Write Grain Averaged Output", "false")
+        self.skipGrainAveragedOutputSteps = int(g("Skip Grain Averaged Output Steps", "-1"))
+        self.numberOfGrainAverageDataOutput = int(g("Number of Grain Averaged Output Variables", "-1"))
+        if self.writeGrainAveragedOutput and b("Output Buffer Layer Removal Feature", "false"):
+            raise native.PlasticityError("the output buffer layer is outside the implemented path")
         self.useVelocityGrad = b("Use velocity gradient BC", "false")  # userInputParameters.cc:78-83
         self.targetVelGrad = [fl(g("Velocity gradient row %d" % (i + 1), "0, 0, 0")) for i in range(3)]
         if self.useVelocityGrad and any(len(r) != 3 for r in self.targetVelGrad):
@@ -143,6 +151,7 @@ class crystalOrientationsIO:
         self.vox = None
         self.numPts = None
         self.eulerAngles = None
+        self.numberOfGrains = 0
 
     def loadOrientations(self, voxelFile, headerLines, orientationFile, numPts, span):
         with open(voxelFile) as f:
@@ -159,12 +168,15 @@ class crystalOrientationsIO:
 
     def loadOrientationVector(self, eulerFile, enableMultiphase=False, numVoxelData=0):
         table = {}
+        last = 0
         with open(eulerFile) as f:
             f.readline()
             for line in f:
                 v = line.split()
                 if len(v) >= 4:
                     table[int(v[0])] = [float(x) for x in v[1:4]]
+                    last = int(v[0])
+        self.numberOfGrains = last  # the last id read (crystalOrientationsIO.cc:61)
         rod = np.zeros((max(table) + 1, 3))
         for k, v in table.items():
             rod[k] = v
@@ -250,6 +262,13 @@ class Context:
             comps = self.n_slip if comps is None else comps
             out = np.zeros((self.nqp, comps))
         check(native.lib().pa_get_state(self._h, ctypes.c_int(which), ptr(out), ctypes.c_longlong(out.size)))
+        return out
+
+    def grain_averages(self, n_grains):
+        """Grain-averaged output (updateAfterIncrement.cc:611-707): rows [volume, Txx, Tyy, Tzz,
+        Txy, Txz, Tyz] for grain ids 1..n_grains, from the Cauchy stress of the last commit."""
+        out = np.zeros((n_grains, 7))
+        check(native.lib().pa_grain_averages(self._h, ctypes.c_int(n_grains), ptr(out)))
         return out
 
     def get_csr(self, values=False):
@@ -421,15 +440,38 @@ class crystalPlasticity:
         n = self.totalIncrements if n_increments is None else n_increments
         rows = np.zeros((n, 15))
         self.lin_iters = np.zeros((n, self.problem.max_newton), dtype=np.int32)
+        ui = self.userInputs
+        outdir = ui.file(ui.outputDirectory)
         for inc in range(n):
             rows[inc], self.lin_iters[inc], res = self.ctx.run_increment(inc)
             if log:
                 log("increment %d: residuals %s, linear iterations %s" % (inc, res, self.lin_iters[inc]))
+            if write and ui.writeGrainAveragedOutput and self._grain_output_due(inc):
+                if ui.numberOfGrainAverageDataOutput != 7:  # updateAfterIncrement.cc:657-660
+                    raise native.PlasticityError("The size of numberOfGrainAverageDataOutput defined in prm.prm is not correct")
+                os.makedirs(outdir, exist_ok=True)
+                write_grain_average(os.path.join(outdir, "GrainAverage%d.csv" % inc),
+                                    self.ctx.grain_averages(self.orientations.numberOfGrains))
         if write:
-            outdir = self.userInputs.file(self.userInputs.outputDirectory)
             os.makedirs(outdir, exist_ok=True)
             write_stressstrain(os.path.join(outdir, "stressstrain.txt"), rows)
         return rows
+
+    def _grain_output_due(self, inc):
+        """updateAfterIncrement.cc:323-339, 612: every skip-th increment, or the increments of the
+        tabular output times (int(t / delT + 0.1))."""
+        ui = self.userInputs
+        if ui.tabularOutput:
+            return sum(int(t / ui.delT + 0.1) == inc + 1 for t in ui.tabularTimeOutput) == 1
+        return (inc + 1) % ui.skipGrainAveragedOutputSteps == 0
+
+
+def write_grain_average(path, data):
+    """GrainAverage<increment>.csv: one row per grain, comma separated, setprecision(9), appended
+    (updateAfterIncrement.cc:686-706)."""
+    with open(path, "a") as f:
+        for row in data:
+            f.write(",".join("%.9g" % v for v in row) + "\n")
 
 
 def write_stressstrain(path, rows):
