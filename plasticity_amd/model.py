@@ -98,6 +98,11 @@ class userInputParameters:
         self.writeGrainAveragedOutput = b("Write Grain Averaged Output", "false")
         self.skipGrainAveragedOutputSteps = int(g("Skip Grain Averaged Output Steps", "-1"))
         self.numberOfGrainAverageDataOutput = int(g("Number of Grain Averaged Output Variables", "-1"))
+        self.writeQuadratureOutput = b("Write Quadrature Output", "false")
+        self.skipQuadratureOutputSteps = int(g("Skip Quadrature Output Steps", "-1"))
+        for key in ("Enable Advanced Twin Model", "Enable Advanced Rate Dependent Model", "Enable User Material Model"):
+            if self.writeQuadratureOutput and b(key, "false"):
+                raise native.PlasticityError("'%s' is outside the implemented path" % key)
         if self.writeGrainAveragedOutput and b("Output Buffer Layer Removal Feature", "false"):
             raise native.PlasticityError("the output buffer layer is outside the implemented path")
         self.useVelocityGrad = b("Use velocity gradient BC", "false")  # userInputParameters.cc:78-83
@@ -446,7 +451,12 @@ class crystalPlasticity:
             rows[inc], self.lin_iters[inc], res = self.ctx.run_increment(inc)
             if log:
                 log("increment %d: residuals %s, linear iterations %s" % (inc, res, self.lin_iters[inc]))
-            if write and ui.writeGrainAveragedOutput and self._grain_output_due(inc):
+            if write and ui.writeQuadratureOutput and self._output_due(inc, ui.skipQuadratureOutputSteps):
+                os.makedirs(outdir, exist_ok=True)
+                write_quadrature_output(os.path.join(outdir, "QuadratureOutputs%d.csv" % inc), self.ctx, self.cellOrientationMap,
+                                        (self.problem.nx, self.problem.ny, self.problem.nz), ui.span,
+                                        dealii_cell_order(ui.subdivisions, ui.meshRefineFactor))
+            if write and ui.writeGrainAveragedOutput and self._output_due(inc, ui.skipGrainAveragedOutputSteps):
                 if ui.numberOfGrainAverageDataOutput != 7:  # updateAfterIncrement.cc:657-660
                     raise native.PlasticityError("The size of numberOfGrainAverageDataOutput defined in prm.prm is not correct")
                 os.makedirs(outdir, exist_ok=True)
@@ -457,13 +467,75 @@ class crystalPlasticity:
             write_stressstrain(os.path.join(outdir, "stressstrain.txt"), rows)
         return rows
 
-    def _grain_output_due(self, inc):
-        """updateAfterIncrement.cc:323-339, 612: every skip-th increment, or the increments of the
-        tabular output times (int(t / delT + 0.1))."""
+    def _output_due(self, inc, skip):
+        """updateAfterIncrement.cc:323-339, 344, 612: every skip-th increment, or the increments of
+        the tabular output times (int(t / delT + 0.1))."""
         ui = self.userInputs
         if ui.tabularOutput:
             return sum(int(t / ui.delT + 0.1) == inc + 1 for t in ui.tabularTimeOutput) == 1
-        return (inc + 1) % ui.skipGrainAveragedOutputSteps == 0
+        return (inc + 1) % skip == 0
+
+
+def dealii_cell_order(subdivisions, refine):
+    """Element index e = ex + nx (ey + ny ez) of each active cell in deal.II's iteration order for
+    subdivided_hyper_rectangle + refine_global (mesh.cc:32,101): coarse cells lexicographic (x
+    fastest); each refinement replaces a cell by its 8 children in place, child c at offset
+    (c & 1, c >> 1 & 1, c >> 2 & 1).  Returns an array of length nelem."""
+    sx, sy, sz = subdivisions
+    n = 2 ** refine
+    nx, ny = sx * n, sy * n
+    fine = np.zeros((n ** 3, 3), dtype=np.int64)  # offsets of the fine cells of one coarse cell, in order
+    for k in range(n ** 3):
+        x = y = z = 0
+        for lev in range(refine):  # child digits from the coarsest level down
+            c = (k >> (3 * (refine - 1 - lev))) & 7
+            x, y, z = 2 * x + (c & 1), 2 * y + ((c >> 1) & 1), 2 * z + ((c >> 2) & 1)
+        fine[k] = (x, y, z)
+    order = []
+    for cz in range(sz):
+        for cy in range(sy):
+            for cx in range(sx):
+                ex = cx * n + fine[:, 0]
+                ey = cy * n + fine[:, 1]
+                ez = cz * n + fine[:, 2]
+                order.append(ex + nx * (ey + ny * ez))
+    return np.concatenate(order)
+
+
+def write_quadrature_output(path, ctx, elem_grain, nel, span, cell_order):
+    """QuadratureOutputs<increment>.csv (updateAfterIncrement.cc:343-608, writeQuadratureOutput.cc:4-49):
+    one row per quadrature point, cells in deal.II order, values as "%8.5e ,".  Columns: grain id,
+    phase (1), JxW, twin flag (0), x, y, z, rotnew (3), Fe_conv, Fp_conv and Cauchy stress (each
+    11,22,33,12,13,21,23,31,32), slip fractions 0..83, twin fractions 0..5.  The reference reads
+    the slip and twin fraction vectors up to index 83 and 5 whatever their length (n_slip and 1
+    with twinning off), past their end; those entries are written as 0 here."""
+    nx, ny, nz = nel
+    h = np.asarray(span, float) / np.array([nx, ny, nz])
+    jxw = h.prod() / 8.0
+    nq = ctx.nqp
+    rot = ctx.get_state("rotnew").reshape(nq, 3)
+    Fe = ctx.get_state("Fe").reshape(nq, 9)
+    Fp = ctx.get_state("Fp").reshape(nq, 9)
+    T = ctx.get_state("T").reshape(nq, 9)
+    slip = ctx.get_state("slip").reshape(nq, -1)
+    g = np.array([0.5 - 0.5 / np.sqrt(3.0), 0.5 + 0.5 / np.sqrt(3.0)])
+    mix = [0, 4, 8, 1, 2, 3, 5, 6, 7]
+    e = np.asarray(cell_order, np.int64)
+    ex, ey, ez = e % nx, (e // nx) % ny, e // (nx * ny)
+    cols = []
+    for q in range(8):
+        Q = 8 * e + q
+        X = np.stack([(ex + g[q & 1]) * h[0], (ey + g[(q >> 1) & 1]) * h[1], (ez + g[(q >> 2) & 1]) * h[2]], axis=1)
+        sl = np.zeros((len(e), 84))
+        sl[:, :min(84, slip.shape[1])] = slip[Q, :84]
+        block = np.hstack([np.asarray(elem_grain, float)[e][:, None], np.ones((len(e), 1)), np.full((len(e), 1), jxw),
+                           np.zeros((len(e), 1)), X, rot[Q], Fe[Q][:, mix], Fp[Q][:, mix], T[Q][:, mix], sl,
+                           np.zeros((len(e), 6))])
+        cols.append(block)
+    rows = np.stack(cols, axis=1).reshape(len(e) * 8, -1)  # cell-major, qp-minor
+    with open(path, "w") as f:
+        for r in rows:
+            f.write("".join("%8.5e ," % v for v in r) + "\n")
 
 
 def write_grain_average(path, data):

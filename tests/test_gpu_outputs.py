@@ -77,3 +77,51 @@ def test_grain_average_files_from_prm(tmp_path):
     got = np.array([[float(v) for v in r] for r in rows])
     assert np.allclose(got, ga, rtol=1e-8, atol=0)  # setprecision(9)
     assert abs(got[:, 0].sum() - np.prod(ui.span)) <= 1e-8 * np.prod(ui.span)
+
+
+def test_quadrature_output_file_matches_state(tmp_path):
+    """QuadratureOutputs<inc>.csv of the fcc/simpleTension inputs (8^3 by refine factor 3, Krylov
+    solves to 1e-14): 127 columns, one row per qp in deal.II cell order; stresses, Fe, Fp,
+    orientation and slip equal the oracle's per-qp state to print precision (%8.5e)."""
+    src = os.path.join(refcase.GOLDEN, "fcc_simpleTension")
+    for f in os.listdir(src):
+        shutil.copy(os.path.join(src, f), tmp_path / f)
+    with open(tmp_path / "prm.prm", "a") as f:
+        f.write("\nset Write Quadrature Output = true\nset Skip Quadrature Output Steps = 2\nset Output Directory = out\n"
+                "set Relative linear solver tolerance = 1.0e-14\n")
+    ui = pa.userInputParameters(str(tmp_path / "prm.prm"))
+    problem = pa.crystalPlasticity(ui)
+    problem.orientations.loadOrientations(ui.file(ui.grainIDFile), ui.headerLinesGrainIDFile,
+                                          ui.file(ui.grainOrientationsFile), ui.numPts, ui.span)
+    problem.orientations.loadOrientationVector(ui.file(ui.grainOrientationsFile))
+    problem.run(n_increments=2)
+    out = tmp_path / "out"
+    assert sorted(p for p in os.listdir(out) if p.startswith("Quadrature")) == ["QuadratureOutputs1.csv"]
+    text = open(out / "QuadratureOutputs1.csv").read().splitlines()
+    rows = np.array([[float(v) for v in line.split(",")[:-1]] for line in text])
+    nel = (problem.problem.nx, problem.problem.ny, problem.problem.nz)
+    nq = 8 * int(np.prod(nel))
+    assert rows.shape == (nq, 127) and all(line.endswith(",") for line in text)
+    case = refcase.load_reference_case("fcc_simpleTension")
+    case.lin_tol = 1e-14
+    st = refcase.oracle_simulate(case, 2, want_state=True)["state"]
+    order = pa.model.dealii_cell_order(ui.subdivisions, ui.meshRefineFactor)
+    Q = (8 * order[:, None] + np.arange(8)[None, :]).ravel()
+    mix = [0, 4, 8, 1, 2, 3, 5, 6, 7]
+    nT = case.nT
+    want = {"grain": np.repeat(case.elem_grain[order], 8), "T": st[Q][:, mix], "Fp": st[Q][:, 9 + np.array(mix)],
+            "Fe": st[Q][:, 18 + np.array(mix)], "rot": st[Q][:, 27 + nT:30 + nT], "slip": st[Q][:, 30 + nT:30 + nT + 12]}
+    close = lambda a, b: np.allclose(a, b, rtol=2e-5, atol=1e-12 * max(1.0, np.abs(b).max()))
+    assert np.array_equal(rows[:, 0], want["grain"])
+    assert (rows[:, 1] == 1).all() and (rows[:, 3] == 0).all()
+    assert close(rows[:, 2], np.prod(np.array(ui.span) / np.array(nel)) / 8)
+    assert close(rows[:, 7:10], want["rot"])
+    assert close(rows[:, 10:19], want["Fe"]) and close(rows[:, 19:28], want["Fp"])
+    assert np.allclose(rows[:, 28:37], want["T"], rtol=2e-5, atol=1e-6 * np.abs(want["T"]).max())
+    assert np.allclose(rows[:, 37:49], want["slip"], rtol=2e-5, atol=1e-9)
+    assert not rows[:, 49:].any()
+    # qp coordinates lie inside their cells, cell by cell in the given order
+    h = np.array(ui.span) / np.array(nel)
+    e = np.repeat(order, 8)
+    cell = np.stack([e % nel[0], (e // nel[0]) % nel[1], e // (nel[0] * nel[1])], axis=1)
+    assert (np.floor(rows[:, 4:7] / h + 1e-9).astype(int) == cell).all()
