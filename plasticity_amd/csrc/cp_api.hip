@@ -1955,6 +1955,14 @@ int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out) {
   if (!mat || !pb || !out) return fail(PA_ERR_ARG, "null argument");
   *out = nullptr;
   if (mat->n_slip < 1 || mat->n_slip > PA_MAX_SLIP) return fail(PA_ERR_ARG, "n_slip out of range");
+  // The quadrature-point kernels hold each lane's active-set and tangent matrices in private
+  // memory: 8.6 KB per lane for up to 12 systems, 24 KB for up to 24, 84 KB for up to 48.  The
+  // runtime refuses the scratch reservation of the 48-system variant on MI355X
+  // (HSA_STATUS_ERROR_OUT_OF_RESOURCES at its first dispatch, even for two waves), so more than
+  // 24 systems are rejected here rather than failing inside a Newton step.
+  if (mat->n_slip > 24)
+    return fail(PA_ERR_UNSUPPORTED, "more than 24 slip systems: the 48-system kernels need 84 KB of private memory per "
+                                    "lane, which the runtime does not reserve on this GPU");
   if (mat->max_slip_search != 1) return fail(PA_ERR_UNSUPPORTED, "Max Slip Search Iterations must be 1");
   if (mat->max_solver_iter < 1) return fail(PA_ERR_ARG, "Max Solver Iterations must be >= 1");
   if (mat->twinning) return fail(PA_ERR_UNSUPPORTED, "twinning is not implemented");
