@@ -893,15 +893,16 @@ __device__ __attribute__((noinline)) bool post_qp_one(const PostArgs &a, long lo
     for (int i = 0; i < a.ns; ++i) a.slip_iter[i * a.nq + Q] = a.slip_conv[i * a.nq + Q] + x[i];
   return true;
 }
+// lane = qp, grid-stride over one-wave blocks (as the first assembly pass)
 template <int NS, bool KIN>
-__global__ void __launch_bounds__(256) k_post_qp(PostArgs a) {
-  const long long Q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (Q >= a.nq) return;
-  SingAny<NS> f{SING_CLAIM, a.sing, a.qcount, a.queue, (unsigned)Q, 0u, 0, 0};
-  post_qp_one<NS, KIN>(a, Q, f);
+__global__ void __launch_bounds__(64) QP_WAVES k_post_qp(PostArgs a) {
+  for (long long Q = blockIdx.x * (long long)blockDim.x + threadIdx.x; Q < a.nq; Q += (long long)gridDim.x * blockDim.x) {
+    SingAny<NS> f{SING_CLAIM, a.sing, a.qcount, a.queue, (unsigned)Q, 0u, 0, 0};
+    post_qp_one<NS, KIN>(a, Q, f);
+  }
 }
 template <int NS, bool KIN>
-__global__ void __launch_bounds__(64) k_post_qp_replay(PostArgs a) {
+__global__ void __launch_bounds__(64) QP_WAVES k_post_qp_replay(PostArgs a) {
   const unsigned n = min(*a.qcount, a.sing.cap);
   for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
     if (a.sing.done[e]) continue;
@@ -910,7 +911,7 @@ __global__ void __launch_bounds__(64) k_post_qp_replay(PostArgs a) {
   }
 }
 template <int NS, bool KIN>
-__global__ void __launch_bounds__(64) k_post_qp_deferred(PostArgs a) {
+__global__ void __launch_bounds__(64) QP_WAVES k_post_qp_deferred(PostArgs a) {
   const unsigned n = *a.qcount;
   for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     if (i >= a.sing.cap || !a.sing.done[i]) {
@@ -1488,7 +1489,7 @@ struct pa_context {
   pa_material mat;
   pa_problem pb;
   MeshDims m;
-  double h[3], JxW;
+  double h[3], JxW, microvol = 0.0;
   int ns = 0;
   long long nq = 0, ndof = 0, nnz = 0;
   int method = PA_SOLVER_BICG, precision = PA_PREC_FP64;
@@ -1598,7 +1599,8 @@ int launch_qp_tangent(pa_context *c, bool main_only = false) {
 
 template <int NS, bool KIN>
 void post_passes_k(pa_context *c, const PostArgs &a, unsigned grid) {
-  hipLaunchKernelGGL((k_post_qp<NS, KIN>), dim3(grid), dim3(256), 0, c->stream, a);
+  (void)grid;
+  hipLaunchKernelGGL((k_post_qp<NS, KIN>), dim3(std::min(2048u, nblk(c->nq, 64))), dim3(64), 0, c->stream, a);
   for (int r = 0; r < SING_ROUNDS; ++r) {
     sing_svd_round<NS>(c, a.qcount);
     hipLaunchKernelGGL((k_post_qp_replay<NS, KIN>), dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
@@ -1992,6 +1994,8 @@ int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out) {
   c->ndof = 3 * m.nnode;
   for (int d = 0; d < 3; ++d) c->h[d] = pb->span[d] / (d == 0 ? m.nx : d == 1 ? m.ny : m.nz);
   c->JxW = c->h[0] * c->h[1] * c->h[2] / 8.0;
+  c->microvol = 0.0;  // microvol = the sum of JxW over every qp in order (updateAfterIncrement.cc:221)
+  for (long long e = 0; e < 8 * c->m.nelem; ++e) c->microvol += c->JxW;
   auto cnt = [](int i, int nn) { return (long long)(std::min(i + 1, nn - 1) - std::max(i - 1, 0) + 1); };
   auto tot = [&](int nn) { long long s = 0; for (int i = 0; i < nn; ++i) s += cnt(i, nn); return s; };
   c->nnz = 9 * tot(m.nnx) * tot(m.nny) * tot(m.nnz);
@@ -2239,8 +2243,7 @@ int pa_commit_increment(pa_context *c, double row[15]) {
   HIPCHK(hipStreamSynchronize(c->stream));
   float ms = 0;
   if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) c->last_ms[2] = ms;
-  double vol = 0;
-  for (long long e = 0; e < 8 * c->m.nelem; ++e) vol += c->JxW;  // microvol = sum of JxW (updateAfterIncrement.cc:221)
+  const double vol = c->microvol;
   if (row) {
     for (int k = 0; k < 12; ++k) row[k] = h[k] / vol;
     row[12] = 0.0;  // TwinRealVF: no twin system is ever active
