@@ -386,6 +386,34 @@ def test_first_newton_iteration_stagewise():
     ctx.close()
 
 
+def test_mixed_form_first_assembly_stagewise():
+    """The mixed form's condensed Jacobian (FP32 element tangents, products and sums) equals the
+    oracle's FP64 one to FP32 rounding (1e-6 of the largest entry), and so does its residual, whose
+    iteration-0 condensation term -K_ij g_j uses the FP32 entries.  (The mixed form's plastic runs
+    are checked end to end in test_mixed_precision_form_error_vs_oracle.)"""
+    import ctypes
+    case = refcase.fcc_case(nx=4, ny=3, nz=5, delT=1.0, total_time=10.0, lin_tol=1e-14)
+    rng = np.random.default_rng(8)
+    case.grain_rod = rng.uniform(-0.4, 0.4, size=(4, 3))
+    case.elem_grain = rng.integers(0, 4, size=60).astype(np.uint32)
+    lib = refcase.oracle()
+    pb = case.problem()
+    nnz = lib.or_sparsity(ctypes.byref(pb), None, None)
+    ndof = 3 * 5 * 4 * 6
+    K, R, du = np.zeros(nnz), np.zeros(ndof), np.zeros(ndof)
+    mat = case.material(1)
+    lib.or_first_newton(ctypes.byref(mat), ctypes.byref(pb), refcase.ptr(case.elem_grain), refcase.ptr(case.grain_rod),
+                        ctypes.c_int(4), refcase.ptr(K), refcase.ptr(R), refcase.ptr(du))
+    ctx = _ctx(case)
+    ctx.set_grains(case.elem_grain, case.grain_rod)
+    ctx.set_solver(pa.native.PA_SOLVER_BICG, pa.native.PA_PREC_MIXED)
+    ctx.assemble(0, 0)
+    rp, ci, gK = ctx.get_csr(values=True)
+    assert _rel(gK, K) < 1e-6, _rel(gK, K)
+    assert _rel(ctx.get_state("R"), R) < 1e-6
+    ctx.close()
+
+
 def test_mixed_precision_form_error_vs_oracle():
     """Mixed form (FP32 tangents and matrix values, FP64 return map, residual and Krylov
     recurrences): the Jacobian only steers Newton, so after the 4 Newton iterations of each
