@@ -43,6 +43,28 @@ def test_pinv_solve_rejects_bad_arguments_before_any_device_call():
         pa.model.pinv_solve(np.zeros((1, 49, 49)), np.zeros((1, 49)))
     assert pa.model.pinv_solve(np.zeros((0, 4, 4)), np.zeros((0, 4))).shape == (0, 4)
 
+def test_warm_start_and_output_entry_points_reject_bad_arguments_before_any_device_call():
+    """PA_ERR_ARG (1) for null handles and out-of-range sizes, checked before the device is touched."""
+    import ctypes
+    import numpy as np
+    from plasticity_amd import native
+    L = native.lib()
+    X = np.zeros((4, 18), np.float32)
+    Y = np.zeros((4, 12), np.float32)
+    g = np.zeros(11472, np.float32)  # PA_MLP_NPARAM
+    loss = ctypes.c_double()
+    assert L.pa_mlp_create(None) == 1
+    assert L.pa_mlp_set_params(None, native.ptr(g)) == 1
+    assert L.pa_mlp_get_params(None, native.ptr(g)) == 1
+    assert L.pa_mlp_predict(None, ctypes.c_longlong(4), native.ptr(X), native.ptr(Y)) == 1
+    assert L.pa_mlp_grad(None, ctypes.c_longlong(4), native.ptr(X), native.ptr(Y), native.ptr(g), ctypes.byref(loss)) == 1
+    assert L.pa_mlp_train(None, ctypes.c_longlong(4), native.ptr(X), native.ptr(Y), 1, 4, ctypes.c_float(1e-3),
+                          ctypes.c_ulonglong(0), None) == 1
+    assert L.pa_grain_averages(None, 4, native.ptr(np.zeros(28))) == 1
+    assert L.pa_qp_update_warm(None, None, ctypes.c_longlong(0), *([None] * 17)) == 1
+    assert L.pa_mlp_destroy(None) == 0
+
+
 def test_struct_layout_matches_header(tmp_path):
     """ctypes mirrors of pa_material / pa_problem have the sizes and field offsets the C compiler
     gives the header's structs."""
