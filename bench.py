@@ -78,7 +78,7 @@ def main():
     ap.add_argument("--no-warm", action="store_true", help="skip the learned warm-start measurement")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md for how it would shard)",
+        print("bench.py runs on one GPU only: the library has no multi-GPU path",
               file=sys.stderr)
         sys.exit(2)
 
