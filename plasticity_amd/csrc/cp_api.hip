@@ -1531,6 +1531,10 @@ struct pa_context {
   double *mg_dinv[MG_MAX_LEVELS] = {nullptr}, *mg_b[MG_MAX_LEVELS] = {nullptr}, *mg_x[MG_MAX_LEVELS] = {nullptr};
   double *mg_r[MG_MAX_LEVELS] = {nullptr}, *mg_d[MG_MAX_LEVELS] = {nullptr};
   double *mg_lam = nullptr, *mg_minv = nullptr, *mg_part = nullptr;
+  // the V-cycle as a HIP graph, one per arithmetic form (its kernels take only level pointers,
+  // sizes and device-side scalars, all fixed once the levels are allocated)
+  hipGraph_t mg_graph[2] = {nullptr, nullptr};
+  hipGraphExec_t mg_exec[2] = {nullptr, nullptr};
   std::vector<void *> allocs;
 };
 
@@ -1854,7 +1858,24 @@ int mg_vcycle_t(pa_context *c) {
   HIPCHK(hipGetLastError());
   return 0;
 }
-int mg_vcycle(pa_context *c) { return mixed(c) ? mg_vcycle_t<float>(c) : mg_vcycle_t<double>(c); }
+// The V-cycle's ~20 launches are captured once per form into a graph and replayed.
+int mg_vcycle(pa_context *c) {
+  const int f = mixed(c) ? 1 : 0;
+  if (!c->mg_exec[f]) {
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = f ? mg_vcycle_t<float>(c) : mg_vcycle_t<double>(c);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc ? rc : fail(PA_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    }
+    c->mg_graph[f] = g;
+    HIPCHK(hipGraphInstantiate(&c->mg_exec[f], g, nullptr, nullptr, 0));
+  }
+  HIPCHK(hipGraphLaunch(c->mg_exec[f], c->stream));
+  return 0;
+}
 
 // One Krylov launch sequence for iteration k: p update, operator (CG: K p; BiCG: K p and K^T p~), update.
 int launch_krylov_iteration(pa_context *c, int k) {
@@ -2092,6 +2113,10 @@ int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out) {
 int pa_destroy(pa_context *c) {
   if (!c) return 0;
   if (c->stream) hipStreamSynchronize(c->stream);
+  for (int f = 0; f < 2; ++f) {
+    if (c->mg_exec[f]) (void)hipGraphExecDestroy(c->mg_exec[f]);
+    if (c->mg_graph[f]) (void)hipGraphDestroy(c->mg_graph[f]);
+  }
   for (void *p : c->allocs) hipFree(p);
   for (int i = 0; i < 4; ++i)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
