@@ -129,17 +129,23 @@ def qp_update_warm(ctx, F, Fp, Fe_conv, s, grain, net=None, flag=0):
     return out
 
 
-def qp_deformation_gradients(u, nel, span):
+def qp_deformation_gradients(u, nel, span, elems=None):
     """F = I + grad u at the 2x2x2 Gauss points of every Q1 element of the structured box
-    (getElementalValues.cc:129-138), qp order Q = 8e + qx + 2qy + 4qz.  u: 3 per node."""
+    (getElementalValues.cc:129-138), qp order Q = 8e + qx + 2qy + 4qz.  u: 3 per node.  elems
+    (optional): only these elements, in the given order."""
     nx, ny, nz = nel
     h = np.asarray(span, float) / np.array([nx, ny, nz])
     U = np.asarray(u, float).reshape(nz + 1, ny + 1, nx + 1, 3)
     g = np.array([0.5 - 0.5 / np.sqrt(3.0), 0.5 + 0.5 / np.sqrt(3.0)])
-    # corner displacements of every element, vertex v = vx + 2vy + 4vz
-    corners = np.stack([U[vz:vz + nz, vy:vy + ny, vx:vx + nx] for vz in (0, 1) for vy in (0, 1) for vx in (0, 1)], axis=3)
-    corners = corners.reshape(nx * ny * nz, 8, 3)  # element e = ex + nx (ey + ny ez)
-    F = np.zeros((nx * ny * nz, 8, 3, 3))
+    if elems is None:
+        # corner displacements of every element, vertex v = vx + 2vy + 4vz
+        corners = np.stack([U[vz:vz + nz, vy:vy + ny, vx:vx + nx] for vz in (0, 1) for vy in (0, 1) for vx in (0, 1)], axis=3)
+        corners = corners.reshape(nx * ny * nz, 8, 3)  # element e = ex + nx (ey + ny ez)
+    else:
+        e = np.asarray(elems, np.int64)
+        ex, ey, ez = e % nx, (e // nx) % ny, e // (nx * ny)
+        corners = np.stack([U[ez + vz, ey + vy, ex + vx] for vz in (0, 1) for vy in (0, 1) for vx in (0, 1)], axis=1)
+    F = np.zeros((len(corners), 8, 3, 3))
     for q in range(8):
         xi = (g[q & 1], g[(q >> 1) & 1], g[(q >> 2) & 1])
         dN = np.zeros((8, 3))

@@ -131,3 +131,13 @@ def test_qp_deformation_gradients_affine_closed_form():
         y = (2 + g[(q >> 1) & 1]) * span[1] / ny
         z = (1 + g[(q >> 2) & 1]) * span[2] / nz
         assert abs(F2[e, q, 0, 1] - 0.01 * z) < 1e-14 and abs(F2[e, q, 0, 2] - 0.01 * y) < 1e-14
+
+
+def test_qp_deformation_gradients_element_subset():
+    rng = np.random.default_rng(5)
+    nel, span = (4, 3, 5), (1.0, 0.8, 1.2)
+    u = rng.normal(size=3 * 5 * 4 * 6) * 1e-3
+    full = ws.qp_deformation_gradients(u, nel, span).reshape(-1, 8, 9)
+    elems = np.array([7, 0, 59, 23])
+    sub = ws.qp_deformation_gradients(u, nel, span, elems).reshape(-1, 8, 9)
+    assert np.array_equal(sub, full[elems])
