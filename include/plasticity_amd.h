@@ -36,6 +36,7 @@ enum {
 
 enum { PA_SOLVER_BICG = 0, PA_SOLVER_CG = 1, PA_SOLVER_MGCG = 2 };
 enum { PA_PREC_FP64 = 0, PA_PREC_MIXED = 1 };
+enum { PA_QP_LANE = 0, PA_QP_COOP = 1 };
 
 /* Single-phase material tables, as crystalPlasticity<dim>::init() builds them from the prm
  * inputs (src/materialModels/crystalPlasticity/init.cc:6-196, multiphaseInit.cc:4-67). */
@@ -107,6 +108,15 @@ int pa_set_grains(pa_context *ctx, const unsigned *elem_grain, const double *gra
  * FP64 residual, return map and Krylov recurrences). */
 int pa_set_solver(pa_context *ctx, int method, int precision);
 
+/* Layout of the Newton step's quadrature-point passes (calculatePlasticity over every qp in
+ * getElementalValues.cc:141 and updateAfterIncrement.cc:105).  No counterpart in the reference
+ * (one CPU call per qp); the results agree to rounding in both forms.
+ *   PA_QP_LANE (default): one lane per qp; numerically singular local systems are deferred to a
+ *     cooperative pseudo-inverse pass and replayed.
+ *   PA_QP_COOP: a group of lanes per qp (lane = slip system / tangent column, matrices in LDS),
+ *     persistent wavefronts on a work queue, singular systems solved in place by the group. */
+int pa_set_qp_form(pa_context *ctx, int form);
+
 /* One Newton iteration of solveNonLinearSystem (src/ellipticBVP/solveNonLinearSystem.cc:14-47):
  * applyDirichletBCs + assemble (boundaryConditions.cc:452-514, assemble.cc:6-104,
  * getElementalValues.cc:5-168) then solveLinearSystem (solveLinearSystem.cc:8-36).
@@ -161,7 +171,8 @@ int pa_qp_update(pa_context *ctx, long long nqp, const double *F, const double *
  * active-set matrix A_PA is singular, with the same small-singular-value rule as the quadrature-
  * point kernels.  path 0 solves each system in one lane (the routine a quadrature point runs in
  * its own lane), path 1 with the cooperative kernel that solves queued singular systems (a group
- * of lanes per system); both give bit-identical x.  No context needed. */
+ * of lanes per system), path 2 with the routine the PA_QP_COOP qp kernels run in place (a group of
+ * lanes per system, rows in LDS); all three give bit-identical x.  No context needed. */
 int pa_pinv_solve(int path, int n, long long count, const double *A, const double *b, double *x);
 
 /* Timing of the last pa_newton_iteration / pa_run_increment phases on the context stream (ms):

@@ -21,6 +21,7 @@
 
 #include "../../include/plasticity_amd.h"
 #include "cp_qp.h"
+#include "cp_coop.h"
 #include "cp_tables.h"
 #include "cp_mlp.h"
 
@@ -119,31 +120,12 @@ __device__ bool grid_sum(double (&v)[NV], double *partials, unsigned *counter, d
 }
 
 // ------------------------------------------------------------ mesh helpers
-struct MeshDims {
-  int nx, ny, nz, nnx, nny, nnz;
-  long long nnode, nelem;
-};
+#include "cp_passes.h"
 
 __device__ __forceinline__ void node_ijk(const MeshDims &m, long long n, int &i, int &j, int &k) {
   i = (int)(n % m.nnx);
   j = (int)((n / m.nnx) % m.nny);
   k = (int)(n / ((long long)m.nnx * m.nny));
-}
-
-// Q1 gradient of vertex v at Gauss point q of a box cell with sides h
-__device__ __forceinline__ void shape_grad(int q, int v, const double *h, double g[3]) {
-  const double g0 = 0.5 - 0.5 / sqrt(3.0), g1 = 0.5 + 0.5 / sqrt(3.0);
-  double N[3], dN[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const double xi = ((q >> d) & 1) ? g1 : g0;
-    const int b = (v >> d) & 1;
-    N[d] = b ? xi : 1.0 - xi;
-    dN[d] = (b ? 1.0 : -1.0) / h[d];
-  }
-  g[0] = dN[0] * N[1] * N[2];
-  g[1] = N[0] * dN[1] * N[2];
-  g[2] = N[0] * N[1] * dN[2];
 }
 
 // ---------------------------------------------------- grain ids (integer work)
@@ -290,18 +272,6 @@ __global__ void k_dirichlet(MeshDims m, BCArgs bc, unsigned char *__restrict__ c
 // (few qps meet a second one, and a round costs the latency of a whole qp), otherwise the qp
 // leaves its next system.  Entries above capacity are finished with the pseudo-inverse in their
 // own lane.  Every path computes the same x (same Jacobi sweep order and rules).
-constexpr int SING_ROUNDS = 1;
-struct SingStore {
-  unsigned cap;  // entries with storage
-  double *A;     // [cap][NS*NS] A_PA, leading dimension NS
-  double *b;     // [cap][NS]
-  int *n;        // [cap]
-  double *X;     // [cap][SING_ROUNDS][NS] solutions in the order the qp meets its singular systems
-  unsigned long long *key;  // [cap][SING_ROUNDS] hash of the bits of each solved system
-  int *have;     // [cap] solutions stored
-  int *done;     // [cap] 1 once the qp is finished
-  unsigned *miss;  // replays whose system differed from the stored one (solved in their lane)
-};
 // FNV-1a over the bit patterns of n, A_PA and b_PA
 template <int NS>
 __device__ __forceinline__ unsigned long long sing_key(int n, const double *A, const double *b) {
@@ -533,23 +503,6 @@ __global__ void __launch_bounds__(64) k_pinv_lane(unsigned count, int n, const d
 // ------------------------------------------------------------ assembly
 // ---- assembly, stage 1: one lane per qp.  F from u, calculatePlasticity(.,.,1)
 // (getElementalValues.cc:129-141); writes the iter history and JxW * dP/dF (as DT) and JxW * P.
-struct QpArgs {
-  MeshDims m;
-  double h[3], JxW;
-  const double *u;
-  const unsigned *egrain;
-  const double *gtab;
-  int gstride;
-  const double *mat;
-  int ns, max_solver_iter;
-  long long nq;
-  const double *Fp_conv, *s_conv, *slip_conv, *W_conv;
-  double *Fe_iter, *Fp_iter, *s_iter, *slip_iter, *W_iter;  // W_*: backstress (kinematic hardening only)
-  void *D;    // [81][nq] of DT
-  double *P;  // [9][nq]
-  unsigned *queue, *qcount;  // deferred qps
-  SingStore sing;            // their singular systems
-};
 
 // Lanes whose local problem meets a numerically singular A_PA (the pseudo-inverse branch of
 // inactive-slip removal) are deferred: pass 1 runs every qp on the LU path and appends the
@@ -769,24 +722,6 @@ __global__ void __launch_bounds__(64) k_element(ElemArgs a) {
 }
 
 // ------------------------------------------------ post-increment qp pass
-struct PostArgs {
-  MeshDims m;
-  double h[3], JxW;
-  const double *u;
-  const unsigned *egrain;
-  const double *gtab;
-  int gstride;
-  const double *mat;
-  int ns, max_solver_iter;
-  long long nq;
-  const double *Fp_conv, *s_conv, *slip_conv, *Fe_conv, *W_conv;
-  double *Fe_iter, *Fp_iter, *s_iter, *slip_iter, *T, *rotnew, *W_iter;
-  double *partials;
-  unsigned *counter;
-  double *out;  // 13: E(6) T(6) slip
-  unsigned *queue, *qcount;  // deferred qps
-  SingStore sing;            // their singular systems
-};
 
 // symmetric 3x3 eigen-decomposition by cyclic Jacobi rotations.  The sweeps end when the
 // off-diagonal is zero or a sweep leaves the eigenvalues (diagonal of C) and V unchanged: the
@@ -844,27 +779,6 @@ __device__ void polar_R(const double *Fe, double *R) {
   cp::m3_mult(VL, V, U);
   cp::m3_inv(U, Ui);
   cp::m3_mul(Fe, Ui, R);
-}
-
-__device__ __forceinline__ void post_F(const PostArgs &a, long long Q, double F[9]) {
-  const long long e = Q >> 3;
-  const int q = (int)(Q & 7);
-  const int ex = (int)(e % a.m.nx), ey = (int)((e / a.m.nx) % a.m.ny), ez = (int)(e / ((long long)a.m.nx * a.m.ny));
-#pragma unroll
-  for (int i = 0; i < 9; ++i) F[i] = 0.0;
-#pragma unroll
-  for (int v = 0; v < 8; ++v) {
-    const long long nd = (ex + (v & 1)) + (long long)a.m.nnx * ((ey + ((v >> 1) & 1)) + (long long)a.m.nny * (ez + ((v >> 2) & 1)));
-    double g[3];
-    shape_grad(q, v, a.h, g);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const double uc = a.u[3 * nd + c];
-#pragma unroll
-      for (int j = 0; j < 3; ++j) F[3 * c + j] += uc * g[j];
-    }
-  }
-  F[0] += 1; F[4] += 1; F[8] += 1;
 }
 
 // updateAfterIncrement's qp loop (updateAfterIncrement.cc:77-126): calculatePlasticity(.,.,0) at
@@ -972,41 +886,54 @@ __global__ void __launch_bounds__(256) k_post_reduce(PostArgs a) {
   }
 }
 
-// batch of calculatePlasticity calls with host-provided inputs (parity entry point); W (KIN):
-// backstress in (W_kh_conv) and out (W_kh_tau), n_slip per qp
+// ---- cooperative qp kernels (bodies in cp_passes.h): one-wave blocks, one LDS slice per group of
+// lanes.  The batch entry point (pa_qp_update) always runs in this form; the Newton-step passes run
+// in it when the context's qp form is PA_QP_COOP.
 template <int NS, bool KIN>
-__global__ void __launch_bounds__(64) k_qp_batch(long long n, const double *__restrict__ F, const double *__restrict__ Fpc,
-                                                 const double *__restrict__ sc, const unsigned *__restrict__ grain,
-                                                 const double *gtab, int gstride, const double *mat, int ns, int msi, int flag,
-                                                 double *P, double *T, double *D, double *Fe, double *Fp, double *so, double *xo,
-                                                 int *na, double *Wio) {
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double f[9], fp[9], s[NS], x[NS], p[9], t[9], fe[9], fpo[9], w[KIN ? NS : 1];
-  for (int k = 0; k < 9; ++k) { f[k] = F[9 * i + k]; fp[k] = Fpc[9 * i + k]; }
-  for (int k = 0; k < ns; ++k) s[k] = sc[(long long)ns * i + k];
-  if constexpr (KIN)
-    for (int k = 0; k < ns; ++k) w[k] = Wio[(long long)ns * i + k];
-  const double *gt = gtab + (long long)grain[i] * gstride;
-  const int r = cp::qp_update<NS, double, KIN>(mat, gt, ns, msi, f, fp, s, x, flag, p, t, fe, fpo, flag ? D + 81 * i : nullptr, 1,
-                                               nullptr, 1.0, cp::SvdInLane<NS>(), w);
-  for (int k = 0; k < 9; ++k) { P[9 * i + k] = p[k]; T[9 * i + k] = t[k]; Fe[9 * i + k] = fe[k]; Fp[9 * i + k] = fpo[k]; }
-  for (int k = 0; k < ns; ++k) { so[(long long)ns * i + k] = s[k]; xo[(long long)ns * i + k] = x[k]; }
-  if constexpr (KIN)
-    for (int k = 0; k < ns; ++k) Wio[(long long)ns * i + k] = w[k];
-  na[i] = r;
+__global__ void __launch_bounds__(64) k_qp_batch(BatchArgs b) {
+  __shared__ double lds[64 / coop::group_size<NS>() * coop::Slice<NS, KIN>::size];
+  qp_batch_body<NS, KIN>(b, lds);
+}
+template <int NS, typename DT, bool KIN>
+__global__ void __launch_bounds__(64) k_qp_tangent_coop(QpArgs a) {
+  __shared__ double lds[64 / coop::group_size<NS>() * coop::Slice<NS, KIN>::size];
+  qp_tangent_coop_pass<NS, DT, KIN>(a, lds);
+}
+template <int NS, bool KIN>
+__global__ void __launch_bounds__(64) k_post_qp_coop(PostArgs a) {
+  __shared__ double lds[64 / coop::group_size<NS>() * coop::Slice<NS, KIN>::size];
+  post_qp_coop_pass<NS, KIN>(a, lds);
 }
 
 template <int NS>
-void launch_qp_batch(hipStream_t st, bool kin, unsigned grid, long long n, const double *F, const double *Fpc, const double *sc,
-                     const unsigned *grain, const double *gtab, int gstride, const double *mat, int ns, int msi, int flag, double *P,
-                     double *T, double *D, double *Fe, double *Fp, double *so, double *xo, int *na, double *Wio) {
-  if (kin)
-    hipLaunchKernelGGL((k_qp_batch<NS, true>), dim3(grid), dim3(64), 0, st, n, F, Fpc, sc, grain, gtab, gstride, mat, ns, msi, flag, P,
-                       T, D, Fe, Fp, so, xo, na, Wio);
-  else
-    hipLaunchKernelGGL((k_qp_batch<NS, false>), dim3(grid), dim3(64), 0, st, n, F, Fpc, sc, grain, gtab, gstride, mat, ns, msi, flag, P,
-                       T, D, Fe, Fp, so, xo, na, Wio);
+void launch_qp_batch(hipStream_t st, bool kin, const BatchArgs &b) {
+  const unsigned grid = (unsigned)((b.n + 64 / coop::group_size<NS>() - 1) / (64 / coop::group_size<NS>()));
+  if (kin) hipLaunchKernelGGL((k_qp_batch<NS, true>), dim3(grid), dim3(64), 0, st, b);
+  else hipLaunchKernelGGL((k_qp_batch<NS, false>), dim3(grid), dim3(64), 0, st, b);
+}
+
+// x = pinv(A) b for dense systems, one group of lanes per system: the cooperative pseudo-inverse
+// (coop::gsvd_pinv) of the cooperative qp kernels, for pa_pinv_solve (A: [count][NS*NS], b and x:
+// [count][NS]).
+template <int NS>
+__global__ void __launch_bounds__(64) k_pinv_group(unsigned count, int n, const double *A, const double *b, double *x) {
+  constexpr int G = coop::group_size<NS>(), GPW = 64 / G;
+  typedef coop::Slice<NS, false> SL;
+  __shared__ double lds[GPW * SL::size];
+  const coop::Group<G> g;
+  const int gi = threadIdx.x / G;
+  double *L = lds + gi * SL::size;
+  const unsigned e = blockIdx.x * GPW + gi;
+  if (e >= count) return;  // uniform within the group
+  const unsigned long long mask = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
+  double *M0 = L + SL::M0, *bv = L + SL::b, *yv = L + SL::y;
+  if (g.l < n) {
+    for (int j = 0; j < n; ++j) M0[g.l * NS + j] = A[(size_t)e * NS * NS + g.l * NS + j];
+    bv[g.l] = b[(size_t)e * NS + g.l];
+  }
+  coop::gsync();
+  coop::gsvd_pinv<NS, G, false>(g, mask, M0, bv, L, yv);
+  if (g.l < n) x[(size_t)e * NS + g.l] = yv[g.l];
 }
 
 // Per-grain sums of JxW and JxW * Cauchy stress (updateAfterIncrement.cc:611-668): one block per
@@ -1493,6 +1420,7 @@ struct pa_context {
   int ns = 0;
   long long nq = 0, ndof = 0, nnz = 0;
   int method = PA_SOLVER_BICG, precision = PA_PREC_FP64;
+  int qp_form = PA_QP_LANE;  // layout of the Newton-step qp passes (pa_set_qp_form)
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t kev[2] = {nullptr, nullptr};  // Krylov flag copies, one per chunk parity
@@ -1573,9 +1501,34 @@ void qp_tangent_passes_k(pa_context *c, const QpArgs &a, bool main_only) {
   }
   hipLaunchKernelGGL((k_qp_tangent_deferred<NS, DT, KIN>), dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
 }
+// Blocks of a persistent cooperative qp kernel: as many one-wave blocks as the device holds at
+// once (from the kernel's occupancy), at most one per chunk of qps.
+template <typename K>
+unsigned resident_grid(K kernel, long long nq) {
+  int dev = 0, ncu = 0, nb = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 64, 0) != hipSuccess)
+    nb = 1, ncu = 256;
+  const long long full = (long long)std::max(nb, 1) * std::max(ncu, 1);
+  return (unsigned)std::max(1LL, std::min(full, (nq + QP_CHUNK - 1) / QP_CHUNK));
+}
+// cooperative form: one persistent launch on a work queue (counters 8: pseudo-inverse solves,
+// 11: the queue)
+template <int NS, typename DT, bool KIN>
+void qp_tangent_coop_k(pa_context *c, const QpArgs &a) {
+  static const unsigned grid_full = resident_grid(k_qp_tangent_coop<NS, DT, KIN>, 1LL << 40);
+  const unsigned grid = (unsigned)std::min<long long>(grid_full, (c->nq + QP_CHUNK - 1) / QP_CHUNK);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, c->stream, c->counters + 11);
+  hipLaunchKernelGGL((k_qp_tangent_coop<NS, DT, KIN>), dim3(grid), dim3(64), 0, c->stream, a);
+}
 template <int NS, typename DT>
 void qp_tangent_passes(pa_context *c, const QpArgs &a, bool main_only) {
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, c->stream, a.qcount);
+  if (c->qp_form == PA_QP_COOP) {
+    if (c->kin) qp_tangent_coop_k<NS, DT, true>(c, a);
+    else qp_tangent_coop_k<NS, DT, false>(c, a);
+    return;
+  }
   if (c->kin) qp_tangent_passes_k<NS, DT, true>(c, a, main_only);
   else qp_tangent_passes_k<NS, DT, false>(c, a, main_only);
 }
@@ -1592,6 +1545,7 @@ int launch_qp_tangent(pa_context *c, bool main_only = false) {
   a.Fe_iter = c->Fe_iter; a.Fp_iter = c->Fp_iter; a.s_iter = c->s_iter; a.slip_iter = c->slip_iter; a.W_iter = c->W_iter;
   a.D = c->Dq; a.P = c->Pq;
   a.queue = c->queue; a.qcount = c->counters + 8; a.sing = c->sing;
+  a.nsing = c->counters + 8; a.work = c->counters + 11;
   const int NS = ns_template(c->ns);
   const bool f = mixed(c);
   if (NS == 12) { if (f) qp_tangent_passes<12, float>(c, a, main_only); else qp_tangent_passes<12, double>(c, a, main_only); }
@@ -1611,8 +1565,20 @@ void post_passes_k(pa_context *c, const PostArgs &a, unsigned grid) {
   }
   hipLaunchKernelGGL((k_post_qp_deferred<NS, KIN>), dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
 }
+template <int NS, bool KIN>
+void post_coop_k(pa_context *c, const PostArgs &a) {
+  static const unsigned grid_full = resident_grid(k_post_qp_coop<NS, KIN>, 1LL << 40);
+  const unsigned grid = (unsigned)std::min<long long>(grid_full, (c->nq + QP_CHUNK - 1) / QP_CHUNK);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, c->stream, c->counters + 12);
+  hipLaunchKernelGGL((k_post_qp_coop<NS, KIN>), dim3(grid), dim3(64), 0, c->stream, a);
+}
 template <int NS>
 void post_passes(pa_context *c, const PostArgs &a, unsigned grid) {
+  if (c->qp_form == PA_QP_COOP) {
+    if (c->kin) post_coop_k<NS, true>(c, a);
+    else post_coop_k<NS, false>(c, a);
+    return;
+  }
   if (c->kin) post_passes_k<NS, true>(c, a, grid);
   else post_passes_k<NS, false>(c, a, grid);
 }
@@ -1970,6 +1936,21 @@ int do_solve(pa_context *c, int it, int *iters) {
 
 }  // namespace
 
+namespace {
+// The one-lane qp passes hold each lane's active-set and tangent matrices in private memory: 8.6 KB
+// per lane for up to 12 systems, 24 KB for up to 24, 84 KB for up to 48.  The runtime refuses the
+// scratch reservation of the 48-system variant on MI355X (HSA_STATUS_ERROR_OUT_OF_RESOURCES at its
+// first dispatch, even for two waves), so a Newton step with more than 24 systems needs the
+// cooperative form (no private memory).  The batch entry point always runs cooperatively.
+int check_qp_form(const pa_context *c) {
+  if (c->ns > 24 && c->qp_form == PA_QP_LANE)
+    return fail(PA_ERR_UNSUPPORTED, "more than 24 slip systems in a Newton step need the cooperative qp form "
+                                    "(pa_set_qp_form PA_QP_COOP): the one-lane 48-system kernels need 84 KB of private "
+                                    "memory per lane, which the runtime does not reserve on this GPU");
+  return 0;
+}
+}  // namespace
+
 extern "C" {
 
 const char *pa_last_error(void) { return g_err.c_str(); }
@@ -1978,14 +1959,6 @@ int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out) {
   if (!mat || !pb || !out) return fail(PA_ERR_ARG, "null argument");
   *out = nullptr;
   if (mat->n_slip < 1 || mat->n_slip > PA_MAX_SLIP) return fail(PA_ERR_ARG, "n_slip out of range");
-  // The quadrature-point kernels hold each lane's active-set and tangent matrices in private
-  // memory: 8.6 KB per lane for up to 12 systems, 24 KB for up to 24, 84 KB for up to 48.  The
-  // runtime refuses the scratch reservation of the 48-system variant on MI355X
-  // (HSA_STATUS_ERROR_OUT_OF_RESOURCES at its first dispatch, even for two waves), so more than
-  // 24 systems are rejected here rather than failing inside a Newton step.
-  if (mat->n_slip > 24)
-    return fail(PA_ERR_UNSUPPORTED, "more than 24 slip systems: the 48-system kernels need 84 KB of private memory per "
-                                    "lane, which the runtime does not reserve on this GPU");
   if (mat->max_slip_search != 1) return fail(PA_ERR_UNSUPPORTED, "Max Slip Search Iterations must be 1");
   if (mat->max_solver_iter < 1) return fail(PA_ERR_ARG, "Max Solver Iterations must be >= 1");
   if (mat->twinning) return fail(PA_ERR_UNSUPPORTED, "twinning is not implemented");
@@ -2202,9 +2175,17 @@ int pa_set_solver(pa_context *c, int method, int precision) {
   return 0;
 }
 
+int pa_set_qp_form(pa_context *c, int form) {
+  if (!c) return fail(PA_ERR_ARG, "null context");
+  if (form != PA_QP_LANE && form != PA_QP_COOP) return fail(PA_ERR_ARG, "unknown qp form");
+  c->qp_form = form;
+  return 0;
+}
+
 int pa_assemble(pa_context *c, int inc, int it, double *res_norm) {
   if (!c) return fail(PA_ERR_ARG, "null context");
   if (!c->grains_set) return fail(PA_ERR_STATE, "pa_set_grains has not been called");
+  if (int rc = check_qp_form(c)) return rc;
   HIPCHK(hipEventRecord(c->ev[0], c->stream));
   int rc = do_assemble(c, inc, it, res_norm);
   if (rc) return rc;
@@ -2235,6 +2216,7 @@ int pa_newton_iteration(pa_context *c, int inc, int it, double *res_norm, int *i
 int pa_commit_increment(pa_context *c, double row[15]) {
   if (!c) return fail(PA_ERR_ARG, "null context");
   if (!c->grains_set) return fail(PA_ERR_STATE, "pa_set_grains has not been called");
+  if (int rc = check_qp_form(c)) return rc;
   PostArgs a;
   a.m = c->m;
   for (int d = 0; d < 3; ++d) a.h[d] = c->h[d];
@@ -2246,6 +2228,7 @@ int pa_commit_increment(pa_context *c, double row[15]) {
   a.W_iter = c->W_iter;
   a.partials = c->partials; a.counter = c->counters + 4; a.out = c->d_out;
   a.queue = c->queue; a.qcount = c->counters + 9; a.sing = c->sing;
+  a.nsing = c->counters + 9; a.work = c->counters + 12;
   HIPCHK(hipEventRecord(c->ev[0], c->stream));
   const int NS = ns_template(c->ns);
   const unsigned grid = nblk(c->nq, 256);
@@ -2412,11 +2395,11 @@ int pa_qp_update(pa_context *c, long long n, const double *F, const double *Fp_c
     rc = fail(PA_ERR_HIP, "copy inputs");
   if (!rc) {
     const int NS = ns_template(ns);
-    const unsigned grid = nblk(n, 64);
     const int msi = c->mat.max_solver_iter;
-    if (NS == 12) launch_qp_batch<12>(c->stream, c->kin, grid, n, dF, dFp, ds, dg, c->d_gtab, c->gstride, c->d_mat, ns, msi, flag, dP, dT, dD, dFe, dFpo, dso, dx, dna, dW);
-    else if (NS == 24) launch_qp_batch<24>(c->stream, c->kin, grid, n, dF, dFp, ds, dg, c->d_gtab, c->gstride, c->d_mat, ns, msi, flag, dP, dT, dD, dFe, dFpo, dso, dx, dna, dW);
-    else launch_qp_batch<48>(c->stream, c->kin, grid, n, dF, dFp, ds, dg, c->d_gtab, c->gstride, c->d_mat, ns, msi, flag, dP, dT, dD, dFe, dFpo, dso, dx, dna, dW);
+    const BatchArgs b{n, dF, dFp, ds, dg, c->d_gtab, c->gstride, c->d_mat, ns, msi, flag, dP, dT, dD, dFe, dFpo, dso, dx, dna, dW};
+    if (NS == 12) launch_qp_batch<12>(c->stream, c->kin, b);
+    else if (NS == 24) launch_qp_batch<24>(c->stream, c->kin, b);
+    else launch_qp_batch<48>(c->stream, c->kin, b);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(PA_ERR_HIP, "qp kernel");
   }
   if (!rc && (hipMemcpy(P, dP, b9, hipMemcpyDeviceToHost) != hipSuccess || hipMemcpy(T, dT, b9, hipMemcpyDeviceToHost) != hipSuccess ||
@@ -2691,7 +2674,7 @@ int pa_grain_averages(pa_context *c, int n_grains, double *out) {
 }
 
 int pa_pinv_solve(int path, int n, long long count, const double *A, const double *b, double *x) {
-  if ((path != 0 && path != 1) || n < 1 || n > 48 || count < 0 || count > (1ll << 20) || (count && (!A || !b || !x)))
+  if (path < 0 || path > 2 || n < 1 || n > 48 || count < 0 || count > (1ll << 20) || (count && (!A || !b || !x)))
     return fail(PA_ERR_ARG, "bad argument");
   if (count == 0) return 0;
   const int NS = ns_template(n);
@@ -2725,10 +2708,14 @@ int pa_pinv_solve(int path, int n, long long count, const double *A, const doubl
       if (NS == 12) hipLaunchKernelGGL(k_pinv_lane<12>, dim3(nblk(cnt, 64)), dim3(64), 0, 0, cnt, n, st.A, st.b, st.X);
       else if (NS == 24) hipLaunchKernelGGL(k_pinv_lane<24>, dim3(nblk(cnt, 64)), dim3(64), 0, 0, cnt, n, st.A, st.b, st.X);
       else hipLaunchKernelGGL(k_pinv_lane<48>, dim3(nblk(cnt, 64)), dim3(64), 0, 0, cnt, n, st.A, st.b, st.X);
-    } else {
+    } else if (path == 1) {
       if (NS == 12) hipLaunchKernelGGL(k_sing_svd<12>, dim3(SVD_GRID), dim3(sing_per_block<12>() * sing_group<12>()), 0, 0, st, (const unsigned *)dcount);
       else if (NS == 24) hipLaunchKernelGGL(k_sing_svd<24>, dim3(SVD_GRID), dim3(sing_per_block<24>() * sing_group<24>()), 0, 0, st, (const unsigned *)dcount);
       else hipLaunchKernelGGL(k_sing_svd<48>, dim3(SVD_GRID), dim3(sing_per_block<48>() * sing_group<48>()), 0, 0, st, (const unsigned *)dcount);
+    } else {  // the cooperative qp kernels' routine, one group of lanes per system
+      if (NS == 12) hipLaunchKernelGGL(k_pinv_group<12>, dim3(nblk(cnt, 64 / coop::group_size<12>())), dim3(64), 0, 0, cnt, n, st.A, st.b, st.X);
+      else if (NS == 24) hipLaunchKernelGGL(k_pinv_group<24>, dim3(nblk(cnt, 64 / coop::group_size<24>())), dim3(64), 0, 0, cnt, n, st.A, st.b, st.X);
+      else hipLaunchKernelGGL(k_pinv_group<48>, dim3(nblk(cnt, 64 / coop::group_size<48>())), dim3(64), 0, 0, cnt, n, st.A, st.b, st.X);
     }
     if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = fail(PA_ERR_HIP, "pinv kernel");
   }
@@ -2743,6 +2730,7 @@ int pa_pinv_solve(int path, int n, long long count, const double *A, const doubl
 int pa_time_kernel(pa_context *c, int which, int reps, double *avg_ms) {
   if (!c || !avg_ms || reps < 1) return fail(PA_ERR_ARG, "bad argument");
   if (!c->grains_set) return fail(PA_ERR_STATE, "pa_set_grains has not been called");
+  if (int rc = check_qp_form(c)) return rc;
   const unsigned gn = nblk(c->m.nnode, VB), gu = ugrid(c->ndof);
   double *partA = c->partials, *partU = c->partials + spmv_partials_offset(c);
   // solver state that never ends (tol < 0, no iteration limit) with unit r.z history, and update

@@ -228,6 +228,10 @@ class Context:
     def set_solver(self, method=native.PA_SOLVER_BICG, precision=native.PA_PREC_FP64):
         check(native.lib().pa_set_solver(self._h, ctypes.c_int(method), ctypes.c_int(precision)))
 
+    def set_qp_form(self, form):
+        """Layout of the Newton step's qp passes: native.PA_QP_LANE (default) or native.PA_QP_COOP."""
+        check(native.lib().pa_set_qp_form(self._h, ctypes.c_int(form)))
+
     def assemble(self, inc, it):
         rn = ctypes.c_double()
         check(native.lib().pa_assemble(self._h, ctypes.c_int(inc), ctypes.c_int(it), ctypes.byref(rn)))
@@ -319,7 +323,8 @@ class Context:
 def pinv_solve(A, b, path=0):
     """x = pinv(A) b for a batch of n x n systems (A: (count, n, n), b: (count, n)) on the GPU:
     the pseudo-inverse step of inactiveSlipRemoval.cc:26,99; path 0 = one lane per system,
-    1 = the cooperative kernel (a group of lanes per system)."""
+    1 = the cooperative kernel for queued systems, 2 = the group routine of the cooperative qp
+    kernels (a group of lanes per system in both)."""
     A = f64(A)
     count, n = A.shape[0], A.shape[1]
     b = f64(b).reshape(count, n)

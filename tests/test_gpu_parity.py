@@ -163,13 +163,15 @@ def _bcc48_systems():
     return np.array(m, float), np.array(n, float)
 
 
-@pytest.mark.skip(reason="the 48-system qp kernels need 84 KB of private memory per lane; their first dispatch "
-                         "failed with HSA_STATUS_ERROR_OUT_OF_RESOURCES on MI355X, so pa_create rejects n_slip > 24")
 def test_qp_update_bcc48_matches_oracle():
-    """BCC with 48 slip systems (the 48-system kernel variant; C11/C12/C44 = 231/135/117 GPa), per
-    qp at 1e-10 against the oracle, with and without the tangent."""
+    """BCC with 48 slip systems (C11/C12/C44 = 231/135/117 GPa) through the batch entry point, whose
+    cooperative kernel (a wavefront per qp, matrices in LDS) needs no private memory; per qp at
+    1e-10 against the oracle, with and without the tangent.  The oracle takes unit slip vectors
+    (as the reference normalises the file rows, init.cc:73-123); the library normalises its own."""
     m, nrm = _bcc48_systems()
     assert len(m) == 48 and np.allclose((m * nrm).sum(axis=1), 0)
+    m = m / np.linalg.norm(m, axis=1)[:, None]
+    nrm = nrm / np.linalg.norm(nrm, axis=1)[:, None]
     q = np.full((48, 48), 1.4)
     for a in range(48):
         for b in range(48):
@@ -190,13 +192,20 @@ def test_qp_update_bcc48_matches_oracle():
     grain = rng.integers(0, 8, size=n).astype(np.uint32)
     ctx = _ctx(case, nel=(1, 1, 1))
     ctx.set_grains(np.zeros(1, np.uint32), rod)
+    # states where the reference's own result depends on round-off (rank-deficient A_PA whose
+    # smallest singular values sit at round-off): the oracle moves by > 1e-9 under 1e-14 relative
+    # perturbations of F; no implementation can be held to 1e-10 there
+    ill = {i for i in range(n) if _roundoff_dependent(case, rod[grain[i]], F[i], Fp[i], s[i])}
+    assert len(ill) <= n // 20, sorted(ill)
     for flag in (0, 1):
         g = ctx.qp_update(F.reshape(n, 9), Fp.reshape(n, 9), s, grain, flag)
         nplastic = 0
         for i in range(n):
             o = refcase.oracle_qp(case, rod[grain[i]], F[i], Fp[i], np.append(s[i], 10e14), flag)
-            assert g["n_active"][i] == o["n_active"], (i, g["n_active"][i], o["n_active"])
             nplastic += o["n_active"] > 0
+            if i in ill:
+                continue
+            assert g["n_active"][i] == o["n_active"], (i, g["n_active"][i], o["n_active"])
             for k in ("P", "T", "Fe", "Fp"):
                 assert _rel(g[k][i], o[k].ravel()) < 1e-10, (k, i)
             assert _rel(g["s"][i], o["s"][:48]) < 1e-10
@@ -204,6 +213,18 @@ def test_qp_update_bcc48_matches_oracle():
                 assert _rel(g["dPdF"][i], o["dPdF"].ravel()) < 1e-10, (i, _rel(g["dPdF"][i], o["dPdF"].ravel()))
         assert nplastic > n // 4
     ctx.close()
+
+
+def _roundoff_dependent(case, rod, F, Fp, s, eps=1e-14, tries=6):
+    """True when the oracle's stress moves by more than 1e-9 under `tries` random relative
+    perturbations of F of size eps."""
+    o = refcase.oracle_qp(case, rod, F, Fp, np.append(s, 10e14), 0)
+    r = np.random.default_rng(12345)
+    for _ in range(tries):
+        o2 = refcase.oracle_qp(case, rod, F * (1 + eps * r.standard_normal((3, 3))), Fp, np.append(s, 10e14), 0)
+        if _rel(o2["P"], o["P"]) > 1e-9 or o2["n_active"] != o["n_active"]:
+            return True
+    return False
 
 
 def test_hcp_slip_curve_matches_oracle():

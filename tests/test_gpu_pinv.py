@@ -1,8 +1,9 @@
 """The pseudo-inverse step of inactive-slip removal (inactiveSlipRemoval.cc:26,99) on the GPU.
 
-Both device paths, the one-lane routine and the cooperative kernel that solves queued singular
-systems, must give bit-identical x (a rerun of a quadrature point feeds one path's solution to a
-computation that elsewhere takes the other).  Against the oracle (or_pinv_solve, mode 1) and the
+The three device paths, the one-lane routine, the cooperative kernel that solves queued singular
+systems, and the group routine the cooperative qp kernels run in place, must give bit-identical x
+(a rerun of a quadrature point feeds one path's solution to a computation that elsewhere takes
+another).  Against the oracle (or_pinv_solve, mode 1) and the
 closed form V diag(1/sigma) U^T b: 1e-11 relative on full-rank systems, 1e-8 on exact rank
 deficiency with b off the range (minimum-norm solution), 1e-3 where one singular value of
 1e-11 sigma_max is inverted (its relative round-off is about eps / 1e-11).
@@ -65,7 +66,9 @@ def test_pinv_paths_bit_identical_and_match_oracle(n, kind):
     print("pinv", n, kind, count, flush=True)
     x0 = pa.model.pinv_solve(A, b, path=0)
     x1 = pa.model.pinv_solve(A, b, path=1)
+    x2 = pa.model.pinv_solve(A, b, path=2)
     assert np.array_equal(x0.view(np.int64), x1.view(np.int64)), "lane and cooperative paths differ"
+    assert np.array_equal(x0.view(np.int64), x2.view(np.int64)), "lane and in-place group paths differ"
     xo = np.zeros_like(x0)
     for e in range(count):
         Ae = np.ascontiguousarray(A[e])
