@@ -129,14 +129,21 @@ def test_qp_update_hcp_slip_matches_oracle():
     ctx.close()
 
 
-def test_more_than_24_slip_systems_rejected():
-    """pa_create refuses the 48-system configuration with PA_ERR_UNSUPPORTED (no GPU work)."""
+def test_more_than_24_slip_systems_need_the_cooperative_form():
+    """A 48-system context is created, but a Newton step in the one-lane form (whose 48-system
+    kernels need 84 KB of private memory per lane) is refused with PA_ERR_UNSUPPORTED before any
+    qp kernel runs."""
     m, nrm = _bcc48_systems()
     C = np.diag([231e3] * 3 + [117e3] * 3)
     mat = pa.make_material(m, nrm, np.ones((48, 48)), C, [30.0] * 48, [400.0] * 48, [150.0] * 48, [1.5] * 48, 1e-9)
     pb = pa.make_problem((2, 2, 2), (1.0, 1.0, 1.0), 10, 1.0, [(1, 1, 0.0), (2, 1, 0.01)])
+    ctx = pa.Context(mat, pb)
+    ctx.set_grains(np.zeros(8, np.uint32), np.zeros((1, 3)))
     with pytest.raises(pa.PlasticityError, match="more than 24 slip systems"):
-        pa.Context(mat, pb)
+        ctx.newton_iteration(0, 0)
+    with pytest.raises(pa.PlasticityError, match="more than 24 slip systems"):
+        ctx.commit_increment()
+    ctx.close()
 
 
 def _bcc48_systems():
