@@ -19,6 +19,18 @@
 
 namespace {
 
+// Test-only knob (or_tangent_perturbation): scales one block of the consistent tangent by
+// (1 + g_pert_rel) so that a test can show that its checks see an error of that size there.
+// Block ids: 1 delb, 2 the elastic part of A2, 3 the hardening part of A2, 4 the k >= 1 terms of
+// the d exp series, 5 S_PA, 6 dT*/dF in the final tangent, 7 the plastic part of dFe/dF,
+// 8 the q5 term of the final tangent; 9 (rel ignored) replaces the kinematic-hardening term of A2 by
+// the exact linearisation of b_i = |tau_i - W_i| - s_i with dW_i = (sgn_i C_1 - C_2 W_i) dgamma_i,
+// i.e. C_1 - sgn_i C_2 W_i on the diagonal, where the reference adds sgn_i (C_1 - C_2 W_i) to every
+// column of the row (:700-703).  0 (the default) leaves the tangent as the reference forms it.
+int g_pert_block = 0;
+double g_pert_rel = 0.0;
+inline double pert(int block) { return g_pert_block == block ? 1.0 + g_pert_rel : 1.0; }
+
 // ---------------------------------------------------------------- 3x3 algebra
 struct M3 {
   double a[3][3];
@@ -391,7 +403,7 @@ M3 dexp_series(const M3 &X, const M3 &Sj, int count3, double sign) {
       M3 t6 = mul(t5, mpow(X, k - l));
       addto(t4, 1.0, t6);
     }
-    const double c = (sign < 0 ? std::pow(-1.0, k) : 1.0) / std::tgamma((double)(k + 2));
+    const double c = pert(4) * (sign < 0 ? std::pow(-1.0, k) : 1.0) / std::tgamma((double)(k + 2));
     t4 = scale(t4, c);
     addto(diff, 1.0, t4);
   }
@@ -600,7 +612,7 @@ int qp_update(const or_material *mat, const Grain &g, const double *Fin, const d
       double dFe[81] = {0}, dE[81] = {0}, dT[81] = {0}, dtau[81] = {0};
       for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) for (int k = 0; k < 3; ++k) for (int l = 0; l < 3; ++l) {
         double &v = dFe[(3 * i + j) * 9 + 3 * k + l];
-        for (int a = 0; a < 3; ++a) for (int bb = 0; bb < 3; ++bb) v = v - t1[i][a] * delFp[(3 * a + bb) * 9 + 3 * k + l] * Fpn_inv[bb][j];
+        for (int a = 0; a < 3; ++a) for (int bb = 0; bb < 3; ++bb) v = v - pert(7) * t1[i][a] * delFp[(3 * a + bb) * 9 + 3 * k + l] * Fpn_inv[bb][j];
         if (i == k) v = v + Fpn_inv[l][j];
       }
       for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) for (int k = 0; k < 3; ++k) for (int l = 0; l < 3; ++l)
@@ -630,7 +642,7 @@ int qp_update(const or_material *mat, const Grain &g, const double *Fin, const d
         for (int l = 0; l < 9; ++l) {
           double s = 0;
           for (int m = 0; m < 9; ++m) s += dtau[m * 9 + l] * S[m / 3][m % 3];
-          delb[k * 9 + l] = s;
+          delb[k * 9 + l] = pert(1) * s;
         }
         if ((tau_tr[PA[k]] - W_t[PA[k]]) < 0) for (int l = 0; l < 9; ++l) delb[k * 9 + l] = -1.0 * delb[k * 9 + l];
       }
@@ -647,7 +659,7 @@ int qp_update(const or_material *mat, const Grain &g, const double *Fin, const d
         }
       }
       std::vector<double> A2(nPA * nPA);                   // :622-709
-      for (int i = 0; i < nPA; ++i) for (int j = 0; j < nPA; ++j) A2[i * nPA + j] = hab[PA[i] * nT + PA[j]];
+      for (int i = 0; i < nPA; ++i) for (int j = 0; j < nPA; ++j) A2[i * nPA + j] = pert(3) * hab[PA[i] * nT + PA[j]];
       const M3 eneg = expm3(scale(del_FP, -1.0));
       for (int j = 0; j < nPA; ++j) {
         const M3 diff = dexp_series(del_FP, g.S[PA[j]], count3, -1.0);
@@ -661,9 +673,14 @@ int qp_update(const or_material *mat, const Grain &g, const double *Fin, const d
           double &a = A2[i * nPA + j];
           const bool ni = (tau_tr[PA[i]] - W_t[PA[i]]) < 0.0, nj = (tau_tr[PA[j]] - W_t[PA[j]]) < 0.0;
           for (int k = 0; k < 3; ++k) for (int l = 0; l < 3; ++l) {
-            if (ni ^ nj) a -= g.S[PA[i]][k][l] * tm[k][l]; else a += g.S[PA[i]][k][l] * tm[k][l];
+            const double e = pert(2) * g.S[PA[i]][k][l] * tm[k][l];
+            if (ni ^ nj) a -= e; else a += e;
           }
-          if (ni) a -= mat->C1[PA[i]] - mat->C2[PA[i]] * W_tau[PA[i]]; else a += mat->C1[PA[i]] - mat->C2[PA[i]] * W_tau[PA[i]];
+          if (g_pert_block == 9) {
+            if (i == j) a += mat->C1[PA[i]] - (ni ? -1.0 : 1.0) * mat->C2[PA[i]] * W_tau[PA[i]];
+          } else {
+            if (ni) a -= mat->C1[PA[i]] - mat->C2[PA[i]] * W_tau[PA[i]]; else a += mat->C1[PA[i]] - mat->C2[PA[i]] * W_tau[PA[i]];
+          }
         }
       }
       std::vector<double> dg(nPA * 9);                     // :712-722 delgamma = A2^-1 (delb - dels_PA)
@@ -677,7 +694,7 @@ int qp_update(const or_material *mat, const Grain &g, const double *Fin, const d
         const M3 diff = dexp_series(del_FP, g.S[PA[j]], count3, +1.0);
         const M3 t = mul(diff, FP_t2);
         const bool nj = (tau_tr[PA[j]] - W_t[PA[j]]) < 0;
-        for (int k = 0; k < 3; ++k) for (int l = 0; l < 3; ++l) SPA[(3 * k + l) * nPA + j] = nj ? -t[k][l] : t[k][l];
+        for (int k = 0; k < 3; ++k) for (int l = 0; l < 3; ++l) SPA[(3 * k + l) * nPA + j] = pert(5) * (nj ? -t[k][l] : t[k][l]);
       }
       const M3 epos = expm3(del_FP);                       // :799-821
       std::vector<double> dFp2(81, 0.0);
@@ -702,12 +719,12 @@ int qp_update(const or_material *mat, const Grain &g, const double *Fin, const d
     double dFe[81] = {0}, dTs[81] = {0};
     for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) for (int k = 0; k < 3; ++k) for (int l = 0; l < 3; ++l) {
       double &v = dFe[(3 * i + j) * 9 + 3 * k + l];
-      for (int a = 0; a < 3; ++a) for (int bb = 0; bb < 3; ++bb) v = v - t1[i][a] * delFp[(3 * a + bb) * 9 + 3 * k + l] * Fpn_inv[bb][j];
+      for (int a = 0; a < 3; ++a) for (int bb = 0; bb < 3; ++bb) v = v - pert(7) * t1[i][a] * delFp[(3 * a + bb) * 9 + 3 * k + l] * Fpn_inv[bb][j];
       if (i == k) v = v + Fpn_inv[l][j];
     }
     for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) for (int k = 0; k < 3; ++k) for (int l = 0; l < 3; ++l)
       for (int a = 0; a < 3; ++a) for (int bb = 0; bb < 3; ++bb) for (int c = 0; c < 3; ++c)
-        dTs[(3 * i + j) * 9 + 3 * k + l] = dTs[(3 * i + j) * 9 + 3 * k + l] + g.TM[3 * i + j][3 * a + bb] * dFe[(3 * c + a) * 9 + 3 * k + l] * FE_tau[c][bb];
+        dTs[(3 * i + j) * 9 + 3 * k + l] = dTs[(3 * i + j) * 9 + 3 * k + l] + pert(6) * g.TM[3 * i + j][3 * a + bb] * dFe[(3 * c + a) * 9 + 3 * k + l] * FE_tau[c][bb];
     const M3 t4 = inv3(F);                                // :885-897
     const M3 tt = mul(t4, FE_tau);
     const M3 q1 = mult(Tstar_tau, tt);
@@ -720,7 +737,7 @@ int qp_update(const or_material *mat, const Grain &g, const double *Fin, const d
         for (int bb = 0; bb < 3; ++bb)
           v = v + (i == a ? 1.0 : 0.0) * dFe[(3 * a + bb) * 9 + 3 * k + l] * q1[bb][j] + FE_tau[i][a] * dTs[(3 * a + bb) * 9 + 3 * k + l] * q2[j][bb] -
               q3[i][a] * dFe[(3 * a + bb) * 9 + 3 * k + l] * t4[j][bb];
-        v = v - q5[i][a] * t4[j][k] * t4[l][a];
+        v = v - pert(8) * q5[i][a] * t4[j][k] * t4[l][a];
       }
       dPdF[(3 * i + j) * 9 + 3 * k + l] = v;              // :916-939 with L = I
     }
@@ -1069,6 +1086,11 @@ void reorient_qp(const M3 &Fe_old, const M3 &Fe_new, double r[3]) {
 
 // ======================================================================= C API
 extern "C" {
+
+void or_tangent_perturbation(int block, double rel) {
+  g_pert_block = block;
+  g_pert_rel = rel;
+}
 
 int or_qp_update(const or_material *mat, const double rot[3], const double F[9], const double Fp_conv[9],
                  const double *s_conv, const double *W_conv, const double *slip_conv, int flag, double P[9], double T[9],

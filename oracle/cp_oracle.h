@@ -75,6 +75,11 @@ int or_qp_update(const or_material *mat, const double rot[3], const double F[9],
                  double P[9], double T[9], double dPdF[81], double Fe_iter[9],
                  double Fp_iter[9], double *s_iter, double *W_iter, double *slip_iter);
 
+/* Test-only: scale block `block` of the consistent tangent by (1 + rel) in every later
+ * or_qp_update call (block 0 = none; ids in cp_oracle.cpp at g_pert_block).  Lets a test show
+ * that its tangent checks detect an error of that size in each block. */
+void or_tangent_perturbation(int block, double rel);
+
 /* or_qp_update with a warm start (no counterpart in the reference): x0 (n_slip values, may be
  * NULL = the reference's start from zero) holds predicted slip increments; those of the
  * potentially-active systems that are positive are applied once before the first solve of the
