@@ -31,7 +31,9 @@ enum {
   PA_ERR_UNSUPPORTED = 2, /* configuration outside what the kernels implement */
   PA_ERR_HIP = 3,         /* HIP runtime error (no GPU, allocation, launch) */
   PA_ERR_IO = 4,          /* unreadable input file (the reference exits with status 1) */
-  PA_ERR_STATE = 5        /* call out of order (e.g. no grains set) */
+  PA_ERR_STATE = 5,       /* call out of order (e.g. no grains set) */
+  PA_ERR_NONFINITE = 6    /* a NaN/Inf residual, Krylov norm or averaged stress: the reference's PETSc
+                           * solve stops on it (KSP_DIVERGED_NANORINF); no correction is applied */
 };
 
 enum { PA_SOLVER_BICG = 0, PA_SOLVER_CG = 1, PA_SOLVER_MGCG = 2 };
@@ -123,7 +125,10 @@ int pa_set_qp_form(pa_context *ctx, int form);
  * res_norm: ||R||_2 of the condensed residual; lin_iters: Krylov iterations. */
 int pa_newton_iteration(pa_context *ctx, int increment, int iteration, double *res_norm, int *lin_iters);
 
-/* Split halves of pa_newton_iteration, for timing and parity. */
+/* Split halves of pa_newton_iteration, for timing and parity.  pa_assemble returns
+ * PA_ERR_NONFINITE when the residual norm it reports is NaN/Inf; pa_linear_solve returns it when the
+ * Krylov solve stops on a NaN/Inf norm (u is then left unchanged) and PA_ERR_STATE when the
+ * arithmetic form changed (pa_set_solver) since the last assembly. */
 int pa_assemble(pa_context *ctx, int increment, int iteration, double *res_norm);
 int pa_linear_solve(pa_context *ctx, int increment, int iteration, int *lin_iters);
 
@@ -174,6 +179,13 @@ int pa_qp_update(pa_context *ctx, long long nqp, const double *F, const double *
  * of lanes per system), path 2 with the routine the PA_QP_COOP qp kernels run in place (a group of
  * lanes per system, rows in LDS); all three give bit-identical x.  No context needed. */
 int pa_pinv_solve(int path, int n, long long count, const double *A, const double *b, double *x);
+
+/* Residuals of the last linear solve (no counterpart in the reference, whose PETSc solve reports
+ * only its own stopping norm): with B the condensed residual and X the correction it solved for,
+ * out[0] = ||B - K X|| / ||B|| (true residual, K the matrix in memory: FP32 values in the mixed
+ * form), out[1] = ||D^-1 (B - K X)|| / ||B|| (recomputed stopping quantity), out[2] = the solver's own
+ * recurrence value of that quantity / ||B||.  Valid until the next assembly. */
+int pa_last_residual(pa_context *ctx, double out[3]);
 
 /* Timing of the last pa_newton_iteration / pa_run_increment phases on the context stream (ms):
  * [0] assembly (qp + tangent + K_e + scatter), [1] Krylov, [2] post-increment pass. */

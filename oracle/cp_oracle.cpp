@@ -1214,6 +1214,7 @@ long long or_sparsity(const or_problem *pb, long long *row_ptr, int *col_idx) {
 }
 
 static double *g_dump_K = nullptr, *g_dump_R = nullptr, *g_dump_du = nullptr;
+static int g_dump_inc = 0, g_dump_it = 0;  // the Newton system that is dumped
 
 int or_krylov(int method, long long n, const long long *rp, const int *ci, const double *val, const double *b,
               double *x, double tol, int maxit) {
@@ -1227,9 +1228,24 @@ int or_krylov(int method, long long n, const long long *rp, const int *ci, const
   return it;
 }
 
+int or_newton_system(const or_material *mat, const or_problem *pb, const unsigned *elem_grain, const double *grain_rod,
+                     int n_grain_rows, int inc, int it, double *K_vals, double *R, double *du) {
+  if (inc < 0 || it < 0 || it >= pb->max_newton) return 1;
+  g_dump_K = K_vals; g_dump_R = R; g_dump_du = du;
+  g_dump_inc = inc; g_dump_it = it;
+  or_problem p1 = *pb;
+  p1.n_increments = inc + 1;
+  std::vector<double> rows(15 * (size_t)(inc + 1));
+  const int rc = or_simulate(mat, &p1, elem_grain, grain_rod, n_grain_rows, rows.data(), nullptr, nullptr, nullptr, nullptr);
+  g_dump_K = g_dump_R = g_dump_du = nullptr;
+  g_dump_inc = g_dump_it = 0;
+  return rc;
+}
+
 int or_first_newton(const or_material *mat, const or_problem *pb, const unsigned *elem_grain, const double *grain_rod,
                     int n_grain_rows, double *K_vals, double *R, double *du) {
   g_dump_K = K_vals; g_dump_R = R; g_dump_du = du;
+  g_dump_inc = g_dump_it = 0;
   or_problem p1 = *pb;
   p1.n_increments = 1;
   p1.max_newton = 1;
@@ -1348,14 +1364,15 @@ int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *el
           }
         }
       }
-      if (g_dump_K) std::copy(A.val.begin(), A.val.end(), g_dump_K);
-      if (g_dump_R) std::copy(R.begin(), R.end(), g_dump_R);
+      const bool dump = inc == g_dump_inc && it == g_dump_it;
+      if (g_dump_K && dump) std::copy(A.val.begin(), A.val.end(), g_dump_K);
+      if (g_dump_R && dump) std::copy(R.begin(), R.end(), g_dump_R);
       const double rn = std::sqrt(dot(R, R));
       if (res_norms) res_norms[inc * pb->max_newton + it] = rn;
       const int its = (pb->solver == 1) ? pcg(A, R, du, pb->lin_tol * rn, pb->lin_maxit) : bicg(A, R, du, pb->lin_tol * rn, pb->lin_maxit);
       if (lin_iters) lin_iters[inc * pb->max_newton + it] = its;
       for (long long r = 0; r < m.ndof; ++r) if (con[r]) du[r] = (it == 0) ? delu[r] : 0.0;  // constraints.distribute
-      if (g_dump_du) std::copy(du.begin(), du.end(), g_dump_du);
+      if (g_dump_du && dump) std::copy(du.begin(), du.end(), g_dump_du);
       for (long long r = 0; r < m.ndof; ++r) x[r] += du[r];
     }
     // updateAfterIncrement (CP updateAfterIncrement.cc:6-305, 710-771)

@@ -120,6 +120,12 @@ int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *el
 int or_first_newton(const or_material *mat, const or_problem *pb, const unsigned *elem_grain,
                     const double *grain_rod, int n_grain_rows, double *K_vals, double *R, double *du);
 
+/* Increment inc, Newton iteration it (after inc full increments and `it` Newton iterations of
+ * increment inc): the condensed CSR values, residual and Newton correction, as or_first_newton. */
+int or_newton_system(const or_material *mat, const or_problem *pb, const unsigned *elem_grain,
+                     const double *grain_rod, int n_grain_rows, int inc, int it, double *K_vals, double *R,
+                     double *du);
+
 /* Jacobi-BiCG (method 0) / Jacobi-PCG (1) on a CSR system with absolute tolerance tol on the
  * preconditioned residual norm; returns the iteration count. */
 int or_krylov(int method, long long n, const long long *rp, const int *ci, const double *val, const double *b,

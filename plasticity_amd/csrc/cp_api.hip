@@ -1012,6 +1012,9 @@ __global__ void __launch_bounds__(64) k_qp_batch_warm(long long n, const double 
 // writes) adds them in every block in the same fixed order, so every block holds the same
 // bitwise total.  rzh[i & 1] holds r.z of iteration i (PETSc's beta), written by block 0 of the
 // kernel that first forms it and read one kernel later.
+// breakdown: 0 none, 1 r.z = 0 before convergence, KRYLOV_NONFINITE a norm or r.z that is NaN or
+// infinite (the solve stops at once, as PETSc's KSP_DIVERGED_NANORINF)
+constexpr int KRYLOV_NONFINITE = 2;
 struct KState {
   double beta, dp, tol, bnorm;
   double rzh[2];
@@ -1099,13 +1102,14 @@ __device__ __forceinline__ bool krylov_next(KState *st, const double *upart, uns
   sum_partials<2>(upart, n_upart, v, lds);
   const double dp = sqrt(v[0]);
   const bool breakdown = (v[1] == 0.0);
-  const bool done = (dp <= st->tol) || it >= st->maxit || breakdown;
+  const bool nonfinite = !isfinite(dp) || !isfinite(v[1]);
+  const bool done = (dp <= st->tol) || it >= st->maxit || breakdown || nonfinite;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->dp = dp;
     if (done) {
       st->done = 1;
       st->it = it;
-      st->breakdown = (breakdown && !(dp <= st->tol)) ? 1 : 0;
+      st->breakdown = nonfinite ? KRYLOV_NONFINITE : ((breakdown && !(dp <= st->tol)) ? 1 : 0);
     } else {
       st->rzh[it & 1] = v[1];
     }
@@ -1267,6 +1271,7 @@ __global__ void __launch_bounds__(VB) k_krylov_init(long long N, const double *_
     st->breakdown = 0;
     st->done = (st->dp <= st->tol) ? 1 : 0;
     if (!st->done && st->beta == 0.0) { st->done = 1; st->breakdown = 1; }
+    if (!isfinite(acc[0]) || !isfinite(acc[1]) || !isfinite(acc[2])) { st->done = 1; st->breakdown = KRYLOV_NONFINITE; }
   }
 }
 
@@ -1329,6 +1334,37 @@ __global__ void __launch_bounds__(VB) k_cg_spmv(MeshDims m, const VT *__restrict
     acc[0] = y[0] * pc[0] + y[1] * pc[1] + y[2] * pc[2];
   }
   block_partials<1>(acc, opart, lds);
+}
+
+// r = B - K X for the last solve (B: the condensed residual, X: its solution), with block partials
+// of r.r, (D^-1 r).(D^-1 r) and B.B (for pa_last_residual)
+template <typename VT>
+__global__ void __launch_bounds__(VB) k_true_residual(MeshDims m, const VT *__restrict__ K, const double *__restrict__ X,
+                                                      const double *__restrict__ B, const double *__restrict__ dinv,
+                                                      double *part) {
+  __shared__ double lds[3 * 16];
+  const long long n = blockIdx.x * (long long)VB + threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (n < m.nnode) {
+    int i, j, k;
+    node_ijk32(m, n, i, j, k);
+    double y[3] = {0, 0, 0};
+    const long long nn = m.nnode;
+    for (int s = 0; s < 27; ++s) {
+      bool ok;
+      const long long mn = nb_masked(m, n, i, j, k, s, ok);
+      const double w = ok ? 1.0 : 0.0;
+      block_mv(K + ((long long)s * nn + n) * 9, w * X[3 * mn], w * X[3 * mn + 1], w * X[3 * mn + 2], y);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double b = B[3 * n + c], r = b - y[c], z = dinv[3 * n + c] * r;
+      acc[0] += r * r;
+      acc[1] += z * z;
+      acc[2] += b * b;
+    }
+  }
+  block_partials<3>(acc, part, lds);
 }
 
 // a = (r.z)_it / (p.q); x += a p; r -= a q; z = D^-1 r; block partials of z.z and r.z.
@@ -1421,6 +1457,10 @@ struct pa_context {
   long long nq = 0, ndof = 0, nnz = 0;
   int method = PA_SOLVER_BICG, precision = PA_PREC_FP64;
   int qp_form = PA_QP_LANE;  // layout of the Newton-step qp passes (pa_set_qp_form)
+  int asm_precision = -1;    // arithmetic form of the last assembly (-1: none yet)
+  bool solved = false;       // a solve has run on the current assembly (pa_last_residual)
+  double *resid_part = nullptr;  // block partials of k_true_residual (allocated on first use)
+  KState last_krylov = {};   // solver state at the end of the last Krylov solve
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t kev[2] = {nullptr, nullptr};  // Krylov flag copies, one per chunk parity
@@ -1676,6 +1716,8 @@ int set_dirichlet(pa_context *c, int inc) {
 }
 
 int do_assemble(pa_context *c, int inc, int it, double *res_norm) {
+  c->asm_precision = -1;
+  c->solved = false;
   int rc = set_dirichlet(c, inc);
   if (rc) return rc;
   HIPCHK(hipMemsetAsync(c->K, 0, kbytes(c), c->stream));
@@ -1693,6 +1735,9 @@ int do_assemble(pa_context *c, int inc, int it, double *res_norm) {
     HIPCHK(hipMemcpyAsync(res_norm, c->d_out + 16, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   }
+  c->asm_precision = c->precision;
+  if (res_norm && !std::isfinite(*res_norm))
+    return fail(PA_ERR_NONFINITE, "assembled residual norm is " + std::to_string(*res_norm));
   return 0;
 }
 
@@ -1836,8 +1881,14 @@ int mg_vcycle(pa_context *c) {
       if (g) (void)hipGraphDestroy(g);
       return rc ? rc : fail(PA_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
     }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      return fail(PA_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    }
     c->mg_graph[f] = g;
-    HIPCHK(hipGraphInstantiate(&c->mg_exec[f], g, nullptr, nullptr, 0));
+    c->mg_exec[f] = ex;
   }
   HIPCHK(hipGraphLaunch(c->mg_exec[f], c->stream));
   return 0;
@@ -1891,6 +1942,8 @@ int launch_krylov_iteration(pa_context *c, int k) {
 }
 
 int do_solve(pa_context *c, int it, int *iters) {
+  if (c->asm_precision != c->precision)
+    return fail(PA_ERR_STATE, "no assembly in the current arithmetic form: call pa_assemble after pa_set_solver");
   const long long N = c->ndof;
   double *X = c->kv[0], *Rr = c->kv[1], *Rl = c->kv[2], *Zr = c->kv[3], *Zl = c->kv[4];
   const bool cg = c->method != PA_SOLVER_BICG;
@@ -1927,6 +1980,11 @@ int do_solve(pa_context *c, int it, int *iters) {
     if (hs.done) break;
   }
   if (iters) *iters = hs.it;
+  c->last_krylov = hs;
+  c->solved = true;
+  if (hs.breakdown == KRYLOV_NONFINITE)  // as PETSc's KSP_DIVERGED_NANORINF: no correction is applied
+    return fail(PA_ERR_NONFINITE, "Krylov solve stopped on a NaN/Inf residual norm at iteration " + std::to_string(hs.it) +
+                                      " (right-hand side norm " + std::to_string(hs.bnorm) + ")");
   // the CG iterate x is complete; BiCG likewise (x lives in kv[0] for both)
   hipLaunchKernelGGL(k_apply_du, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, c->u, (const double *)X, (const unsigned char *)c->con,
                      (const double *)c->delu, (int)(it == 0));
@@ -2259,6 +2317,8 @@ int pa_commit_increment(pa_context *c, double row[15]) {
     row[14] = h[12] / vol;
   }
   c->bc_inc = -1;
+  for (int k = 0; k < 13; ++k)
+    if (!std::isfinite(h[k])) return fail(PA_ERR_NONFINITE, "post-increment averages are not finite (NaN/Inf stress or strain)");
   return 0;
 }
 
@@ -2279,6 +2339,34 @@ int pa_run_increment(pa_context *c, int inc, double row[15], int *lin_iters, dou
   c->last_ms[0] = t_asm;
   c->last_ms[1] = t_kry;
   return rc;
+}
+
+int pa_last_residual(pa_context *c, double out[3]) {
+  if (!c || !out) return fail(PA_ERR_ARG, "null argument");
+  if (!c->solved || c->asm_precision != c->precision) return fail(PA_ERR_STATE, "no linear solve on the current assembly");
+  const unsigned g = nblk(c->m.nnode, VB);
+  if (!c->resid_part) {
+    int rc = dalloc(c, &c->resid_part, 3 * (size_t)g);
+    if (rc) return rc;
+  }
+  if (mixed(c))
+    hipLaunchKernelGGL(k_true_residual<float>, dim3(g), dim3(VB), 0, c->stream, c->m, (const float *)c->K, (const double *)c->kv[0],
+                       (const double *)c->R, (const double *)c->dinv, c->resid_part);
+  else
+    hipLaunchKernelGGL(k_true_residual<double>, dim3(g), dim3(VB), 0, c->stream, c->m, (const double *)c->K, (const double *)c->kv[0],
+                       (const double *)c->R, (const double *)c->dinv, c->resid_part);
+  HIPCHK(hipGetLastError());
+  std::vector<double> h(3 * (size_t)g);
+  HIPCHK(hipMemcpyAsync(h.data(), c->resid_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double t[3] = {0, 0, 0};
+  for (unsigned b = 0; b < g; ++b)
+    for (int k = 0; k < 3; ++k) t[k] += h[3 * (size_t)b + k];
+  const double bn = std::sqrt(t[2]);
+  out[0] = bn > 0 ? std::sqrt(t[0]) / bn : 0.0;
+  out[1] = bn > 0 ? std::sqrt(t[1]) / bn : 0.0;
+  out[2] = c->last_krylov.dp / (bn > 0 ? bn : 1.0);
+  return 0;
 }
 
 int pa_last_timings(pa_context *c, double ms[3]) {

@@ -513,13 +513,14 @@ __global__ void __launch_bounds__(VB) k_mgcg_p(long long N, const double *__rest
     sum_partials<1>(upart, n_upart, dd, lds);
     const double dp = sqrt(dd[0]);
     const bool breakdown = (rz[0] == 0.0);
-    const bool done = (dp <= st->tol) || it >= st->maxit || breakdown;
+    const bool nonfinite = !isfinite(dp) || !isfinite(rz[0]);
+    const bool done = (dp <= st->tol) || it >= st->maxit || breakdown || nonfinite;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->dp = dp;
       if (done) {
         st->done = 1;
         st->it = it;
-        st->breakdown = (breakdown && !(dp <= st->tol)) ? 1 : 0;
+        st->breakdown = nonfinite ? KRYLOV_NONFINITE : ((breakdown && !(dp <= st->tol)) ? 1 : 0);
       } else {
         st->rzh[it & 1] = rz[0];
       }

@@ -228,6 +228,12 @@ class Context:
     def set_solver(self, method=native.PA_SOLVER_BICG, precision=native.PA_PREC_FP64):
         check(native.lib().pa_set_solver(self._h, ctypes.c_int(method), ctypes.c_int(precision)))
 
+    def last_residual(self):
+        """(true ||B - K X|| / ||B||, ||D^-1(B - K X)|| / ||B||, the solver's own value) of the last solve."""
+        out = np.zeros(3)
+        check(native.lib().pa_last_residual(self._h, ptr(out)))
+        return out
+
     def set_qp_form(self, form):
         """Layout of the Newton step's qp passes: native.PA_QP_LANE (default) or native.PA_QP_COOP."""
         check(native.lib().pa_set_qp_form(self._h, ctypes.c_int(form)))

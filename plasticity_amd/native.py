@@ -16,10 +16,11 @@ MAX_TABLE = 32
 PA_SOLVER_BICG, PA_SOLVER_CG, PA_SOLVER_MGCG = 0, 1, 2
 PA_PREC_FP64, PA_PREC_MIXED = 0, 1
 PA_QP_LANE, PA_QP_COOP = 0, 1
+PA_ERR_ARG, PA_ERR_UNSUPPORTED, PA_ERR_HIP, PA_ERR_IO, PA_ERR_STATE, PA_ERR_NONFINITE = 1, 2, 3, 4, 5, 6
 
 # every symbol include/plasticity_amd.h declares
 EXPORTS = [
-    "pa_last_error", "pa_create", "pa_destroy", "pa_element_grains", "pa_set_grains", "pa_set_solver", "pa_set_qp_form",
+    "pa_last_error", "pa_create", "pa_destroy", "pa_element_grains", "pa_set_grains", "pa_set_solver", "pa_set_qp_form", "pa_last_residual",
     "pa_newton_iteration", "pa_assemble", "pa_linear_solve", "pa_commit_increment", "pa_run_increment",
     "pa_get_state", "pa_get_csr", "pa_sizes", "pa_qp_update", "pa_last_timings", "pa_time_kernel",
     "pa_last_counts", "pa_pinv_solve", "pa_mlp_create", "pa_mlp_destroy", "pa_mlp_set_params", "pa_mlp_get_params",
