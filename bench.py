@@ -25,44 +25,85 @@ sys.path.insert(0, ROOT)
 
 METRIC = "qp updates/s + Newton-step time, 64^3 FCC: mixed-precision vs plain FP64 GPU"
 SOLVER_NAMES = {"bicg": "Jacobi-BiCG", "cg": "Jacobi-PCG", "mgcg": "multigrid-PCG"}
-FP64_PEAK_TFLOPS = 78.6
-# FP64 operations issued by one launch of the first qp pass (k_qp_tangent, every qp on the LU path)
-# at the state bench.py times it on, from rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64
-# (wave instructions x 64 lanes, FMA = 2), mean over the timed launches; filled in from the
-# committed profile.  Masked-off lanes are counted, so this is an upper bound of the useful work.
-PMC_QP_FLOPS = {("mixed", 64): (57438700 + 108138000 + 2 * 375480000 + 4213720) * 64}
-# HBM bytes (KiB) per launch of the same pass: FETCH_SIZE x 2 + WRITE_SIZE, as for the Krylov operator
-PMC_QP_TRAFFIC_KIB = {("mixed", 64): 23369400 * 2 + 18264900}
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (and matrix) peak
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the Jacobi-PCG operator kernel k_cg_spmv at the 64^3 workload, from two
-# rocprofv3 PMC passes (FETCH_SIZE x 2 + WRITE_SIZE x 1 KiB: the factors that make both counters
-# read the known byte count of the history-commit copies); filled in from the committed profile
+# Algorithmic FP64 flops of the first qp pass per qp and Newton step, counted by the flop-counting
+# build of the oracle on a 24^3 sample of this workload (tests/golden/make_flop_fixture.py): the
+# flops of each qp up to its first numerically singular local system (those go to the
+# pseudo-inverse pass that follows), all of them for the others.
+FLOP_FIXTURE = os.path.join(ROOT, "tests", "golden", "qp_flops_fcc24.json")
+# FP64 operations issued by one launch of the same pass at 64^3 (k_qp_tangent<12,float>), from
+# rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 (wave instructions x 64 lanes, FMA = 2; masked
+# lanes count) and its HBM bytes (FETCH_SIZE x 2 + WRITE_SIZE KiB, the factors that make both
+# counters read the known byte count of a plain copy): profiles/k_qp_tangent_mixed_64cube_fcc.md
+PMC_QP_FLOPS = {("mixed", 64): (57438700 + 108138000 + 2 * 375480000 + 4213720) * 64}
+PMC_QP_TRAFFIC_KIB = {("mixed", 64): 23369400 * 2 + 18264900}
+# HBM bytes per launch of the Jacobi-PCG operator kernel k_cg_spmv at 64^3, measured the same way
+# (profiles/k_cg_spmv_mixed_64cube_fcc.md)
 PMC_TRAFFIC_KIB = {("cg", "mixed", 64): 142996 * 2 + 6373}
 
 
+def first_pass_flops_per_qp(steps):
+    """Mean algorithmic flops per qp of the first qp pass over the given Newton steps (indices from
+    the start of the load path), from the flop fixture; None when it is absent."""
+    if not os.path.exists(FLOP_FIXTURE):
+        return None, None
+    with open(FLOP_FIXTURE) as f:
+        fx = json.load(f)
+    rows = fx["assembly_pass"]
+    per = [rows[s]["flops_to_first_singular"] / rows[s]["calls"] for s in steps if s < len(rows)]
+    return (float(np.mean(per)) if per else None), fx["workload"]
+
+
+def oracle_native_build():
+    """The oracle compiled for this host (g++ -O3 -march=native -fopenmp) into a temporary
+    directory, or the prebuilt portable one when that fails."""
+    import shutil
+    import subprocess
+    import tempfile
+    src = os.path.join(ROOT, "oracle", "cp_oracle.cpp")
+    cxx = shutil.which("g++")
+    if cxx:
+        out = os.path.join(tempfile.mkdtemp(prefix="oracle_native_"), "liboracle_native.so")
+        flags = ["-O3", "-march=native", "-fopenmp", "-fPIC", "-std=c++17", "-shared"]
+        r = subprocess.run([cxx] + flags + ["-o", out, src], capture_output=True, text=True, timeout=600)
+        if r.returncode == 0:
+            return out, "g++ " + " ".join(flags[:3])
+    return os.path.join(ROOT, "oracle", "liboracle.so"), "prebuilt -O2 -march=x86-64-v2 (native build failed)"
+
+
 def cpu_baseline(args, quick):
-    """Oracle (CPU FP64 restatement, test infrastructure) on a bounded sample of the same workload:
-    a 24^3 mesh with the same material and grain statistics, 2 increments x 4 Newton steps
-    (about 15-20 s on 16 host cores)."""
+    """Oracle (CPU FP64 restatement, test infrastructure) built for the host, on a bounded sample of
+    the same workload: a 32^3 mesh with the same material and 32 grains (1,024 elements per grain,
+    as at 64^3 with 256), Jacobi-PCG (the solver the FP64 workload names), 3 increments x 4 Newton
+    steps (about 10-15 s on 16 cores), with the assembly / solve / post-increment split."""
+    import ctypes as ct
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import refcase
     from plasticity_amd import synth
-    n = 8 if quick else 24
-    mat, pb, vox, rod, bcs, t = synth.fcc_workload(n, 16, 20, 5e-4, seed=2024)
+    path, how = oracle_native_build()
+    refcase.use_oracle_build(path)
+    n, grains, incs = (8, 4, 2) if quick else (32, 32, 3)
+    mat, pb, vox, rod, bcs, t = synth.fcc_workload(n, grains, 20, 5e-4, seed=2024)
     case = refcase.Case(t["m"] / np.linalg.norm(t["m"], axis=1)[:, None], t["n"] / np.linalg.norm(t["n"], axis=1)[:, None],
                         t["q"], t["C"], t["s0"], t["h0"], t["ss"], t["pw"], tol=t["tol"], nx=n, ny=n, nz=n, delT=1.0,
-                        total_time=20.0, bcs=bcs)
+                        total_time=20.0, bcs=bcs, solver=1)
     case.elem_grain = vox.transpose(2, 1, 0).ravel().astype(np.uint32)  # element e = ex + n(ey + n ez)
     case.grain_rod = rod
-    cores = os.cpu_count() or 1
+    cores = min(os.cpu_count() or 1, 16)
     t0 = time.time()
-    out = refcase.oracle_simulate(case, 2, nthreads=cores)
+    out = refcase.oracle_simulate(case, incs, nthreads=cores)
     dt = time.time() - t0
-    steps = 2 * case.max_newton
+    ns = (ct.c_longlong * 3)()
+    refcase.oracle().or_last_timings(ns)
+    steps = incs * case.max_newton
     nq = 8 * n ** 3
     return {"value": nq * steps / dt, "unit": "qp updates/s", "cores": cores, "kind": "port",
-            "sample": "oracle (CPU FP64 restatement) on %d^3 elements, 2 increments x 4 Newton steps (%d qp updates with tangent), %.1f s"
-                      % (n, nq * steps, dt)}
+            "sample": "oracle (CPU FP64 restatement, %s) on %d^3 elements (%d qps, %d grains), Jacobi-PCG, %d increments "
+                      "x 4 Newton steps: %.1f s" % (how, n, nq, grains, incs, dt),
+            "assembly_s": ns[0] * 1e-9, "solve_s": ns[1] * 1e-9, "post_s": ns[2] * 1e-9,
+            "krylov_iters_per_step": float(np.mean(out["lin_iters"])),
+            "assembly_qp_updates_per_s": nq * steps / (ns[0] * 1e-9)}
 
 
 def main():
@@ -76,6 +117,8 @@ def main():
     ap.add_argument("--solver", choices=["bicg", "cg", "mgcg", "all"], default="all")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-warm", action="store_true", help="skip the learned warm-start measurement")
+    ap.add_argument("--full-path", type=int, default=1,
+                    help="after the timed steps, run the multigrid-PCG forms to the end of the load path (untimed for value)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the library has no multi-GPU path",
@@ -122,7 +165,7 @@ def main():
     }
     keys = ("ms_per_step", "qp_updates_per_s", "krylov_iters_per_step", "assembly_ms_per_step", "krylov_ms_per_step",
             "post_ms_per_increment", "qp_tangent_ms", "element_ms", "operator_us", "update_us", "update_GBps", "p_us",
-            "singular_local_solves_per_step", "rerun_lane_solves", "qp_main_pass_ms")
+            "singular_local_solves_per_step", "rerun_lane_solves", "qp_main_pass_ms", "true_rel_residual", "full_load_path")
     out["config"]["roofline_krylov_operator"] = head["roofline_krylov"]
     for (form, solver), r in results.items():
         out["config"]["%s_%s" % (form, solver)] = {k: r[k] for k in keys}
@@ -204,13 +247,17 @@ def run_form(args, form, solver, pa, native, synth, torch):
     state = {"step": 0}
     log = []
 
-    def newton_step():
+    def newton_step(check=True):
+        """One Newton iteration; check: also the true relative residual ||b - K du|| / ||b|| of its
+        Krylov solve (an extra operator product, so never inside the timed region)."""
         s = state["step"]
         inc, it = divmod(s, nmax)
         rn, its = ctx.newton_iteration(inc, it)
         asm_ms, kry_ms, _ = ctx.last_timings()
         rec = {"inc": inc, "it": it, "res": rn, "lin_iters": its, "asm_ms": asm_ms, "krylov_ms": kry_ms,
-               "deferred": int(ctx.last_counts()[0])}
+               "deferred": int(ctx.last_counts()[0]), "qp_pass_ms": ctx.last_qp_pass_ms()}
+        if check:
+            rec["true_rel_res"] = float(ctx.last_residual()[0])
         if it == nmax - 1:
             rec["row"] = ctx.commit_increment().tolist()
             rec["post_ms"] = ctx.last_timings()[2]
@@ -225,7 +272,7 @@ def run_form(args, form, solver, pa, native, synth, torch):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        newton_step()
+        newton_step(check=False)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     timed = log[args.warmup:]
@@ -257,12 +304,32 @@ def run_form(args, form, solver, pa, native, synth, torch):
     # the first qp pass alone (the dominant kernel): return map + dP/dF of every qp
     native.check(native.lib().pa_time_kernel(ctx._h, 7, 3, ctypes.byref(avg)))
     qp_main_ms = avg.value
+    full = None
+    if args.full_path and solver == "mgcg":
+        # the rest of the load path (all increments), untimed for `value`: phase times from the
+        # library's HIP events, plus the true residual of every solve
+        while state["step"] < pb.total_increments * nmax:
+            newton_step()
+        steps_all = [r for r in log]
+        full = {"steps": len(steps_all),
+                "ms_per_step_events": float(np.mean([r["asm_ms"] + r["krylov_ms"] for r in steps_all]) +
+                                      np.sum([r.get("post_ms", 0.0) for r in steps_all]) / len(steps_all)),
+                "assembly_ms_per_step": float(np.mean([r["asm_ms"] for r in steps_all])),
+                "krylov_ms_per_step": float(np.mean([r["krylov_ms"] for r in steps_all])),
+                "krylov_iters_per_step": float(np.mean([r["lin_iters"] for r in steps_all])),
+                "first_qp_pass_ms": float(np.mean([r["qp_pass_ms"] for r in steps_all]))}
+    checked = [r["true_rel_res"] for r in log if "true_rel_res" in r]
     posts = [r["post_ms"] for r in timed if "post_ms" in r]
     kib = PMC_TRAFFIC_KIB.get(("cg" if solver == "mgcg" else solver, form, n)) if args.grains == 256 else None
     traffic = kib * 1024 if kib else None
-    qp_flops = PMC_QP_FLOPS.get((form, n)) if args.grains == 256 else None
+    issued = PMC_QP_FLOPS.get((form, n)) if args.grains == 256 else None
     qp_kib = PMC_QP_TRAFFIC_KIB.get((form, n)) if args.grains == 256 else None
-    qp_tflops = qp_flops / (qp_main_ms * 1e-3) / 1e12 if qp_flops else None
+    # the first qp pass over the timed steps: HIP-event durations of each launch, and the flop
+    # fixture's algorithmic flops per qp for the same Newton steps of the load path
+    live_ms = float(np.mean([r["qp_pass_ms"] for r in timed]))
+    fpq, fx_workload = first_pass_flops_per_qp(range(args.warmup, args.warmup + args.steps))
+    qp_flops = fpq * ctx.nqp if fpq else None
+    qp_tflops = qp_flops / (live_ms * 1e-3) / 1e12 if qp_flops else None
     res = {
         "nq": ctx.nqp, "ndof": ctx.ndof, "nnz": ctx.nnz,
         "ms_per_step": ms_per_step, "qp_updates_per_s": ctx.nqp / (ms_per_step * 1e-3),
@@ -277,13 +344,20 @@ def run_form(args, form, solver, pa, native, synth, torch):
         "rows": [r["row"] for r in log if "row" in r],
         "roofline": {"kernel": "k_qp_tangent<%d, %s> (first qp pass: return map + dP/dF of all %d qps, singular ones queued)"
                                % (12, "float" if form == "mixed" else "double", ctx.nqp),
-                     "bound": "mfma", "achieved": qp_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "bound": "fp64-valu", "achieved": qp_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": qp_tflops / FP64_PEAK_TFLOPS if qp_tflops else None,
-                     "traffic": qp_kib * 1024 if qp_kib else None, "avg_launch_ms": qp_main_ms, "flops_per_launch": qp_flops,
-                     "flops_source": "rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 x 64 lanes, FMA = 2 (upper bound: "
-                                     "masked lanes count) and FETCH_SIZE x2 + WRITE_SIZE (profiles/k_qp_tangent_mixed_64cube_fcc.md); "
-                                     "FP64 VALU, whose MI355X peak equals the FP64 matrix figure"
-                                     if qp_flops else "not measured for this form/size"},
+                     "traffic": qp_kib * 1024 if qp_kib else None, "avg_launch_ms": live_ms,
+                     "avg_launch_source": "HIP events around each first-pass launch on the context stream, mean over the "
+                                          "%d timed steps" % args.steps,
+                     "flops_per_launch": qp_flops, "flops_per_qp": fpq,
+                     "flops_source": ("algorithmic FP64 flops per qp of the same Newton steps, counted by the flop-counting "
+                                      "oracle build on the %s (tests/golden/qp_flops_fcc24.json), x %d qps" % (fx_workload, ctx.nqp))
+                                     if fpq else "flop fixture absent",
+                     "issued_flops_per_launch": issued,
+                     "issued_source": "rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 x 64 lanes, FMA = 2, masked lanes "
+                                      "count (profiles/k_qp_tangent_mixed_64cube_fcc.md)" if issued else None,
+                     "traffic_source": "rocprofv3 FETCH_SIZE x2 + WRITE_SIZE (profiles/k_qp_tangent_mixed_64cube_fcc.md)"
+                                       if qp_kib else None},
         "roofline_krylov": {"kernel": "%s<%s> (27-point node-block %s)" % ("k_bicg_spmv" if solver == "bicg" else "k_cg_spmv",
                                                                       "float" if form == "mixed" else "double",
                                                                       "K p and K^T p~" if solver == "bicg" else "K p"),
@@ -293,6 +367,12 @@ def run_form(args, form, solver, pa, native, synth, torch):
                      "traffic_source": "rocprofv3 FETCH_SIZE x2 + WRITE_SIZE (profiles/k_cg_spmv_mixed_64cube_fcc.md)"
                                        if traffic else "not measured for this kernel/size"},
     }
+    res["true_rel_residual"] = {"max": float(np.max(checked)) if checked else None,
+                                "median": float(np.median(checked)) if checked else None,
+                                "solves_checked": len(checked),
+                                "note": "||b - K du|| / ||b|| after each Krylov solve outside the timed steps (the stopping "
+                                        "test is ||D^-1 r|| <= 1e-10 ||b||, on the Jacobi-preconditioned residual)"}
+    res["full_load_path"] = full
     res["rerun_lane_solves"] = int(ctx.last_counts()[2])
     res["T"] = ctx.get_state("T")
     res["s"] = ctx.get_state("s")

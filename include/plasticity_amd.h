@@ -191,6 +191,12 @@ int pa_last_residual(pa_context *ctx, double out[3]);
  * [0] assembly (qp + tangent + K_e + scatter), [1] Krylov, [2] post-increment pass. */
 int pa_last_timings(pa_context *ctx, double ms[3]);
 
+/* Instrumentation (no counterpart in the reference): duration (ms, HIP events on the context
+ * stream) of the first qp pass of the latest assembly, the kernel that runs the return map and
+ * the consistent tangent at every quadrature point (numerically singular local systems are handed
+ * to the pseudo-inverse pass that follows it).  PA_ERR_STATE before the first assembly. */
+int pa_last_qp_pass_ms(pa_context *ctx, double *ms);
+
 /* Instrumentation (no counterpart in the reference): quadrature points whose local active-set
  * matrix was numerically singular and went through the pseudo-inverse pass, in the last assembly
  * (counts[0]) and the last post-increment pass (counts[1]); counts[2]: reruns, since the context

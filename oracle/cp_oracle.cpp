@@ -12,9 +12,18 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <ctime>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
+#endif
+
+// Hooks of the flop-counting build of this file (cp_oracle_flops.cpp); empty in the oracle itself.
+#ifndef OR_FLOP_QP_BEGIN
+#define OR_FLOP_QP_BEGIN(flag)
+#define OR_FLOP_QP_END(flag)
+#define OR_FLOP_STEP(inc, it, max_newton)
+#define OR_FLOP_SINGULAR()
 #endif
 
 namespace {
@@ -30,6 +39,15 @@ namespace {
 int g_pert_block = 0;
 double g_pert_rel = 0.0;
 inline double pert(int block) { return g_pert_block == block ? 1.0 + g_pert_rel : 1.0; }
+
+// wall-clock split of the latest or_simulate (ns): assembly (qp passes, K_e, scatter), Krylov
+// solves, post-increment passes; read by or_last_timings
+long long g_phase_ns[3];
+long long now_ns() {
+  timespec t;
+  clock_gettime(CLOCK_MONOTONIC, &t);
+  return (long long)t.tv_sec * 1000000000LL + t.tv_nsec;
+}
 
 // ---------------------------------------------------------------- 3x3 algebra
 struct M3 {
@@ -325,6 +343,7 @@ void pinv_solve(int n, const double *A, const double *b, double *x, int mode) {
     std::vector<double> Av(A, A + n * n);
     for (int i = 0; i < n; ++i) x[i] = b[i];
     if (lu_solve(n, Av, x, 1)) return;
+    OR_FLOP_SINGULAR();
     if (g_pinv_singular++ == 0) {
       g_sample_n = n;
       for (int i = 0; i < n * n; ++i) g_sample_A[i] = A[i];
@@ -411,10 +430,10 @@ M3 dexp_series(const M3 &X, const M3 &Sj, int count3, double sign) {
 }
 
 // calculatePlasticity(cellID, q, flag), CP calculatePlasticity.cc:13-959 (twinning branch :962-1019 off)
-int qp_update(const or_material *mat, const Grain &g, const double *Fin, const double *Fp_conv, const double *s_conv,
-              const double *W_conv, const double *slip_conv, int flag, double *Pout, double *Tout, double *dPdF,
-              double *Fe_out, double *Fp_out, double *s_out, double *W_out, double *slip_iter,
-              const double *x0 = nullptr, int *iters = nullptr) {
+int qp_update_body(const or_material *mat, const Grain &g, const double *Fin, const double *Fp_conv, const double *s_conv,
+                   const double *W_conv, const double *slip_conv, int flag, double *Pout, double *Tout, double *dPdF,
+                   double *Fe_out, double *Fp_out, double *s_out, double *W_out, double *slip_iter, const double *x0,
+                   int *iters) {
   const int nT = mat->nT, ns = mat->n_slip, ntw = mat->n_twin;
   const double tol1 = mat->tol;
   if (iters) *iters = 0;
@@ -650,7 +669,7 @@ int qp_update(const or_material *mat, const Grain &g, const double *Fin, const d
       int count3 = 0;
       {
         M3 s1 = zero3(), s2 = eye3();
-        while (tol2 > std::max(tol1 / 1e4, 1e-12)) {
+        while (tol2 > std::max<double>(tol1 / 1e4, 1e-12)) {
           count3 = count3 + 1;
           s1 = mul(del_FP, s2);
           s1 = scale(s1, (1.0 / count3));
@@ -749,6 +768,17 @@ int qp_update(const or_material *mat, const Grain &g, const double *Fin, const d
   store3(FP_tau, Fp_out);
   for (int i = 0; i < nT; ++i) { s_out[i] = s_tau[i]; W_out[i] = W_tau[i]; }
   return n_active;
+}
+
+int qp_update(const or_material *mat, const Grain &g, const double *Fin, const double *Fp_conv, const double *s_conv,
+              const double *W_conv, const double *slip_conv, int flag, double *Pout, double *Tout, double *dPdF,
+              double *Fe_out, double *Fp_out, double *s_out, double *W_out, double *slip_iter,
+              const double *x0 = nullptr, int *iters = nullptr) {
+  OR_FLOP_QP_BEGIN(flag);
+  const int rc = qp_update_body(mat, g, Fin, Fp_conv, s_conv, W_conv, slip_conv, flag, Pout, Tout, dPdF, Fe_out, Fp_out,
+                                s_out, W_out, slip_iter, x0, iters);
+  OR_FLOP_QP_END(flag);
+  return rc;
 }
 
 // ----------------------------------------------------------- finite elements
@@ -1116,6 +1146,7 @@ void or_pinv_stats(long long *calls, long long *singular) {
 }
 
 void or_set_pinv_rtol(double rtol) { kPinvRtol = rtol; }
+void or_last_timings(long long ns[3]) { for (int i = 0; i < 3; ++i) ns[i] = g_phase_ns[i]; }
 
 int or_pinv_records(double *out, int max_records) {
   const int n = std::min((int)(g_records.size() / 4), max_records);
@@ -1264,6 +1295,7 @@ int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *el
   const Shape sh = make_shape(m.h);
   const int nT = mat->nT, ns = mat->n_slip;
   const long long nq = m.nelem * 8;
+  g_phase_ns[0] = g_phase_ns[1] = g_phase_ns[2] = 0;
   for (long long e = 0; e < m.nelem; ++e)
     if ((int)elem_grain[e] >= n_grain_rows) return -1;
   // per-grain constants, identical to what calculatePlasticity rebuilds on every call
@@ -1296,6 +1328,8 @@ int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *el
   for (int inc = 0; inc < pb->n_increments; ++inc) {
     dirichlet(m, pb, inc, con, delu);
     for (int it = 0; it < pb->max_newton; ++it) {
+      OR_FLOP_STEP(inc, it, pb->max_newton);
+      const long long t_asm = now_ns();
       // assemble (BVP assemble.cc:6-104) with getElementalValues (CP getElementalValues.cc:5-168)
 #pragma omp parallel for schedule(dynamic, 64)
       for (long long e = 0; e < m.nelem; ++e) {
@@ -1368,13 +1402,17 @@ int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *el
       if (g_dump_K && dump) std::copy(A.val.begin(), A.val.end(), g_dump_K);
       if (g_dump_R && dump) std::copy(R.begin(), R.end(), g_dump_R);
       const double rn = std::sqrt(dot(R, R));
+      const long long t_solve = now_ns();
+      g_phase_ns[0] += t_solve - t_asm;
       if (res_norms) res_norms[inc * pb->max_newton + it] = rn;
       const int its = (pb->solver == 1) ? pcg(A, R, du, pb->lin_tol * rn, pb->lin_maxit) : bicg(A, R, du, pb->lin_tol * rn, pb->lin_maxit);
       if (lin_iters) lin_iters[inc * pb->max_newton + it] = its;
       for (long long r = 0; r < m.ndof; ++r) if (con[r]) du[r] = (it == 0) ? delu[r] : 0.0;  // constraints.distribute
       if (g_dump_du && dump) std::copy(du.begin(), du.end(), g_dump_du);
       for (long long r = 0; r < m.ndof; ++r) x[r] += du[r];
+      g_phase_ns[1] += now_ns() - t_solve;
     }
+    const long long t_post = now_ns();
     // updateAfterIncrement (CP updateAfterIncrement.cc:6-305, 710-771)
     double sT[3][3] = {{0}}, sE[3][3] = {{0}}, Fs = 0;
     vol_total = 0;
@@ -1408,6 +1446,7 @@ int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *el
     double *row = rows + 15 * inc;
     const int vi[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {0, 2}, {0, 1}};
     for (int c = 0; c < 6; ++c) { row[c] = sE[vi[c][0]][vi[c][1]] / vol_total; row[6 + c] = sT[vi[c][0]][vi[c][1]] / vol_total; }
+    g_phase_ns[2] += now_ns() - t_post;
     row[12] = 0.0;  // TwinRealVF: the inert twin system never activates
     row[13] = 0.0;  // TwinMade
     row[14] = Fs / vol_total;

@@ -145,6 +145,9 @@ void or_pinv_histogram(int enable, long long hist[24]);
 int or_pinv_records(double *out, int max_records);
 /* Overrides the relative pivot / singular-value level of pinv_mode 1 (experiments only). */
 void or_set_pinv_rtol(double rtol);
+/* Wall-clock split of the latest or_simulate (ns): [0] assembly (qp passes, element matrices,
+ * scatter), [1] Krylov solves and updates, [2] post-increment passes. */
+void or_last_timings(long long ns[3]);
 
 /* Single increment helpers exposed for property tests. */
 void or_odfpoint(const double r[3], double R[9]);

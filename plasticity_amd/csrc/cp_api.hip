@@ -1464,6 +1464,8 @@ struct pa_context {
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t kev[2] = {nullptr, nullptr};  // Krylov flag copies, one per chunk parity
+  hipEvent_t qev[2] = {nullptr, nullptr};  // around the first qp pass of the latest assembly
+  bool qev_set = false;
   KState *hks = nullptr;                   // pinned host copies of the solver state (2)
   double last_ms[3] = {0, 0, 0};
   bool grains_set = false;
@@ -1533,7 +1535,10 @@ void sing_svd_round(pa_context *c, const unsigned *count) {
 template <int NS, typename DT, bool KIN>
 void qp_tangent_passes_k(pa_context *c, const QpArgs &a, bool main_only) {
   // one-wave blocks, a grid of 2048 (eight per CU) striding over the qps
+  hipEventRecord(c->qev[0], c->stream);
   hipLaunchKernelGGL((k_qp_tangent<NS, DT, KIN>), dim3(std::min(2048u, nblk(c->nq, 64))), dim3(64), 0, c->stream, a);
+  hipEventRecord(c->qev[1], c->stream);
+  c->qev_set = true;
   if (main_only) return;
   for (int r = 0; r < SING_ROUNDS; ++r) {
     sing_svd_round<NS>(c, a.qcount);
@@ -1559,7 +1564,10 @@ void qp_tangent_coop_k(pa_context *c, const QpArgs &a) {
   static const unsigned grid_full = resident_grid(k_qp_tangent_coop<NS, DT, KIN>, 1LL << 40);
   const unsigned grid = (unsigned)std::min<long long>(grid_full, (c->nq + QP_CHUNK - 1) / QP_CHUNK);
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, c->stream, c->counters + 11);
+  hipEventRecord(c->qev[0], c->stream);
   hipLaunchKernelGGL((k_qp_tangent_coop<NS, DT, KIN>), dim3(grid), dim3(64), 0, c->stream, a);
+  hipEventRecord(c->qev[1], c->stream);
+  c->qev_set = true;
 }
 template <int NS, typename DT>
 void qp_tangent_passes(pa_context *c, const QpArgs &a, bool main_only) {
@@ -2056,6 +2064,7 @@ int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out) {
   if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return fail(PA_ERR_HIP, "hipStreamCreate failed"); }
   for (int i = 0; i < 4; ++i) hipEventCreate(&c->ev[i]);
   for (int i = 0; i < 2; ++i) hipEventCreateWithFlags(&c->kev[i], hipEventDisableTiming);
+  for (int i = 0; i < 2; ++i) hipEventCreate(&c->qev[i]);
   if (hipHostMalloc((void **)&c->hks, 2 * sizeof(KState), hipHostMallocDefault) != hipSuccess) {
     pa_destroy(c);
     return fail(PA_ERR_HIP, "hipHostMalloc failed");
@@ -2153,6 +2162,8 @@ int pa_destroy(pa_context *c) {
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
   for (int i = 0; i < 2; ++i)
     if (c->kev[i]) hipEventDestroy(c->kev[i]);
+  for (int i = 0; i < 2; ++i)
+    if (c->qev[i]) hipEventDestroy(c->qev[i]);
   if (c->hks) hipHostFree(c->hks);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -2372,6 +2383,16 @@ int pa_last_residual(pa_context *c, double out[3]) {
 int pa_last_timings(pa_context *c, double ms[3]) {
   if (!c || !ms) return fail(PA_ERR_ARG, "null argument");
   for (int i = 0; i < 3; ++i) ms[i] = c->last_ms[i];
+  return 0;
+}
+
+int pa_last_qp_pass_ms(pa_context *c, double *ms) {
+  if (!c || !ms) return fail(PA_ERR_ARG, "null argument");
+  if (!c->qev_set) return fail(PA_ERR_STATE, "no assembly has run");
+  HIPCHK(hipEventSynchronize(c->qev[1]));
+  float t = 0;
+  HIPCHK(hipEventElapsedTime(&t, c->qev[0], c->qev[1]));
+  *ms = t;
   return 0;
 }
 

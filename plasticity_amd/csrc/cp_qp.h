@@ -14,7 +14,8 @@
 //   * Max Slip Search Iterations is 1, so the tangent's "previous" terms
 //     delFp_delF_prev and dels_delF_prev are zero (:147-148, :476-477) and
 //     dels_delF stays zero (:579); Max Solver Iterations may be > 1;
-//   * kinematic hardening is off (C_1 = C_2 = 0, backstress W = 0).
+//   * kinematic hardening (C_1, C_2 and the backstress W) is carried by the KIN
+//     instantiation of qp_update; without it the backstress is zero.
 #pragma once
 #include <hip/hip_runtime.h>
 

@@ -298,6 +298,13 @@ class Context:
         check(native.lib().pa_last_timings(self._h, ptr(ms)))
         return ms
 
+    def last_qp_pass_ms(self):
+        """HIP-event duration of the first qp pass (return map + tangent at every qp) of the latest
+        assembly, in ms."""
+        ms = ctypes.c_double()
+        check(native.lib().pa_last_qp_pass_ms(self._h, ctypes.byref(ms)))
+        return ms.value
+
     def last_counts(self):
         """Quadrature points that went through the pseudo-inverse pass in the last assembly and
         the last post-increment pass, and the reruns (since creation) that met a singular system

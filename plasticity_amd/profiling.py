@@ -9,8 +9,9 @@
   python -m plasticity_amd.profiling counters <results.db> <kernel name part> <grid> <n>
       every counter of a --pmc run, summed over its per-unit rows, averaged over the last n
       dispatches of the kernel with that grid size
-  python -m plasticity_amd.profiling dispatches <results.db> <kernel name part> <n>
-      durations (us) of the last n dispatches of the kernel (--kernel-trace) and their mean
+  python -m plasticity_amd.profiling dispatches <results.db> <kernel name part> <n> [<first>]
+      durations (us) of the last n dispatches of the kernel (--kernel-trace) and their mean, or of
+      the n dispatches from the first-th (0-based, in launch order)
 """
 import sqlite3
 import sys
@@ -65,15 +66,15 @@ def counters(db, part, grid, n):
         print("  %s %.6g" % (cname, sum(per[d][cname] for d in last) / len(last)))
 
 
-def dispatches(db, part, n):
+def dispatches(db, part, n, first=None):
     c = sqlite3.connect(db)
     rows = [(name, start, end) for name, start, end in
             c.execute("select name, start, end from kernels order by start") if part in name]
-    last = rows[-n:]
-    durs = [(e - s) / 1e3 for _, s, e in last]
-    print("%d dispatches of `%s`; the last %d: %s us; mean %.2f us" % (len(rows), part, len(last),
-                                                                     ", ".join("%.2f" % d for d in durs),
-                                                                     sum(durs) / max(len(durs), 1)))
+    sel = rows[-n:] if first is None else rows[first:first + n]
+    durs = [(e - s) / 1e3 for _, s, e in sel]
+    which = "the last %d" % len(sel) if first is None else "%d from dispatch %d" % (len(sel), first)
+    print("%d dispatches of `%s`; %s: %s us; mean %.2f us" % (len(rows), part, which, ", ".join("%.2f" % d for d in durs),
+                                                            sum(durs) / max(len(durs), 1)))
 
 
 if __name__ == "__main__":
@@ -81,8 +82,8 @@ if __name__ == "__main__":
         trace(sys.argv[2], sys.argv[3])
     elif len(sys.argv) == 4 and sys.argv[1] == "traffic":
         traffic(sys.argv[2], sys.argv[3])
-    elif len(sys.argv) == 5 and sys.argv[1] == "dispatches":
-        dispatches(sys.argv[2], sys.argv[3], int(sys.argv[4]))
+    elif len(sys.argv) in (5, 6) and sys.argv[1] == "dispatches":
+        dispatches(sys.argv[2], sys.argv[3], int(sys.argv[4]), int(sys.argv[5]) if len(sys.argv) == 6 else None)
     elif len(sys.argv) == 6 and sys.argv[1] == "counters":
         counters(sys.argv[2], sys.argv[3], int(sys.argv[4]), int(sys.argv[5]))
     else:

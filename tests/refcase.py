@@ -278,14 +278,21 @@ def load_reference_case(folder):
 
 # ------------------------------------------------------------------ oracle
 _ORACLE = None
+_ORACLE_PATH = None
+
+
+def use_oracle_build(path):
+    """Load the oracle from another build of oracle/cp_oracle.cpp (same source, other flags)."""
+    global _ORACLE, _ORACLE_PATH
+    _ORACLE, _ORACLE_PATH = None, path
 
 
 def oracle():
     global _ORACLE
     if _ORACLE is None:
-        path = os.path.join(ROOT, "oracle", "liboracle.so")
+        path = _ORACLE_PATH or os.path.join(ROOT, "oracle", "liboracle.so")
         if not os.path.exists(path):
-            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+            raise RuntimeError("%s is not built (run __graft_entry__.build())" % path)
         lib = ctypes.CDLL(path)
         D, I, P = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), ctypes.c_void_p
         lib.or_qp_update.restype = ctypes.c_int
