@@ -180,6 +180,20 @@ int pa_qp_update(pa_context *ctx, long long nqp, const double *F, const double *
  * lanes per system, rows in LDS); all three give bit-identical x.  No context needed. */
 int pa_pinv_solve(int path, int n, long long count, const double *A, const double *b, double *x);
 
+/* pa_pinv_solve without device allocation.  No counterpart in the reference: it is for callers
+ * that cannot call hipMalloc (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation).  Same arguments, kernels and results (bit for bit) as pa_pinv_solve, but
+ * every device buffer the solve needs (the staged A, b, x, the padded store and its per-system
+ * flags and queue length) lives in one file-scope device array of PA_NOALLOC_SCRATCH_BYTES per
+ * GPU, held until the process ends; no hipMalloc/hipFree or other allocating call is made.
+ * Capacity: count systems of size n (leading dimension NS = 12, 24 or 48 for n <= 12, 24, 48)
+ * need 8 (n^2 + 2n + NS^2 + 2 NS) + 12 bytes each (each of ten segments rounded up to 256
+ * bytes): about 12,400 systems at n = 12 and 870 at n = 48.  A larger count returns PA_ERR_ARG with the limit
+ * in pa_last_error.  All calls on one GPU share its array: calls that overlap on one GPU (from
+ * several host threads) race and must be serialised by the caller. */
+#define PA_NOALLOC_SCRATCH_BYTES (32u << 20)
+int pa_pinv_solve_noalloc(int path, int n, long long count, const double *A, const double *b, double *x);
+
 /* Residuals of the last linear solve (no counterpart in the reference, whose PETSc solve reports
  * only its own stopping norm): with B the condensed residual and X the correction it solved for,
  * out[0] = ||B - K X|| / ||B|| (true residual, K the matrix in memory: FP32 values in the mixed

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2834,6 +2835,143 @@ int pa_pinv_solve(int path, int n, long long count, const double *A, const doubl
       for (int i = 0; i < n; ++i) x[e * n + i] = hx[e * NS + i];
   for (void *p : tmp) hipFree(p);
   return rc;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ pa_pinv_solve without allocation
+// Device memory of pa_pinv_solve_noalloc: one array per GPU, held until the process ends.  Host
+// code never passes `scratch` itself to a kernel or a copy (in host code the name is a host-side
+// stand-in, not a device address); it uses the address hipGetSymbolAddress returns for the
+// current device, kept per device in g_scratch_dev.
+static __device__ __align__(256) unsigned char scratch[PA_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+unsigned char *g_scratch_dev[kMaxDevices] = {};
+
+// the current device's copy of `scratch` (looked up on the first call on each device)
+int scratch_on_device(unsigned char **out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return fail(PA_ERR_HIP, "hipGetDevice");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+      return fail(PA_ERR_HIP, "hipGetSymbolAddress(scratch)");
+    g_scratch_dev[dev] = (unsigned char *)p;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+// Partition of the scratch for `count` systems of size n (leading dimension NS): compact staging
+// of A, b, x as the caller lays them out, the padded store the pinv kernels read, and the store's
+// per-system words; every segment starts on a 256-byte boundary.
+struct NoallocLayout {
+  size_t Ac, bc, xc, Ap, bp, Xp, nn, have, done, cnt, end;
+};
+NoallocLayout noalloc_layout(int n, int NS, size_t count) {
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  NoallocLayout L;
+  size_t o = 0;
+  L.Ac = o; o += up(count * n * n * sizeof(double));
+  L.bc = o; o += up(count * n * sizeof(double));
+  L.xc = o; o += up(count * n * sizeof(double));
+  L.Ap = o; o += up(count * NS * NS * sizeof(double));
+  L.bp = o; o += up(count * NS * sizeof(double));
+  L.Xp = o; o += up(count * SING_ROUNDS * NS * sizeof(double));
+  L.nn = o; o += up(count * sizeof(int));
+  L.have = o; o += up(count * sizeof(int));
+  L.done = o; o += up(count * sizeof(int));
+  L.cnt = o; o += up(sizeof(unsigned));
+  L.end = o;
+  return L;
+}
+
+// compact (count x n x n, count x n) -> padded store (leading dimension NS), per-system words
+// cleared (the array keeps the previous call's contents), queue length = count
+__global__ void __launch_bounds__(256) k_noalloc_pack(unsigned count, int n, int NS, const double *Ac, const double *bc,
+                                                      double *Ap, double *bp, double *Xp, int *nn, int *have, int *done,
+                                                      unsigned *cnt) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t per = (size_t)NS * NS;
+  if (t < count * per) {
+    const size_t e = t / per;
+    const int i = (int)((t % per) / NS), j = (int)(t % NS);
+    Ap[t] = (i < n && j < n) ? Ac[(e * n + i) * n + j] : 0.0;
+  }
+  if (t < (size_t)count * NS) {
+    const size_t e = t / NS;
+    const int i = (int)(t % NS);
+    bp[t] = i < n ? bc[e * n + i] : 0.0;
+  }
+  if (t < (size_t)count * SING_ROUNDS * NS) Xp[t] = 0.0;
+  if (t < count) { nn[t] = n; have[t] = 0; done[t] = 0; }
+  if (t == 0) *cnt = count;
+}
+
+__global__ void __launch_bounds__(256) k_noalloc_unpack(unsigned count, int n, int NS, const double *Xp, double *xc) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)count * n) return;
+  const size_t e = t / n;
+  xc[t] = Xp[e * SING_ROUNDS * NS + t % n];
+}
+}  // namespace
+
+extern "C" {
+
+int pa_pinv_solve_noalloc(int path, int n, long long count, const double *A, const double *b, double *x) {
+  if (path < 0 || path > 2 || n < 1 || n > 48 || count < 0 || (count && (!A || !b || !x)))
+    return fail(PA_ERR_ARG, "bad argument");
+  if (count == 0) return 0;
+  const int NS = ns_template(n);
+  if (count > (1ll << 20) || noalloc_layout(n, NS, (size_t)count).end > (size_t)PA_NOALLOC_SCRATCH_BYTES) {
+    long long cap = 0;
+    while (noalloc_layout(n, NS, (size_t)cap + 1).end <= (size_t)PA_NOALLOC_SCRATCH_BYTES) ++cap;
+    return fail(PA_ERR_ARG, "pa_pinv_solve_noalloc: " + std::to_string(count) + " systems of size " + std::to_string(n) +
+                                " exceed the scratch array (" + std::to_string(PA_NOALLOC_SCRATCH_BYTES) + " bytes: at most " +
+                                std::to_string(cap) + " systems of this size)");
+  }
+  unsigned char *base = nullptr;
+  if (int rc = scratch_on_device(&base)) return rc;
+  const unsigned cnt = (unsigned)count;
+  const NoallocLayout L = noalloc_layout(n, NS, cnt);
+  double *Ac = (double *)(base + L.Ac), *bc = (double *)(base + L.bc), *xc = (double *)(base + L.xc);
+  SingStore st{};
+  st.cap = cnt;
+  st.A = (double *)(base + L.Ap);
+  st.b = (double *)(base + L.bp);
+  st.X = (double *)(base + L.Xp);
+  st.n = (int *)(base + L.nn);
+  st.have = (int *)(base + L.have);
+  st.done = (int *)(base + L.done);
+  unsigned *dcount = (unsigned *)(base + L.cnt);
+  if (hipMemcpy(Ac, A, (size_t)cnt * n * n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(bc, b, (size_t)cnt * n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(PA_ERR_HIP, "copy inputs");
+  const size_t npack = std::max((size_t)cnt * NS * NS, (size_t)cnt * SING_ROUNDS * NS);
+  hipLaunchKernelGGL(k_noalloc_pack, dim3((unsigned)((npack + 255) / 256)), dim3(256), 0, 0, cnt, n, NS, (const double *)Ac,
+                     (const double *)bc, st.A, st.b, st.X, st.n, st.have, st.done, dcount);
+  if (path == 0) {
+    if (NS == 12) hipLaunchKernelGGL(k_pinv_lane<12>, dim3(nblk(cnt, 64)), dim3(64), 0, 0, cnt, n, st.A, st.b, st.X);
+    else if (NS == 24) hipLaunchKernelGGL(k_pinv_lane<24>, dim3(nblk(cnt, 64)), dim3(64), 0, 0, cnt, n, st.A, st.b, st.X);
+    else hipLaunchKernelGGL(k_pinv_lane<48>, dim3(nblk(cnt, 64)), dim3(64), 0, 0, cnt, n, st.A, st.b, st.X);
+  } else if (path == 1) {
+    if (NS == 12) hipLaunchKernelGGL(k_sing_svd<12>, dim3(SVD_GRID), dim3(sing_per_block<12>() * sing_group<12>()), 0, 0, st, (const unsigned *)dcount);
+    else if (NS == 24) hipLaunchKernelGGL(k_sing_svd<24>, dim3(SVD_GRID), dim3(sing_per_block<24>() * sing_group<24>()), 0, 0, st, (const unsigned *)dcount);
+    else hipLaunchKernelGGL(k_sing_svd<48>, dim3(SVD_GRID), dim3(sing_per_block<48>() * sing_group<48>()), 0, 0, st, (const unsigned *)dcount);
+  } else {
+    if (NS == 12) hipLaunchKernelGGL(k_pinv_group<12>, dim3(nblk(cnt, 64 / coop::group_size<12>())), dim3(64), 0, 0, cnt, n, st.A, st.b, st.X);
+    else if (NS == 24) hipLaunchKernelGGL(k_pinv_group<24>, dim3(nblk(cnt, 64 / coop::group_size<24>())), dim3(64), 0, 0, cnt, n, st.A, st.b, st.X);
+    else hipLaunchKernelGGL(k_pinv_group<48>, dim3(nblk(cnt, 64 / coop::group_size<48>())), dim3(64), 0, 0, cnt, n, st.A, st.b, st.X);
+  }
+  hipLaunchKernelGGL(k_noalloc_unpack, dim3((unsigned)(((size_t)cnt * n + 255) / 256)), dim3(256), 0, 0, cnt, n, NS,
+                     (const double *)st.X, xc);
+  if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) return fail(PA_ERR_HIP, "pinv kernel");
+  if (hipMemcpy(x, xc, (size_t)cnt * n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return fail(PA_ERR_HIP, "copy outputs");
+  return 0;
 }
 
 int pa_time_kernel(pa_context *c, int which, int reps, double *avg_ms) {

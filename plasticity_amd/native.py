@@ -23,7 +23,7 @@ EXPORTS = [
     "pa_last_error", "pa_create", "pa_destroy", "pa_element_grains", "pa_set_grains", "pa_set_solver", "pa_set_qp_form", "pa_last_residual",
     "pa_newton_iteration", "pa_assemble", "pa_linear_solve", "pa_commit_increment", "pa_run_increment",
     "pa_get_state", "pa_get_csr", "pa_sizes", "pa_qp_update", "pa_last_timings", "pa_last_qp_pass_ms", "pa_time_kernel",
-    "pa_last_counts", "pa_pinv_solve", "pa_mlp_create", "pa_mlp_destroy", "pa_mlp_set_params", "pa_mlp_get_params",
+    "pa_last_counts", "pa_pinv_solve", "pa_pinv_solve_noalloc", "pa_mlp_create", "pa_mlp_destroy", "pa_mlp_set_params", "pa_mlp_get_params",
     "pa_mlp_predict", "pa_mlp_grad", "pa_mlp_train", "pa_qp_update_warm", "pa_grain_averages",
 ]
 
