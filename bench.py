@@ -36,8 +36,9 @@ FLOP_FIXTURE = os.path.join(ROOT, "tests", "golden", "qp_flops_fcc24.json")
 # rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 (wave instructions x 64 lanes, FMA = 2; masked
 # lanes count) and its HBM bytes (FETCH_SIZE x 2 + WRITE_SIZE KiB, the factors that make both
 # counters read the known byte count of a plain copy): profiles/k_qp_tangent_mixed_64cube_fcc.md
-PMC_QP_FLOPS = {("mixed", 64): (57438700 + 108138000 + 2 * 375480000 + 4213720) * 64}
-PMC_QP_TRAFFIC_KIB = {("mixed", 64): 23369400 * 2 + 18264900}
+# (means over the timed dispatches of the default window, --warmup 5 --steps 20)
+PMC_QP_FLOPS = {("mixed", 64): (55784300 + 100712000 + 2 * 347436000 + 3884150) * 64}
+PMC_QP_TRAFFIC_KIB = {("mixed", 64): 20712600 * 2 + 15814300}
 # HBM bytes per launch of the Jacobi-PCG operator kernel k_cg_spmv at 64^3, measured the same way
 # (profiles/k_cg_spmv_mixed_64cube_fcc.md)
 PMC_TRAFFIC_KIB = {("cg", "mixed", 64): 142996 * 2 + 6373}

@@ -6,9 +6,9 @@
       mean FETCH_SIZE and WRITE_SIZE (KiB) per dispatch of every kernel, grouped by grid size;
       the device-to-device copies (__amd_rocclr_copyBuffer) move a known number of bytes and
       calibrate the counters
-  python -m plasticity_amd.profiling counters <results.db> <kernel name part> <grid> <n>
+  python -m plasticity_amd.profiling counters <results.db> <kernel name part> <grid> <n> [<first>]
       every counter of a --pmc run, summed over its per-unit rows, averaged over the last n
-      dispatches of the kernel with that grid size
+      dispatches of the kernel with that grid size, or over the n from the first-th (0-based)
   python -m plasticity_amd.profiling dispatches <results.db> <kernel name part> <n> [<first>]
       durations (us) of the last n dispatches of the kernel (--kernel-trace) and their mean, or of
       the n dispatches from the first-th (0-based, in launch order)
@@ -53,15 +53,16 @@ def traffic(fetch_db, write_db, min_kib=1024):
                                                    sum(fv) / max(len(fv), 1), sum(wv) / max(len(wv), 1)))
 
 
-def counters(db, part, grid, n):
+def counters(db, part, grid, n, first=None):
     c = sqlite3.connect(db)
     per = defaultdict(lambda: defaultdict(float))
     for name, g, cname, val, did in c.execute(
             "select kernel_name, grid_size, counter_name, value, dispatch_id from counters_collection"):
         if part in name and g == grid:
             per[did][cname] += val
-    last = sorted(per)[-n:]
-    print("%d dispatches of `%s` (grid %d); mean over the last %d:" % (len(per), part, grid, len(last)))
+    last = sorted(per)[-n:] if first is None else sorted(per)[first:first + n]
+    which = "the last %d" % len(last) if first is None else "%d from dispatch %d" % (len(last), first)
+    print("%d dispatches of `%s` (grid %d); mean over %s:" % (len(per), part, grid, which))
     for cname in sorted({k for d in last for k in per[d]}):
         print("  %s %.6g" % (cname, sum(per[d][cname] for d in last) / len(last)))
 
@@ -84,7 +85,7 @@ if __name__ == "__main__":
         traffic(sys.argv[2], sys.argv[3])
     elif len(sys.argv) in (5, 6) and sys.argv[1] == "dispatches":
         dispatches(sys.argv[2], sys.argv[3], int(sys.argv[4]), int(sys.argv[5]) if len(sys.argv) == 6 else None)
-    elif len(sys.argv) == 6 and sys.argv[1] == "counters":
-        counters(sys.argv[2], sys.argv[3], int(sys.argv[4]), int(sys.argv[5]))
+    elif len(sys.argv) in (6, 7) and sys.argv[1] == "counters":
+        counters(sys.argv[2], sys.argv[3], int(sys.argv[4]), int(sys.argv[5]), int(sys.argv[6]) if len(sys.argv) == 7 else None)
     else:
         sys.exit(__doc__)
