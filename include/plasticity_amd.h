@@ -56,9 +56,18 @@ typedef struct {
   int max_slip_search;                   /* Max Slip Search Iterations (must be 1) */
   int max_solver_iter;                   /* Max Solver Iterations */
   int kinematic_hardening;               /* Enable Kinematic Hardening */
-  int twinning;                          /* Twinning enabled (must be 0) */
+  int twinning;                          /* Twinning enabled */
   double C1[PA_MAX_SLIP];                /* C_1 Slip Kinematic Hardening (used when kinematic_hardening) */
   double C2[PA_MAX_SLIP];                /* C_2 Slip Kinematic Hardening */
+  /* Twinning (calculatePlasticity.cc:421-452, 962-1019; used when twinning): the n_twin twin
+   * systems follow the slip systems in m, n (Twin Directions / Twin Normals File rows), s0, h0,
+   * ss, pw (the Twin parameters: Initial Slip Resistance / Initial Hardening Modulus / Saturation
+   * Stress / Power Law Exponent Twin) and q (latent ratios over all n_slip + n_twin systems);
+   * n_slip + n_twin <= PA_MAX_SLIP. */
+  int n_twin;                            /* Number of Twin Systems */
+  double twin_shear;                     /* Characteristic Twin Shear */
+  double twin_threshold;                 /* Twin Threshold Fraction */
+  double twin_saturation;                /* Twin Saturation Factor */
 } pa_material;
 
 /* Mesh, loading and solver inputs of ellipticBVP<dim> (userInputParameters.cc:17-235). */
@@ -134,16 +143,19 @@ int pa_linear_solve(pa_context *ctx, int increment, int iteration, int *lin_iter
 
 /* updateAfterIncrement (src/materialModels/crystalPlasticity/updateAfterIncrement.cc:6-305,
  * 710-771): stress-only qp pass, volume averages, reorient() (reorient.cc:4-100), conv <- iter.
- * row: Exx Eyy Ezz Eyz Exz Exy Txx Tyy Tzz Tyz Txz Txy TwinRealVF TwinMade SlipTotal. */
+ * row: Exx Eyy Ezz Eyz Exz Exy Txx Tyy Tzz Tyz Txz Txy TwinRealVF TwinMade SlipTotal (TwinRealVF,
+ * TwinMade: volume-averaged twin fraction and reoriented volume; 0 without twinning). */
 int pa_commit_increment(pa_context *ctx, double row[15]);
 
 /* solve() body for one increment (src/ellipticBVP/solve.cc:69-91): max_newton Newton iterations
  * then pa_commit_increment.  lin_iters / res_norms (optional) receive max_newton entries. */
 int pa_run_increment(pa_context *ctx, int increment, double row[15], int *lin_iters, double *res_norms);
 
-/* State readback for parity.  which: 0 T (9/qp), 1 Fp_conv (9), 2 Fe_conv (9), 3 s_conv (n_slip),
+/* State readback for parity.  which: 0 T (9/qp), 1 Fp_conv (9), 2 Fe_conv (9), 3 s_conv (n_sys),
  * 4 rotnew (3), 5 slip_conv (n_slip), 6 u (3/node), 7 R (3/node), 8 Fp_iter, 9 Fe_iter, 10 s_iter,
- * 11 W_kh_conv (n_slip, backstress; kinematic hardening only), 12 W_kh_iter.
+ * 11 W_kh_conv (n_sys, backstress; kinematic hardening only), 12 W_kh_iter; with twinning
+ * 13 twinfraction_conv (n_twin), 14 twin_conv (1: reoriented by twinning, else 0), 15 rot_conv (3,
+ * the orientation of the constitutive update).  n_sys = n_slip (+ n_twin with twinning).
  * Output is qp-major (or node-major).  n = number of doubles the caller provides. */
 int pa_get_state(pa_context *ctx, int which, double *host, long long n);
 

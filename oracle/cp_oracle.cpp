@@ -433,7 +433,7 @@ M3 dexp_series(const M3 &X, const M3 &Sj, int count3, double sign) {
 int qp_update_body(const or_material *mat, const Grain &g, const double *Fin, const double *Fp_conv, const double *s_conv,
                    const double *W_conv, const double *slip_conv, int flag, double *Pout, double *Tout, double *dPdF,
                    double *Fe_out, double *Fp_out, double *s_out, double *W_out, double *slip_iter, const double *x0,
-                   int *iters) {
+                   int *iters, const double *tw_conv, double *tw_iter) {
   const int nT = mat->nT, ns = mat->n_slip, ntw = mat->n_twin;
   const double tol1 = mat->tol;
   if (iters) *iters = 0;
@@ -613,6 +613,8 @@ int qp_update_body(const or_material *mat, const Grain &g, const double *Fin, co
     }
 
     if (iters) *iters = n_solves;
+    if (mat->twinning && tw_iter)                         // :450-452
+      for (int i = 0; i < ntw; ++i) tw_iter[i] = tw_conv[i] + x_old[ns + i] / mat->twin_shear;
     for (int i = 0; i < ns; ++i) slip_iter[i] = slip_conv[i] + x_old[i];  // :454-456
 
     Fpn_inv = inv3(FP_tau);                               // :458-468
@@ -773,12 +775,58 @@ int qp_update_body(const or_material *mat, const Grain &g, const double *Fin, co
 int qp_update(const or_material *mat, const Grain &g, const double *Fin, const double *Fp_conv, const double *s_conv,
               const double *W_conv, const double *slip_conv, int flag, double *Pout, double *Tout, double *dPdF,
               double *Fe_out, double *Fp_out, double *s_out, double *W_out, double *slip_iter,
-              const double *x0 = nullptr, int *iters = nullptr) {
+              const double *x0 = nullptr, int *iters = nullptr, const double *tw_conv = nullptr,
+              double *tw_iter = nullptr) {
   OR_FLOP_QP_BEGIN(flag);
   const int rc = qp_update_body(mat, g, Fin, Fp_conv, s_conv, W_conv, slip_conv, flag, Pout, Tout, dPdF, Fe_out, Fp_out,
-                                s_out, W_out, slip_iter, x0, iters);
+                                s_out, W_out, slip_iter, x0, iters, tw_conv, tw_iter);
   OR_FLOP_QP_END(flag);
   return rc;
+}
+
+// ------------------------------------------------------------ twinning
+// Quaternion helpers of updateAfterIncrement.cc:780-821: q = cos(phi/2) (1, r) from a Rodrigues
+// vector; the product p = q2 q1 (R(p) = R(q2) R(q1)) with the sign chosen so that p0 >= 0; and
+// r = q_vec / q0 with |q0| raised to 1e-4 when smaller.
+void rod2quat(const double r[3], double q[4]) {
+  const double c = std::cos(std::atan(std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2])));
+  q[0] = c; q[1] = c * r[0]; q[2] = c * r[1]; q[3] = c * r[2];
+}
+void quatproduct(const double q2[4], const double q1[4], double p[4]) {
+  const double a = q2[0], b = q1[0];
+  const double dot1 = q1[1] * q2[1] + q1[2] * q2[2] + q1[3] * q2[3];
+  p[0] = a * b - dot1;
+  p[1] = a * q1[1] + b * q2[1] + q2[2] * q1[3] - q1[2] * q2[3];
+  p[2] = a * q1[2] + b * q2[2] - q2[1] * q1[3] + q1[1] * q2[3];
+  p[3] = a * q1[3] + b * q2[3] + q2[1] * q1[2] - q1[1] * q2[2];
+  if (p[0] < 0) for (int i = 0; i < 4; ++i) p[i] = -p[i];
+}
+void quat2rod(double q[4], double r[3]) {
+  if (std::fabs(q[0]) < 1e-4) q[0] = 1e-4;
+  const double inv = 1 / q[0];
+  for (int i = 0; i < 3; ++i) r[i] = q[i + 1] * inv;
+}
+// the orientation after twinning on system k: rot rotated by 180 degrees about the twin normal
+// (calculatePlasticity.cc:995-1008)
+void twin_rotation(const or_material *mat, int k, const double rot[3], double out[3]) {
+  double q2[4], q1[4] = {0.0, mat->n[mat->n_slip + k][0], mat->n[mat->n_slip + k][1], mat->n[mat->n_slip + k][2]}, p[4];
+  rod2quat(rot, q2);
+  quatproduct(q2, q1, p);
+  quat2rod(p, out);
+}
+// calculatePlasticity.cc:962-1019 for one qp (see or_twin_event)
+int twin_event(const or_material *mat, double F_r, double F_e, const double *tw_iter, int twin_conv, const double rot_conv[3],
+               double rot_iter[3], int *twin_iter, double *s_iter) {
+  if (!mat->twinning) return -1;
+  double F_T = F_r > 0 ? mat->twin_threshold + (mat->twin_saturation * F_e / F_r) : mat->twin_threshold;  // :970-980
+  if (F_T > 1.0) F_T = 1.0;                                                                             // :983-985
+  int pos = 0;                                                                                          // :986-990
+  for (int i = 1; i < mat->n_twin; ++i) if (tw_iter[i] > tw_iter[pos]) pos = i;
+  if (twin_conv == 1 || !(F_r > 0) || !(tw_iter[pos] > F_T)) return -1;                                // :991-993
+  twin_rotation(mat, pos, rot_conv, rot_iter);
+  *twin_iter = 1;
+  for (int i = 0; i < mat->n_twin; ++i) s_iter[mat->n_slip + i] = 100000;                              // :1013-1015
+  return pos;
 }
 
 // ----------------------------------------------------------- finite elements
@@ -821,6 +869,11 @@ struct QPState {  // SoA-free per-qp history (oracle only)
   std::vector<double> Fp_conv, Fe_conv, s_conv, W_conv, slip_conv, rotnew_conv;
   std::vector<double> Fp_iter, Fe_iter, s_iter, W_iter, slip_iter;
   std::vector<double> T;  // Cauchy stress of the last flag-0 pass
+  // twinning: orientation of the constitutive update (rot_conv/iter), output orientation before
+  // reorient() (rotnew_iter), twinned flags, twin fractions (n_twin per qp)
+  std::vector<double> rot_conv, rot_iter, rotnew_iter, tw_conv, tw_iter;
+  std::vector<int> twin_conv, twin_iter;
+  std::vector<int> tgrain;  // index into the twinned grains (-1: the element's grain)
 };
 
 struct Mesh {
@@ -1130,6 +1183,21 @@ int or_qp_update(const or_material *mat, const double rot[3], const double F[9],
   return qp_update(mat, g, F, Fp_conv, s_conv, W_conv, slip_conv, flag, P, T, dPdF, Fe_iter, Fp_iter, s_iter, W_iter, slip_iter);
 }
 
+int or_twin_event(const or_material *mat, double F_r, double F_e, const double *tw_iter, int twin_conv,
+                  const double rot_conv[3], double rot_iter[3], int *twin_iter, double *s_iter) {
+  return twin_event(mat, F_r, F_e, tw_iter, twin_conv, rot_conv, rot_iter, twin_iter, s_iter);
+}
+
+int or_qp_update_twin(const or_material *mat, const double rot[3], const double F[9], const double Fp_conv[9],
+                      const double *s_conv, const double *W_conv, const double *slip_conv, const double *tw_conv, int flag,
+                      double P[9], double T[9], double dPdF[81], double Fe_iter[9], double Fp_iter[9], double *s_iter,
+                      double *W_iter, double *slip_iter, double *tw_iter) {
+  Grain g;
+  grain_setup(mat, rot, g);
+  return qp_update(mat, g, F, Fp_conv, s_conv, W_conv, slip_conv, flag, P, T, dPdF, Fe_iter, Fp_iter, s_iter, W_iter, slip_iter,
+                   nullptr, nullptr, tw_conv, tw_iter);
+}
+
 int or_qp_update_warm(const or_material *mat, const double rot[3], const double F[9], const double Fp_conv[9],
                       const double *s_conv, const double *W_conv, const double *slip_conv, int flag, const double *x0,
                       double P[9], double T[9], double dPdF[81], double Fe_iter[9], double Fp_iter[9], double *s_iter,
@@ -1317,6 +1385,27 @@ int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *el
   st.rotnew_conv.resize(3 * nq);
   for (long long e = 0; e < m.nelem; ++e)
     for (int q = 0; q < 8; ++q) for (int i = 0; i < 3; ++i) st.rotnew_conv[3 * (8 * e + q) + i] = grain_rod[3 * elem_grain[e] + i];
+  const int ntw = mat->n_twin;
+  const bool twinning = mat->twinning != 0;
+  double F_r = 0.0, F_e = 0.0;  // volume-averaged twin fraction / reoriented volume of the last increment
+  std::vector<Grain> twin_grains;
+  if (twinning) {
+    st.rot_conv = st.rotnew_conv; st.rot_iter = st.rotnew_conv; st.rotnew_iter = st.rotnew_conv;
+    st.tw_conv.assign((size_t)ntw * nq, 0.0); st.tw_iter = st.tw_conv;
+    st.twin_conv.assign(nq, 0); st.twin_iter = st.twin_conv;
+    st.tgrain.assign(nq, -1);
+  }
+  // the orientation tables of qp Q: its grain's, or its own once it has twinned
+  auto qgrain = [&](long long e, long long Q) -> const Grain & {
+    return (twinning && st.tgrain[Q] >= 0) ? twin_grains[st.tgrain[Q]] : grains[elem_grain[e]];
+  };
+  // reorientation by twinning after a qp update (calculatePlasticity.cc:962-1019)
+  auto twin_step = [&](long long Q) {
+    if (!twinning) return;
+    if (twin_event(mat, F_r, F_e, &st.tw_iter[(size_t)ntw * Q], st.twin_conv[Q], &st.rot_conv[3 * Q], &st.rot_iter[3 * Q],
+                   &st.twin_iter[Q], &st.s_iter[(size_t)nT * Q]) >= 0)
+      for (int i = 0; i < 3; ++i) st.rotnew_iter[3 * Q + i] = st.rot_iter[3 * Q + i];
+  };
 
   CSR A = make_csr(m);
   std::vector<double> x(m.ndof, 0.0), R(m.ndof), du;
@@ -1340,14 +1429,16 @@ int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *el
         double *Ke = &Kbuf[576 * e], *Re = &Rbuf[24 * e];
         std::memset(Ke, 0, 576 * sizeof(double));
         std::memset(Re, 0, 24 * sizeof(double));
-        const Grain &g = grains[elem_grain[e]];
         for (int q = 0; q < 8; ++q) {
           const long long Q = 8 * e + q;
+          const Grain &g = qgrain(e, Q);
           const M3 F = def_grad(sh, q, ue);
           double Fv[9], P[9], T[9], D[81];
           store3(F, Fv);
           qp_update(mat, g, Fv, &st.Fp_conv[9 * Q], &st.s_conv[nT * Q], &st.W_conv[nT * Q], &st.slip_conv[ns * Q], 1, P, T, D,
-                    &st.Fe_iter[9 * Q], &st.Fp_iter[9 * Q], &st.s_iter[nT * Q], &st.W_iter[nT * Q], &st.slip_iter[ns * Q]);
+                    &st.Fe_iter[9 * Q], &st.Fp_iter[9 * Q], &st.s_iter[nT * Q], &st.W_iter[nT * Q], &st.slip_iter[ns * Q], nullptr,
+                    nullptr, twinning ? &st.tw_conv[(size_t)ntw * Q] : nullptr, twinning ? &st.tw_iter[(size_t)ntw * Q] : nullptr);
+          twin_step(Q);
           for (int d = 0; d < 24; ++d) {
             const int i = d % 3, v = d / 3;
             for (int j = 0; j < 3; ++j) Re[d] -= sh.grad[q][v][j] * P[3 * i + j] * sh.JxW;
@@ -1414,21 +1505,27 @@ int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *el
     }
     const long long t_post = now_ns();
     // updateAfterIncrement (CP updateAfterIncrement.cc:6-305, 710-771)
-    double sT[3][3] = {{0}}, sE[3][3] = {{0}}, Fs = 0;
+    double sT[3][3] = {{0}}, sE[3][3] = {{0}}, Fs = 0, Fr_sum = 0, Fe_sum = 0;
     vol_total = 0;
     for (long long e = 0; e < m.nelem; ++e) {
       int dofs[24];
       m.elem_dofs(e, dofs);
       double ue[24];
       for (int d = 0; d < 24; ++d) ue[d] = x[dofs[d]];
-      const Grain &g = grains[elem_grain[e]];
       for (int q = 0; q < 8; ++q) {
         const long long Q = 8 * e + q;
+        const Grain &g = qgrain(e, Q);
         const M3 F = def_grad(sh, q, ue);
         double Fv[9], P[9], D[81];
         store3(F, Fv);
         qp_update(mat, g, Fv, &st.Fp_conv[9 * Q], &st.s_conv[nT * Q], &st.W_conv[nT * Q], &st.slip_conv[ns * Q], 0, P, &st.T[9 * Q], D,
-                  &st.Fe_iter[9 * Q], &st.Fp_iter[9 * Q], &st.s_iter[nT * Q], &st.W_iter[nT * Q], &st.slip_iter[ns * Q]);
+                  &st.Fe_iter[9 * Q], &st.Fp_iter[9 * Q], &st.s_iter[nT * Q], &st.W_iter[nT * Q], &st.slip_iter[ns * Q], nullptr,
+                  nullptr, twinning ? &st.tw_conv[(size_t)ntw * Q] : nullptr, twinning ? &st.tw_iter[(size_t)ntw * Q] : nullptr);
+        twin_step(Q);
+        if (twinning) {                                   // updateAfterIncrement.cc:176, 222-228
+          for (int i = 0; i < ntw; ++i) Fr_sum += st.tw_iter[(size_t)ntw * Q + i] * sh.JxW;
+          Fe_sum += st.twin_iter[Q] * sh.JxW;
+        }
         const M3 C = tmul(F, F);
         for (int i = 0; i < 3; ++i)
           for (int j = 0; j < 3; ++j) {
@@ -1440,19 +1537,35 @@ int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *el
         for (int a = 0; a < ns; ++a) Fs += st.slip_iter[ns * Q + a] * sh.JxW;
       }
     }
-    // reorient (CP reorient.cc) then the _conv <- _iter commit (:287-305)
+    // reorient (CP reorient.cc) then the _conv <- _iter commit (:287-305); with twinning the output
+    // orientation first takes the twinned one (rotnew_conv = rotnew_iter, :287)
+    if (twinning) st.rotnew_conv = st.rotnew_iter;
     for (long long Q = 0; Q < nq; ++Q) reorient_qp(load3(&st.Fe_conv[9 * Q]), load3(&st.Fe_iter[9 * Q]), &st.rotnew_conv[3 * Q]);
     st.Fe_conv = st.Fe_iter; st.Fp_conv = st.Fp_iter; st.s_conv = st.s_iter; st.W_conv = st.W_iter; st.slip_conv = st.slip_iter;
+    if (twinning) {
+      st.rotnew_iter = st.rotnew_conv;                    // :293
+      st.tw_conv = st.tw_iter;                            // :302-305
+      for (long long Q = 0; Q < nq; ++Q)
+        if (st.twin_iter[Q] && !st.twin_conv[Q]) {        // newly twinned: its own orientation tables
+          st.tgrain[Q] = (int)twin_grains.size();
+          twin_grains.emplace_back();
+          grain_setup(mat, &st.rot_iter[3 * Q], twin_grains.back());
+        }
+      st.rot_conv = st.rot_iter;
+      st.twin_conv = st.twin_iter;
+      F_r = Fr_sum / vol_total;                           // :719-720
+      F_e = Fe_sum / vol_total;
+    }
     double *row = rows + 15 * inc;
     const int vi[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {0, 2}, {0, 1}};
     for (int c = 0; c < 6; ++c) { row[c] = sE[vi[c][0]][vi[c][1]] / vol_total; row[6 + c] = sT[vi[c][0]][vi[c][1]] / vol_total; }
     g_phase_ns[2] += now_ns() - t_post;
-    row[12] = 0.0;  // TwinRealVF: the inert twin system never activates
-    row[13] = 0.0;  // TwinMade
+    row[12] = twinning ? F_r : 0.0;  // TwinRealVF (without twinning the inert system never activates)
+    row[13] = twinning ? F_e : 0.0;  // TwinMade
     row[14] = Fs / vol_total;
   }
   if (state_out) {
-    const int W = 9 + 9 + 9 + nT + 3 + ns;
+    const int W = 9 + 9 + 9 + nT + 3 + ns + (twinning ? 4 + ntw : 0);
     for (long long Q = 0; Q < nq; ++Q) {
       double *o = state_out + W * Q;
       std::copy(&st.T[9 * Q], &st.T[9 * Q] + 9, o);
@@ -1461,6 +1574,11 @@ int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *el
       std::copy(&st.s_conv[nT * Q], &st.s_conv[nT * Q] + nT, o + 27);
       std::copy(&st.rotnew_conv[3 * Q], &st.rotnew_conv[3 * Q] + 3, o + 27 + nT);
       std::copy(&st.slip_conv[ns * Q], &st.slip_conv[ns * Q] + ns, o + 30 + nT);
+      if (twinning) {
+        std::copy(&st.rot_conv[3 * Q], &st.rot_conv[3 * Q] + 3, o + 30 + nT + ns);
+        o[33 + nT + ns] = st.twin_conv[Q];
+        std::copy(&st.tw_conv[(size_t)ntw * Q], &st.tw_conv[(size_t)ntw * Q] + ntw, o + 34 + nT + ns);
+      }
     }
   }
   if (u_out) std::copy(x.begin(), x.end(), u_out);

@@ -38,6 +38,13 @@ typedef struct {
   double tol;                       /* Stress Tolerance */
   int max_slip_search, max_solver_iter;
   int pinv_mode;
+  /* Twinning (calculatePlasticity.cc:421-451, 962-1019): with twinning = 1 the n_twin systems after
+   * the n_slip slip systems are real twin systems (their s0/h0/ss/pw, latent rows and normalised
+   * m/n in the arrays above); twin_shear = Characteristic Twin Shear, twin_threshold = Twin
+   * Threshold Fraction, twin_saturation = Twin Saturation Factor.  twinning = 0: one inert
+   * system (n_twin = 1, s0 = 1e15) that never activates. */
+  int twinning;
+  double twin_shear, twin_threshold, twin_saturation;
 } or_material;
 
 typedef struct {
@@ -64,6 +71,17 @@ typedef struct {
   double vel_grad[3][3];            /* L */
 } or_problem;
 
+/* Reorientation by twinning at the end of calculatePlasticity (calculatePlasticity.cc:962-1019),
+ * for one qp after its update: F_r, F_e = the volume-averaged twin fraction and reoriented volume
+ * of the previous increment (updateAfterIncrement.cc:222-231, 719-720); tw_iter = the qp's twin
+ * fractions (n_twin); twin_conv = 1 once the qp has twinned in an earlier increment; rot_conv
+ * its orientation.  When the largest twin fraction exceeds F_T = min(1, threshold + saturation
+ * F_e / F_r) (F_r > 0, not yet twinned), rot_iter receives the orientation rotated by 180 degrees
+ * about that twin system's normal (quaternion product), *twin_iter = 1 and the twin systems'
+ * s_iter entries 1e5; returns the twin system (0-based) or -1 (nothing written). */
+int or_twin_event(const or_material *mat, double F_r, double F_e, const double *tw_iter, int twin_conv,
+                  const double rot_conv[3], double rot_iter[3], int *twin_iter, double *s_iter);
+
 /* One calculatePlasticity(cellID,q,flag) call.  In/out arrays follow the
  * reference member layout (row-major 3x3, dP_dF[i][k][j][l] flattened as
  * (3i+k)*9 + 3j+l).  slip_iter is only written on the plastic path
@@ -74,6 +92,12 @@ int or_qp_update(const or_material *mat, const double rot[3], const double F[9],
                  const double *slip_conv, int flag,
                  double P[9], double T[9], double dPdF[81], double Fe_iter[9],
                  double Fp_iter[9], double *s_iter, double *W_iter, double *slip_iter);
+/* or_qp_update with twin fractions: tw_conv (n_twin) in, tw_iter (n_twin) in/out (written on the
+ * plastic path as tw_conv + slip increment / twin_shear, calculatePlasticity.cc:450-452). */
+int or_qp_update_twin(const or_material *mat, const double rot[3], const double F[9], const double Fp_conv[9],
+                      const double *s_conv, const double *W_conv, const double *slip_conv, const double *tw_conv, int flag,
+                      double P[9], double T[9], double dPdF[81], double Fe_iter[9], double Fp_iter[9], double *s_iter,
+                      double *W_iter, double *slip_iter, double *tw_iter);
 
 /* Test-only: scale block `block` of the consistent tangent by (1 + rel) in every later
  * or_qp_update call (block 0 = none; ids in cp_oracle.cpp at g_pert_block).  Lets a test show
@@ -108,7 +132,10 @@ long long or_sparsity(const or_problem *pb, long long *row_ptr, int *col_idx);
  * lin_iters (optional) receives n_increments*max_newton Krylov iteration counts,
  * res_norms (optional) the Newton residual norms.
  * state_out (optional) receives the final per-qp state, qp-major:
- *   [T 9 | Fp 9 | Fe 9 | s nT | rotnew 3 | slip n_slip] per qp.
+ *   [T 9 | Fp 9 | Fe 9 | s nT | rotnew 3 | slip n_slip] per qp, followed with twinning by
+ *   [rot 3 | twinned 1 | twin fractions n_twin].
+ * With twinning, rows[12] = TwinRealVF (volume-averaged twin fraction) and rows[13] = TwinMade
+ * (reoriented volume fraction), updateAfterIncrement.cc:719-720, 755.
  * u_out (optional) receives the final displacement (3 per node). */
 int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *elem_grain,
                 const double *grain_rod, int n_grain_rows,

@@ -7,7 +7,7 @@
 // exports the oracle's functions unchanged (Counted has the layout and the calling convention of a
 // double: one double member, trivially copyable) plus the readout below.
 //
-// Counting convention (algorithmic flops): + - * / each 1, sqrt 1, exp/log/pow/tgamma 1 each
+// Counting convention (algorithmic flops): + - * / each 1, sqrt 1, exp/log/pow/tgamma/atan/cos 1 each
 // (kept apart as "functions"); comparisons, fabs, negation, copies and conversions 0.  Written
 // a*b + c counts 2.  Single-threaded (built without OpenMP).
 //
@@ -67,6 +67,8 @@ inline Counted abs(Counted a) { return Counted(std::fabs(a.v)); }
 inline Counted sqrt(Counted a) { ++flops::g_count[flops::SQRT]; return Counted(std::sqrt(a.v)); }
 inline Counted exp(Counted a) { ++flops::g_count[flops::FUN]; return Counted(std::exp(a.v)); }
 inline Counted log(Counted a) { ++flops::g_count[flops::FUN]; return Counted(std::log(a.v)); }
+inline Counted atan(Counted a) { ++flops::g_count[flops::FUN]; return Counted(std::atan(a.v)); }
+inline Counted cos(Counted a) { ++flops::g_count[flops::FUN]; return Counted(std::cos(a.v)); }
 inline Counted log10(Counted a) { ++flops::g_count[flops::FUN]; return Counted(std::log10(a.v)); }
 inline Counted tgamma(Counted a) { ++flops::g_count[flops::FUN]; return Counted(std::tgamma(a.v)); }
 inline Counted pow(Counted a, Counted b) { ++flops::g_count[flops::FUN]; return Counted(std::pow(a.v, b.v)); }

@@ -534,10 +534,11 @@ __device__ __attribute__((noinline)) bool qp_tangent_one(const QpArgs &a, long l
   for (int i = 0; i < a.ns; ++i) s[i] = a.s_conv[i * a.nq + Q];
   if constexpr (KIN)
     for (int i = 0; i < a.ns; ++i) W[i] = a.W_conv[i * a.nq + Q];
-  const double *gt = a.gtab + (long long)a.egrain[e] * a.gstride;
+  const double *gt = a.gtab + table_row(a.tw, a.egrain, Q) * a.gstride;
+  const int nslip = a.tw.on ? a.tw.nslip : a.ns;
   bool plastic = false;
   const int na = cp::qp_update<NS, DT, KIN>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 1, P, T, Fe, Fp, (DT *)a.D + Q,
-                                            a.nq, &plastic, a.JxW, SingRef<NS>{&sing}, W);
+                                            a.nq, &plastic, a.JxW, SingRef<NS>{&sing}, W, nullptr, 1, nullptr, nslip);
   if (na < 0) return false;
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
@@ -550,7 +551,8 @@ __device__ __attribute__((noinline)) bool qp_tangent_one(const QpArgs &a, long l
     for (int i = 0; i < a.ns; ++i) a.W_iter[i * a.nq + Q] = W[i];
   // slipfraction_iter is written only on the plastic path (calculatePlasticity.cc:454-456)
   if (plastic)
-    for (int i = 0; i < a.ns; ++i) a.slip_iter[i * a.nq + Q] = a.slip_conv[i * a.nq + Q] + x[i];
+    for (int i = 0; i < nslip; ++i) a.slip_iter[i * a.nq + Q] = a.slip_conv[i * a.nq + Q] + x[i];
+  if (a.tw.on) twin_epilogue(a.tw, Q, a.nq, plastic, x, a.s_iter);
   return true;
 }
 
@@ -794,10 +796,11 @@ __device__ __attribute__((noinline)) bool post_qp_one(const PostArgs &a, long lo
   for (int i = 0; i < a.ns; ++i) s[i] = a.s_conv[i * a.nq + Q];
   if constexpr (KIN)
     for (int i = 0; i < a.ns; ++i) W[i] = a.W_conv[i * a.nq + Q];
-  const double *gt = a.gtab + (long long)a.egrain[Q >> 3] * a.gstride;
+  const double *gt = a.gtab + table_row(a.tw, a.egrain, Q) * a.gstride;
+  const int nslip = a.tw.on ? a.tw.nslip : a.ns;
   bool plastic = false;
   const int na = cp::qp_update<NS, double, KIN>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 0, P, T, Fe, Fp, (double *)nullptr,
-                                                0, &plastic, 1.0, SingRef<NS>{&sing}, W);
+                                                0, &plastic, 1.0, SingRef<NS>{&sing}, W, nullptr, 1, nullptr, nslip);
   if (na < 0) return false;
 #pragma unroll
   for (int i = 0; i < 9; ++i) { a.Fe_iter[i * a.nq + Q] = Fe[i]; a.Fp_iter[i * a.nq + Q] = Fp[i]; a.T[i * a.nq + Q] = T[i]; }
@@ -805,7 +808,8 @@ __device__ __attribute__((noinline)) bool post_qp_one(const PostArgs &a, long lo
   if constexpr (KIN)
     for (int i = 0; i < a.ns; ++i) a.W_iter[i * a.nq + Q] = W[i];
   if (plastic)
-    for (int i = 0; i < a.ns; ++i) a.slip_iter[i * a.nq + Q] = a.slip_conv[i * a.nq + Q] + x[i];
+    for (int i = 0; i < nslip; ++i) a.slip_iter[i * a.nq + Q] = a.slip_conv[i * a.nq + Q] + x[i];
+  if (a.tw.on) twin_epilogue(a.tw, Q, a.nq, plastic, x, a.s_iter);
   return true;
 }
 // lane = qp, grid-stride over one-wave blocks (as the first assembly pass)
@@ -838,11 +842,11 @@ __global__ void __launch_bounds__(64) QP_WAVES k_post_qp_deferred(PostArgs a) {
 // Green-Lagrange strain, JxW-weighted sums (updateAfterIncrement.cc:107-126, 218-245) and
 // reorient() (reorient.cc:18-96) with Fe_old = Fe_conv, Fe_new = Fe_iter; deterministic sums.
 __global__ void __launch_bounds__(256) k_post_reduce(PostArgs a) {
-  __shared__ double lds[13 * 16];
+  __shared__ double lds[15 * 16];
   const long long Q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  double acc[13];
+  double acc[15];
 #pragma unroll
-  for (int i = 0; i < 13; ++i) acc[i] = 0.0;
+  for (int i = 0; i < 15; ++i) acc[i] = 0.0;
   if (Q < a.nq) {
     double F[9], C[9], T[9];
     post_F(a, Q, F);
@@ -856,8 +860,15 @@ __global__ void __launch_bounds__(256) k_post_reduce(PostArgs a) {
       acc[6 + c] = T[vi[c]] * a.JxW;
     }
     double slip_sum = 0;
-    for (int i = 0; i < a.ns; ++i) slip_sum += a.slip_iter[i * a.nq + Q] * a.JxW;
+    const int nslip = a.tw.on ? a.tw.nslip : a.ns;
+    for (int i = 0; i < nslip; ++i) slip_sum += a.slip_iter[i * a.nq + Q] * a.JxW;
     acc[12] = slip_sum;
+    if (a.tw.on) {  // TwinRealVF and TwinMade numerators (updateAfterIncrement.cc:176, 222-228)
+      double fr = 0;
+      for (int k = 0; k < a.tw.ntw; ++k) fr += a.tw.tw_iter[k * a.nq + Q] * a.JxW;
+      acc[13] = fr;
+      acc[14] = (a.tw.var_iter[Q] != 0 ? 1.0 : 0.0) * a.JxW;
+    }
     double Fo[9], Fn[9], Ro[9], Rn[9], Om[9], Om2[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) { Fo[i] = a.Fe_conv[i * a.nq + Q]; Fn[i] = a.Fe_iter[i * a.nq + Q]; }
@@ -870,6 +881,11 @@ __global__ void __launch_bounds__(256) k_post_reduce(PostArgs a) {
     double r[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) r[i] = a.rotnew[i * a.nq + Q];
+    if (a.tw.on && a.tw.var_iter[Q] != a.tw.var_conv[Q]) {  // reoriented by twinning in this increment (:1009-1010, :287)
+      const long long row = (long long)a.egrain[Q >> 3] * (1 + a.tw.ntw) + a.tw.var_iter[Q];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) r[i] = a.tw.vrod[3 * row + i];
+    }
     const double dt = w[0] * r[0] + w[1] * r[1] + w[2] * r[2];
     const double cr[3] = {w[1] * r[2] - w[2] * r[1], w[2] * r[0] - w[0] * r[2], w[0] * r[1] - w[1] * r[0]};
     double rn[3];
@@ -881,9 +897,9 @@ __global__ void __launch_bounds__(256) k_post_reduce(PostArgs a) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) a.rotnew[i * a.nq + Q] = rn[i];
   }
-  if (grid_sum<13>(acc, a.partials, a.counter, lds)) {
+  if (grid_sum<15>(acc, a.partials, a.counter, lds)) {
 #pragma unroll
-    for (int i = 0; i < 13; ++i) a.out[i] = acc[i];
+    for (int i = 0; i < 15; ++i) a.out[i] = acc[i];
   }
 }
 
@@ -1479,6 +1495,13 @@ struct pa_context {
   double *slip_conv = nullptr, *slip_iter = nullptr, *rotnew = nullptr, *T = nullptr;
   bool kin = false;                                // kinematic hardening: backstress state below
   double *W_conv = nullptr, *W_iter = nullptr;     // [ns][nq], allocated only with kinematic hardening
+  // twinning: ns counts slip + twin systems; nslip the slip systems, ntw the twin systems
+  int nslip = 0, ntw = 0;
+  double *tw_conv = nullptr, *tw_iter = nullptr;   // [ntw][nq] twin fractions
+  int *var_conv = nullptr, *var_iter = nullptr;    // [nq] orientation variant (0: the grain's)
+  double *d_vrod = nullptr;                        // [grain][1 + ntw][3] variant orientations
+  std::vector<double> h_vrod;                      // the same on the host
+  double F_r = 0.0, F_e = 0.0;                     // TwinRealVF / TwinMade of the last increment
   double *u = nullptr, *R = nullptr, *dinv = nullptr;
   void *K = nullptr;   // node-block Jacobian, double (FP64 form) or float (mixed form)
   void *Dq = nullptr;  // [81][nq] JxW dP/dF of the last assembly, double or float
@@ -1582,6 +1605,26 @@ void qp_tangent_passes(pa_context *c, const QpArgs &a, bool main_only) {
   else qp_tangent_passes_k<NS, DT, false>(c, a, main_only);
 }
 
+// twinning arguments of the qp passes of the current increment: F_T from the last increment's
+// TwinRealVF (F_r) and TwinMade (F_e), calculatePlasticity.cc:969-985
+TwinArgs twin_args(const pa_context *c) {
+  TwinArgs t;
+  std::memset(&t, 0, sizeof(t));
+  if (!c->ntw) return t;
+  t.on = 1;
+  t.nslip = c->nslip;
+  t.ntw = c->ntw;
+  t.shear = c->mat.twin_shear;
+  double F_T = c->F_r > 0 ? c->mat.twin_threshold + (c->mat.twin_saturation * c->F_e / c->F_r) : c->mat.twin_threshold;
+  if (F_T > 1.0) F_T = 1.0;
+  t.F_T = F_T;
+  t.armed = c->F_r > 0;
+  t.tw_conv = c->tw_conv; t.tw_iter = c->tw_iter;
+  t.var_conv = c->var_conv; t.var_iter = c->var_iter;
+  t.vrod = c->d_vrod;
+  return t;
+}
+
 // main_only: pass 1 alone (every qp on the LU path; the singular ones are only queued)
 int launch_qp_tangent(pa_context *c, bool main_only = false) {
   QpArgs a;
@@ -1595,6 +1638,7 @@ int launch_qp_tangent(pa_context *c, bool main_only = false) {
   a.D = c->Dq; a.P = c->Pq;
   a.queue = c->queue; a.qcount = c->counters + 8; a.sing = c->sing;
   a.nsing = c->counters + 8; a.work = c->counters + 11;
+  a.tw = twin_args(c);
   const int NS = ns_template(c->ns);
   const bool f = mixed(c);
   if (NS == 12) { if (f) qp_tangent_passes<12, float>(c, a, main_only); else qp_tangent_passes<12, double>(c, a, main_only); }
@@ -1668,7 +1712,7 @@ long long update_partials_end(const pa_context *c) { return (spmv_partials_offse
 
 size_t partials_size(const pa_context *c) {
   const size_t vec = (size_t)update_partials_end(c) + 3 * VGRID_MAX + 64;
-  const size_t post = 13 * (size_t)nblk(c->nq, 256) + 64;
+  const size_t post = 15 * (size_t)nblk(c->nq, 256) + 64;
   return std::max(vec, post);
 }
 
@@ -2028,7 +2072,11 @@ int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out) {
   if (mat->n_slip < 1 || mat->n_slip > PA_MAX_SLIP) return fail(PA_ERR_ARG, "n_slip out of range");
   if (mat->max_slip_search != 1) return fail(PA_ERR_UNSUPPORTED, "Max Slip Search Iterations must be 1");
   if (mat->max_solver_iter < 1) return fail(PA_ERR_ARG, "Max Solver Iterations must be >= 1");
-  if (mat->twinning) return fail(PA_ERR_UNSUPPORTED, "twinning is not implemented");
+  if (mat->twinning) {
+    if (mat->n_twin < 1 || mat->n_slip + mat->n_twin > PA_MAX_SLIP)
+      return fail(PA_ERR_ARG, "twinning: need 1 <= n_twin and n_slip + n_twin <= PA_MAX_SLIP");
+    if (!(mat->twin_shear > 0)) return fail(PA_ERR_ARG, "twinning: Characteristic Twin Shear must be > 0");
+  }
   if (pb->nx < 1 || pb->ny < 1 || pb->nz < 1) return fail(PA_ERR_ARG, "mesh size must be >= 1");
   if (pb->total_increments < 1 || pb->max_newton < 1 || pb->lin_maxit < 1) return fail(PA_ERR_ARG, "bad increment/solver counts");
   if (pb->tabular && (pb->n_table < 2 || pb->n_table > PA_MAX_TABLE)) return fail(PA_ERR_ARG, "tabular BCs need 2..PA_MAX_TABLE time points");
@@ -2045,7 +2093,9 @@ int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out) {
   c->mat = *mat;
   c->pb = *pb;
   c->kin = mat->kinematic_hardening != 0;
-  c->ns = mat->n_slip;
+  c->nslip = mat->n_slip;
+  c->ntw = mat->twinning ? mat->n_twin : 0;
+  c->ns = c->nslip + c->ntw;
   MeshDims &m = c->m;
   m.nx = pb->nx; m.ny = pb->ny; m.nz = pb->nz;
   m.nnx = m.nx + 1; m.nny = m.ny + 1; m.nnz = m.nz + 1;
@@ -2099,6 +2149,19 @@ int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out) {
       (rc = dalloc(c, &c->d_out, 64))) {
     pa_destroy(c);
     return rc;
+  }
+  if (c->ntw) {  // twin fractions start at zero, every qp in its grain's orientation (init.cc:274-324)
+    if ((rc = dalloc(c, &c->tw_conv, (long long)c->ntw * nq)) || (rc = dalloc(c, &c->tw_iter, (long long)c->ntw * nq)) ||
+        (rc = dalloc(c, &c->var_conv, nq)) || (rc = dalloc(c, &c->var_iter, nq))) {
+      pa_destroy(c);
+      return rc;
+    }
+    const size_t bt = sizeof(double) * c->ntw * nq, bv = sizeof(int) * nq;
+    if (hipMemset(c->tw_conv, 0, bt) != hipSuccess || hipMemset(c->tw_iter, 0, bt) != hipSuccess ||
+        hipMemset(c->var_conv, 0, bv) != hipSuccess || hipMemset(c->var_iter, 0, bv) != hipSuccess) {
+      pa_destroy(c);
+      return fail(PA_ERR_HIP, "state init");
+    }
   }
   for (int i = 0; i < 7; ++i)
     if ((rc = dalloc(c, &c->kv[i], N))) { pa_destroy(c); return rc; }
@@ -2159,6 +2222,8 @@ int pa_destroy(pa_context *c) {
     if (c->mg_graph[f]) (void)hipGraphDestroy(c->mg_graph[f]);
   }
   for (void *p : c->allocs) hipFree(p);
+  if (c->d_gtab) hipFree(c->d_gtab);
+  if (c->d_vrod) hipFree(c->d_vrod);
   for (int i = 0; i < 4; ++i)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
   for (int i = 0; i < 2; ++i)
@@ -2213,9 +2278,26 @@ int pa_set_grains(pa_context *c, const unsigned *elem_grain, const double *grain
     mn = std::sqrt(mn); nn = std::sqrt(nn);
     for (int j = 0; j < 3; ++j) { mv[3 * a + j] = c->mat.m[a][j] / mn; nv[3 * a + j] = c->mat.n[a][j] / nn; }
   }
-  std::vector<double> tab((size_t)n_grain_rows * c->gstride, 0.0);
+  // orientation tables: one row per grain, or with twinning one per (grain, variant): the grain's
+  // orientation and its 180-degree turns about each twin normal
+  const int nvar = 1 + c->ntw;
+  c->h_vrod.assign((size_t)n_grain_rows * nvar * 3, 0.0);
   for (int g = 0; g < n_grain_rows; ++g)
-    grain_table(c->mat, (const double(*)[3])mv.data(), (const double(*)[3])nv.data(), grain_rod + 3 * g, c->ns, NS, &tab[(size_t)g * c->gstride]);
+    for (int v = 0; v < nvar; ++v) {
+      double *r = &c->h_vrod[((size_t)g * nvar + v) * 3];
+      if (v == 0) for (int i = 0; i < 3; ++i) r[i] = grain_rod[3 * g + i];
+      else twin_variant_rod(grain_rod + 3 * g, &nv[3 * (c->nslip + v - 1)], r);
+    }
+  std::vector<double> tab((size_t)n_grain_rows * nvar * c->gstride, 0.0);
+  for (size_t row = 0; row < (size_t)n_grain_rows * nvar; ++row)
+    grain_table(c->mat, (const double(*)[3])mv.data(), (const double(*)[3])nv.data(), &c->h_vrod[3 * row], c->ns, NS,
+                &tab[row * c->gstride]);
+  if (c->ntw) {
+    if (c->d_vrod) hipFree(c->d_vrod);
+    c->d_vrod = nullptr;
+    HIPCHK(hipMalloc(&c->d_vrod, c->h_vrod.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(c->d_vrod, c->h_vrod.data(), c->h_vrod.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   if (c->d_gtab) hipFree(c->d_gtab);
   c->d_gtab = nullptr;
   HIPCHK(hipMalloc(&c->d_gtab, tab.size() * sizeof(double)));
@@ -2299,6 +2381,7 @@ int pa_commit_increment(pa_context *c, double row[15]) {
   a.partials = c->partials; a.counter = c->counters + 4; a.out = c->d_out;
   a.queue = c->queue; a.qcount = c->counters + 9; a.sing = c->sing;
   a.nsing = c->counters + 9; a.work = c->counters + 12;
+  a.tw = twin_args(c);
   HIPCHK(hipEventRecord(c->ev[0], c->stream));
   const int NS = ns_template(c->ns);
   const unsigned grid = nblk(c->nq, 256);
@@ -2315,21 +2398,29 @@ int pa_commit_increment(pa_context *c, double row[15]) {
   HIPCHK(hipMemcpyAsync(c->s_conv, c->s_iter, bs, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->slip_conv, c->slip_iter, bs, hipMemcpyDeviceToDevice, c->stream));
   if (c->kin) HIPCHK(hipMemcpyAsync(c->W_conv, c->W_iter, bs, hipMemcpyDeviceToDevice, c->stream));
-  double h[13];
+  if (c->ntw) {  // twinfraction_conv, twin_conv and rot_conv (as the variant) <- _iter (:302-305)
+    HIPCHK(hipMemcpyAsync(c->tw_conv, c->tw_iter, sizeof(double) * c->ntw * c->nq, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->var_conv, c->var_iter, sizeof(int) * c->nq, hipMemcpyDeviceToDevice, c->stream));
+  }
+  double h[15];
   HIPCHK(hipMemcpyAsync(h, c->d_out, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipEventRecord(c->ev[1], c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   float ms = 0;
   if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) c->last_ms[2] = ms;
   const double vol = c->microvol;
+  if (c->ntw) {  // updateAfterIncrement.cc:719-720: the F_T of the next increment uses these
+    c->F_r = h[13] / vol;
+    c->F_e = h[14] / vol;
+  }
   if (row) {
     for (int k = 0; k < 12; ++k) row[k] = h[k] / vol;
-    row[12] = 0.0;  // TwinRealVF: no twin system is ever active
-    row[13] = 0.0;  // TwinMade
+    row[12] = c->ntw ? c->F_r : 0.0;  // TwinRealVF (without twinning no twin system is ever active)
+    row[13] = c->ntw ? c->F_e : 0.0;  // TwinMade
     row[14] = h[12] / vol;
   }
   c->bc_inc = -1;
-  for (int k = 0; k < 13; ++k)
+  for (int k = 0; k < 15; ++k)
     if (!std::isfinite(h[k])) return fail(PA_ERR_NONFINITE, "post-increment averages are not finite (NaN/Inf stress or strain)");
   return 0;
 }
@@ -2418,7 +2509,7 @@ int pa_get_state(pa_context *c, int which, double *host, long long n) {
     case 2: src = c->Fe_conv; comps = 9; break;
     case 3: src = c->s_conv; comps = c->ns; break;
     case 4: src = c->rotnew; comps = 3; break;
-    case 5: src = c->slip_conv; comps = c->ns; break;
+    case 5: src = c->slip_conv; comps = c->nslip; break;
     case 6: src = c->u; comps = 3; node = true; break;
     case 7: src = c->R; comps = 3; node = true; break;
     case 8: src = c->Fp_iter; comps = 9; break;
@@ -2426,10 +2517,30 @@ int pa_get_state(pa_context *c, int which, double *host, long long n) {
     case 10: src = c->s_iter; comps = c->ns; break;
     case 11: src = c->W_conv; comps = c->ns; break;
     case 12: src = c->W_iter; comps = c->ns; break;
+    case 13: src = c->tw_conv; comps = c->ntw; break;
+    case 14:    // twinned (0/1), and 15: the orientation of the constitutive update (rot_conv)
+    case 15: {
+      if (!c->ntw) return fail(PA_ERR_STATE, "no twin state: twinning is off");
+      const int k = which == 14 ? 1 : 3;
+      if (n < k * nq) return fail(PA_ERR_ARG, "host buffer too small");
+      if (!c->grains_set) return fail(PA_ERR_STATE, "pa_set_grains has not been called");
+      std::vector<int> var(nq);
+      std::vector<unsigned> eg(c->m.nelem);
+      HIPCHK(hipStreamSynchronize(c->stream));
+      HIPCHK(hipMemcpy(var.data(), c->var_conv, nq * sizeof(int), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(eg.data(), c->d_egrain, c->m.nelem * sizeof(unsigned), hipMemcpyDeviceToHost));
+      for (long long Q = 0; Q < nq; ++Q) {
+        if (which == 14) { host[Q] = var[Q] != 0 ? 1.0 : 0.0; continue; }
+        const double *r = &c->h_vrod[((size_t)eg[Q >> 3] * (1 + c->ntw) + var[Q]) * 3];
+        for (int i = 0; i < 3; ++i) host[3 * Q + i] = r[i];
+      }
+      return 0;
+    }
     default: return fail(PA_ERR_ARG, "unknown state id");
   }
   if (!src) {
     if (which == 11 || which == 12) return fail(PA_ERR_STATE, "no backstress state: kinematic hardening is off");
+    if (which == 13) return fail(PA_ERR_STATE, "no twin state: twinning is off");
     return fail(PA_ERR_STATE, "state not allocated");
   }
   const long long cnt = node ? c->ndof : comps * nq;
@@ -2481,6 +2592,7 @@ int pa_qp_update(pa_context *c, long long n, const double *F, const double *Fp_c
     return fail(PA_ERR_ARG, "null argument");
   if (flag && !dPdF) return fail(PA_ERR_ARG, "dPdF required for flag 1");
   if (!c->grains_set) return fail(PA_ERR_STATE, "pa_set_grains has not been called");
+  if (c->ntw) return fail(PA_ERR_UNSUPPORTED, "pa_qp_update: the batch form has no twin systems (the Newton step has)");
   for (long long i = 0; i < n; ++i)
     if ((int)grain[i] >= c->n_grain_rows) return fail(PA_ERR_ARG, "grain index out of range");
   if (n == 0) return 0;

@@ -87,6 +87,51 @@ __device__ __forceinline__ void qp_queue(unsigned *work, long long nq, F &&one) 
   }
 }
 
+// ---- twinning (calculatePlasticity.cc:421-452, 962-1019).  The twin systems follow the slip
+// systems in the system arrays (s, x); a qp's lattice orientation is one of 1 + ntw variants of
+// its grain's (0: the grain's, 1 + k: turned 180 degrees about twin system k's normal, at most one
+// reorientation per qp), each with its own orientation tables (row grain * (1 + ntw) + variant).
+struct TwinArgs {
+  int on;                 // twinning enabled (everything below unused otherwise)
+  int nslip, ntw;         // slip systems; twin systems
+  double shear;           // Characteristic Twin Shear
+  double F_T;             // min(1, Twin Threshold Fraction + Twin Saturation Factor F_e / F_r) of this increment
+  int armed;              // F_r > 0 (calculatePlasticity.cc:992): a qp may reorient in this increment
+  const double *tw_conv;  // [ntw][nq] twin fractions
+  double *tw_iter;
+  const int *var_conv;    // [nq] orientation variant
+  int *var_iter;
+  const double *vrod;     // [grain][1 + ntw][3] Rodrigues vectors of the variants
+};
+
+// After a qp update: the twin fractions (written on the plastic path only, as the reference writes
+// twinfraction_iter, :450-452; otherwise the last call's stand) and the reorientation test of
+// :986-1016 (largest fraction, first of equals, above F_T; once per qp): the variant, and twin
+// slip resistances of 1e5.
+__device__ __forceinline__ void twin_epilogue(const TwinArgs &t, long long Q, long long nq, bool plastic, const double *x,
+                                              double *s_iter) {
+  double fmax = 0.0;
+  int pos = 0;
+  for (int k = 0; k < t.ntw; ++k) {
+    double f;
+    if (plastic) {
+      f = t.tw_conv[k * nq + Q] + x[t.nslip + k] / t.shear;
+      t.tw_iter[k * nq + Q] = f;
+    } else {
+      f = t.tw_iter[k * nq + Q];
+    }
+    if (k == 0 || f > fmax) { fmax = f; pos = k; }
+  }
+  if (t.var_conv[Q] == 0 && t.armed && fmax > t.F_T) {
+    t.var_iter[Q] = 1 + pos;
+    for (int k = 0; k < t.ntw; ++k) s_iter[(t.nslip + k) * nq + Q] = 100000.0;
+  }
+}
+// orientation-table row of qp Q in element e
+__device__ __forceinline__ long long table_row(const TwinArgs &t, const unsigned *egrain, long long Q) {
+  return t.on ? (long long)egrain[Q >> 3] * (1 + t.ntw) + t.var_conv[Q] : (long long)egrain[Q >> 3];
+}
+
 // ---- assembly, stage 1: F from u, calculatePlasticity(.,.,1); writes the iter history,
 // JxW * dP/dF (as DT) and JxW * P.
 struct QpArgs {
@@ -109,6 +154,7 @@ struct QpArgs {
   // cooperative form
   unsigned *work;   // work queue: the next unclaimed qp (zeroed before each launch)
   unsigned *nsing;  // pseudo-inverse solves (numerically singular A_PA) of the pass
+  TwinArgs tw;      // twinning (one-lane form)
 };
 
 template <int NS, typename DT, bool KIN, int G>
@@ -166,11 +212,12 @@ struct PostArgs {
   double *Fe_iter, *Fp_iter, *s_iter, *slip_iter, *T, *rotnew, *W_iter;
   double *partials;
   unsigned *counter;
-  double *out;      // 13: E(6) T(6) slip
+  double *out;      // 15: E(6) T(6) slip, twin fraction, twinned volume (JxW sums)
   unsigned *queue, *qcount;  // one-lane form: deferred qps
   SingStore sing;            // and their singular systems
   unsigned *work;   // cooperative form: work queue of the qp pass (zeroed before the launch)
   unsigned *nsing;  // pseudo-inverse solves of the pass
+  TwinArgs tw;      // twinning (one-lane form)
 };
 
 __device__ __forceinline__ void post_F(const PostArgs &a, long long Q, double F[9]) { qp_F(a.m, a.h, a.u, Q, F); }

@@ -315,6 +315,9 @@ CP_DEV void dexp_series(const double *X, const double *S, int count3, bool neg, 
   }
 }
 
+// nslip: systems a >= nslip are twin systems (twinning), whose resolved shear below the backstress
+// is replaced by the backstress (they shear in one sense only, calculatePlasticity.cc:213-216,
+// 390-393); the default (no twin systems) leaves every system two-sided.
 // Return: number of systems left active after inactive-slip removal (0 = elastic), or -1 when
 // the singular-solve functor declined a numerically singular A_PA (nothing was written then).
 // s holds s_conv on entry and s_tau on exit; x receives x_beta_old (the slip increments).
@@ -336,7 +339,7 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
                      const double *F, const double *Fp_t, double *s, double *x, int flag, double *P, double *T,
                      double *Fe_out, double *Fp_out, DT *D, long long dstride, bool *plastic_path = nullptr,
                      double dscale = 1.0, SING sing = SING(), double *W = nullptr, const float *x0 = nullptr,
-                     long long x0stride = 1, int *iters = nullptr) {
+                     long long x0stride = 1, int *iters = nullptr, int nslip = 1 << 30) {
   static_assert(!(WARM && KIN), "the warm start does not cover kinematic hardening");
   typedef MatLayout<NS> ML;
   typedef GrainLayout<NS> GL;
@@ -367,7 +370,8 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
   int nPA = 0;
   m3_mul(CE, Tst, tmp);                                  // :203-240
   for (int a = 0; a < ns; ++a) {
-    const double r = m3_ddot(tmp, Sg + 9 * a);
+    double r = m3_ddot(tmp, Sg + 9 * a);
+    if (a >= nslip && (r - wtau(a)) < 0) r = wtau(a);  // twin systems shear one way only (:213-216)
     tau_tr[a] = r;
     b[a] = fabs(r - wtau(a)) - s[a];
     x[a] = 0.0;
@@ -472,6 +476,8 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
       dmat_apply(Dm, E, TstTau);                         // :376-377
       m3_mul(Ce, TstTau, tmp);
       for (int a = 0; a < ns; ++a) {                     // :380-440
+        double ta = m3_ddot(tmp, Sg + 9 * a);
+        if (a >= nslip && (tau_tr[a] - wtau(a)) < 0) ta = wtau(a);  // :390-393, the backstress before :431
         double h1 = 0;
         for (int c = 0; c < ns; ++c) h1 += mat[ML::q + a * NS + c] * h[c] * xb[c];
         double sa = s[a] + h1;
@@ -481,7 +487,7 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
           const double c1 = mat[ML::c1 + a], c2 = mat[ML::c2 + a];
           W[a] += (((tau_tr[a] - W[a]) > 0) ? (c1 - c2 * W[a]) : (-c1 - c2 * W[a])) * xb[a];
         }
-        b[a] = fabs(m3_ddot(tmp, Sg + 9 * a) - wtau(a)) - sa;
+        b[a] = fabs(ta - wtau(a)) - sa;
       }
     };
     // the reference's loop from `done` solves already counted; stop_on_drop: stop as soon as

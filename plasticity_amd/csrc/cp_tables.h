@@ -56,4 +56,25 @@ inline void grain_table(const pa_material &mat, const double mv[][3], const doub
     for (int j = 0; j < 3; ++j) Rout[3 * i + j] = R[i][j];
 }
 
+// Orientation of a grain after twinning on a system with unit normal nt (calculatePlasticity.cc:
+// 995-1008 with updateAfterIncrement.cc:780-821): the quaternion of r, q = cos(phi/2) (1, r) with
+// phi/2 = atan|r|, times (0, nt) as R(p) = R(q) R(0, nt), sign chosen so that p0 >= 0, back to
+// r = p_vec / p0 (|p0| raised to 1e-4 when smaller).
+inline void twin_variant_rod(const double r[3], const double nt[3], double out[3]) {
+  const double c = std::cos(std::atan(std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2])));
+  const double q[4] = {c, c * r[0], c * r[1], c * r[2]};
+  const double t[4] = {0.0, nt[0], nt[1], nt[2]};
+  const double dot = t[1] * q[1] + t[2] * q[2] + t[3] * q[3];
+  double p[4];
+  p[0] = q[0] * t[0] - dot;
+  p[1] = q[0] * t[1] + t[0] * q[1] + q[2] * t[3] - t[2] * q[3];
+  p[2] = q[0] * t[2] + t[0] * q[2] - q[1] * t[3] + t[1] * q[3];
+  p[3] = q[0] * t[3] + t[0] * q[3] + q[1] * t[2] - t[1] * q[2];
+  if (p[0] < 0)
+    for (int i = 0; i < 4; ++i) p[i] = -p[i];
+  if (std::fabs(p[0]) < 1e-4) p[0] = 1e-4;
+  const double inv = 1 / p[0];
+  for (int i = 0; i < 3; ++i) out[i] = p[i + 1] * inv;
+}
+
 }  // namespace

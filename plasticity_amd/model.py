@@ -72,7 +72,18 @@ class userInputParameters:
         self.enableKinematicHardening1 = b("Enable Kinematic Hardening", "false")  # userInputParameters.cc:274-279
         self.C_1_slip1 = fl(g("C_1 Slip Kinematic Hardening", "")) if self.enableKinematicHardening1 else []
         self.C_2_slip1 = fl(g("C_2 Slip Kinematic Hardening", "")) if self.enableKinematicHardening1 else []
-        self.enableTwinning1 = b("Twinning enabled", "false")
+        self.enableTwinning1 = b("Twinning enabled", "false")  # userInputParameters.cc twin block
+        if self.enableTwinning1:
+            self.numTwinSystems1 = int(g("Number of Twin Systems", "0"))
+            self.initialSlipResistanceTwin1 = fl(g("Initial Slip Resistance Twin", ""))
+            self.initialHardeningModulusTwin1 = fl(g("Initial Hardening Modulus Twin", ""))
+            self.powerLawExponentTwin1 = fl(g("Power Law Exponent Twin", ""))
+            self.saturationStressTwin1 = fl(g("Saturation Stress Twin", ""))
+            self.twinSaturationFactor = float(g("Twin Saturation Factor", "0"))
+            self.twinThresholdFraction = float(g("Twin Threshold Fraction", "0"))
+            self.twinShear = float(g("Characteristic Twin Shear", "0"))
+            self.twinDirectionsFile1 = g("Twin Directions File", "")
+            self.twinNormalsFile1 = g("Twin Normals File", "")
         self.enableMultiphase = b("Enable Multiphase", "false")
         self.modelStressTolerance = float(g("Stress Tolerance", "-1"))
         self.modelMaxSlipSearchIterations = int(g("Max Slip Search Iterations", "-1"))
@@ -101,7 +112,7 @@ class userInputParameters:
         self.writeQuadratureOutput = b("Write Quadrature Output", "false")
         self.skipQuadratureOutputSteps = int(g("Skip Quadrature Output Steps", "-1"))
         for key in ("Enable Advanced Twin Model", "Enable Advanced Rate Dependent Model", "Enable User Material Model"):
-            if self.writeQuadratureOutput and b(key, "false"):
+            if b(key, "false"):
                 raise native.PlasticityError("'%s' is outside the implemented path" % key)
         if self.writeGrainAveragedOutput and b("Output Buffer Layer Removal Feature", "false"):
             raise native.PlasticityError("the output buffer layer is outside the implemented path")
@@ -201,6 +212,8 @@ class Context:
         check(L.pa_sizes(self._h, ctypes.byref(ne), ctypes.byref(nq), ctypes.byref(nd), ctypes.byref(nz)))
         self.nelem, self.nqp, self.ndof, self.nnz = ne.value, nq.value, nd.value, nz.value
         self.n_slip = mat.n_slip
+        self.n_twin = mat.n_twin if mat.twinning else 0
+        self.n_sys = self.n_slip + self.n_twin  # systems of the return map (slip, then twin)
 
     def close(self):
         if self._h:
@@ -265,16 +278,18 @@ class Context:
         check(native.lib().pa_run_increment(self._h, ctypes.c_int(inc), ptr(row), ptr(its), ptr(res)))
         return row, its, res
 
-    STATE = {"T": (0, 9), "Fp": (1, 9), "Fe": (2, 9), "s": (3, None), "rotnew": (4, 3), "slip": (5, None),
-             "u": (6, None), "R": (7, None), "Fp_iter": (8, 9), "Fe_iter": (9, 9), "s_iter": (10, None),
-             "W": (11, None), "W_iter": (12, None)}
+    # name: (pa_get_state id, components per qp; "sys": slip + twin systems, "slip": slip systems,
+    # "twin": twin systems, None: per node)
+    STATE = {"T": (0, 9), "Fp": (1, 9), "Fe": (2, 9), "s": (3, "sys"), "rotnew": (4, 3), "slip": (5, "slip"),
+             "u": (6, None), "R": (7, None), "Fp_iter": (8, 9), "Fe_iter": (9, 9), "s_iter": (10, "sys"),
+             "W": (11, "sys"), "W_iter": (12, "sys"), "twin_fraction": (13, "twin"), "twinned": (14, 1), "rot": (15, 3)}
 
     def get_state(self, name):
         which, comps = self.STATE[name]
-        if name in ("u", "R"):
+        if comps is None:
             out = np.zeros(self.ndof)
         else:
-            comps = self.n_slip if comps is None else comps
+            comps = {"sys": self.n_sys, "slip": self.n_slip, "twin": self.n_twin}.get(comps, comps)
             out = np.zeros((self.nqp, comps))
         check(native.lib().pa_get_state(self._h, ctypes.c_int(which), ptr(out), ctypes.c_longlong(out.size)))
         return out
@@ -346,21 +361,34 @@ def pinv_solve(A, b, path=0):
     return x
 
 
-def make_material(m, n, q, C, s0, h0, ss, pw, tol, max_slip_search=1, max_solver_iter=1, C1=None, C2=None):
+def make_material(m, n, q, C, s0, h0, ss, pw, tol, max_slip_search=1, max_solver_iter=1, C1=None, C2=None, twin=None):
     """pa_material from the prm inputs; C1, C2: kinematic hardening (C_1 / C_2 Slip Kinematic
-    Hardening), off when None."""
+    Hardening), off when None; twin: dict(m, n, s0, h0, ss, pw, shear, threshold, saturation) of the
+    twin systems (Twin Directions / Normals File rows and the Twin parameters), with q then the
+    latent ratios over slip + twin systems; None = twinning off."""
     mat = PaMaterial()
     ns = len(m)
-    if ns > native.MAX_SLIP:
-        raise native.PlasticityError("at most %d slip systems" % native.MAX_SLIP)
+    ntw = len(twin["m"]) if twin else 0
+    if ns + ntw > native.MAX_SLIP:
+        raise native.PlasticityError("at most %d slip + twin systems" % native.MAX_SLIP)
     mat.n_slip = ns
     for a in range(ns):
         for j in range(3):
             mat.m[a][j] = m[a][j]
             mat.n[a][j] = n[a][j]
         mat.s0[a], mat.h0[a], mat.ss[a], mat.pw[a] = s0[a], h0[a], ss[a], pw[a]
-        for b in range(ns):
+    for k in range(ntw):
+        a = ns + k
+        for j in range(3):
+            mat.m[a][j] = twin["m"][k][j]
+            mat.n[a][j] = twin["n"][k][j]
+        mat.s0[a], mat.h0[a], mat.ss[a], mat.pw[a] = twin["s0"][k], twin["h0"][k], twin["ss"][k], twin["pw"][k]
+    for a in range(ns + ntw):
+        for b in range(ns + ntw):
             mat.q[a][b] = q[a][b]
+    if twin:
+        mat.twinning, mat.n_twin = 1, ntw
+        mat.twin_shear, mat.twin_threshold, mat.twin_saturation = twin["shear"], twin["threshold"], twin["saturation"]
     for i in range(6):
         for j in range(6):
             mat.C[i][j] = C[i][j]
@@ -418,13 +446,19 @@ class crystalPlasticity:
         ns = ui.numSlipSystems1
         m = read_vectors(ui.file(ui.slipDirectionsFile1), ns)
         n = read_vectors(ui.file(ui.slipNormalsFile1), ns)
-        q = read_table(ui.file(ui.latentHardeningRatioFileName1), ns)
+        twin = None
+        if ui.enableTwinning1:  # init.cc:125-175: twin systems after the slip systems
+            ntw = ui.numTwinSystems1
+            twin = dict(m=read_vectors(ui.file(ui.twinDirectionsFile1), ntw), n=read_vectors(ui.file(ui.twinNormalsFile1), ntw),
+                        s0=ui.initialSlipResistanceTwin1, h0=ui.initialHardeningModulusTwin1, ss=ui.saturationStressTwin1,
+                        pw=ui.powerLawExponentTwin1, shear=ui.twinShear, threshold=ui.twinThresholdFraction,
+                        saturation=ui.twinSaturationFactor)
+        q = read_table(ui.file(ui.latentHardeningRatioFileName1), ns + (len(twin["m"]) if twin else 0))
         kin = ui.enableKinematicHardening1
         self.material = make_material(m, n, q, ui.elasticStiffness1, ui.initialSlipResistance1, ui.initialHardeningModulus1,
                                       ui.saturationStress1, ui.powerLawExponent1, ui.modelStressTolerance,
                                       ui.modelMaxSlipSearchIterations, ui.modelMaxSolverIterations,
-                                      ui.C_1_slip1 if kin else None, ui.C_2_slip1 if kin else None)
-        self.material.twinning = int(ui.enableTwinning1)
+                                      ui.C_1_slip1 if kin else None, ui.C_2_slip1 if kin else None, twin)
         nel = [s * 2 ** ui.meshRefineFactor for s in ui.subdivisions]
         self.totalIncrements = int(round(ui.totalTime / ui.delT))  # ellipticBVP.cc:40
         bcs = []
@@ -523,10 +557,11 @@ def dealii_cell_order(subdivisions, refine):
 def write_quadrature_output(path, ctx, elem_grain, nel, span, cell_order):
     """QuadratureOutputs<increment>.csv (updateAfterIncrement.cc:343-608, writeQuadratureOutput.cc:4-49):
     one row per quadrature point, cells in deal.II order, values as "%8.5e ,".  Columns: grain id,
-    phase (1), JxW, twin flag (0), x, y, z, rotnew (3), Fe_conv, Fp_conv and Cauchy stress (each
-    11,22,33,12,13,21,23,31,32), slip fractions 0..83, twin fractions 0..5.  The reference reads
-    the slip and twin fraction vectors up to index 83 and 5 whatever their length (n_slip and 1
-    with twinning off), past their end; those entries are written as 0 here."""
+    phase (1), JxW, twin flag (twin_ouput: 1 once reoriented by twinning), x, y, z, rotnew (3),
+    Fe_conv, Fp_conv and Cauchy stress (each 11,22,33,12,13,21,23,31,32), slip fractions 0..83,
+    twin fractions 0..5.  The reference reads the slip and twin fraction vectors up to index 83 and
+    5 whatever their length (n_slip, and n_twin or 1), past their end; those entries are written
+    as 0 here."""
     nx, ny, nz = nel
     h = np.asarray(span, float) / np.array([nx, ny, nz])
     jxw = h.prod() / 8.0
@@ -536,6 +571,8 @@ def write_quadrature_output(path, ctx, elem_grain, nel, span, cell_order):
     Fp = ctx.get_state("Fp").reshape(nq, 9)
     T = ctx.get_state("T").reshape(nq, 9)
     slip = ctx.get_state("slip").reshape(nq, -1)
+    twinned = ctx.get_state("twinned").reshape(nq) if ctx.n_twin else np.zeros(nq)
+    twf = ctx.get_state("twin_fraction").reshape(nq, -1) if ctx.n_twin else np.zeros((nq, 0))
     g = np.array([0.5 - 0.5 / np.sqrt(3.0), 0.5 + 0.5 / np.sqrt(3.0)])
     mix = [0, 4, 8, 1, 2, 3, 5, 6, 7]
     e = np.asarray(cell_order, np.int64)
@@ -546,9 +583,10 @@ def write_quadrature_output(path, ctx, elem_grain, nel, span, cell_order):
         X = np.stack([(ex + g[q & 1]) * h[0], (ey + g[(q >> 1) & 1]) * h[1], (ez + g[(q >> 2) & 1]) * h[2]], axis=1)
         sl = np.zeros((len(e), 84))
         sl[:, :min(84, slip.shape[1])] = slip[Q, :84]
+        tw = np.zeros((len(e), 6))
+        tw[:, :min(6, twf.shape[1])] = twf[Q, :6]
         block = np.hstack([np.asarray(elem_grain, float)[e][:, None], np.ones((len(e), 1)), np.full((len(e), 1), jxw),
-                           np.zeros((len(e), 1)), X, rot[Q], Fe[Q][:, mix], Fp[Q][:, mix], T[Q][:, mix], sl,
-                           np.zeros((len(e), 6))])
+                           twinned[Q][:, None], X, rot[Q], Fe[Q][:, mix], Fp[Q][:, mix], T[Q][:, mix], sl, tw])
         cols.append(block)
     rows = np.stack(cols, axis=1).reshape(len(e) * 8, -1)  # cell-major, qp-minor
     with open(path, "w") as f:

@@ -38,6 +38,8 @@ class PaMaterial(ctypes.Structure):
         ("stress_tol", ctypes.c_double), ("max_slip_search", ctypes.c_int), ("max_solver_iter", ctypes.c_int),
         ("kinematic_hardening", ctypes.c_int), ("twinning", ctypes.c_int),
         ("C1", ctypes.c_double * MAX_SLIP), ("C2", ctypes.c_double * MAX_SLIP),
+        ("n_twin", ctypes.c_int), ("twin_shear", ctypes.c_double), ("twin_threshold", ctypes.c_double),
+        ("twin_saturation", ctypes.c_double),
     ]
 
 

@@ -109,6 +109,8 @@ class Material(ctypes.Structure):
         ("C1", ctypes.c_double * MAXT), ("C2", ctypes.c_double * MAXT),
         ("tol", ctypes.c_double), ("max_slip_search", ctypes.c_int),
         ("max_solver_iter", ctypes.c_int), ("pinv_mode", ctypes.c_int),
+        ("twinning", ctypes.c_int), ("twin_shear", ctypes.c_double),
+        ("twin_threshold", ctypes.c_double), ("twin_saturation", ctypes.c_double),
     ]
 
 
@@ -155,12 +157,17 @@ class Case:
         self.C1 = self.C2 = None  # kinematic hardening off
         self.tabular = None       # (times, [(face, dof, displacements)]) for tabular BCs
         self.vel_grad = None      # 3x3 velocity gradient L for velocity-gradient BCs
+        # twinning: dict(m, n (normalised), s0, h0, ss, pw per twin system, q (latent ratios over
+        # slip + twin systems), shear, threshold, saturation); None = the inert twin system
+        self.twin = None
 
     # --- reference defaults for the inert twin system (userInputParameters.cc:300-311)
     def material(self, pinv_mode=1):
         mat = Material()
         ns = self.n_slip
-        mat.n_slip, mat.n_twin, mat.nT = ns, 1, ns + 1
+        tw = self.twin
+        ntw = len(tw["m"]) if tw else 1
+        mat.n_slip, mat.n_twin, mat.nT = ns, ntw, ns + ntw
         for a in range(ns):
             for j in range(3):
                 mat.m[a][j] = self.m[a, j]
@@ -168,9 +175,22 @@ class Case:
             mat.s0[a], mat.h0[a], mat.ss[a], mat.pw[a] = self.s0[a], self.h0[a], self.ss[a], self.pw[a]
             for b in range(ns):
                 mat.q[a][b] = self.q[a, b]
-        # dummy twin: n=(1,0,0), m=(0,1,0), s0=10e14, h0=1, a=0, s_s=10e15 (init.cc:178-185)
-        mat.n[ns][0], mat.m[ns][1] = 1.0, 1.0
-        mat.s0[ns], mat.h0[ns], mat.pw[ns], mat.ss[ns] = 10e14, 1.0, 0.0, 10e15
+        if tw:  # twin systems after the slip systems (init.cc:125-175, 250-262)
+            for k in range(ntw):
+                a = ns + k
+                for j in range(3):
+                    mat.m[a][j] = tw["m"][k][j]
+                    mat.n[a][j] = tw["n"][k][j]
+                mat.s0[a], mat.h0[a], mat.ss[a], mat.pw[a] = tw["s0"][k], tw["h0"][k], tw["ss"][k], tw["pw"][k]
+            for a in range(ns + ntw):
+                for b in range(ns + ntw):
+                    mat.q[a][b] = tw["q"][a][b]
+            mat.twinning, mat.twin_shear = 1, tw["shear"]
+            mat.twin_threshold, mat.twin_saturation = tw["threshold"], tw["saturation"]
+        else:
+            # dummy twin: n=(1,0,0), m=(0,1,0), s0=10e14, h0=1, a=0, s_s=10e15 (init.cc:178-185)
+            mat.n[ns][0], mat.m[ns][1] = 1.0, 1.0
+            mat.s0[ns], mat.h0[ns], mat.pw[ns], mat.ss[ns] = 10e14, 1.0, 0.0, 10e15
         for i in range(6):
             for j in range(6):
                 mat.C[i][j] = self.C[i, j]
@@ -212,16 +232,22 @@ class Case:
         return pb
 
     @property
+    def n_twin(self):
+        return len(self.twin["m"]) if self.twin else 1
+
+    @property
     def nT(self):
-        return self.n_slip + 1
+        return self.n_slip + self.n_twin
 
 
 def _true(prm, key):
     return prm.get(key, "false").strip().lower() == "true"
 
 
-def load_reference_case(folder):
-    """A Case built from one of the reference application folders copied under tests/golden."""
+def load_reference_case(folder, twinning=False):
+    """A Case built from one of the reference application folders copied under tests/golden; with
+    twinning=True the prm's twin systems and parameters are read (userInputParameters.cc twin
+    block), otherwise the case runs with the inert twin system."""
     d = os.path.join(GOLDEN, folder)
     prm = parse_prm(os.path.join(d, "prm.prm"))
     ns = int(prm["Number of Slip Systems"])
@@ -260,6 +286,15 @@ def load_reference_case(folder):
                 max_newton=int(prm["Maximum non linear iterations"]),
                 lin_tol=float(prm["Relative linear solver tolerance"]),
                 lin_maxit=int(prm["Maximum linear solver iterations"]))
+    if twinning and _true(prm, "Twinning enabled"):
+        ntw = int(prm["Number of Twin Systems"])
+        case.twin = dict(m=read_vectors(os.path.join(d, prm["Twin Directions File"]), ntw),
+                         n=read_vectors(os.path.join(d, prm["Twin Normals File"]), ntw),
+                         s0=_floats(prm["Initial Slip Resistance Twin"]), h0=_floats(prm["Initial Hardening Modulus Twin"]),
+                         ss=_floats(prm["Saturation Stress Twin"]), pw=_floats(prm["Power Law Exponent Twin"]),
+                         q=read_latent(os.path.join(d, prm["Latent Hardening Ratio filename"]), ns + ntw),
+                         shear=float(prm["Characteristic Twin Shear"]), threshold=float(prm["Twin Threshold Fraction"]),
+                         saturation=float(prm["Twin Saturation Factor"]))
     if prm.get("Enable Kinematic Hardening", "false").strip().lower() == "true":  # userInputParameters.cc:274-279
         case.C1 = np.array(_floats(prm["C_1 Slip Kinematic Hardening"]))
         case.C2 = np.array(_floats(prm["C_2 Slip Kinematic Hardening"]))
@@ -328,7 +363,7 @@ def oracle_simulate(case, n_increments, pinv_mode=1, nthreads=1, want_state=Fals
     its = np.zeros(n_increments * case.max_newton, np.int32)
     res = np.zeros(n_increments * case.max_newton)
     nq = case.nx * case.ny * case.nz * 8
-    W = 27 + case.nT + 3 + case.n_slip
+    W = 27 + case.nT + 3 + case.n_slip + (4 + case.n_twin if case.twin else 0)
     state = np.zeros((nq, W)) if want_state else None
     u = np.zeros(3 * (case.nx + 1) * (case.ny + 1) * (case.nz + 1)) if want_state else None
     rod = np.ascontiguousarray(case.grain_rod, float)
