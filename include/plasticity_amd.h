@@ -177,7 +177,10 @@ int pa_sizes(pa_context *ctx, long long *nelem, long long *nqp, long long *ndof,
  * F, Fp_conv: 9/qp; s_conv: n_slip/qp.  Outputs P, T, Fe, Fp: 9/qp; dPdF: 81/qp (flag 1, may be
  * NULL for flag 0); s_out, slip_inc (x_beta_old): n_slip/qp; n_active: 1/qp.  With kinematic
  * hardening, W_conv (n_slip/qp, NULL = zero) is the backstress W_kh_conv and W_out (may be NULL)
- * receives W_kh_tau; both are ignored otherwise. */
+ * receives W_kh_tau; both are ignored otherwise.  With twinning the per-system arrays (s_conv,
+ * s_out, slip_inc, W) have n_slip + n_twin entries per qp (slip systems, then twin systems) and
+ * each qp uses its grain's own orientation (no reorientation test: that belongs to the Newton
+ * step's history). */
 int pa_qp_update(pa_context *ctx, long long nqp, const double *F, const double *Fp_conv, const double *s_conv,
                  const unsigned *grain, int flag, double *P, double *T, double *dPdF, double *Fe, double *Fp,
                  double *s_out, double *slip_inc, int *n_active, const double *W_conv, double *W_out);

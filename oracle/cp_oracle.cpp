@@ -816,9 +816,10 @@ void twin_rotation(const or_material *mat, int k, const double rot[3], double ou
 }
 // calculatePlasticity.cc:962-1019 for one qp (see or_twin_event)
 int twin_event(const or_material *mat, double F_r, double F_e, const double *tw_iter, int twin_conv, const double rot_conv[3],
-               double rot_iter[3], int *twin_iter, double *s_iter) {
+               double rot_iter[3], int *twin_iter, double *s_iter, bool multiphase = false) {
   if (!mat->twinning) return -1;
-  double F_T = F_r > 0 ? mat->twin_threshold + (mat->twin_saturation * F_e / F_r) : mat->twin_threshold;  // :970-980
+  // :970-980; with several phases F_T is the phase's threshold fraction alone
+  double F_T = (!multiphase && F_r > 0) ? mat->twin_threshold + (mat->twin_saturation * F_e / F_r) : mat->twin_threshold;
   if (F_T > 1.0) F_T = 1.0;                                                                             // :983-985
   int pos = 0;                                                                                          // :986-990
   for (int i = 1; i < mat->n_twin; ++i) if (tw_iter[i] > tw_iter[pos]) pos = i;
@@ -1354,30 +1355,48 @@ int or_first_newton(const or_material *mat, const or_problem *pb, const unsigned
   return rc;
 }
 
-int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *elem_grain, const double *grain_rod,
-                int n_grain_rows, double *rows, int *lin_iters, double *res_norms, double *state_out, double *u_out) {
+// or_simulate over nph phases: grain_phase (1-based, per grain row; null = one phase) selects the
+// material of each grain (multiphaseInit.cc:68-361).  Per-qp state arrays have the largest phase's
+// size; each qp uses its phase's leading part.
+int simulate(const or_material *mats, int nph, const int *grain_phase, const or_problem *pb, const unsigned *elem_grain,
+             const double *grain_rod, int n_grain_rows, double *rows, int *lin_iters, double *res_norms, double *state_out,
+             double *u_out) {
 #ifdef _OPENMP
   if (pb->nthreads > 0) omp_set_num_threads(pb->nthreads);
 #endif
   const Mesh m = make_mesh(pb);
   const Shape sh = make_shape(m.h);
-  const int nT = mat->nT, ns = mat->n_slip;
+  const bool multiphase = grain_phase != nullptr;
+  int nT = 0, ns = 0, ntw = 0;
+  bool twinning = false;
+  for (int k = 0; k < nph; ++k) {
+    nT = std::max(nT, mats[k].nT);
+    ns = std::max(ns, mats[k].n_slip);
+    if (mats[k].twinning) { twinning = true; ntw = std::max(ntw, mats[k].n_twin); }
+  }
   const long long nq = m.nelem * 8;
   g_phase_ns[0] = g_phase_ns[1] = g_phase_ns[2] = 0;
-  for (long long e = 0; e < m.nelem; ++e)
+  for (long long e = 0; e < m.nelem; ++e) {
     if ((int)elem_grain[e] >= n_grain_rows) return -1;
+    if (multiphase && (grain_phase[elem_grain[e]] < 1 || grain_phase[elem_grain[e]] > nph)) return -1;
+  }
+  auto gmat = [&](int g) -> const or_material * { return mats + (multiphase ? grain_phase[g] - 1 : 0); };
+  auto emat = [&](long long e) -> const or_material * { return gmat((int)elem_grain[e]); };
   // per-grain constants, identical to what calculatePlasticity rebuilds on every call
   std::vector<Grain> grains(n_grain_rows);
   std::vector<char> used(n_grain_rows, 0);
   for (long long e = 0; e < m.nelem; ++e) used[elem_grain[e]] = 1;
-  for (int g = 0; g < n_grain_rows; ++g) if (used[g]) grain_setup(mat, grain_rod + 3 * g, grains[g]);
+  for (int g = 0; g < n_grain_rows; ++g) if (used[g]) grain_setup(gmat(g), grain_rod + 3 * g, grains[g]);
 
   QPState st;  // crystalPlasticity::init (CP init.cc:302-330)
   st.Fp_conv.assign(9 * nq, 0.0); st.Fe_conv.assign(9 * nq, 0.0);
   for (long long q = 0; q < nq; ++q) for (int i = 0; i < 3; ++i) { st.Fp_conv[9 * q + 4 * i] = 1.0; st.Fe_conv[9 * q + 4 * i] = 1.0; }
   st.Fp_iter = st.Fp_conv; st.Fe_iter = st.Fe_conv;
-  st.s_conv.resize(nT * nq);
-  for (long long q = 0; q < nq; ++q) for (int a = 0; a < nT; ++a) st.s_conv[nT * q + a] = mat->s0[a];
+  st.s_conv.assign(nT * nq, 0.0);
+  for (long long q = 0; q < nq; ++q) {
+    const or_material *mq = emat(q / 8);
+    for (int a = 0; a < mq->nT; ++a) st.s_conv[nT * q + a] = mq->s0[a];
+  }
   st.s_iter = st.s_conv;
   st.W_conv.assign(nT * nq, 0.0); st.W_iter = st.W_conv;
   st.slip_conv.assign(ns * nq, 0.0); st.slip_iter = st.slip_conv;
@@ -1385,9 +1404,10 @@ int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *el
   st.rotnew_conv.resize(3 * nq);
   for (long long e = 0; e < m.nelem; ++e)
     for (int q = 0; q < 8; ++q) for (int i = 0; i < 3; ++i) st.rotnew_conv[3 * (8 * e + q) + i] = grain_rod[3 * elem_grain[e] + i];
-  const int ntw = mat->n_twin;
-  const bool twinning = mat->twinning != 0;
   double F_r = 0.0, F_e = 0.0;  // volume-averaged twin fraction / reoriented volume of the last increment
+  // the averages sum the first numTwinSystems1 twin and numSlipSystems1 slip fractions of every qp,
+  // phase 1's counts whatever the qp's phase (updateAfterIncrement.cc:222-236)
+  const int nTw1 = mats[0].twinning ? mats[0].n_twin : 1, nSl1 = mats[0].n_slip;
   std::vector<Grain> twin_grains;
   if (twinning) {
     st.rot_conv = st.rotnew_conv; st.rot_iter = st.rotnew_conv; st.rotnew_iter = st.rotnew_conv;
@@ -1402,8 +1422,8 @@ int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *el
   // reorientation by twinning after a qp update (calculatePlasticity.cc:962-1019)
   auto twin_step = [&](long long Q) {
     if (!twinning) return;
-    if (twin_event(mat, F_r, F_e, &st.tw_iter[(size_t)ntw * Q], st.twin_conv[Q], &st.rot_conv[3 * Q], &st.rot_iter[3 * Q],
-                   &st.twin_iter[Q], &st.s_iter[(size_t)nT * Q]) >= 0)
+    if (twin_event(emat(Q / 8), F_r, F_e, &st.tw_iter[(size_t)ntw * Q], st.twin_conv[Q], &st.rot_conv[3 * Q], &st.rot_iter[3 * Q],
+                   &st.twin_iter[Q], &st.s_iter[(size_t)nT * Q], multiphase) >= 0)
       for (int i = 0; i < 3; ++i) st.rotnew_iter[3 * Q + i] = st.rot_iter[3 * Q + i];
   };
 
@@ -1435,7 +1455,7 @@ int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *el
           const M3 F = def_grad(sh, q, ue);
           double Fv[9], P[9], T[9], D[81];
           store3(F, Fv);
-          qp_update(mat, g, Fv, &st.Fp_conv[9 * Q], &st.s_conv[nT * Q], &st.W_conv[nT * Q], &st.slip_conv[ns * Q], 1, P, T, D,
+          qp_update(emat(e), g, Fv, &st.Fp_conv[9 * Q], &st.s_conv[nT * Q], &st.W_conv[nT * Q], &st.slip_conv[ns * Q], 1, P, T, D,
                     &st.Fe_iter[9 * Q], &st.Fp_iter[9 * Q], &st.s_iter[nT * Q], &st.W_iter[nT * Q], &st.slip_iter[ns * Q], nullptr,
                     nullptr, twinning ? &st.tw_conv[(size_t)ntw * Q] : nullptr, twinning ? &st.tw_iter[(size_t)ntw * Q] : nullptr);
           twin_step(Q);
@@ -1518,12 +1538,12 @@ int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *el
         const M3 F = def_grad(sh, q, ue);
         double Fv[9], P[9], D[81];
         store3(F, Fv);
-        qp_update(mat, g, Fv, &st.Fp_conv[9 * Q], &st.s_conv[nT * Q], &st.W_conv[nT * Q], &st.slip_conv[ns * Q], 0, P, &st.T[9 * Q], D,
+        qp_update(emat(e), g, Fv, &st.Fp_conv[9 * Q], &st.s_conv[nT * Q], &st.W_conv[nT * Q], &st.slip_conv[ns * Q], 0, P, &st.T[9 * Q], D,
                   &st.Fe_iter[9 * Q], &st.Fp_iter[9 * Q], &st.s_iter[nT * Q], &st.W_iter[nT * Q], &st.slip_iter[ns * Q], nullptr,
                   nullptr, twinning ? &st.tw_conv[(size_t)ntw * Q] : nullptr, twinning ? &st.tw_iter[(size_t)ntw * Q] : nullptr);
         twin_step(Q);
-        if (twinning) {                                   // updateAfterIncrement.cc:176, 222-228
-          for (int i = 0; i < ntw; ++i) Fr_sum += st.tw_iter[(size_t)ntw * Q + i] * sh.JxW;
+        if (twinning) {  // updateAfterIncrement.cc:176, 222-228: the first numTwinSystems1 fractions (phase 1's count, 1 without its twinning)
+          for (int i = 0; i < std::min(nTw1, ntw); ++i) Fr_sum += st.tw_iter[(size_t)ntw * Q + i] * sh.JxW;
           Fe_sum += st.twin_iter[Q] * sh.JxW;
         }
         const M3 C = tmul(F, F);
@@ -1534,7 +1554,7 @@ int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *el
             sE[i][j] += E * sh.JxW;
           }
         vol_total += sh.JxW;
-        for (int a = 0; a < ns; ++a) Fs += st.slip_iter[ns * Q + a] * sh.JxW;
+        for (int a = 0; a < nSl1; ++a) Fs += st.slip_iter[ns * Q + a] * sh.JxW;  // :234-236: phase 1's slip count
       }
     }
     // reorient (CP reorient.cc) then the _conv <- _iter commit (:287-305); with twinning the output
@@ -1549,7 +1569,7 @@ int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *el
         if (st.twin_iter[Q] && !st.twin_conv[Q]) {        // newly twinned: its own orientation tables
           st.tgrain[Q] = (int)twin_grains.size();
           twin_grains.emplace_back();
-          grain_setup(mat, &st.rot_iter[3 * Q], twin_grains.back());
+          grain_setup(emat(Q / 8), &st.rot_iter[3 * Q], twin_grains.back());
         }
       st.rot_conv = st.rot_iter;
       st.twin_conv = st.twin_iter;
@@ -1583,6 +1603,19 @@ int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *el
   }
   if (u_out) std::copy(x.begin(), x.end(), u_out);
   return 0;
+}
+
+int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *elem_grain, const double *grain_rod,
+                int n_grain_rows, double *rows, int *lin_iters, double *res_norms, double *state_out, double *u_out) {
+  return simulate(mat, 1, nullptr, pb, elem_grain, grain_rod, n_grain_rows, rows, lin_iters, res_norms, state_out, u_out);
+}
+
+int or_simulate_phases(const or_material *mats, int n_phases, const int *grain_phase, const or_problem *pb,
+                       const unsigned *elem_grain, const double *grain_rod, int n_grain_rows, double *rows, int *lin_iters,
+                       double *res_norms, double *state_out, double *u_out) {
+  if (n_phases < 1 || n_phases > 4 || !grain_phase) return -1;
+  return simulate(mats, n_phases, grain_phase, pb, elem_grain, grain_rod, n_grain_rows, rows, lin_iters, res_norms, state_out,
+                  u_out);
 }
 
 }  // extern "C"

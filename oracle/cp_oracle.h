@@ -142,6 +142,16 @@ int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *el
                 double *rows, int *lin_iters, double *res_norms, double *state_out,
                 double *u_out);
 
+/* or_simulate with several phases (multiphaseInit.cc:68-361, Enable Multiphase): mats[k] is phase
+ * k + 1's material (n_phases <= 4), grain_phase[g] (1-based) the phase of grain row g.  Per-qp state
+ * has the largest phase's size (state_out rows as or_simulate with those sizes); TwinRealVF and
+ * SlipTotal sum the first numTwinSystems1 / numSlipSystems1 fractions of every qp (phase 1's counts,
+ * numTwinSystems1 = 1 without its twinning; updateAfterIncrement.cc:222-236) and F_T is each
+ * phase's threshold fraction alone (calculatePlasticity.cc:969-980). */
+int or_simulate_phases(const or_material *mats, int n_phases, const int *grain_phase, const or_problem *pb,
+                       const unsigned *elem_grain, const double *grain_rod, int n_grain_rows, double *rows, int *lin_iters,
+                       double *res_norms, double *state_out, double *u_out);
+
 /* Increment 0, Newton iteration 0 only: the condensed CSR values (pattern of or_sparsity),
  * the condensed residual and the Newton correction du (after constraints.distribute). */
 int or_first_newton(const or_material *mat, const or_problem *pb, const unsigned *elem_grain,

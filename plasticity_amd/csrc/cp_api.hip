@@ -2592,7 +2592,6 @@ int pa_qp_update(pa_context *c, long long n, const double *F, const double *Fp_c
     return fail(PA_ERR_ARG, "null argument");
   if (flag && !dPdF) return fail(PA_ERR_ARG, "dPdF required for flag 1");
   if (!c->grains_set) return fail(PA_ERR_STATE, "pa_set_grains has not been called");
-  if (c->ntw) return fail(PA_ERR_UNSUPPORTED, "pa_qp_update: the batch form has no twin systems (the Newton step has)");
   for (long long i = 0; i < n; ++i)
     if ((int)grain[i] >= c->n_grain_rows) return fail(PA_ERR_ARG, "grain index out of range");
   if (n == 0) return 0;
@@ -2609,8 +2608,13 @@ int pa_qp_update(pa_context *c, long long n, const double *F, const double *Fp_c
             (!c->kin || al((void **)&dW, bs));
   int rc = 0;
   if (!ok) rc = fail(PA_ERR_HIP, "hipMalloc");
+  // orientation-table rows: with twinning each grain has 1 + n_twin rows; a batch call uses the
+  // grain's own orientation (row 0 of its block)
+  std::vector<unsigned> rows(grain, grain + n);
+  if (c->ntw)
+    for (auto &r : rows) r *= (unsigned)(1 + c->ntw);
   if (!rc && (hipMemcpy(dF, F, b9, hipMemcpyHostToDevice) != hipSuccess || hipMemcpy(dFp, Fp_conv, b9, hipMemcpyHostToDevice) != hipSuccess ||
-              hipMemcpy(ds, s_conv, bs, hipMemcpyHostToDevice) != hipSuccess || hipMemcpy(dg, grain, n * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess))
+              hipMemcpy(ds, s_conv, bs, hipMemcpyHostToDevice) != hipSuccess || hipMemcpy(dg, rows.data(), n * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess))
     rc = fail(PA_ERR_HIP, "copy inputs");
   if (!rc && c->kin &&
       (W_conv ? hipMemcpy(dW, W_conv, bs, hipMemcpyHostToDevice) : hipMemset(dW, 0, bs)) != hipSuccess)
@@ -2618,7 +2622,8 @@ int pa_qp_update(pa_context *c, long long n, const double *F, const double *Fp_c
   if (!rc) {
     const int NS = ns_template(ns);
     const int msi = c->mat.max_solver_iter;
-    const BatchArgs b{n, dF, dFp, ds, dg, c->d_gtab, c->gstride, c->d_mat, ns, msi, flag, dP, dT, dD, dFe, dFpo, dso, dx, dna, dW};
+    BatchArgs b{n, dF, dFp, ds, dg, c->d_gtab, c->gstride, c->d_mat, ns, msi, flag, dP, dT, dD, dFe, dFpo, dso, dx, dna, dW};
+    if (c->ntw) b.nslip = c->nslip;
     if (NS == 12) launch_qp_batch<12>(c->stream, c->kin, b);
     else if (NS == 24) launch_qp_batch<24>(c->stream, c->kin, b);
     else launch_qp_batch<48>(c->stream, c->kin, b);

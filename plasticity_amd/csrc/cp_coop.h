@@ -286,7 +286,7 @@ __device__ __forceinline__ int qp_update(const Group<G> &g, double *__restrict__
                                          const double *__restrict__ gt, int ns, int max_solver_iter, const double *F,
                                          const double *Fp_t, double s_a, double W_a, int flag, double *P, double *T,
                                          double *Fe_out, double *Fp_out, DT *D, long long dstride, double dscale, bool &plastic,
-                                         unsigned *nsing) {
+                                         unsigned *nsing, int nslip = 1 << 30) {
   using namespace cp;
   typedef MatLayout<NS> ML;
   typedef GrainLayout<NS> GL;
@@ -316,8 +316,9 @@ __device__ __forceinline__ int qp_update(const Group<G> &g, double *__restrict__
   m3_mul(CE, Tst, tmp);
   bool act = false;
   if (sys) {
-    const double r = m3_ddot(tmp, Sg + 9 * l);
+    double r = m3_ddot(tmp, Sg + 9 * l);
     const double w = KIN ? W_a : 0.0;
+    if (l >= nslip && (r - w) < 0) r = w;  // twin systems shear one way only (:213-216)
     const double bb = fabs(r - w) - s_a;
     tau[l] = r;
     bv[l] = bb;
@@ -418,6 +419,8 @@ __device__ __forceinline__ int qp_update(const Group<G> &g, double *__restrict__
       m3_mul(Ce, TstTau, tmp);
       gsync();  // every lane has read the backstress before it changes
       if (sys) {  // :380-440
+        double ta = m3_ddot(tmp, Sg + 9 * l);
+        if (l >= nslip && (tau[l] - wtau(l)) < 0) ta = wtau(l);  // :390-393, the backstress before :431
         double h1 = 0;
         for (int c = 0; c < ns; ++c) h1 += mat[ML::q + l * NS + c] * hv[c] * xbv[c];
         double sa = sv[l] + h1;
@@ -430,7 +433,7 @@ __device__ __forceinline__ int qp_update(const Group<G> &g, double *__restrict__
           wl += (((tau[l] - wl) > 0) ? (c1 - c2 * wl) : (-c1 - c2 * wl)) * xbv[l];
           Wv[l] = wl;
         }
-        bv[l] = fabs(m3_ddot(tmp, Sg + 9 * l) - wl) - sa;
+        bv[l] = fabs(ta - wl) - sa;
       }
       gsync();
     }

@@ -277,6 +277,7 @@ struct BatchArgs {
   double *P, *T, *D, *Fe, *Fp, *so, *xo;
   int *na;
   double *Wio;
+  int nslip = 1 << 30;  // systems from nslip on are twin systems (one-sided); default: none
 };
 
 template <int NS, bool KIN>
@@ -300,7 +301,7 @@ __device__ __forceinline__ void qp_batch_body(const BatchArgs &b, double *lds) {
   const double *gt = b.gtab + (long long)b.grain[i] * b.gstride;
   bool plastic = false;
   const int r = coop::qp_update<NS, G, double, KIN>(g, L, b.mat, gt, ns, b.msi, f, fp, s_a, W_a, b.flag, p, t, fe, fpo,
-                                                    b.flag ? b.D + 81 * i : nullptr, 1, 1.0, plastic, nullptr);
+                                                    b.flag ? b.D + 81 * i : nullptr, 1, 1.0, plastic, nullptr, b.nslip);
   if (l < 9) {
     b.P[9 * i + l] = coop::pick<9>(p, l);
     b.T[9 * i + l] = coop::pick<9>(t, l);

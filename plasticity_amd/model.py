@@ -329,19 +329,21 @@ class Context:
         return c
 
     def qp_update(self, F, Fp, s, grain, flag=1, W=None):
-        """calculatePlasticity over a batch; W: backstress W_kh_conv (kinematic hardening only)."""
+        """calculatePlasticity over a batch; W: backstress W_kh_conv (kinematic hardening only).  The
+        per-system arrays have n_sys = n_slip (+ n_twin with twinning) entries per qp."""
         F = f64(F).reshape(-1, 9)
         n = len(F)
+        ns = self.n_sys
         Fp = f64(Fp).reshape(n, 9)
-        s = f64(s).reshape(n, self.n_slip)
+        s = f64(s).reshape(n, ns)
         grain = np.ascontiguousarray(grain, dtype=np.uint32).reshape(n)
         out = {k: np.zeros((n, 9)) for k in ("P", "T", "Fe", "Fp")}
         out["dPdF"] = np.zeros((n, 81)) if flag else None
-        out["s"] = np.zeros((n, self.n_slip))
-        out["slip_inc"] = np.zeros((n, self.n_slip))
+        out["s"] = np.zeros((n, ns))
+        out["slip_inc"] = np.zeros((n, ns))
         out["n_active"] = np.zeros(n, dtype=np.int32)
-        out["W"] = np.zeros((n, self.n_slip))
-        Win = None if W is None else f64(W).reshape(n, self.n_slip)
+        out["W"] = np.zeros((n, ns))
+        Win = None if W is None else f64(W).reshape(n, ns)
         check(native.lib().pa_qp_update(self._h, ctypes.c_longlong(n), ptr(F), ptr(Fp), ptr(s), ptr(grain), ctypes.c_int(flag),
                                         ptr(out["P"]), ptr(out["T"]), ptr(out["dPdF"]), ptr(out["Fe"]), ptr(out["Fp"]),
                                         ptr(out["s"]), ptr(out["slip_inc"]), ptr(out["n_active"]), ptr(Win), ptr(out["W"])))

@@ -115,6 +115,10 @@ std::vector<T> dup(const T *p, size_t n) { return std::vector<T>(p, p + n); }
 extern "C" {
 
 // pa_qp_update's batch on the CPU: same arguments (W may be null without kinematic hardening).
+// systems from g_nslip on are twin systems (one-sided shear); set by emu_set_nslip, default none
+int g_nslip = 1 << 30;
+void emu_set_nslip(int nslip) { g_nslip = nslip; }
+
 int emu_qp_batch(const pa_material *mat, const double *rod, int nrows, long long n, const double *F, const double *Fp,
                  const double *s, const unsigned *grain, int flag, double *P, double *T, double *D, double *Fe, double *Fpo,
                  double *so, double *xo, int *na, double *W) {
@@ -130,6 +134,7 @@ int emu_qp_batch(const pa_material *mat, const double *rod, int nrows, long long
   BatchArgs b{n, vF.data(), vFp.data(), vs.data(), vg.data(), tab.data(), gstride, mb.data(), ns, mat->max_solver_iter, flag,
               vP.data(), vT.data(), flag ? vD.data() : nullptr, vFe.data(), vFpo.data(), vso.data(), vxo.data(), vna.data(),
               kin ? vW.data() : nullptr};
+  b.nslip = g_nslip;
   if (NS == 12) { if (kin) batch_t<12, true>(b); else batch_t<12, false>(b); }
   else if (NS == 24) { if (kin) batch_t<24, true>(b); else batch_t<24, false>(b); }
   else { if (kin) batch_t<48, true>(b); else batch_t<48, false>(b); }
@@ -160,7 +165,7 @@ int emu_qp_lane(const pa_material *mat, const double *rod, int nrows, long long 
       constexpr bool K = decltype(kinc)::value;
       return cp::qp_update<NSc, double, K>(mb.data(), gt, ns, mat->max_solver_iter, F + 9 * i, Fp + 9 * i, s_, x_, flag, P + 9 * i,
                                            T + 9 * i, Fe + 9 * i, Fpo + 9 * i, flag ? D + 81 * i : nullptr, 1, nullptr, 1.0,
-                                           cp::SvdInLane<NSc>(), W_);
+                                           cp::SvdInLane<NSc>(), W_, nullptr, 1, nullptr, g_nslip);
     };
     using I12 = std::integral_constant<int, 12>;
     using I24 = std::integral_constant<int, 24>;

@@ -254,7 +254,7 @@ def check_golden(folder, k):
     print(json.dumps(out), flush=True)
 
 
-if __name__ == "__main__" and sys.argv[1] != "lane":
+if __name__ == "__main__" and sys.argv[1] not in ("lane", "twin"):
     if sys.argv[1] == "golden":
         check_golden(sys.argv[2], int(sys.argv[3]))
         sys.exit(0)
@@ -306,3 +306,46 @@ def check_lane(case, strain, n, seed):
 
 if __name__ == "__main__" and sys.argv[1] == "lane":
     check_lane(make_case(sys.argv[2]), float(sys.argv[3]), int(sys.argv[4]), int(sys.argv[5]))
+
+
+def check_twin(n, seed):
+    """hcp/simpleTension's 18 slip + 6 twin systems: the cooperative form against the one-lane
+    routine (bit for bit) and against the oracle's twin return map, on the random states of the GPU
+    test (tests/test_gpu_twinning.py::test_qp_update_with_twin_systems_matches_oracle)."""
+    L = lib()
+    case = refcase.load_reference_case("hcp_simpleTension", twinning=True)
+    tw = case.twin
+    ns = case.n_slip
+    # the emulator's material carries all 24 systems; the twin ones are marked by emu_set_nslip
+    sysc = refcase.Case(np.vstack([case.m, tw["m"]]), np.vstack([case.n, tw["n"]]), tw["q"], case.C,
+                        np.concatenate([case.s0, tw["s0"]]), np.concatenate([case.h0, tw["h0"]]),
+                        np.concatenate([case.ss, tw["ss"]]), np.concatenate([case.pw, tw["pw"]]), tol=case.tol)
+    mat = material(sysc)
+    L.emu_set_nslip(ctypes.c_int(ns))
+    rng = np.random.default_rng(seed)
+    rod = rng.uniform(-0.6, 0.6, size=(8, 3))
+    F = np.eye(3) + rng.normal(scale=2.5e-3, size=(n, 3, 3))
+    Fp = np.eye(3) + rng.normal(scale=4e-4, size=(n, 3, 3))
+    Fp /= np.cbrt(np.linalg.det(Fp))[:, None, None]
+    s0 = np.concatenate([case.s0, tw["s0"]])
+    s = np.array([rng.uniform(1.0, 1.3) * s0 for _ in range(n)])
+    grain = rng.integers(0, 8, size=n).astype(np.uint32)
+    out = {"check": "twin", "n": n}
+    for flag in (0, 1):
+        a = run_form(L, "emu_qp_batch", sysc, mat, rod, F, Fp, s, grain, flag, None)
+        b = run_form(L, "emu_qp_lane", sysc, mat, rod, F, Fp, s, grain, flag, None)
+        bad_lane, bad_coop, twins = [], [], 0
+        for i in range(n):
+            o = refcase.oracle_qp_twin(case, rod[grain[i]], F[i], Fp[i], s[i], flag)
+            twins += int((o["x"][ns:] > 0).any())
+            if rel(b["P"][i], o["P"].ravel()) > 1e-10 or rel(b["s"][i], o["s"]) > 1e-10:
+                bad_lane.append(i)
+            if rel(a["P"][i], o["P"].ravel()) > 1e-10 or rel(a["s"][i], o["s"]) > 1e-10:
+                bad_coop.append(i)
+        out["flag%d" % flag] = {"coop_vs_lane": {k: rel(a[k], b[k]) for k in ("P", "s", "x")}, "twin_qps": twins,
+                                "lane_vs_oracle_bad": bad_lane[:10], "coop_vs_oracle_bad": bad_coop[:10]}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__" and sys.argv[1] == "twin":
+    check_twin(int(sys.argv[2]), int(sys.argv[3]))

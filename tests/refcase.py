@@ -74,19 +74,28 @@ def read_voxels(path, header, nv):
     return vox
 
 
-def read_orientations(path):
-    table = {}
+def read_orientations(path, with_phase=False):
+    """id -> Rodrigues rows; with_phase: also the fifth column, the grain's phase (multiphase
+    orientation files, crystalOrientationsIO.cc:30-50)."""
+    table, phase = {}, {}
     with open(path) as f:
         f.readline()
         for line in f:
             vals = line.split()
             if len(vals) >= 4:
                 table[int(vals[0])] = [float(v) for v in vals[1:4]]
+                if with_phase and len(vals) >= 5:
+                    phase[int(vals[0])] = int(float(vals[4]))
     n = max(table) + 1
     rod = np.zeros((n, 3))
     for k, v in table.items():
         rod[k] = v
-    return rod
+    if not with_phase:
+        return rod
+    ph = np.ones(n, np.int32)
+    for k, v in phase.items():
+        ph[k] = v
+    return rod, ph
 
 
 def read_bcs(path, header, count):
@@ -244,6 +253,33 @@ def _true(prm, key):
     return prm.get(key, "false").strip().lower() == "true"
 
 
+def _phase_case(prm, d, k, base):
+    """Phase k's material (keys with the suffix " k" for k > 1, userInputParameters.cc:286-545) as a
+    Case sharing the base case's mesh and loading."""
+    sfx = "" if k == 1 else " %d" % k
+    ns = int(prm["Number of Slip Systems" + sfx])
+    C = np.array([_floats(prm["Elastic Stiffness%s row %d" % (sfx, i + 1)]) for i in range(6)])
+    m = read_vectors(os.path.join(d, prm["Slip Directions File" + sfx]), ns)
+    n = read_vectors(os.path.join(d, prm["Slip Normals File" + sfx]), ns)
+    tw = _true(prm, "Twinning enabled" + sfx)
+    ntw = int(prm["Number of Twin Systems" + sfx]) if tw else 0
+    q = read_latent(os.path.join(d, prm["Latent Hardening Ratio filename" + sfx]), ns + ntw)
+    c = Case(m, n, q[:ns, :ns], C, _floats(prm["Initial Slip Resistance" + sfx]), _floats(prm["Initial Hardening Modulus" + sfx]),
+             _floats(prm["Saturation Stress" + sfx]), _floats(prm["Power Law Exponent" + sfx]), tol=base.tol,
+             max_slip_search=base.max_slip_search, max_solver_iter=base.max_solver_iter)
+    if tw:
+        c.twin = dict(m=read_vectors(os.path.join(d, prm["Twin Directions File" + sfx]), ntw),
+                      n=read_vectors(os.path.join(d, prm["Twin Normals File" + sfx]), ntw),
+                      s0=_floats(prm["Initial Slip Resistance Twin" + sfx]), h0=_floats(prm["Initial Hardening Modulus Twin" + sfx]),
+                      ss=_floats(prm["Saturation Stress Twin" + sfx]), pw=_floats(prm["Power Law Exponent Twin" + sfx]), q=q,
+                      shear=float(prm["Characteristic Twin Shear" + sfx]), threshold=float(prm["Twin Threshold Fraction" + sfx]),
+                      saturation=float(prm.get("Twin Saturation Factor" + sfx, "0")))
+    if _true(prm, "Enable Kinematic Hardening" + sfx):
+        c.C1 = np.array(_floats(prm["C_1 Slip Kinematic Hardening" + sfx]))
+        c.C2 = np.array(_floats(prm["C_2 Slip Kinematic Hardening" + sfx]))
+    return c
+
+
 def load_reference_case(folder, twinning=False):
     """A Case built from one of the reference application folders copied under tests/golden; with
     twinning=True the prm's twin systems and parameters are read (userInputParameters.cc twin
@@ -305,6 +341,10 @@ def load_reference_case(folder, twinning=False):
         case.vel_grad = np.array([_floats(prm["Velocity gradient row %d" % (i + 1)]) for i in range(3)])
     case.vox, case.nv = vox, nv
     case.grain_rod = rod
+    case.phases = None  # multiphase: one Case per phase (materials), grain_phase[grain row] in 1..n
+    if _true(prm, "Enable Multiphase"):
+        case.phases = [_phase_case(prm, d, k, case) for k in range(1, int(prm["Number of Phases"]) + 1)]
+        case.grain_rod, case.grain_phase = read_orientations(os.path.join(d, prm["Orientations file name"]), with_phase=True)
     case.elem_grain = oracle_element_grains(case)
     golden = os.path.join(d, "correct_stressstrain.txt")
     case.golden = np.loadtxt(golden, skiprows=1) if os.path.exists(golden) else None
@@ -363,14 +403,25 @@ def oracle_simulate(case, n_increments, pinv_mode=1, nthreads=1, want_state=Fals
     its = np.zeros(n_increments * case.max_newton, np.int32)
     res = np.zeros(n_increments * case.max_newton)
     nq = case.nx * case.ny * case.nz * 8
-    W = 27 + case.nT + 3 + case.n_slip + (4 + case.n_twin if case.twin else 0)
+    if getattr(case, "phases", None):  # the largest phase's sizes
+        tws = [ph.n_twin for ph in case.phases if ph.twin]
+        W = 27 + max(ph.nT for ph in case.phases) + 3 + max(ph.n_slip for ph in case.phases) + (4 + max(tws) if tws else 0)
+    else:
+        W = 27 + case.nT + 3 + case.n_slip + (4 + case.n_twin if case.twin else 0)
     state = np.zeros((nq, W)) if want_state else None
     u = np.zeros(3 * (case.nx + 1) * (case.ny + 1) * (case.nz + 1)) if want_state else None
     rod = np.ascontiguousarray(case.grain_rod, float)
     eg = np.ascontiguousarray(case.elem_grain, np.uint32)
-    rc = lib.or_simulate(ctypes.byref(mat), ctypes.byref(pb), ptr(eg), ptr(rod), ctypes.c_int(len(rod)),
-                         ptr(rows), ptr(its), ptr(res), ptr(state) if want_state else None,
-                         ptr(u) if want_state else None)
+    if getattr(case, "phases", None):
+        mats = (Material * len(case.phases))(*[ph.material(pinv_mode) for ph in case.phases])
+        gp = np.ascontiguousarray(case.grain_phase, np.int32)
+        rc = lib.or_simulate_phases(mats, ctypes.c_int(len(case.phases)), ptr(gp), ctypes.byref(pb), ptr(eg), ptr(rod),
+                                    ctypes.c_int(len(rod)), ptr(rows), ptr(its), ptr(res), ptr(state) if want_state else None,
+                                    ptr(u) if want_state else None)
+    else:
+        rc = lib.or_simulate(ctypes.byref(mat), ctypes.byref(pb), ptr(eg), ptr(rod), ctypes.c_int(len(rod)),
+                             ptr(rows), ptr(its), ptr(res), ptr(state) if want_state else None,
+                             ptr(u) if want_state else None)
     assert rc == 0, rc
     out = {"rows": rows, "lin_iters": its, "res_norms": res}
     if want_state:
@@ -395,6 +446,23 @@ def oracle_qp(case, rot, F, Fp=None, s=None, flag=1, pinv_mode=1, slip=None, W=N
                           ptr(P), ptr(T), ptr(D), ptr(Fe), ptr(Fpo), ptr(so), ptr(Wo), ptr(sl))
     return {"P": P.reshape(3, 3), "T": T.reshape(3, 3), "dPdF": D.reshape(9, 9), "Fe": Fe.reshape(3, 3),
             "Fp": Fpo.reshape(3, 3), "s": so, "slip": sl, "n_active": na, "W": Wo}
+
+
+def oracle_qp_twin(case, rot, F, Fp, s, flag=1, pinv_mode=1):
+    """or_qp_update_twin for a twinning case: s has n_slip + n_twin entries; returns the oracle's
+    outputs with x = the slip increments of all systems (slip_iter and twin fraction times the
+    characteristic twin shear, from zero histories)."""
+    lib = oracle()
+    mat = case.material(pinv_mode)
+    nT, ns, ntw = case.nT, case.n_slip, case.n_twin
+    P, T, D, Fe, Fpo = np.zeros(9), np.zeros(9), np.zeros(81), np.zeros(9), np.zeros(9)
+    so, Wo, sl, tw = np.zeros(nT), np.zeros(nT), np.zeros(ns), np.zeros(ntw)
+    na = lib.or_qp_update_twin(ctypes.byref(mat), ptr(np.ascontiguousarray(rot, float)),
+                               ptr(np.ascontiguousarray(F, float).reshape(9)), ptr(np.ascontiguousarray(Fp, float).reshape(9)),
+                               ptr(np.ascontiguousarray(s, float)), ptr(np.zeros(nT)), ptr(np.zeros(ns)), ptr(np.zeros(ntw)),
+                               ctypes.c_int(flag), ptr(P), ptr(T), ptr(D), ptr(Fe), ptr(Fpo), ptr(so), ptr(Wo), ptr(sl), ptr(tw))
+    return {"P": P.reshape(3, 3), "T": T.reshape(3, 3), "dPdF": D.reshape(9, 9), "Fe": Fe.reshape(3, 3),
+            "Fp": Fpo.reshape(3, 3), "s": so, "x": np.concatenate([sl, tw * case.twin["shear"]]), "n_active": na}
 
 
 def oracle_qp_warm(case, rot, F, Fp, s, x0=None, flag=0, pinv_mode=1):

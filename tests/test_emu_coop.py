@@ -85,3 +85,18 @@ def test_coop_passes_on_golden_case_states(emu_env):
     for flag in ("flag0", "flag1"):
         for k, v in out[flag]["err"].items():
             assert v < 1e-10, (flag, k, v)
+
+
+def test_coop_with_twin_systems_equals_one_lane_routine(emu_env):
+    """hcp/simpleTension's 18 slip + 6 twin systems (one-sided twin shear): the cooperative form
+    equals the one-lane routine bit for bit; both equal the oracle except on the qps whose local
+    problem meets the exactly singular twin system, where round-off picks the twin pair
+    (tests/test_gpu_twinning.py)."""
+    out = _run(emu_env, "twin", 256, 17)
+    for flag in ("flag0", "flag1"):
+        for k, v in out[flag]["coop_vs_lane"].items():
+            assert v == 0.0, (flag, k, v)
+        assert out[flag]["twin_qps"] > 60
+        assert out[flag]["coop_vs_oracle_bad"] == out[flag]["lane_vs_oracle_bad"]
+        assert len(out[flag]["lane_vs_oracle_bad"]) <= 8
+

@@ -6,6 +6,8 @@ golden curves.
 * Tension, 45 increments (twinning from increment 38): rows against the oracle's (to round-off
   before twinning, within the oracle's own round-off sensitivity after) and the golden,
   TwinRealVF against both, per-qp state against the oracle's before twinning.
+* The return map with twin systems per qp (batch entry point): the oracle's at 1e-10, on random
+  states where twin systems shear.
 * Compression to the reference's first reorientation by twinning (golden row 589: TwinMade =
   1/4096, one qp of the 8^3 mesh): the GPU reorients its first qp there too, and that qp's new
   orientation is the oracle's half turn about the twin normal of its largest fraction.
@@ -35,6 +37,55 @@ def _ctx(case):
 def _rel(a, b):
     a, b = np.asarray(a), np.asarray(b)
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def test_qp_update_with_twin_systems_matches_oracle():
+    """18 slip + 6 twin systems of hcp/simpleTension per qp: stress, tangent, CRSS of every system
+    and the slip increments (twin ones included) at 1e-10 against the oracle, on states where twin
+    systems are active.  Where the six twin systems are all potentially active their 6 x 6 A_PA is
+    exactly singular (one zero singular value, b orthogonal to it) and the minimum-norm increments
+    of a symmetric twin pair are equal in exact arithmetic, so which of the two inactive-slip
+    removal drops next is decided by round-off: on those qps (the oracle's local problem met a
+    numerically singular system) GPU and oracle may settle on different pairs (5 of 26 here, the
+    same qps in the CPU build of the kernels); every qp whose local systems are regular matches."""
+    case = refcase.load_reference_case("hcp_simpleTension", twinning=True)
+    rng = np.random.default_rng(17)
+    rod = rng.uniform(-0.6, 0.6, size=(8, 3))
+    n = 256
+    F = np.eye(3) + rng.normal(scale=2.5e-3, size=(n, 3, 3))
+    Fp = np.eye(3) + rng.normal(scale=4e-4, size=(n, 3, 3))
+    Fp /= np.cbrt(np.linalg.det(Fp))[:, None, None]
+    s0 = np.concatenate([case.s0, case.twin["s0"]])
+    s = np.array([rng.uniform(1.0, 1.3) * s0 for _ in range(n)])
+    grain = rng.integers(0, 8, size=n).astype(np.uint32)
+    ctx = _ctx(case)
+    ctx.set_grains(np.zeros(case.nx * case.ny * case.nz, np.uint32), rod)
+    ns = case.n_slip
+    lib = refcase.oracle()
+    for flag in (0, 1):
+        g = ctx.qp_update(F.reshape(n, 9), Fp.reshape(n, 9), s, grain, flag)
+        twins, singular, parted = 0, 0, 0
+        for i in range(n):
+            c1, c2 = ctypes.c_longlong(), ctypes.c_longlong()
+            lib.or_pinv_stats(ctypes.byref(c1), ctypes.byref(c2))
+            o = refcase.oracle_qp_twin(case, rod[grain[i]], F[i], Fp[i], s[i], flag)
+            lib.or_pinv_stats(ctypes.byref(c1), ctypes.byref(c2))
+            if c2.value > 0:  # a numerically singular local system: round-off picks the twin pair
+                singular += 1
+                parted += _rel(g["P"][i], o["P"].ravel()) > 1e-10
+                continue
+            assert g["n_active"][i] == o["n_active"], (i, g["n_active"][i], o["n_active"])
+            twins += (o["x"][ns:] > 0).any()
+            for k in ("P", "T", "Fe", "Fp"):
+                assert _rel(g[k][i], o[k].ravel()) < 1e-10, (k, i)
+            assert _rel(g["s"][i], o["s"]) < 1e-10
+            assert np.abs(g["slip_inc"][i] - o["x"]).max() < 1e-10 * max(np.abs(o["x"]).max(), 1e-6), i
+            if flag:
+                assert _rel(g["dPdF"][i], o["dPdF"].ravel()) < 1e-10, i
+        print("flag %d: regular qps with an active twin system %d; singular local systems %d, of which parted %d"
+              % (flag, twins, singular, parted))
+        assert twins > 40 and parted <= singular // 3
+    ctx.close()
 
 
 def test_twinning_hcp_tension_matches_oracle_and_golden():
