@@ -123,9 +123,12 @@ int pa_set_solver(pa_context *ctx, int method, int precision);
  * getElementalValues.cc:141 and updateAfterIncrement.cc:105).  No counterpart in the reference
  * (one CPU call per qp); the results agree to rounding in both forms.
  *   PA_QP_LANE (default): one lane per qp; numerically singular local systems are deferred to a
- *     cooperative pseudo-inverse pass and replayed.
+ *     cooperative pseudo-inverse pass and replayed.  With 48 slip systems the lane's active-set
+ *     matrices live in a context-owned workspace of global memory (80 KB per lane, 2.6 GB).
  *   PA_QP_COOP: a group of lanes per qp (lane = slip system / tangent column, matrices in LDS),
- *     persistent wavefronts on a work queue, singular systems solved in place by the group. */
+ *     persistent wavefronts on a work queue, singular systems solved in place by the group.
+ *     Refused (PA_ERR_UNSUPPORTED): these passes took a GPU memory fault whose cause is not
+ *     found; the batch entry point pa_qp_update keeps the cooperative routine. */
 int pa_set_qp_form(pa_context *ctx, int form);
 
 /* One Newton iteration of solveNonLinearSystem (src/ellipticBVP/solveNonLinearSystem.cc:14-47):

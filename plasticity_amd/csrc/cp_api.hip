@@ -306,9 +306,14 @@ struct SingAny {
   unsigned *count, *queue;
   unsigned Q, e;
   int have, k;
+  double *ws = nullptr;  // this lane's cp::LaneWs block (48-system variant), else null
+  __device__ void svd(int n, const double *A, const double *b, double *x) const {
+    if constexpr (cp::lane_ws<NS>()) cp::svd_pinv_solve_ws<NS>(n, A, b, x, ws + cp::LaneWs<NS>::UV);
+    else cp::svd_pinv_solve<NS>(n, A, b, x);
+  }
   __device__ bool operator()(int n, const double *A, const double *b, double *x) {
     if (mode == SING_LANE) {
-      cp::svd_pinv_solve<NS>(n, A, b, x);
+      svd(n, A, b, x);
       return true;
     }
     if (mode == SING_CLAIM) {
@@ -327,13 +332,13 @@ struct SingAny {
         for (int i = 0; i < n; ++i) x[i] = src[i];
       } else {
         atomicAdd(st.miss, 1u);
-        cp::svd_pinv_solve<NS>(n, A, b, x);
+        svd(n, A, b, x);
       }
       ++k;
       return true;
     }
     if (have >= SING_ROUNDS) {  // last round: solve here
-      cp::svd_pinv_solve<NS>(n, A, b, x);
+      svd(n, A, b, x);
       ++k;
       return true;
     }
@@ -538,7 +543,7 @@ __device__ __attribute__((noinline)) bool qp_tangent_one(const QpArgs &a, long l
   const int nslip = a.tw.on ? a.tw.nslip : a.ns;
   bool plastic = false;
   const int na = cp::qp_update<NS, DT, KIN>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 1, P, T, Fe, Fp, (DT *)a.D + Q,
-                                            a.nq, &plastic, a.JxW, SingRef<NS>{&sing}, W, nullptr, 1, nullptr, nslip);
+                                            a.nq, &plastic, a.JxW, SingRef<NS>{&sing}, W, nullptr, 1, nullptr, nslip, sing.ws);
   if (na < 0) return false;
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
@@ -562,11 +567,19 @@ __device__ __attribute__((noinline)) bool qp_tangent_one(const QpArgs &a, long l
 // on every kernel that calls the per-qp function, which is compiled once for all of them.
 #define QP_WAVES __attribute__((amdgpu_waves_per_eu(2)))
 
+// this lane's block of the 48-system variant's workspace (null for the others, whose matrices stay
+// in private memory); the grids of those passes are capped at WS_BLOCKS one-wave blocks
+template <int NS>
+__device__ __forceinline__ double *lane_ws(double *ws) {
+  if constexpr (cp::lane_ws<NS>()) return ws + (size_t)(blockIdx.x * blockDim.x + threadIdx.x) * cp::LaneWs<NS>::size;
+  else return nullptr;
+}
+
 // pass 1: lane = qp (grid-stride), LU path only; singular local problems are queued
 template <int NS, typename DT, bool KIN>
 __global__ void __launch_bounds__(256) QP_WAVES k_qp_tangent(QpArgs a) {
   for (long long Q = blockIdx.x * (long long)blockDim.x + threadIdx.x; Q < a.nq; Q += (long long)gridDim.x * blockDim.x) {
-    SingAny<NS> f{SING_CLAIM, a.sing, a.qcount, a.queue, (unsigned)Q, 0u, 0, 0};
+    SingAny<NS> f{SING_CLAIM, a.sing, a.qcount, a.queue, (unsigned)Q, 0u, 0, 0, lane_ws<NS>(a.ws)};
     qp_tangent_one<NS, DT, KIN>(a, Q, f);  // singular: queued
   }
 }
@@ -576,7 +589,7 @@ __global__ void __launch_bounds__(64) QP_WAVES k_qp_tangent_replay(QpArgs a) {
   const unsigned n = min(*a.qcount, a.sing.cap);
   for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
     if (a.sing.done[e]) continue;
-    SingAny<NS> f{SING_REPLAY, a.sing, a.qcount, a.queue, 0u, e, a.sing.have[e], 0};
+    SingAny<NS> f{SING_REPLAY, a.sing, a.qcount, a.queue, 0u, e, a.sing.have[e], 0, lane_ws<NS>(a.ws)};
     if (qp_tangent_one<NS, DT, KIN>(a, a.queue[e], f)) a.sing.done[e] = 1;
   }
 }
@@ -586,7 +599,7 @@ __global__ void __launch_bounds__(64) QP_WAVES k_qp_tangent_deferred(QpArgs a) {
   const unsigned n = *a.qcount;
   for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     if (i >= a.sing.cap || !a.sing.done[i]) {
-      SingAny<NS> f{SING_LANE, a.sing, a.qcount, a.queue, 0u, i, 0, 0};
+      SingAny<NS> f{SING_LANE, a.sing, a.qcount, a.queue, 0u, i, 0, 0, lane_ws<NS>(a.ws)};
       qp_tangent_one<NS, DT, KIN>(a, a.queue[i], f);
     }
 }
@@ -800,7 +813,7 @@ __device__ __attribute__((noinline)) bool post_qp_one(const PostArgs &a, long lo
   const int nslip = a.tw.on ? a.tw.nslip : a.ns;
   bool plastic = false;
   const int na = cp::qp_update<NS, double, KIN>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 0, P, T, Fe, Fp, (double *)nullptr,
-                                                0, &plastic, 1.0, SingRef<NS>{&sing}, W, nullptr, 1, nullptr, nslip);
+                                                0, &plastic, 1.0, SingRef<NS>{&sing}, W, nullptr, 1, nullptr, nslip, sing.ws);
   if (na < 0) return false;
 #pragma unroll
   for (int i = 0; i < 9; ++i) { a.Fe_iter[i * a.nq + Q] = Fe[i]; a.Fp_iter[i * a.nq + Q] = Fp[i]; a.T[i * a.nq + Q] = T[i]; }
@@ -816,7 +829,7 @@ __device__ __attribute__((noinline)) bool post_qp_one(const PostArgs &a, long lo
 template <int NS, bool KIN>
 __global__ void __launch_bounds__(64) QP_WAVES k_post_qp(PostArgs a) {
   for (long long Q = blockIdx.x * (long long)blockDim.x + threadIdx.x; Q < a.nq; Q += (long long)gridDim.x * blockDim.x) {
-    SingAny<NS> f{SING_CLAIM, a.sing, a.qcount, a.queue, (unsigned)Q, 0u, 0, 0};
+    SingAny<NS> f{SING_CLAIM, a.sing, a.qcount, a.queue, (unsigned)Q, 0u, 0, 0, lane_ws<NS>(a.ws)};
     post_qp_one<NS, KIN>(a, Q, f);
   }
 }
@@ -825,7 +838,7 @@ __global__ void __launch_bounds__(64) QP_WAVES k_post_qp_replay(PostArgs a) {
   const unsigned n = min(*a.qcount, a.sing.cap);
   for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
     if (a.sing.done[e]) continue;
-    SingAny<NS> f{SING_REPLAY, a.sing, a.qcount, a.queue, 0u, e, a.sing.have[e], 0};
+    SingAny<NS> f{SING_REPLAY, a.sing, a.qcount, a.queue, 0u, e, a.sing.have[e], 0, lane_ws<NS>(a.ws)};
     if (post_qp_one<NS, KIN>(a, a.queue[e], f)) a.sing.done[e] = 1;
   }
 }
@@ -834,7 +847,7 @@ __global__ void __launch_bounds__(64) QP_WAVES k_post_qp_deferred(PostArgs a) {
   const unsigned n = *a.qcount;
   for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     if (i >= a.sing.cap || !a.sing.done[i]) {
-      SingAny<NS> f{SING_LANE, a.sing, a.qcount, a.queue, 0u, i, 0, 0};
+      SingAny<NS> f{SING_LANE, a.sing, a.qcount, a.queue, 0u, i, 0, 0, lane_ws<NS>(a.ws)};
       post_qp_one<NS, KIN>(a, a.queue[i], f);
     }
 }
@@ -1508,6 +1521,7 @@ struct pa_context {
   double *Pq = nullptr;  // [9][nq] JxW P
   unsigned *queue = nullptr;  // deferred qps of the pseudo-inverse pass
   SingStore sing = {};        // their singular systems and solutions
+  double *ws = nullptr;       // 48-system variant: per-lane matrix workspace (WS_BLOCKS x 64 lanes)
   unsigned char *con = nullptr;
   double *delu = nullptr;
   double *kv[7] = {nullptr};  // Krylov vectors: X Rr Rl Zr Zl Pr Pl (CG: X R - Z Q P -)
@@ -1545,6 +1559,11 @@ int dalloc(pa_context *c, T **p, size_t count) {
 
 int ns_template(int ns) { return ns <= 12 ? 12 : (ns <= 24 ? 24 : 48); }
 
+// One-wave blocks of the one-lane qp passes of the 48-system variant: each lane of them owns a
+// cp::LaneWs<48> block (80,640 bytes), 2.6 GB in all.
+constexpr unsigned WS_BLOCKS = 512;
+unsigned lane_grid(const pa_context *c, unsigned g) { return c->ws ? std::min(g, WS_BLOCKS) : g; }
+
 bool mixed(const pa_context *c) { return c->precision == PA_PREC_MIXED; }
 
 constexpr unsigned DEFER_GRID = 2048;  // 64-lane blocks of the replay / deferred passes (two waves per SIMD)
@@ -1560,15 +1579,15 @@ template <int NS, typename DT, bool KIN>
 void qp_tangent_passes_k(pa_context *c, const QpArgs &a, bool main_only) {
   // one-wave blocks, a grid of 2048 (eight per CU) striding over the qps
   hipEventRecord(c->qev[0], c->stream);
-  hipLaunchKernelGGL((k_qp_tangent<NS, DT, KIN>), dim3(std::min(2048u, nblk(c->nq, 64))), dim3(64), 0, c->stream, a);
+  hipLaunchKernelGGL((k_qp_tangent<NS, DT, KIN>), dim3(lane_grid(c, std::min(2048u, nblk(c->nq, 64)))), dim3(64), 0, c->stream, a);
   hipEventRecord(c->qev[1], c->stream);
   c->qev_set = true;
   if (main_only) return;
   for (int r = 0; r < SING_ROUNDS; ++r) {
     sing_svd_round<NS>(c, a.qcount);
-    hipLaunchKernelGGL((k_qp_tangent_replay<NS, DT, KIN>), dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
+    hipLaunchKernelGGL((k_qp_tangent_replay<NS, DT, KIN>), dim3(lane_grid(c, DEFER_GRID)), dim3(64), 0, c->stream, a);
   }
-  hipLaunchKernelGGL((k_qp_tangent_deferred<NS, DT, KIN>), dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
+  hipLaunchKernelGGL((k_qp_tangent_deferred<NS, DT, KIN>), dim3(lane_grid(c, DEFER_GRID)), dim3(64), 0, c->stream, a);
 }
 // Blocks of a persistent cooperative qp kernel: as many one-wave blocks as the device holds at
 // once (from the kernel's occupancy), at most one per chunk of qps.
@@ -1639,6 +1658,7 @@ int launch_qp_tangent(pa_context *c, bool main_only = false) {
   a.queue = c->queue; a.qcount = c->counters + 8; a.sing = c->sing;
   a.nsing = c->counters + 8; a.work = c->counters + 11;
   a.tw = twin_args(c);
+  a.ws = c->ws;
   const int NS = ns_template(c->ns);
   const bool f = mixed(c);
   if (NS == 12) { if (f) qp_tangent_passes<12, float>(c, a, main_only); else qp_tangent_passes<12, double>(c, a, main_only); }
@@ -1651,12 +1671,12 @@ int launch_qp_tangent(pa_context *c, bool main_only = false) {
 template <int NS, bool KIN>
 void post_passes_k(pa_context *c, const PostArgs &a, unsigned grid) {
   (void)grid;
-  hipLaunchKernelGGL((k_post_qp<NS, KIN>), dim3(std::min(2048u, nblk(c->nq, 64))), dim3(64), 0, c->stream, a);
+  hipLaunchKernelGGL((k_post_qp<NS, KIN>), dim3(lane_grid(c, std::min(2048u, nblk(c->nq, 64)))), dim3(64), 0, c->stream, a);
   for (int r = 0; r < SING_ROUNDS; ++r) {
     sing_svd_round<NS>(c, a.qcount);
-    hipLaunchKernelGGL((k_post_qp_replay<NS, KIN>), dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
+    hipLaunchKernelGGL((k_post_qp_replay<NS, KIN>), dim3(lane_grid(c, DEFER_GRID)), dim3(64), 0, c->stream, a);
   }
-  hipLaunchKernelGGL((k_post_qp_deferred<NS, KIN>), dim3(DEFER_GRID), dim3(64), 0, c->stream, a);
+  hipLaunchKernelGGL((k_post_qp_deferred<NS, KIN>), dim3(lane_grid(c, DEFER_GRID)), dim3(64), 0, c->stream, a);
 }
 template <int NS, bool KIN>
 void post_coop_k(pa_context *c, const PostArgs &a) {
@@ -2054,10 +2074,7 @@ namespace {
 // first dispatch, even for two waves), so a Newton step with more than 24 systems needs the
 // cooperative form (no private memory).  The batch entry point always runs cooperatively.
 int check_qp_form(const pa_context *c) {
-  if (c->ns > 24 && c->qp_form == PA_QP_LANE)
-    return fail(PA_ERR_UNSUPPORTED, "more than 24 slip systems in a Newton step need the cooperative qp form "
-                                    "(pa_set_qp_form PA_QP_COOP): the one-lane 48-system kernels need 84 KB of private "
-                                    "memory per lane, which the runtime does not reserve on this GPU");
+  if (c->qp_form != PA_QP_LANE) return fail(PA_ERR_UNSUPPORTED, "only the one-lane qp form takes Newton steps");
   return 0;
 }
 }  // namespace
@@ -2165,6 +2182,11 @@ int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out) {
   }
   for (int i = 0; i < 7; ++i)
     if ((rc = dalloc(c, &c->kv[i], N))) { pa_destroy(c); return rc; }
+  if (cp::lane_ws<48>() && ns_template(c->ns) == 48 &&
+      (rc = dalloc(c, &c->ws, (size_t)WS_BLOCKS * 64 * cp::LaneWs<48>::size))) {
+    pa_destroy(c);
+    return rc;
+  }
   {  // singular-system store: as many entries as fit 768 MiB (at most one per qp)
     const size_t ns_t = (size_t)ns_template(c->ns);
     const size_t per = sizeof(double) * (ns_t * ns_t + ns_t + SING_ROUNDS * ns_t) + SING_ROUNDS * 8 + 3 * sizeof(int);
@@ -2330,6 +2352,12 @@ int pa_set_solver(pa_context *c, int method, int precision) {
 int pa_set_qp_form(pa_context *c, int form) {
   if (!c) return fail(PA_ERR_ARG, "null context");
   if (form != PA_QP_LANE && form != PA_QP_COOP) return fail(PA_ERR_ARG, "unknown qp form");
+  // The cooperative Newton-step passes took a GPU memory fault whose cause is not found; until it
+  // is, they are refused (the batch entry point and the pseudo-inverse paths keep the cooperative
+  // routine, which runs clean there).
+  if (form == PA_QP_COOP)
+    return fail(PA_ERR_UNSUPPORTED, "the cooperative Newton-step qp passes are switched off (an unexplained GPU memory "
+                                    "fault); use PA_QP_LANE");
   c->qp_form = form;
   return 0;
 }
@@ -2382,6 +2410,7 @@ int pa_commit_increment(pa_context *c, double row[15]) {
   a.queue = c->queue; a.qcount = c->counters + 9; a.sing = c->sing;
   a.nsing = c->counters + 9; a.work = c->counters + 12;
   a.tw = twin_args(c);
+  a.ws = c->ws;
   HIPCHK(hipEventRecord(c->ev[0], c->stream));
   const int NS = ns_template(c->ns);
   const unsigned grid = nblk(c->nq, 256);

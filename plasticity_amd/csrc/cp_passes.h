@@ -155,6 +155,7 @@ struct QpArgs {
   unsigned *work;   // work queue: the next unclaimed qp (zeroed before each launch)
   unsigned *nsing;  // pseudo-inverse solves (numerically singular A_PA) of the pass
   TwinArgs tw;      // twinning (one-lane form)
+  double *ws;       // one-lane form, 48 systems: per-lane matrix workspace (cp::LaneWs), else null
 };
 
 template <int NS, typename DT, bool KIN, int G>
@@ -218,6 +219,7 @@ struct PostArgs {
   unsigned *work;   // cooperative form: work queue of the qp pass (zeroed before the launch)
   unsigned *nsing;  // pseudo-inverse solves of the pass
   TwinArgs tw;      // twinning (one-lane form)
+  double *ws;       // one-lane form, 48 systems: per-lane matrix workspace (cp::LaneWs), else null
 };
 
 __device__ __forceinline__ void post_F(const PostArgs &a, long long Q, double F[9]) { qp_F(a.m, a.h, a.u, Q, F); }

@@ -168,9 +168,9 @@ CP_DEV void rr_pair(int m, int r, int k, int &p, int &q) {
 // x = pinv(A) b by one-sided Jacobi SVD (round-robin sweeps); singular values <= CP_PINV_RTOL *
 // sigma_max are dropped unless there is exactly one and its step is determined by the data
 // (below).  Used only when LU finds A_PA numerically singular.
+// U, V: n x n work matrices (leading dimension NS) of the caller.
 template <int NS>
-__host__ __device__ __attribute__((noinline)) void svd_pinv_solve(int n, const double *A, const double *b, double *x) {
-  double U[NS * NS], V[NS * NS];
+__host__ __device__ __forceinline__ void svd_pinv_core(int n, const double *A, const double *b, double *x, double *U, double *V) {
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) { U[i * NS + j] = A[i * NS + j]; V[i * NS + j] = (i == j) ? 1.0 : 0.0; }
   const int m = n + (n & 1);
@@ -236,20 +236,43 @@ __host__ __device__ __attribute__((noinline)) void svd_pinv_solve(int n, const d
     for (int i = 0; i < n; ++i) x[i] += V[i * NS + k] * ub;
   }
 }
+template <int NS>
+__host__ __device__ __attribute__((noinline)) void svd_pinv_solve(int n, const double *A, const double *b, double *x) {
+  double U[NS * NS], V[NS * NS];
+  svd_pinv_core<NS>(n, A, b, x, U, V);
+}
+// The same with U and V in the caller's workspace (2 NS^2 doubles): for the 48-system variant,
+// whose matrices do not fit the private segment the runtime grants a lane.
+template <int NS>
+__host__ __device__ __attribute__((noinline)) void svd_pinv_solve_ws(int n, const double *A, const double *b, double *x, double *UV) {
+  svd_pinv_core<NS>(n, A, b, x, UV, UV + NS * NS);
+}
 
 // What qp_update does when LU finds A_PA numerically singular: the functor receives A_PA
 // (n x n, leading dimension NS, intact) and b_PA and either writes x = pinv(A_PA) b_PA and
 // returns true, or returns false, which makes qp_update return -1 at once.
 template <int NS>
 struct SvdInLane {  // the pseudo-inverse in this lane
+  double *uv = nullptr;  // U and V in the lane's workspace (2 NS^2 doubles) when lane_ws<NS>()
   CP_DEV bool operator()(int n, const double *A, const double *b, double *x) const {
-    svd_pinv_solve<NS>(n, A, b, x);
+    if (NS > 24) svd_pinv_solve_ws<NS>(n, A, b, x, uv);
+    else svd_pinv_solve<NS>(n, A, b, x);
     return true;
   }
 };
 struct LuOnly {  // give up: the caller handles the qp elsewhere
   CP_DEV bool operator()(int, const double *, const double *, double *) const { return false; }
 };
+
+// Per-lane workspace of the 48-system variant (doubles): its NS x NS matrices take 4 NS^2 doubles
+// (74 KB), more private memory than the runtime grants a lane, so they live in a block of global
+// memory the caller gives each lane: A over the potentially-active set, the compacted copy A_PA
+// (then A2 of the tangent), U and V of the pseudo-inverse, and the tangent's delgamma and S_PA.
+template <int NS> struct LaneWs {
+  static constexpr int A = 0, Ap = NS * NS, UV = 2 * NS * NS, rhs = 4 * NS * NS, SV = 4 * NS * NS + 9 * NS,
+                       size = 4 * NS * NS + 18 * NS;
+};
+template <int NS> constexpr bool lane_ws() { return NS > 24; }
 
 // ------------------------------------------------------------ table layouts
 // material block (doubles): h0[NS] ss[NS] pw[NS] q[NS*NS] tol
@@ -334,13 +357,16 @@ CP_DEV void dexp_series(const double *X, const double *S, int count3, bool neg, 
 // start stands only if the loop converges on that set without dropping a system; otherwise the
 // loop restarts from zero on the trial set.  iters (may be null) receives the local solves of both
 // attempts.  Not combined with kinematic hardening.
+// ws: this lane's LaneWs<NS> block when lane_ws<NS>() (NS = 48), unused otherwise.
 template <int NS, typename DT = double, bool KIN = false, typename SING = SvdInLane<NS>, bool WARM = false>
 CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ gt, int ns, int max_solver_iter,
                      const double *F, const double *Fp_t, double *s, double *x, int flag, double *P, double *T,
                      double *Fe_out, double *Fp_out, DT *D, long long dstride, bool *plastic_path = nullptr,
                      double dscale = 1.0, SING sing = SING(), double *W = nullptr, const float *x0 = nullptr,
-                     long long x0stride = 1, int *iters = nullptr, int nslip = 1 << 30) {
+                     long long x0stride = 1, int *iters = nullptr, int nslip = 1 << 30, double *ws = nullptr) {
   static_assert(!(WARM && KIN), "the warm start does not cover kinematic hardening");
+  constexpr bool WSP = lane_ws<NS>();
+  typedef LaneWs<NS> LW;
   typedef MatLayout<NS> ML;
   typedef GrainLayout<NS> GL;
   const double tol1 = mat[ML::tol];
@@ -384,7 +410,8 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
   int n_active = 0;
 
   // ---- plastic correction (:250-468)
-  double A[NS * NS];  // A over the initial potentially-active set, indexed by positions in PA0
+  double A_l[WSP ? 1 : NS * NS];
+  double *const A = WSP ? ws + LW::A : A_l;  // A over the initial potentially-active set, indexed by positions in PA0
   int PA0[NS];
   int n0 = nPA;
   double h[NS];
@@ -431,7 +458,8 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
       // inactive_slip_removal (inactiveSlipRemoval.cc:4-124)
       for (int a = 0; a < ns; ++a) xb[a] = 0.0;
       while (true) {
-        double Ap[NS * NS], bp[NS];
+        double Ap_l[WSP ? 1 : NS * NS], bp[NS];
+        double *const Ap = WSP ? ws + LW::Ap : Ap_l;
         for (int i = 0; i < npos; ++i) {
           bp[i] = b[PA0[pos[i]]];
           for (int j = 0; j < npos; ++j) Ap[i * NS + j] = A[pos[i] * NS + pos[j]];
@@ -595,7 +623,9 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
   // ---- tangent: in-loop block (:473-828) with delFp_delF_prev = 0 and dels_delF = 0
   // delFp_delF = sum_j SV_j (x) delgamma_j (rank nPA): SV[j] (9) and delgamma (rhs, 9 per j) are
   // kept and each column kl of it is formed where the final tangent needs it
-  double rhs[NS * 9], SV[NS * 9];
+  double rhs_l[WSP ? 1 : NS * 9], SV_l[WSP ? 1 : NS * 9];
+  double *const rhs = WSP ? ws + LW::rhs : rhs_l;
+  double *const SV = WSP ? ws + LW::SV : SV_l;
   const int nsv = (plastic && nPA > 0) ? nPA : 0;
   if (nsv > 0) {
     // count3 (:607-619): number of series terms until ||X^k / k!||_F <= max(tol/1e4, 1e-12)
@@ -646,7 +676,8 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
       }
     }
     // A2 (:622-709)
-    double A2[NS * NS];
+    double A2_l[WSP ? 1 : NS * NS];
+    double *const A2 = WSP ? ws + LW::Ap : A2_l;  // the compacted A_PA is no longer needed
     double eneg[9], Xneg[9], t6[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) Xneg[i] = -XdelFp[i];

@@ -248,7 +248,8 @@ class Context:
         return out
 
     def set_qp_form(self, form):
-        """Layout of the Newton step's qp passes: native.PA_QP_LANE (default) or native.PA_QP_COOP."""
+        """Layout of the Newton step's qp passes: native.PA_QP_LANE (the default and the only one
+        accepted; native.PA_QP_COOP is refused while its GPU memory fault is unexplained)."""
         check(native.lib().pa_set_qp_form(self._h, ctypes.c_int(form)))
 
     def assemble(self, inc, it):

@@ -156,6 +156,7 @@ int emu_qp_lane(const pa_material *mat, const double *rod, int nrows, long long 
   const bool kin = mat->kinematic_hardening != 0;
   int gstride = 0;
   std::vector<double> mb = material_block(*mat, NS), tab = grain_tables(*mat, rod, nrows, NS, gstride);
+  std::vector<double> ws(cp::LaneWs<48>::size);  // the 48-system variant's matrices (per-lane workspace)
   for (long long i = 0; i < n; ++i) {
     double s_[48], x_[48], W_[48];
     for (int a = 0; a < ns; ++a) { s_[a] = s[ns * i + a]; W_[a] = (kin && W) ? W[ns * i + a] : 0.0; }
@@ -165,7 +166,8 @@ int emu_qp_lane(const pa_material *mat, const double *rod, int nrows, long long 
       constexpr bool K = decltype(kinc)::value;
       return cp::qp_update<NSc, double, K>(mb.data(), gt, ns, mat->max_solver_iter, F + 9 * i, Fp + 9 * i, s_, x_, flag, P + 9 * i,
                                            T + 9 * i, Fe + 9 * i, Fpo + 9 * i, flag ? D + 81 * i : nullptr, 1, nullptr, 1.0,
-                                           cp::SvdInLane<NSc>(), W_, nullptr, 1, nullptr, g_nslip);
+                                           cp::SvdInLane<NSc>{ws.data() + cp::LaneWs<NSc>::UV}, W_, nullptr, 1, nullptr, g_nslip,
+                                           ws.data());
     };
     using I12 = std::integral_constant<int, 12>;
     using I24 = std::integral_constant<int, 24>;

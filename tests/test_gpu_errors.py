@@ -63,3 +63,15 @@ def test_solve_after_a_change_of_arithmetic_form_is_refused():
     rn, n = ctx.newton_iteration(0, 0)
     assert np.isfinite(rn) and n > 0
     ctx.close()
+
+
+def test_cooperative_newton_passes_are_refused():
+    """The cooperative Newton-step qp passes are switched off (a GPU memory fault with no cause
+    found); selecting them is refused and the one-lane form stays in effect."""
+    ctx = _ctx(refcase.fcc_case())
+    with pytest.raises(pa.PlasticityError, match="switched off"):
+        ctx.set_qp_form(native.PA_QP_COOP)
+    ctx.set_qp_form(native.PA_QP_LANE)
+    rn, its = ctx.newton_iteration(0, 0)
+    assert np.isfinite(rn) and its > 0
+    ctx.close()

@@ -62,6 +62,76 @@ def test_qp_update_matches_oracle(strain):
 
 
 @pytest.mark.parametrize("strain", [2e-3, 1e-2])
+def test_qp_update_converged_local_loop_matches_oracle(strain):
+    """Max Solver Iterations 50 (the local loop of calculatePlasticity.cc:322-441 run to the stress
+    tolerance): stress, history and the tangent dP/dF of the last pass at 1e-10 of the oracle's."""
+    case = refcase.fcc_case(max_solver_iter=50)
+    rng = np.random.default_rng(23)
+    rod = rng.uniform(-0.5, 0.5, size=(8, 3))
+    n = 192
+    F, Fp, s = _random_states(case, n, rng, strain)
+    grain = rng.integers(0, 8, size=n).astype(np.uint32)
+    ctx = _ctx(case, nel=(1, 1, 1))
+    ctx.set_grains(np.zeros(1, np.uint32), rod)
+    g = ctx.qp_update(F.reshape(n, 9), Fp.reshape(n, 9), s, grain, 1)
+    nplastic = 0
+    for i in range(n):
+        o = refcase.oracle_qp(case, rod[grain[i]], F[i], Fp[i], np.append(s[i], 10e14), 1)
+        assert g["n_active"][i] == o["n_active"], (i, g["n_active"][i], o["n_active"])
+        nplastic += o["n_active"] > 0
+        for k in ("P", "T", "Fe", "Fp"):
+            assert _rel(g[k][i], o[k].ravel()) < 1e-10, (k, i)
+        assert _rel(g["s"][i], o["s"][:12]) < 1e-10
+        assert _rel(g["dPdF"][i], o["dPdF"].ravel()) < 1e-10, (i, _rel(g["dPdF"][i], o["dPdF"].ravel()))
+    assert nplastic > n // 4
+    ctx.close()
+
+
+def test_cyclic_converged_local_loop_newton_matches_oracle():
+    """The learned-warm-start workload's loading (FCC, cyclic +-1 % in steps of 1e-3, Max Solver
+    Iterations 50) on 8^3 elements, 8 grains, through the Newton step's qp passes, 3 increments.
+    The steps are large for 4 Newton iterations (|R| 775 -> 13), and the run is chaotic: the oracle
+    itself moves |R| by 1.6e-5 at the second Newton iteration and the curve by 1e-2 after three
+    increments when only its Krylov tolerance changes (1e-14 -> 3e-15).  So the first Newton
+    iteration is held to 1e-10 and the rest to 5x the oracle's own spread."""
+    from plasticity_amd import synth
+    _, _, vox, rod, bcs, t = synth.fcc_workload(8, 8, 40, 1e-3, seed=2024)
+
+    def case_at(lin_tol):
+        c = refcase.Case(t["m"] / np.linalg.norm(t["m"], axis=1)[:, None], t["n"] / np.linalg.norm(t["n"], axis=1)[:, None],
+                         t["q"], t["C"], t["s0"], t["h0"], t["ss"], t["pw"], tol=t["tol"], nx=8, ny=8, nz=8, delT=1.0,
+                         total_time=40.0, bcs=bcs, lin_tol=lin_tol, max_solver_iter=50)
+        c.elem_grain = vox.transpose(2, 1, 0).ravel().astype(np.uint32)
+        c.grain_rod = rod
+        c.cyclic = (1, 2, 1, 10.0)
+        return c
+
+    case = case_at(1e-14)
+    ref = refcase.oracle_simulate(case, 3, nthreads=8)
+    ref2 = refcase.oracle_simulate(case_at(3e-15), 3, nthreads=8)
+    ctx = _ctx(case)
+    ctx.set_grains(case.elem_grain, case.grain_rod)
+    rows, res = [], []
+    for inc in range(3):
+        for it in range(case.max_newton):
+            rn, _ = ctx.newton_iteration(inc, it)
+            assert np.isfinite(rn)
+            res.append(rn)
+        rows.append(ctx.commit_increment())
+    ctx.close()
+    res, rows = np.array(res), np.array(rows)
+    r1, r2 = np.asarray(ref["res_norms"]).ravel(), np.asarray(ref2["res_norms"]).ravel()
+    print("|R| GPU   ", " ".join("%.6g" % v for v in res))
+    print("|R| oracle", " ".join("%.6g" % v for v in r1))
+    assert abs(res[0] - r1[0]) <= 1e-10 * r1[0]
+    assert np.abs(res - r1).max() / r1.max() <= 5 * np.abs(r2 - r1).max() / r1.max()
+    d_gpu = np.abs(rows[:, :12] - ref["rows"][:, :12]).max()
+    d_self = np.abs(ref2["rows"][:, :12] - ref["rows"][:, :12]).max()
+    assert d_gpu <= 5 * d_self, (d_gpu, d_self)
+    assert rows[-1, 14] > 0.02  # plastic (SlipTotal)
+
+
+@pytest.mark.parametrize("strain", [2e-3, 1e-2])
 def test_qp_update_with_kinematic_hardening_matches_oracle(strain):
     """Backstress (C_1 = 200, C_2 = 1 as in fcc/CyclicLoading) with random W_kh_conv of both signs."""
     case = refcase.fcc_case()
@@ -129,20 +199,60 @@ def test_qp_update_hcp_slip_matches_oracle():
     ctx.close()
 
 
-def test_more_than_24_slip_systems_need_the_cooperative_form():
-    """A 48-system context is created, but a Newton step in the one-lane form (whose 48-system
-    kernels need 84 KB of private memory per lane) is refused with PA_ERR_UNSUPPORTED before any
-    qp kernel runs."""
+def _bcc48_case(n=4, incs=10, eps=5e-4, lin_tol=1e-14):
+    """BCC with 48 slip systems, the stiffness of the batched RVE workload (231/135/117 GPa) and
+    bcc/simpleTension's hardening (s0 200, h0 1500, a 1, s_s 500 MPa), latent ratio 1.4 off-plane;
+    n^3 elements, 4 grains, x-tension in `incs` steps of `eps`."""
     m, nrm = _bcc48_systems()
-    C = np.diag([231e3] * 3 + [117e3] * 3)
-    mat = pa.make_material(m, nrm, np.ones((48, 48)), C, [30.0] * 48, [400.0] * 48, [150.0] * 48, [1.5] * 48, 1e-9)
-    pb = pa.make_problem((2, 2, 2), (1.0, 1.0, 1.0), 10, 1.0, [(1, 1, 0.0), (2, 1, 0.01)])
-    ctx = pa.Context(mat, pb)
-    ctx.set_grains(np.zeros(8, np.uint32), np.zeros((1, 3)))
-    with pytest.raises(pa.PlasticityError, match="more than 24 slip systems"):
-        ctx.newton_iteration(0, 0)
-    with pytest.raises(pa.PlasticityError, match="more than 24 slip systems"):
-        ctx.commit_increment()
+    m = m / np.linalg.norm(m, axis=1)[:, None]
+    nrm = nrm / np.linalg.norm(nrm, axis=1)[:, None]
+    q = np.full((48, 48), 1.4)
+    for a in range(48):
+        for b in range(48):
+            if np.allclose(nrm[a], nrm[b]):
+                q[a, b] = 1.0
+    C = np.zeros((6, 6))
+    C[:3, :3] = 135e3
+    np.fill_diagonal(C[:3, :3], 231e3)
+    C[3, 3] = C[4, 4] = C[5, 5] = 117e3
+    case = refcase.Case(m, nrm, q, C, [200.0] * 48, [1500.0] * 48, [500.0] * 48, [1.0] * 48, nx=n, ny=n, nz=n,
+                        total_time=float(incs), bcs=((1, 1, 0.0), (1, 2, 0.0), (1, 3, 0.0), (2, 1, eps * incs)), lin_tol=lin_tol)
+    rng = np.random.default_rng(8)
+    case.grain_rod = rng.uniform(-0.4, 0.4, size=(4, 3))
+    case.elem_grain = rng.integers(0, 4, size=n ** 3).astype(np.uint32)
+    return case
+
+
+def test_bcc48_newton_steps_match_oracle():
+    """Newton steps with 48 slip systems in the one-lane qp passes, whose active-set matrices live
+    in a per-lane workspace of global memory (80 KB per lane, more than the private segment the
+    runtime grants): 8 increments through yield on 4^3 elements against the oracle (Krylov solves
+    at 1e-14; the oracle moves by <= 5e-11 when that tolerance changes to 3e-15), curve and
+    per-qp state at 1e-9.  Two more increments leave the oracle's trajectory: with 48 systems
+    A_PA is often rank-deficient with singular values at round-off, where the reference's own
+    result depends on rounding (as in the per-qp test below), and the GPU's FMA rounding differs
+    from the oracle's build; they are held to the method's invariants."""
+    case = _bcc48_case()
+    ref = refcase.oracle_simulate(case, 8, nthreads=8, want_state=True)
+    ctx = _ctx(case)
+    ctx.set_grains(case.elem_grain, case.grain_rod)
+    rows = np.array([ctx.run_increment(inc)[0] for inc in range(8)])
+    assert rows[-1, 14] > 0.02  # plastic (SlipTotal)
+    assert _rel(rows[:, :12], ref["rows"][:, :12]) < 1e-9, _rel(rows[:, :12], ref["rows"][:, :12])
+    assert _rel(rows[:, 14], ref["rows"][:, 14]) < 1e-8
+    st = ref["state"]
+    assert _rel(ctx.get_state("s"), st[:, 27:27 + 48]) < 1e-9
+    assert _rel(ctx.get_state("T"), st[:, 0:9]) < 1e-9
+    assert _rel(ctx.get_state("Fp"), st[:, 9:18]) < 1e-9
+    s_prev, slip_prev = ctx.get_state("s"), ctx.get_state("slip")
+    for inc in range(8, 10):
+        row = ctx.run_increment(inc)[0]
+        assert np.all(np.isfinite(row))
+        Fp, s, slip = ctx.get_state("Fp").reshape(-1, 3, 3), ctx.get_state("s"), ctx.get_state("slip")
+        assert np.abs(np.linalg.det(Fp) - 1).max() < 1e-8
+        assert (s >= s_prev - 1e-12 * 500).all() and s.max() <= 500 * (1 + 1e-12)
+        assert (slip >= slip_prev - 1e-15).all()
+        s_prev, slip_prev = s, slip
     ctx.close()
 
 
