@@ -290,6 +290,47 @@ int pa_qp_update_warm(pa_context *ctx, pa_mlp *net, long long nqp, const double 
                       double *T, double *dPdF, double *Fe, double *Fp, double *s_out, double *slip_inc,
                       int *n_active, int *iters, float *feat, float *x0);
 
+/* ---- Batched RVEs.  No counterpart in the reference, which runs one RVE per job; each RVE of a
+ * batch is the reference's boundary-value problem (ellipticBVP<3> + crystalPlasticity<3> on its own
+ * box mesh), and all of them advance together: one Newton iteration of every RVE (assemble.cc:6-104
+ * per RVE, then a Jacobi-preconditioned CG solve per RVE, solveLinearSystem.cc:8-36 with the PCG
+ * option of pa_set_solver) runs in the same launches, the qp work of all RVEs drained by one
+ * persistent kernel from a device work queue, replayed from HIP graphs captured once per batch.
+ *   mats: 1..4 materials (no twinning, no kinematic hardening); rve_mat[r]: material of RVE r;
+ *   rve_nel[3r..3r+2]: its elements along x, y, z; pb: domain span, boundary rules (simple face
+ *   rules only), increments, Newton count and Krylov tolerance, shared by every RVE (pb->nx/ny/nz
+ *   and pb->solver are not used).  Every RVE needs pa_batch_set_grains before the first step. */
+typedef struct pa_batch pa_batch;
+int pa_batch_create(const pa_material *mats, int n_mats, const pa_problem *pb, int n_rve, const int *rve_mat,
+                    const int *rve_nel, pa_batch **out);
+void pa_batch_destroy(pa_batch *batch);
+/* Grain of each element of RVE `rve` (row into grain_rod, n_grain_rows x 3 Rodrigues vectors), as
+ * pa_set_grains for one context (crystalPlasticity::loadOrientations, loadOrientations.cc:4-33). */
+int pa_batch_set_grains(pa_batch *batch, int rve, const unsigned *elem_grain, const double *grain_rod, int n_grain_rows);
+/* PA_PREC_FP64 or PA_PREC_MIXED (FP32 element tangents and matrices), as pa_set_solver. */
+int pa_batch_set_precision(pa_batch *batch, int precision);
+/* Replay the Newton iteration from HIP graphs (1, the default) or launch its kernels directly (0);
+ * the results are bitwise the same. */
+int pa_batch_set_graphs(pa_batch *batch, int on);
+/* One Newton iteration of every RVE (solveNonLinearSystem.cc:14-47 per RVE): res_norms[r] = |R| of
+ * RVE r's assembly, lin_iters[r] its CG iterations (either may be NULL). */
+int pa_batch_newton_iteration(pa_batch *batch, int inc, int it, double *res_norms, int *lin_iters);
+/* updateAfterIncrement for every RVE (updateAfterIncrement.cc:6-305, 710-771): rows[15r..15r+14]
+ * = RVE r's stressstrain.txt row (may be NULL). */
+int pa_batch_commit_increment(pa_batch *batch, double *rows);
+/* Autotuner: times the persistent qp pass at 256/512/768/1024 one-wave blocks and the Krylov chunk
+ * at 8/16/32 iterations on the current state and keeps the fastest.  report (may be NULL, 9
+ * doubles): grid, chunk, qp-pass ms per grid, solve ms per chunk.  Graphs are re-captured after. */
+int pa_batch_tune(pa_batch *batch, double *report);
+/* The persistent qp pass alone (flag 1: with the tangent) on the current state, HIP-event ms. */
+int pa_batch_qp_pass_ms(pa_batch *batch, int flag, double *ms);
+/* ms of the last assembly, Krylov loop, post-increment pass and timed qp pass. */
+int pa_batch_last_timings(pa_batch *batch, double ms[4]);
+/* State of RVE `rve` as pa_get_state ids 0-7 (T, Fp, Fe, s, rotnew, slip, u, R), per qp / node. */
+int pa_batch_get_state(pa_batch *batch, int rve, int which, double *host, long long n);
+/* Totals of the batch: quadrature points, elements, nodes, dofs. */
+int pa_batch_sizes(pa_batch *batch, long long out[4]);
+
 #ifdef __cplusplus
 }
 #endif

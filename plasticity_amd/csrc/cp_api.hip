@@ -630,14 +630,14 @@ struct ElemArgs {
 
 constexpr int ASM_EL = 8;  // elements per 64-lane workgroup
 
+// One block's ASM_EL elements of a colour: element ci0 + le of the colour (lexicographic over the
+// colour's elements), qp 8 e + q at qbase + 8 e + q of the staged arrays; nodes, K, R, con and delu
+// as in the mesh of `a` (a batch passes its RVE's mesh and pointers).  lds: the block's staging.
 template <typename KT>
-__global__ void __launch_bounds__(64) k_element(ElemArgs a) {
-  __shared__ KT sD[64 * 81];
-  __shared__ double sP[64 * 9];
-  __shared__ double sdiag[ASM_EL * 24];
+__device__ __forceinline__ void element_body(const ElemArgs &a, long long ci0, long long qbase, KT *sD, double *sP, double *sdiag) {
   const int t = threadIdx.x, le = t >> 3, sub = t & 7;
   const long long ncol = (long long)a.ncx * a.ncy * a.ncz;
-  const long long ci = (long long)blockIdx.x * ASM_EL + le;
+  const long long ci = ci0 + le;
   const bool active = ci < ncol;
   long long e = 0;
   int ex = 0, ey = 0, ez = 0;
@@ -650,7 +650,7 @@ __global__ void __launch_bounds__(64) k_element(ElemArgs a) {
   // stage the 8 elements' 64 qps: lane t loads qp (le, sub)
   const KT *Dg = (const KT *)a.D;
   if (active) {
-    const long long Q = 8 * e + sub;
+    const long long Q = qbase + 8 * e + sub;
 #pragma unroll 9
     for (int c = 0; c < 81; ++c) sD[c * 64 + t] = Dg[c * a.nq + Q];
 #pragma unroll
@@ -735,6 +735,13 @@ __global__ void __launch_bounds__(64) k_element(ElemArgs a) {
     }
     if (rv != 0.0) a.R[gi] += rv;
   }
+}
+template <typename KT>
+__global__ void __launch_bounds__(64) k_element(ElemArgs a) {
+  __shared__ KT sD[64 * 81];
+  __shared__ double sP[64 * 9];
+  __shared__ double sdiag[ASM_EL * 24];
+  element_body<KT>(a, (long long)blockIdx.x * ASM_EL, 0, sD, sP, sdiag);
 }
 
 // ------------------------------------------------ post-increment qp pass
@@ -2069,10 +2076,10 @@ int do_solve(pa_context *c, int it, int *iters) {
 
 namespace {
 // The one-lane qp passes hold each lane's active-set and tangent matrices in private memory: 8.6 KB
-// per lane for up to 12 systems, 24 KB for up to 24, 84 KB for up to 48.  The runtime refuses the
-// scratch reservation of the 48-system variant on MI355X (HSA_STATUS_ERROR_OUT_OF_RESOURCES at its
-// first dispatch, even for two waves), so a Newton step with more than 24 systems needs the
-// cooperative form (no private memory).  The batch entry point always runs cooperatively.
+// per lane for up to 12 systems, 25 KB for up to 24.  The 48-system variant would need 84 KB, a
+// scratch reservation the runtime refuses on MI355X (HSA_STATUS_ERROR_OUT_OF_RESOURCES at the first
+// dispatch), so its matrices live in the context's per-lane workspace (cp::LaneWs, 5-6 KB private
+// left).  The batch entry point always runs cooperatively.
 int check_qp_form(const pa_context *c) {
   if (c->qp_form != PA_QP_LANE) return fail(PA_ERR_UNSUPPORTED, "only the one-lane qp form takes Newton steps");
   return 0;
@@ -3187,3 +3194,4 @@ int pa_time_kernel(pa_context *c, int which, int reps, double *avg_ms) {
 }
 
 }  // extern "C"
+#include "cp_batch.h"

@@ -351,6 +351,105 @@ class Context:
         return out
 
 
+class Batch:
+    """Many independent RVEs (each the reference's boundary-value problem on its own box mesh)
+    advanced together through the C ABI's batch entry points: one persistent qp pass over all of
+    them per assembly, per-RVE Jacobi-PCG in shared launches, Newton iterations replayed from HIP
+    graphs.  mats: list of PaMaterial (at most 4); rve_mat[r]: material index of RVE r;
+    rve_nel[r]: (nx, ny, nz) elements of RVE r; pb: PaProblem (span, boundary rules, increments,
+    Newton count, Krylov tolerance) shared by all RVEs."""
+
+    STATE = {"T": (0, 9), "Fp": (1, 9), "Fe": (2, 9), "s": (3, "sys"), "rotnew": (4, 3), "slip": (5, "sys"), "u": (6, None),
+             "R": (7, None)}
+
+    def __init__(self, mats, rve_mat, rve_nel, pb: PaProblem):
+        L = native.lib()
+        self.mats = list(mats)
+        self.rve_mat = np.ascontiguousarray(rve_mat, dtype=np.int32)
+        self.rve_nel = np.ascontiguousarray(np.asarray(rve_nel, dtype=np.int32).reshape(-1, 3))
+        self.n_rve = len(self.rve_mat)
+        arr = (PaMaterial * len(self.mats))(*self.mats)
+        self._h = ctypes.c_void_p()
+        check(L.pa_batch_create(arr, ctypes.c_int(len(self.mats)), ctypes.byref(pb), ctypes.c_int(self.n_rve), ptr(self.rve_mat),
+                                ptr(self.rve_nel), ctypes.byref(self._h)))
+        self.pb = pb
+        sz = np.zeros(4, dtype=np.int64)
+        check(L.pa_batch_sizes(self._h, ptr(sz)))
+        self.nqp, self.nelem, self.nnode, self.ndof = (int(v) for v in sz)
+
+    def close(self):
+        if self._h:
+            native.lib().pa_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_grains(self, rve, elem_grain, grain_rod):
+        eg = np.ascontiguousarray(elem_grain, dtype=np.uint32)
+        rod = f64(grain_rod)
+        check(native.lib().pa_batch_set_grains(self._h, ctypes.c_int(rve), ptr(eg), ptr(rod), ctypes.c_int(len(rod))))
+
+    def set_precision(self, precision):
+        check(native.lib().pa_batch_set_precision(self._h, ctypes.c_int(precision)))
+
+    def set_graphs(self, on):
+        check(native.lib().pa_batch_set_graphs(self._h, ctypes.c_int(1 if on else 0)))
+
+    def newton_iteration(self, inc, it):
+        """|R| and CG iterations of every RVE."""
+        rn = np.zeros(self.n_rve)
+        its = np.zeros(self.n_rve, dtype=np.int32)
+        check(native.lib().pa_batch_newton_iteration(self._h, ctypes.c_int(inc), ctypes.c_int(it), ptr(rn), ptr(its)))
+        return rn, its
+
+    def commit_increment(self):
+        """stressstrain.txt rows of every RVE, shape (n_rve, 15)."""
+        rows = np.zeros((self.n_rve, 15))
+        check(native.lib().pa_batch_commit_increment(self._h, ptr(rows)))
+        return rows
+
+    def run_increment(self, inc):
+        res, its = [], []
+        for it in range(self.pb.max_newton):
+            r, k = self.newton_iteration(inc, it)
+            res.append(r)
+            its.append(k)
+        return self.commit_increment(), np.array(its).T, np.array(res).T
+
+    def tune(self):
+        """Autotuner report: dict(grid, chunk, qp_ms per grid candidate, krylov_ms per chunk candidate)."""
+        rep = np.zeros(9)
+        check(native.lib().pa_batch_tune(self._h, ptr(rep)))
+        return {"grid": int(rep[0]), "chunk": int(rep[1]), "qp_ms": dict(zip((256, 512, 768, 1024), rep[2:6].tolist())),
+                "krylov_ms": dict(zip((8, 16, 32), rep[6:9].tolist()))}
+
+    def qp_pass_ms(self, flag=1):
+        ms = ctypes.c_double()
+        check(native.lib().pa_batch_qp_pass_ms(self._h, ctypes.c_int(flag), ctypes.byref(ms)))
+        return ms.value
+
+    def last_timings(self):
+        ms = np.zeros(4)
+        check(native.lib().pa_batch_last_timings(self._h, ptr(ms)))
+        return ms
+
+    def get_state(self, rve, name):
+        which, comps = self.STATE[name]
+        nel = int(np.prod(self.rve_nel[rve]))
+        if comps is None:
+            out = np.zeros(3 * int(np.prod(self.rve_nel[rve] + 1)))
+        else:
+            if comps == "sys":
+                comps = self.mats[self.rve_mat[rve]].n_slip
+            out = np.zeros((8 * nel, comps))
+        check(native.lib().pa_batch_get_state(self._h, ctypes.c_int(rve), ctypes.c_int(which), ptr(out), ctypes.c_longlong(out.size)))
+        return out
+
+
 def pinv_solve(A, b, path=0):
     """x = pinv(A) b for a batch of n x n systems (A: (count, n, n), b: (count, n)) on the GPU:
     the pseudo-inverse step of inactiveSlipRemoval.cc:26,99; path 0 = one lane per system,
