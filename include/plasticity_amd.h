@@ -293,9 +293,9 @@ int pa_qp_update_warm(pa_context *ctx, pa_mlp *net, long long nqp, const double 
 /* ---- Batched RVEs.  No counterpart in the reference, which runs one RVE per job; each RVE of a
  * batch is the reference's boundary-value problem (ellipticBVP<3> + crystalPlasticity<3> on its own
  * box mesh), and all of them advance together: one Newton iteration of every RVE (assemble.cc:6-104
- * per RVE, then a Jacobi-preconditioned CG solve per RVE, solveLinearSystem.cc:8-36 with the PCG
- * option of pa_set_solver) runs in the same launches, the qp work of all RVEs drained by one
- * persistent kernel from a device work queue, replayed from HIP graphs captured once per batch.
+ * per RVE, then a Jacobi-BiCG or Jacobi-PCG solve per RVE, solveLinearSystem.cc:8-36) runs in the
+ * same launches, the qp work of all RVEs drained by one persistent kernel from a device work queue,
+ * replayed from HIP graphs captured once per batch.
  *   mats: 1..4 materials (no twinning, no kinematic hardening); rve_mat[r]: material of RVE r;
  *   rve_nel[3r..3r+2]: its elements along x, y, z; pb: domain span, boundary rules (simple face
  *   rules only), increments, Newton count and Krylov tolerance, shared by every RVE (pb->nx/ny/nz
@@ -307,8 +307,9 @@ void pa_batch_destroy(pa_batch *batch);
 /* Grain of each element of RVE `rve` (row into grain_rod, n_grain_rows x 3 Rodrigues vectors), as
  * pa_set_grains for one context (crystalPlasticity::loadOrientations, loadOrientations.cc:4-33). */
 int pa_batch_set_grains(pa_batch *batch, int rve, const unsigned *elem_grain, const double *grain_rod, int n_grain_rows);
-/* PA_PREC_FP64 or PA_PREC_MIXED (FP32 element tangents and matrices), as pa_set_solver. */
-int pa_batch_set_precision(pa_batch *batch, int precision);
+/* Krylov method (PA_SOLVER_BICG, the reference's and the default, or PA_SOLVER_CG) and arithmetic
+ * form (PA_PREC_FP64 or PA_PREC_MIXED: FP32 element tangents and matrices), as pa_set_solver. */
+int pa_batch_set_solver(pa_batch *batch, int method, int precision);
 /* Replay the Newton iteration from HIP graphs (1, the default) or launch its kernels directly (0);
  * the results are bitwise the same. */
 int pa_batch_set_graphs(pa_batch *batch, int on);

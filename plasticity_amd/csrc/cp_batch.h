@@ -10,9 +10,9 @@
 // material classes (12, 24 or 48 slip systems) form one work queue, drained by one persistent
 // launch of one-wave blocks: a wavefront claims 64 consecutive qps at a time, so qps with many
 // active systems or local iterations do not hold up a fixed share of the grid.  Each RVE's
-// Jacobian is its own node-block matrix; the Krylov solver (Jacobi-PCG) runs all RVEs in the same
-// launches, every block belonging to one RVE, with per-RVE scalars, stopping tests and iteration
-// counts.  A Newton iteration is replayed from HIP graphs (assembly, chunks of Krylov iterations,
+// Jacobian is its own node-block matrix; the Krylov solver (Jacobi-BiCG, the reference's, or
+// Jacobi-PCG) runs all RVEs in the same launches, every block belonging to one RVE, with per-RVE
+// scalars, stopping tests and iteration counts.  A Newton iteration is replayed from HIP graphs (assembly, chunks of Krylov iterations,
 // update), captured once per batch; an autotuner picks the qp pass's grid and the Krylov chunk.
 #pragma once
 
@@ -190,9 +190,10 @@ __global__ void k_zero_words(size_t n, unsigned *p) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
 }
 
-// ---- Jacobi-PCG over every RVE (the PCG option of the single-RVE solver, per RVE): block b of the
-// Krylov kernels covers VB nodes of RVE krve[b]; each kernel sums its RVE's block partials in the
-// same order in every block, so all blocks of an RVE take the same decision.
+// ---- Krylov solvers over every RVE, per RVE the single-RVE solvers' recurrences (Jacobi-PCG; and
+// Jacobi-BiCG, whose left sequence rides along in the same kernels): block b of the Krylov kernels
+// covers VB nodes of RVE krve[b]; each kernel sums its RVE's block partials in the same order in
+// every block, so all blocks of an RVE take the same decision.
 struct BKState {
   double tol, bnorm, dp, rzh[2];
   int it, done, breakdown, maxit, pad;
@@ -203,7 +204,8 @@ struct BKArgs {
   BKState *st;
   const void *K;
   const double *B, *dinv;
-  double *X, *R, *Z, *P, *Q;
+  double *X, *R, *Z, *P, *Q;     // CG; BiCG: R, Z, P the right sequence (Q unused)
+  double *Rl, *Zl, *Pl;          // BiCG: the left sequence (null for CG)
   double *ipart, *opart, *upart;  // 3, 1 and 2 partials per block
 };
 
@@ -237,6 +239,7 @@ __global__ void __launch_bounds__(VB) kb_cg_init(BKArgs a) {
     for (int c = 0; c < 3; ++c) {
       const double b = a.B[g0 + c], z = b * a.dinv[g0 + c];
       a.X[g0 + c] = 0.0; a.R[g0 + c] = b; a.Z[g0 + c] = z;
+      if (a.Rl) { a.Rl[g0 + c] = b; a.Zl[g0 + c] = z; }
       acc[0] += b * b; acc[1] += z * z; acc[2] += z * b;
     }
   }
@@ -265,7 +268,8 @@ __global__ void __launch_bounds__(VB) kb_cg_init2(BKArgs a, double lin_tol, int 
 }
 
 // iteration it (parity par = it & 1): it = 0: p = z; it >= 1: stopping test on ||z|| and
-// b = (r.z)_it / (r.z)_(it-1) from the update kernel's partials, p = z + b p
+// b = (r.z)_it / (r.z)_(it-1) from the update kernel's partials, p = z + b p (BiCG: both sequences,
+// r.z meaning z.rl)
 __global__ void __launch_bounds__(VB) kb_cg_p(BKArgs a, int first, int par) {
   __shared__ double lds[32];
   const int r = a.krve[blockIdx.x];
@@ -295,7 +299,10 @@ __global__ void __launch_bounds__(VB) kb_cg_p(BKArgs a, int first, int par) {
   if (nl < R.m.nnode) {
     const long long g0 = 3 * (R.node_off + nl);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) a.P[g0 + c] = first ? a.Z[g0 + c] : a.Z[g0 + c] + b * a.P[g0 + c];
+    for (int c = 0; c < 3; ++c) {
+      a.P[g0 + c] = first ? a.Z[g0 + c] : a.Z[g0 + c] + b * a.P[g0 + c];
+      if (a.Pl) a.Pl[g0 + c] = first ? a.Zl[g0 + c] : a.Zl[g0 + c] + b * a.Pl[g0 + c];
+    }
   }
 }
 
@@ -355,6 +362,75 @@ __global__ void __launch_bounds__(VB) kb_cg_update(BKArgs a, int par) {
       a.Z[g] = z;
       acc[0] += z * z;
       acc[1] += rn * z;
+    }
+  }
+  __syncthreads();
+  block_partials<2>(acc, a.upart, lds);
+  if (blockIdx.x == R.kblk0 && threadIdx.x == 0) st->it += 1;
+}
+
+// BiCG (the reference's KSPBICG with PCJACOBI, solveLinearSystem.cc:13-16, per RVE): Zr = K Pr,
+// Zl = K^T Pl on the RVE's node-block matrix; partials of Zr.Pl
+template <typename VT>
+__global__ void __launch_bounds__(VB) kb_bicg_spmv(BKArgs a) {
+  __shared__ double lds[16];
+  const int r = a.krve[blockIdx.x];
+  if (__syncthreads_or(a.st[r].done != 0)) return;
+  const BRve R = a.rve[r];
+  const long long nl = (long long)(blockIdx.x - R.kblk0) * VB + threadIdx.x;
+  double acc[1] = {0.0};
+  if (nl < R.m.nnode) {
+    int i, j, k;
+    node_ijk32(R.m, nl, i, j, k);
+    const VT *K = (const VT *)a.K + R.kof;
+    const double *Pr = a.P + 3 * R.node_off, *Pl = a.Pl + 3 * R.node_off;
+    const long long nn = R.m.nnode;
+    double y[3] = {0, 0, 0}, yt[3] = {0, 0, 0};
+#pragma unroll 3
+    for (int s = 0; s < 27; ++s) {
+      bool ok;
+      const long long mn = nb_masked(R.m, nl, i, j, k, s, ok);
+      const double w = ok ? 1.0 : 0.0;
+      double pr[3], pl[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { pr[c] = w * Pr[3 * mn + c]; pl[c] = w * Pl[3 * mn + c]; }
+      block_mv(K + ((long long)s * nn + nl) * 9, pr[0], pr[1], pr[2], y);
+      block_tmv(K + ((long long)(26 - s) * nn + mn) * 9, pl[0], pl[1], pl[2], yt);
+    }
+    double *Zr = a.Z + 3 * R.node_off, *Zl = a.Zl + 3 * R.node_off;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { Zr[3 * nl + c] = y[c]; Zl[3 * nl + c] = yt[c]; }
+    acc[0] = y[0] * Pl[3 * nl] + y[1] * Pl[3 * nl + 1] + y[2] * Pl[3 * nl + 2];
+  }
+  block_partials<1>(acc, a.opart, lds);
+}
+
+// a = (r.z)_it / (Zr.Pl); x += a Pr; r -= a Zr; rl -= a Zl; z = D^-1 r; zl = D^-1 rl; partials of
+// z.z and z.rl; it += 1
+__global__ void __launch_bounds__(VB) kb_bicg_update(BKArgs a, int par) {
+  __shared__ double lds[32];
+  const int r = a.krve[blockIdx.x];
+  BKState *st = a.st + r;
+  if (__syncthreads_or(st->done != 0)) return;
+  const BRve R = a.rve[r];
+  double dpi[1];
+  rve_partials<1>(a.opart, R.kblk0, R.nkblk, dpi, lds);
+  const double al = st->rzh[par] / dpi[0];
+  const long long nl = (long long)(blockIdx.x - R.kblk0) * VB + threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+  if (nl < R.m.nnode) {
+    const long long g0 = 3 * (R.node_off + nl);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long g = g0 + c;
+      const double d = a.dinv[g];
+      a.X[g] += al * a.P[g];
+      const double rrn = a.R[g] - al * a.Z[g], rln = a.Rl[g] - al * a.Zl[g];
+      a.R[g] = rrn; a.Rl[g] = rln;
+      const double zrn = rrn * d, zln = rln * d;
+      a.Z[g] = zrn; a.Zl[g] = zln;
+      acc[0] += zrn * zrn;
+      acc[1] += zrn * rln;
     }
   }
   __syncthreads();
@@ -445,6 +521,7 @@ struct pa_batch {
   pa_problem pb;
   int nrve = 0, ncls = 0, nsmax = 0;
   int precision = PA_PREC_FP64;
+  int method = PA_SOLVER_BICG;  // PA_SOLVER_BICG (the reference's) or PA_SOLVER_CG
   std::vector<BRve> hrve;
   std::vector<int> rve_mat;
   long long nq = 0, nelem = 0, nnode = 0, ndof = 0;
@@ -470,7 +547,7 @@ struct pa_batch {
   double *u = nullptr, *R = nullptr, *dinv = nullptr, *delu = nullptr, *P = nullptr;
   unsigned char *con = nullptr;
   void *K = nullptr, *D = nullptr;
-  double *kv[5] = {nullptr};  // X R Z P Q
+  double *kv[8] = {nullptr};  // X R Z P Q Rl Zl Pl
   BKState *st = nullptr, *hst = nullptr;  // device state; pinned host copies (2 x nrve)
   double *ipart = nullptr, *opart = nullptr, *upart = nullptr, *npart = nullptr, *ppart = nullptr;
   double *ws = nullptr;
@@ -478,8 +555,8 @@ struct pa_batch {
   // launch choices (pa_batch_tune) and graphs
   int qp_grid = 512, chunk = 16;
   bool use_graphs = true;
-  hipGraphExec_t g_asm[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [mixed][iter0]
-  hipGraphExec_t g_chunk[2] = {nullptr, nullptr};                          // [mixed]
+  hipGraphExec_t g_asm[2][2][2] = {};  // [BiCG][mixed][iter0]
+  hipGraphExec_t g_chunk[2][2] = {};   // [BiCG][mixed]
   double last_ms[4] = {0, 0, 0, 0};  // assembly, Krylov, post-increment, qp pass (eager runs)
   std::vector<double> tune;          // last autotuner report
   std::vector<void *> allocs;
@@ -507,12 +584,14 @@ int bupload(pa_batch *b, T **p, const std::vector<T> &h) {
 bool bmixed(const pa_batch *b) { return b->precision == PA_PREC_MIXED; }
 size_t bkbytes(const pa_batch *b) { return (bmixed(b) ? sizeof(float) : sizeof(double)) * 243 * (size_t)b->nnode; }
 
+bool bbicg(const pa_batch *b) { return b->method == PA_SOLVER_BICG; }
 void batch_drop_graphs(pa_batch *b) {
-  for (int f = 0; f < 2; ++f) {
-    for (int i = 0; i < 2; ++i)
-      if (b->g_asm[f][i]) { (void)hipGraphExecDestroy(b->g_asm[f][i]); b->g_asm[f][i] = nullptr; }
-    if (b->g_chunk[f]) { (void)hipGraphExecDestroy(b->g_chunk[f]); b->g_chunk[f] = nullptr; }
-  }
+  for (int m = 0; m < 2; ++m)
+    for (int f = 0; f < 2; ++f) {
+      for (int i = 0; i < 2; ++i)
+        if (b->g_asm[m][f][i]) { (void)hipGraphExecDestroy(b->g_asm[m][f][i]); b->g_asm[m][f][i] = nullptr; }
+      if (b->g_chunk[m][f]) { (void)hipGraphExecDestroy(b->g_chunk[m][f]); b->g_chunk[m][f] = nullptr; }
+    }
 }
 
 BArgs batch_qp_args(pa_batch *b) {
@@ -531,6 +610,8 @@ BKArgs batch_k_args(pa_batch *b) {
   a.rve = b->d_rve; a.krve = b->d_krve; a.st = b->st; a.K = b->K;
   a.B = b->R; a.dinv = b->dinv;
   a.X = b->kv[0]; a.R = b->kv[1]; a.Z = b->kv[2]; a.P = b->kv[3]; a.Q = b->kv[4];
+  const bool bi = bbicg(b);
+  a.Rl = bi ? b->kv[5] : nullptr; a.Zl = bi ? b->kv[6] : nullptr; a.Pl = bi ? b->kv[7] : nullptr;
   a.ipart = b->ipart; a.opart = b->opart; a.upart = b->upart;
   return a;
 }
@@ -543,6 +624,26 @@ int batch_qp_pass(pa_batch *b, int flag, int grid) {
   else hipLaunchKernelGGL(kb_qp_pass<double>, dim3(grid), dim3(64), 0, b->stream, a, flag);
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+// operator and update of one Krylov iteration (parity par)
+void batch_op_update(pa_batch *b, const BKArgs &k, int par) {
+  if (bbicg(b)) {
+    if (bmixed(b)) hipLaunchKernelGGL(kb_bicg_spmv<float>, dim3(b->nkblk), dim3(VB), 0, b->stream, k);
+    else hipLaunchKernelGGL(kb_bicg_spmv<double>, dim3(b->nkblk), dim3(VB), 0, b->stream, k);
+    hipLaunchKernelGGL(kb_bicg_update, dim3(b->nkblk), dim3(VB), 0, b->stream, k, par);
+  } else {
+    if (bmixed(b)) hipLaunchKernelGGL(kb_cg_spmv<float>, dim3(b->nkblk), dim3(VB), 0, b->stream, k);
+    else hipLaunchKernelGGL(kb_cg_spmv<double>, dim3(b->nkblk), dim3(VB), 0, b->stream, k);
+    hipLaunchKernelGGL(kb_cg_update, dim3(b->nkblk), dim3(VB), 0, b->stream, k, par);
+  }
+}
+// Krylov initialisation and iteration 0 on the assembled system
+void batch_krylov_start(pa_batch *b, const BKArgs &k) {
+  hipLaunchKernelGGL(kb_cg_init, dim3(b->nkblk), dim3(VB), 0, b->stream, k);
+  hipLaunchKernelGGL(kb_cg_init2, dim3(b->nrve), dim3(VB), 0, b->stream, k, b->pb.lin_tol, b->pb.lin_maxit);
+  hipLaunchKernelGGL(kb_cg_p, dim3(b->nkblk), dim3(VB), 0, b->stream, k, 1, 0);
+  batch_op_update(b, k, 0);
 }
 
 // assembly of every RVE and Krylov iteration 0: no memset (fill kernels only), so the sequence can
@@ -567,12 +668,7 @@ int batch_assemble_seq(pa_batch *b, int iter0) {
   else hipLaunchKernelGGL(kb_dinv<double>, dim3(gn), dim3(256), 0, b->stream, b->d_rve, b->d_nrve, b->nnode, (const double *)b->K, b->dinv);
   const BKArgs k = batch_k_args(b);
   hipLaunchKernelGGL(kb_norm_parts, dim3(b->nkblk), dim3(VB), 0, b->stream, k, b->npart);
-  hipLaunchKernelGGL(kb_cg_init, dim3(b->nkblk), dim3(VB), 0, b->stream, k);
-  hipLaunchKernelGGL(kb_cg_init2, dim3(b->nrve), dim3(VB), 0, b->stream, k, b->pb.lin_tol, b->pb.lin_maxit);
-  hipLaunchKernelGGL(kb_cg_p, dim3(b->nkblk), dim3(VB), 0, b->stream, k, 1, 0);
-  if (bmixed(b)) hipLaunchKernelGGL(kb_cg_spmv<float>, dim3(b->nkblk), dim3(VB), 0, b->stream, k);
-  else hipLaunchKernelGGL(kb_cg_spmv<double>, dim3(b->nkblk), dim3(VB), 0, b->stream, k);
-  hipLaunchKernelGGL(kb_cg_update, dim3(b->nkblk), dim3(VB), 0, b->stream, k, 0);
+  batch_krylov_start(b, k);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -582,9 +678,7 @@ int batch_chunk_seq(pa_batch *b) {
   for (int i = 0; i < b->chunk; ++i) {
     const int par = (1 + i) & 1;
     hipLaunchKernelGGL(kb_cg_p, dim3(b->nkblk), dim3(VB), 0, b->stream, k, 0, par);
-    if (bmixed(b)) hipLaunchKernelGGL(kb_cg_spmv<float>, dim3(b->nkblk), dim3(VB), 0, b->stream, k);
-    else hipLaunchKernelGGL(kb_cg_spmv<double>, dim3(b->nkblk), dim3(VB), 0, b->stream, k);
-    hipLaunchKernelGGL(kb_cg_update, dim3(b->nkblk), dim3(VB), 0, b->stream, k, par);
+    batch_op_update(b, k, par);
   }
   HIPCHK(hipGetLastError());
   return 0;
@@ -609,7 +703,7 @@ int batch_capture(pa_batch *b, hipGraphExec_t *out, F &&seq) {
 }
 int batch_assemble(pa_batch *b, int iter0) {
   if (!b->use_graphs) return batch_assemble_seq(b, iter0);
-  hipGraphExec_t &g = b->g_asm[bmixed(b) ? 1 : 0][iter0 ? 1 : 0];
+  hipGraphExec_t &g = b->g_asm[bbicg(b) ? 1 : 0][bmixed(b) ? 1 : 0][iter0 ? 1 : 0];
   if (!g)
     if (int rc = batch_capture(b, &g, [&] { return batch_assemble_seq(b, iter0); })) return rc;
   HIPCHK(hipGraphLaunch(g, b->stream));
@@ -617,7 +711,7 @@ int batch_assemble(pa_batch *b, int iter0) {
 }
 int batch_chunk(pa_batch *b) {
   if (!b->use_graphs) return batch_chunk_seq(b);
-  hipGraphExec_t &g = b->g_chunk[bmixed(b) ? 1 : 0];
+  hipGraphExec_t &g = b->g_chunk[bbicg(b) ? 1 : 0][bmixed(b) ? 1 : 0];
   if (!g)
     if (int rc = batch_capture(b, &g, [&] { return batch_chunk_seq(b); })) return rc;
   HIPCHK(hipGraphLaunch(g, b->stream));
@@ -861,7 +955,7 @@ int pa_batch_create(const pa_material *mats, int n_mats, const pa_problem *pb, i
     pa_batch_destroy(b);
     return rc;
   }
-  for (int i = 0; i < 5; ++i)
+  for (int i = 0; i < 8; ++i)
     if ((rc = balloc(b, &b->kv[i], N))) { pa_batch_destroy(b); return rc; }
   if (hipHostMalloc((void **)&b->hst, 2 * sizeof(BKState) * n_rve, hipHostMallocDefault) != hipSuccess) {
     pa_batch_destroy(b);
@@ -886,9 +980,11 @@ int pa_batch_set_grains(pa_batch *b, int rve, const unsigned *elem_grain, const 
   return 0;
 }
 
-int pa_batch_set_precision(pa_batch *b, int precision) {
+int pa_batch_set_solver(pa_batch *b, int method, int precision) {
   if (!b) return fail(PA_ERR_ARG, "null batch");
+  if (method != PA_SOLVER_BICG && method != PA_SOLVER_CG) return fail(PA_ERR_ARG, "batch: Jacobi-BiCG or Jacobi-PCG");
   if (precision != PA_PREC_FP64 && precision != PA_PREC_MIXED) return fail(PA_ERR_ARG, "unknown precision");
+  b->method = method;
   b->precision = precision;
   return 0;
 }
@@ -999,13 +1095,7 @@ int pa_batch_tune(pa_batch *b, double *report) {
   int bk = 0;
   for (int i = 0; i < 3; ++i) {
     b->chunk = chunks[i];
-    const BKArgs k = batch_k_args(b);
-    hipLaunchKernelGGL(kb_cg_init, dim3(b->nkblk), dim3(VB), 0, b->stream, k);
-    hipLaunchKernelGGL(kb_cg_init2, dim3(b->nrve), dim3(VB), 0, b->stream, k, b->pb.lin_tol, b->pb.lin_maxit);
-    hipLaunchKernelGGL(kb_cg_p, dim3(b->nkblk), dim3(VB), 0, b->stream, k, 1, 0);
-    if (bmixed(b)) hipLaunchKernelGGL(kb_cg_spmv<float>, dim3(b->nkblk), dim3(VB), 0, b->stream, k);
-    else hipLaunchKernelGGL(kb_cg_spmv<double>, dim3(b->nkblk), dim3(VB), 0, b->stream, k);
-    hipLaunchKernelGGL(kb_cg_update, dim3(b->nkblk), dim3(VB), 0, b->stream, k, 0);
+    batch_krylov_start(b, batch_k_args(b));
     HIPCHK(hipEventRecord(b->ev[0], b->stream));
     if (int rc = batch_krylov(b, nullptr)) { b->use_graphs = graphs; return rc; }
     HIPCHK(hipEventRecord(b->ev[1], b->stream));

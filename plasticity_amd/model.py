@@ -354,8 +354,8 @@ class Context:
 class Batch:
     """Many independent RVEs (each the reference's boundary-value problem on its own box mesh)
     advanced together through the C ABI's batch entry points: one persistent qp pass over all of
-    them per assembly, per-RVE Jacobi-PCG in shared launches, Newton iterations replayed from HIP
-    graphs.  mats: list of PaMaterial (at most 4); rve_mat[r]: material index of RVE r;
+    them per assembly, per-RVE Jacobi-BiCG (or PCG) in shared launches, Newton iterations replayed
+    from HIP graphs.  mats: list of PaMaterial (at most 4); rve_mat[r]: material index of RVE r;
     rve_nel[r]: (nx, ny, nz) elements of RVE r; pb: PaProblem (span, boundary rules, increments,
     Newton count, Krylov tolerance) shared by all RVEs."""
 
@@ -393,8 +393,8 @@ class Batch:
         rod = f64(grain_rod)
         check(native.lib().pa_batch_set_grains(self._h, ctypes.c_int(rve), ptr(eg), ptr(rod), ctypes.c_int(len(rod))))
 
-    def set_precision(self, precision):
-        check(native.lib().pa_batch_set_precision(self._h, ctypes.c_int(precision)))
+    def set_solver(self, method=native.PA_SOLVER_BICG, precision=native.PA_PREC_FP64):
+        check(native.lib().pa_batch_set_solver(self._h, ctypes.c_int(method), ctypes.c_int(precision)))
 
     def set_graphs(self, on):
         check(native.lib().pa_batch_set_graphs(self._h, ctypes.c_int(1 if on else 0)))

@@ -25,7 +25,7 @@ EXPORTS = [
     "pa_get_state", "pa_get_csr", "pa_sizes", "pa_qp_update", "pa_last_timings", "pa_last_qp_pass_ms", "pa_time_kernel",
     "pa_last_counts", "pa_pinv_solve", "pa_pinv_solve_noalloc", "pa_mlp_create", "pa_mlp_destroy", "pa_mlp_set_params", "pa_mlp_get_params",
     "pa_mlp_predict", "pa_mlp_grad", "pa_mlp_train", "pa_qp_update_warm", "pa_grain_averages",
-    "pa_batch_create", "pa_batch_destroy", "pa_batch_set_grains", "pa_batch_set_precision", "pa_batch_set_graphs",
+    "pa_batch_create", "pa_batch_destroy", "pa_batch_set_grains", "pa_batch_set_solver", "pa_batch_set_graphs",
     "pa_batch_newton_iteration", "pa_batch_commit_increment", "pa_batch_tune", "pa_batch_qp_pass_ms", "pa_batch_last_timings",
     "pa_batch_get_state", "pa_batch_sizes",
 ]

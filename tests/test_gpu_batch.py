@@ -1,6 +1,6 @@
 """Batched RVEs on the GPU (pa_batch_*): RVEs of uneven size and different materials (FCC 12, HCP 24
 and BCC 48 slip systems) advanced in the same launches, each checked against the oracle run on that
-RVE alone, with the Krylov solves (Jacobi-PCG, the PCG option of the single-RVE solver) at 1e-14.
+RVE alone, with the Krylov solves (Jacobi-BiCG, the reference's, and Jacobi-PCG) at 1e-14.
 
 * curves against each RVE's own oracle run at 1e-9 where the oracle's result does not move with
   its Krylov tolerance, and every RVE's committed per-qp state against the oracle's qp update on
@@ -37,7 +37,7 @@ def _tables():
 RVES = [(0, 4, 4, 4, 4), (1, 3, 4, 5, 3), (2, 3, 3, 3, 2), (0, 5, 3, 4, 5), (1, 4, 4, 4, 4)]
 
 
-def _setup(lin_tol=1e-14):
+def _setup(lin_tol=1e-14, solver=1):
     tabs = _tables()
     bcs = [(1, 1, 0.0), (1, 2, 0.0), (1, 3, 0.0), (2, 1, 1e-3 * N_INC)]
     cases, grains = [], []
@@ -45,7 +45,7 @@ def _setup(lin_tol=1e-14):
     for k, nx, ny, nz, ng in RVES:
         t = tabs[k]
         c = refcase.Case(t["m"], t["n"], t["q"], t["C"], t["s0"], t["h0"], t["ss"], t["pw"], tol=t["tol"], nx=nx, ny=ny, nz=nz,
-                         total_time=float(N_INC), bcs=bcs, lin_tol=lin_tol, solver=1)
+                         total_time=float(N_INC), bcs=bcs, lin_tol=lin_tol, solver=solver)
         c.grain_rod = rng.uniform(-0.5, 0.5, size=(ng, 3))
         c.elem_grain = rng.integers(0, ng, size=nx * ny * nz).astype(np.uint32)
         cases.append(c)
@@ -54,11 +54,12 @@ def _setup(lin_tol=1e-14):
     return mats, pb, cases
 
 
-def _batch(mats, pb, cases, graphs=True):
+def _batch(mats, pb, cases, graphs=True, solver=1):
     b = pa.Batch(mats, [r[0] for r in RVES], [r[1:4] for r in RVES], pb)
     for i, c in enumerate(cases):
         b.set_grains(i, c.elem_grain, c.grain_rod)
     b.set_graphs(graphs)
+    b.set_solver(native.PA_SOLVER_CG if solver == 1 else native.PA_SOLVER_BICG, native.PA_PREC_FP64)
     return b
 
 
@@ -95,7 +96,8 @@ def _qp_F(u, nx, ny, nz, h):
     return F
 
 
-def test_batch_rves_match_their_own_oracle_runs():
+@pytest.mark.parametrize("solver", [0, 1], ids=["bicg", "pcg"])
+def test_batch_rves_match_their_own_oracle_runs(solver):
     """Every RVE against the oracle on that RVE alone.
     * Curves: the two HCP RVEs do not move when the oracle's Krylov tolerance goes from 1e-14 to
       3e-15, 1e-12 or 1e-11 (spread <= 1e-11) and are held to 1e-9 over all increments, with their
@@ -108,9 +110,9 @@ def test_batch_rves_match_their_own_oracle_runs():
       batch's u, Fp and s committed after increment 0), at 1e-10, except qps whose oracle result
       moves under 1e-14 relative perturbations of F (round-off-dependent singular active sets)."""
     from test_gpu_parity import _roundoff_dependent
-    mats, pb, cases = _setup()
-    others = [_setup(lin_tol=t)[2] for t in (3e-15, 1e-12, 1e-11)]
-    b = _batch(mats, pb, cases)
+    mats, pb, cases = _setup(solver=solver)
+    others = [_setup(lin_tol=t, solver=solver)[2] for t in (3e-15, 1e-12, 1e-11)]
+    b = _batch(mats, pb, cases, solver=solver)
     print("batch: %d RVEs, %d qps" % (b.n_rve, b.nqp), flush=True)
     rows = [b.run_increment(0)[0]]
     before = [(b.get_state(i, "Fp"), b.get_state(i, "s")) for i in range(len(cases))]
@@ -162,11 +164,12 @@ def test_batch_rves_match_their_own_oracle_runs():
     b.close()
 
 
-def test_batch_graph_replay_equals_direct_launches_bitwise():
+@pytest.mark.parametrize("solver", [0, 1], ids=["bicg", "pcg"])
+def test_batch_graph_replay_equals_direct_launches_bitwise(solver):
     mats, pb, cases = _setup(lin_tol=1e-10)
     out = []
     for graphs in (True, False):
-        b = _batch(mats, pb, cases, graphs)
+        b = _batch(mats, pb, cases, graphs, solver)
         rows, its = _run(b, 3)
         out.append((rows, its, [b.get_state(i, "s") for i in range(len(RVES))]))
         b.close()
