@@ -37,20 +37,18 @@ struct BClass {
   int gstride, ns, NS, msi;
 };
 
-// Per-lane singular-system functor of the batch: the pseudo-inverse in the lane, U and V in its
-// workspace.
-template <int NS>
-struct SvdWs {
-  double *uv;
-  __device__ bool operator()(int n, const double *A, const double *b, double *x) const {
-    cp::svd_pinv_solve_ws<NS>(n, A, b, x, uv);
-    return true;
-  }
+// Numerically singular local systems of one material class: the queue of qps deferred by the
+// first pass and the store of their systems (the single-RVE passes' pipeline, SingAny: queued in
+// pass 1, solved by the cooperative k_sing_svd, replayed; later singular solves in the lane).
+struct BSing {
+  SingStore st;
+  unsigned *queue, *count;
 };
 
 struct BArgs {
   const BRve *rve;
   const BClass *cls;
+  const BSing *sing;           // [class]
   const unsigned short *erve;  // [nelem] RVE of each element
   const unsigned *egrain;      // [nelem] row of the element's grain in its class table
   long long nq;
@@ -65,9 +63,10 @@ struct BArgs {
 };
 
 // calculatePlasticity(cellID, q, flag) at qp Q of the batch (flag 1: assembly stage 1, writes the
-// iter history, JxW dP/dF and JxW P; flag 0: the post-increment pass, writes the history and T)
+// iter history, JxW dP/dF and JxW P; flag 0: the post-increment pass, writes the history and T).
+// False when the singular-system functor deferred the qp (nothing written).
 template <int NS, typename DT>
-__device__ __attribute__((noinline)) void bq_one(const BArgs &a, long long Q, int flag, double *lw) {
+__device__ __attribute__((noinline)) bool bq_one(const BArgs &a, long long Q, int flag, SingAny<NS> &sing) {
   const BRve R = a.rve[a.erve[Q >> 3]];
   const BClass C = a.cls[R.cls];
   double F[9];
@@ -78,9 +77,9 @@ __device__ __attribute__((noinline)) void bq_one(const BArgs &a, long long Q, in
   for (int i = 0; i < C.ns; ++i) s[i] = a.s_conv[i * a.nq + Q];
   const double *gt = C.gtab + (long long)a.egrain[Q >> 3] * C.gstride;
   bool plastic = false;
-  const SvdWs<NS> sv{lw + (cp::lane_ws<NS>() ? cp::LaneWs<NS>::UV : 0)};
-  cp::qp_update<NS, DT, false, SvdWs<NS>>(C.mat, gt, C.ns, C.msi, F, Fp_t, s, x, flag, P, T, Fe, Fp, flag ? (DT *)a.D + Q : nullptr,
-                                         a.nq, &plastic, R.JxW, sv, nullptr, nullptr, 1, nullptr, 1 << 30, lw);
+  const int na = cp::qp_update<NS, DT, false>(C.mat, gt, C.ns, C.msi, F, Fp_t, s, x, flag, P, T, Fe, Fp, flag ? (DT *)a.D + Q : nullptr,
+                                              a.nq, &plastic, R.JxW, SingRef<NS>{&sing}, nullptr, nullptr, 1, nullptr, 1 << 30, sing.ws);
+  if (na < 0) return false;
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
     a.Fe_iter[i * a.nq + Q] = Fe[i];
@@ -91,9 +90,22 @@ __device__ __attribute__((noinline)) void bq_one(const BArgs &a, long long Q, in
   for (int i = 0; i < C.ns; ++i) a.s_iter[i * a.nq + Q] = s[i];
   if (plastic)  // slipfraction_iter is written on the plastic path only (calculatePlasticity.cc:454-456)
     for (int i = 0; i < C.ns; ++i) a.slip_iter[i * a.nq + Q] = a.slip_conv[i * a.nq + Q] + x[i];
+  return true;
+}
+// U and V of the in-lane pseudo-inverse live in the lane's workspace for every class
+template <int NS>
+__device__ __forceinline__ SingAny<NS> bsing(const BSing &S, int mode, unsigned Q, unsigned e, int have, double *lw) {
+  SingAny<NS> f{mode, S.st, S.count, S.queue, Q, e, have, 0, lw};
+  return f;
 }
 
-// The persistent qp pass: one-wave blocks drain the queue of all qps, 64 at a time per wavefront.
+template <int NS, typename DT>
+__device__ __forceinline__ void bq_claim(const BArgs &a, long long Q, int k, int flag, double *lw) {
+  SingAny<NS> f = bsing<NS>(a.sing[k], SING_CLAIM, (unsigned)Q, 0u, 0, lw);
+  bq_one<NS, DT>(a, Q, flag, f);  // a singular system defers the qp to the passes below
+}
+// The persistent qp pass: one-wave blocks drain the queue of all qps, 64 at a time per wavefront;
+// qps that meet a numerically singular A_PA are queued per class.
 template <typename DT>
 __global__ void __launch_bounds__(64) QP_WAVES kb_qp_pass(BArgs a, int flag) {
   __shared__ unsigned sbase;
@@ -106,12 +118,37 @@ __global__ void __launch_bounds__(64) QP_WAVES kb_qp_pass(BArgs a, int flag) {
     if (base >= a.nq) break;
     const long long Q = base + threadIdx.x;
     if (Q < a.nq) {
-      const int NS = a.cls[a.rve[a.erve[Q >> 3]].cls].NS;
-      if (NS == 48) bq_one<48, DT>(a, Q, flag, lw);
-      else if (NS == 24) bq_one<24, DT>(a, Q, flag, lw);
-      else bq_one<12, DT>(a, Q, flag, lw);
+      const int k = a.rve[a.erve[Q >> 3]].cls;
+      const int NS = a.cls[k].NS;
+      if (NS == 48) bq_claim<48, DT>(a, Q, k, flag, lw);
+      else if (NS == 24) bq_claim<24, DT>(a, Q, k, flag, lw);
+      else bq_claim<12, DT>(a, Q, k, flag, lw);
     }
   }
+}
+// class k's queued qps with the cooperative solutions of their first singular system
+template <int NS, typename DT>
+__global__ void __launch_bounds__(64) QP_WAVES kb_qp_replay(BArgs a, int k, int flag) {
+  const BSing S = a.sing[k];
+  double *lw = a.ws + (size_t)(blockIdx.x * 64 + threadIdx.x) * a.ws_lane;
+  const unsigned n = min(*S.count, S.st.cap);
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    if (S.st.done[e]) continue;
+    SingAny<NS> f = bsing<NS>(S, SING_REPLAY, 0u, e, S.st.have[e], lw);
+    if (bq_one<NS, DT>(a, S.queue[e], flag, f)) S.st.done[e] = 1;
+  }
+}
+// the rest of class k's queue (entries beyond the store, or deferred again): pseudo-inverse in the lane
+template <int NS, typename DT>
+__global__ void __launch_bounds__(64) QP_WAVES kb_qp_deferred(BArgs a, int k, int flag) {
+  const BSing S = a.sing[k];
+  double *lw = a.ws + (size_t)(blockIdx.x * 64 + threadIdx.x) * a.ws_lane;
+  const unsigned n = *S.count;
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (i >= S.st.cap || !S.st.done[i]) {
+      SingAny<NS> f = bsing<NS>(S, SING_LANE, 0u, i, 0, lw);
+      bq_one<NS, DT>(a, S.queue[i], flag, f);
+    }
 }
 
 // ---- assembly stage 2, Dirichlet lines, diagonal, residual norms
@@ -552,6 +589,9 @@ struct pa_batch {
   double *ipart = nullptr, *opart = nullptr, *upart = nullptr, *npart = nullptr, *ppart = nullptr;
   double *ws = nullptr;
   long long ws_lane = 0;
+  std::vector<BSing> hsing;  // per class: queue, count and store of the deferred singular systems
+  BSing *d_sing = nullptr;
+  unsigned *d_counts = nullptr;  // [class]: queue lengths; [4 + class]: replay misses
   // launch choices (pa_batch_tune) and graphs
   int qp_grid = 512, chunk = 16;
   bool use_graphs = true;
@@ -596,7 +636,7 @@ void batch_drop_graphs(pa_batch *b) {
 
 BArgs batch_qp_args(pa_batch *b) {
   BArgs a;
-  a.rve = b->d_rve; a.cls = b->d_cls; a.erve = b->d_erve; a.egrain = b->d_egrain;
+  a.rve = b->d_rve; a.cls = b->d_cls; a.sing = b->d_sing; a.erve = b->d_erve; a.egrain = b->d_egrain;
   a.nq = b->nq; a.u = b->u;
   a.Fp_conv = b->Fp_conv; a.s_conv = b->s_conv; a.slip_conv = b->slip_conv;
   a.Fe_iter = b->Fe_iter; a.Fp_iter = b->Fp_iter; a.s_iter = b->s_iter; a.slip_iter = b->slip_iter; a.T = b->T;
@@ -616,12 +656,32 @@ BKArgs batch_k_args(pa_batch *b) {
   return a;
 }
 
-// the qp pass over every qp of the batch (flag 1: assembly stage 1; flag 0: post-increment pass)
+template <int NS, typename DT>
+void batch_sing_passes(pa_batch *b, const BArgs &a, int k, int flag, int grid) {
+  hipLaunchKernelGGL(k_sing_svd<NS>, dim3(SVD_GRID), dim3(sing_per_block<NS>() * sing_group<NS>()), 0, b->stream, b->hsing[k].st,
+                     (const unsigned *)b->hsing[k].count);
+  hipLaunchKernelGGL((kb_qp_replay<NS, DT>), dim3(grid), dim3(64), 0, b->stream, a, k, flag);
+  hipLaunchKernelGGL((kb_qp_deferred<NS, DT>), dim3(grid), dim3(64), 0, b->stream, a, k, flag);
+}
+template <typename DT>
+void batch_qp_passes(pa_batch *b, const BArgs &a, int flag, int grid) {
+  hipLaunchKernelGGL(kb_qp_pass<DT>, dim3(grid), dim3(64), 0, b->stream, a, flag);
+  for (int k = 0; k < b->ncls; ++k) {
+    const int NS = b->hcls[k].NS;
+    if (NS == 48) batch_sing_passes<48, DT>(b, a, k, flag, grid);
+    else if (NS == 24) batch_sing_passes<24, DT>(b, a, k, flag, grid);
+    else batch_sing_passes<12, DT>(b, a, k, flag, grid);
+  }
+}
+// the qp passes over every qp of the batch (flag 1: assembly stage 1; flag 0: post-increment pass):
+// the persistent pass, then per class the cooperative pseudo-inverse round, the replay and the
+// in-lane rest of the qps it deferred
 int batch_qp_pass(pa_batch *b, int flag, int grid) {
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, b->stream, b->d_work);
+  for (int k = 0; k < b->ncls; ++k) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, b->stream, b->d_counts + k);
   const BArgs a = batch_qp_args(b);
-  if (flag && bmixed(b)) hipLaunchKernelGGL(kb_qp_pass<float>, dim3(grid), dim3(64), 0, b->stream, a, flag);
-  else hipLaunchKernelGGL(kb_qp_pass<double>, dim3(grid), dim3(64), 0, b->stream, a, flag);
+  if (flag && bmixed(b)) batch_qp_passes<float>(b, a, flag, grid);
+  else batch_qp_passes<double>(b, a, flag, grid);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -939,7 +999,7 @@ int pa_batch_create(const pa_material *mats, int n_mats, const pa_problem *pb, i
   }
   const long long nq = b->nq, N = b->ndof;
   const int nsm = b->nsmax;
-  b->ws_lane = has48 ? cp::LaneWs<48>::size : 2LL * ns_template(nsm) * ns_template(nsm);
+  b->ws_lane = has48 ? cp::LaneWs<48>::size : 0;  // only the 48-system variant keeps matrices there
   if ((rc = bupload(b, &b->d_rve, b->hrve)) || (rc = bupload(b, &b->d_erve, erve)) || (rc = bupload(b, &b->d_nrve, nrve)) ||
       (rc = bupload(b, &b->d_krve, krve)) || (rc = bupload(b, &b->d_prve, prve)) || (rc = bupload(b, &b->d_colrve, colrve)) ||
       (rc = bupload(b, &b->d_colblk0, b->col_blk0)) || (rc = balloc(b, &b->d_egrain, b->nelem)) || (rc = balloc(b, &b->d_work, 4)) ||
@@ -951,12 +1011,35 @@ int pa_batch_create(const pa_material *mats, int n_mats, const pa_problem *pb, i
       (rc = balloc(b, (double **)&b->K, 243 * b->nnode)) || (rc = balloc(b, (double **)&b->D, 81 * nq)) || (rc = balloc(b, &b->st, n_rve)) ||
       (rc = balloc(b, &b->ipart, 3 * (size_t)b->nkblk)) || (rc = balloc(b, &b->opart, (size_t)b->nkblk)) ||
       (rc = balloc(b, &b->upart, 2 * (size_t)b->nkblk)) || (rc = balloc(b, &b->npart, (size_t)b->nkblk)) ||
-      (rc = balloc(b, &b->ppart, 13 * (size_t)b->npblk)) || (rc = balloc(b, &b->ws, (size_t)BATCH_MAX_GRID * 64 * b->ws_lane))) {
+      (rc = balloc(b, &b->ppart, 13 * (size_t)b->npblk)) || (rc = balloc(b, &b->ws, std::max<size_t>(1, (size_t)BATCH_MAX_GRID * 64 * b->ws_lane))) ||
+      (rc = balloc(b, &b->d_counts, 8))) {
     pa_batch_destroy(b);
     return rc;
   }
   for (int i = 0; i < 8; ++i)
     if ((rc = balloc(b, &b->kv[i], N))) { pa_batch_destroy(b); return rc; }
+  // per class: deferred-qp queue and singular-system store, as many entries as fit 512 MiB
+  if (hipMemset(b->d_counts, 0, 8 * sizeof(unsigned)) != hipSuccess) { pa_batch_destroy(b); return fail(PA_ERR_HIP, "memset"); }
+  b->hsing.resize(n_mats);
+  for (int k = 0; k < n_mats; ++k) {
+    long long nqk = 0;
+    for (int r = 0; r < n_rve; ++r)
+      if (rve_mat[r] == k) nqk += 8 * b->hrve[r].m.nelem;
+    const size_t NS = (size_t)b->hcls[k].NS;
+    const size_t per = sizeof(double) * (NS * NS + NS + SING_ROUNDS * NS) + SING_ROUNDS * 8 + 3 * sizeof(int);
+    const size_t cap = std::max<size_t>(1, std::min<size_t>((size_t)nqk, ((size_t)512 << 20) / per));
+    BSing &S = b->hsing[k];
+    S.count = b->d_counts + k;
+    S.st.cap = (unsigned)cap;
+    S.st.miss = b->d_counts + 4 + k;
+    if ((rc = balloc(b, &S.queue, std::max<long long>(1, nqk))) || (rc = balloc(b, &S.st.A, cap * NS * NS)) ||
+        (rc = balloc(b, &S.st.b, cap * NS)) || (rc = balloc(b, &S.st.n, cap)) || (rc = balloc(b, &S.st.X, cap * SING_ROUNDS * NS)) ||
+        (rc = balloc(b, &S.st.key, cap * SING_ROUNDS)) || (rc = balloc(b, &S.st.have, cap)) || (rc = balloc(b, &S.st.done, cap))) {
+      pa_batch_destroy(b);
+      return rc;
+    }
+  }
+  if ((rc = bupload(b, &b->d_sing, b->hsing))) { pa_batch_destroy(b); return rc; }
   if (hipHostMalloc((void **)&b->hst, 2 * sizeof(BKState) * n_rve, hipHostMallocDefault) != hipSuccess) {
     pa_batch_destroy(b);
     return fail(PA_ERR_HIP, "hipHostMalloc failed");

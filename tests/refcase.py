@@ -374,6 +374,7 @@ def oracle():
         lib.or_simulate.restype = ctypes.c_int
         lib.or_sparsity.restype = ctypes.c_longlong
         lib.or_material_id.restype = ctypes.c_uint
+        lib.or_set_pinv_rtol.argtypes = [ctypes.c_double]
         _ORACLE = lib
     return _ORACLE
 

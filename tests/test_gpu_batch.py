@@ -107,9 +107,14 @@ def test_batch_rves_match_their_own_oracle_runs(solver):
       below instead.
     * Per qp, every RVE: after increment 1 the batch's committed state (T, s, Fp) at every qp equals
       the oracle's calculatePlasticity(.,.,0) on the batch's own inputs at that qp (F from the
-      batch's u, Fp and s committed after increment 0), at 1e-10, except qps whose oracle result
-      moves under 1e-14 relative perturbations of F (round-off-dependent singular active sets)."""
-    from test_gpu_parity import _roundoff_dependent
+      batch's u, Fp and s committed after increment 0) at 1e-10.  A qp whose numerically singular
+      active set leaves one singular value at round-off may differ: the keep/drop decision of
+      its pseudo-inverse rests on rounding (one FCC qp of this run: the oracle drops it, the kernel
+      code keeps it, on the GPU and compiled for the host alike, and the oracle itself keeps it
+      under 6 % of random 1e-12 relative perturbations of F).  Such a qp passes only if the GPU's
+      state is the oracle's at an input within 1e-12 of the GPU's (found among 400 seeded
+      perturbations), and at most 1 % of qps may need it.
+    """
     mats, pb, cases = _setup(solver=solver)
     others = [_setup(lin_tol=t, solver=solver)[2] for t in (3e-15, 1e-12, 1e-11)]
     b = _batch(mats, pb, cases, solver=solver)
@@ -150,16 +155,25 @@ def test_batch_rves_match_their_own_oracle_runs(solver):
         for Q in range(nq):
             rod = c.grain_rod[c.elem_grain[Q // 8]]
             Fpq = Fp0[Q].reshape(3, 3)
-            if _roundoff_dependent(c, rod, F[Q], Fpq, s0[Q]):
-                skipped += 1
-                continue
-            o = refcase.oracle_qp(c, rod, F[Q], Fpq, np.append(s0[Q], 10e14), 0)
+            sq = np.append(s0[Q], 10e14)
+            o = refcase.oracle_qp(c, rod, F[Q], Fpq, sq, 0)
             plastic += o["n_active"] > 0
-            assert _rel(T1[Q], o["T"].ravel()) < 1e-10, (i, Q)
-            assert _rel(s1[Q], o["s"][:c.n_slip]) < 1e-10, (i, Q)
-            assert _rel(Fp1[Q], o["Fp"].ravel()) < 1e-10, (i, Q)
-        print("  per-qp pass of increment 1: %d qps, %d plastic, %d round-off dependent skipped" % (nq, plastic, skipped), flush=True)
-        assert skipped <= nq // 20
+            if _rel(T1[Q], o["T"].ravel()) < 1e-10:
+                assert _rel(s1[Q], o["s"][:c.n_slip]) < 1e-10, (i, Q)
+                assert _rel(Fp1[Q], o["Fp"].ravel()) < 1e-10, (i, Q)
+                continue
+            r = np.random.default_rng(1000 + Q)
+            near = False
+            for _ in range(400):
+                o2 = refcase.oracle_qp(c, rod, F[Q] * (1 + 1e-12 * r.standard_normal((3, 3))), Fpq, sq, 0)
+                if _rel(T1[Q], o2["T"].ravel()) < 1e-6 and _rel(s1[Q], o2["s"][:c.n_slip]) < 1e-6:
+                    near = True
+                    break
+            assert near, (i, Q, _rel(T1[Q], o["T"].ravel()))
+            skipped += 1
+        print("  per-qp pass of increment 1: %d qps, %d plastic, %d on the other side of a round-off decision" % (nq, plastic, skipped),
+              flush=True)
+        assert skipped <= max(1, nq // 100)
     assert stable >= 2
     b.close()
 

@@ -332,14 +332,14 @@ def test_qp_update_bcc48_matches_oracle():
     ctx.close()
 
 
-def _roundoff_dependent(case, rod, F, Fp, s, eps=1e-14, tries=6):
-    """True when the oracle's stress moves by more than 1e-9 under `tries` random relative
-    perturbations of F of size eps."""
+def _roundoff_dependent(case, rod, F, Fp, s, eps=1e-14, tries=6, tol=1e-9):
+    """True when the oracle's stress moves by more than tol (or its active count changes) under
+    `tries` random relative perturbations of F of size eps."""
     o = refcase.oracle_qp(case, rod, F, Fp, np.append(s, 10e14), 0)
     r = np.random.default_rng(12345)
     for _ in range(tries):
         o2 = refcase.oracle_qp(case, rod, F * (1 + eps * r.standard_normal((3, 3))), Fp, np.append(s, 10e14), 0)
-        if _rel(o2["P"], o["P"]) > 1e-9 or o2["n_active"] != o["n_active"]:
+        if _rel(o2["P"], o["P"]) > tol or o2["n_active"] != o["n_active"]:
             return True
     return False
 
