@@ -274,8 +274,8 @@ __global__ void k_dirichlet(MeshDims m, BCArgs bc, unsigned char *__restrict__ c
 // leaves its next system.  Entries above capacity are finished with the pseudo-inverse in their
 // own lane.  Every path computes the same x (same Jacobi sweep order and rules).
 // FNV-1a over the bit patterns of n, A_PA and b_PA
-template <int NS>
-__device__ __forceinline__ unsigned long long sing_key(int n, const double *A, const double *b) {
+template <int NS, typename PA>
+__device__ __forceinline__ unsigned long long sing_key(int n, PA A, const double *b) {
   unsigned long long h = 1469598103934665603ull ^ (unsigned long long)n;
   auto mix = [&](double v) { h = (h ^ (unsigned long long)__double_as_longlong(v)) * 1099511628211ull; };
   for (int i = 0; i < n; ++i) {
@@ -284,8 +284,8 @@ __device__ __forceinline__ unsigned long long sing_key(int n, const double *A, c
   }
   return h;
 }
-template <int NS>
-__device__ __forceinline__ void sing_write(const SingStore &st, unsigned e, int round, int n, const double *A, const double *b) {
+template <int NS, typename PA>
+__device__ __forceinline__ void sing_write(const SingStore &st, unsigned e, int round, int n, PA A, const double *b) {
   double *dA = st.A + (size_t)e * NS * NS;
   for (int i = 0; i < n; ++i) {
     for (int j = 0; j < n; ++j) dA[i * NS + j] = A[i * NS + j];
@@ -306,12 +306,13 @@ struct SingAny {
   unsigned *count, *queue;
   unsigned Q, e;
   int have, k;
-  double *ws = nullptr;  // this lane's cp::LaneWs block (48-system variant), else null
-  __device__ void svd(int n, const double *A, const double *b, double *x) const {
-    if constexpr (cp::lane_ws<NS>()) cp::svd_pinv_solve_ws<NS>(n, A, b, x, ws + cp::LaneWs<NS>::UV);
-    else cp::svd_pinv_solve<NS>(n, A, b, x);
+  cp::LanePtr ws = {nullptr, 1};  // this lane's cp::LaneWs array (48-system variant), else unused
+  __device__ void svd(int n, const double *A, const double *b, double *x) const { cp::svd_pinv_solve<NS>(n, A, b, x); }
+  __device__ void svd(int n, cp::LanePtr A, const double *b, double *x) const {
+    cp::svd_pinv_solve_ws<NS>(n, A, b, x, ws + cp::LaneWs<NS>::UV);
   }
-  __device__ bool operator()(int n, const double *A, const double *b, double *x) {
+  template <typename PA>
+  __device__ bool operator()(int n, PA A, const double *b, double *x) {
     if (mode == SING_LANE) {
       svd(n, A, b, x);
       return true;
@@ -350,7 +351,8 @@ struct SingAny {
 template <int NS>
 struct SingRef {
   SingAny<NS> *f;
-  __device__ bool operator()(int n, const double *A, const double *b, double *x) const { return (*f)(n, A, b, x); }
+  template <typename PA>
+  __device__ bool operator()(int n, PA A, const double *b, double *x) const { return (*f)(n, A, b, x); }
 };
 
 template <int NS>
@@ -567,12 +569,19 @@ __device__ __attribute__((noinline)) bool qp_tangent_one(const QpArgs &a, long l
 // on every kernel that calls the per-qp function, which is compiled once for all of them.
 #define QP_WAVES __attribute__((amdgpu_waves_per_eu(2)))
 
-// this lane's block of the 48-system variant's workspace (null for the others, whose matrices stay
-// in private memory); the grids of those passes are capped at WS_BLOCKS one-wave blocks
+// this lane's array in the 48-system variant's workspace (unused for the others, whose matrices
+// stay in private memory): each wavefront owns a block of 64 arrays interleaved lane by lane, so a
+// wavefront's accesses to one element coalesce and its data stays within a few pages; the grids of
+// those passes are capped at WS_BLOCKS one-wave blocks
+constexpr unsigned WS_BLOCKS = 512;
+__device__ __forceinline__ cp::LanePtr wave_block_ptr(double *ws, long long per_lane) {
+  const unsigned lane = blockIdx.x * blockDim.x + threadIdx.x;
+  return cp::LanePtr{ws + (size_t)(lane >> 6) * 64 * per_lane + (lane & 63), 64};
+}
 template <int NS>
-__device__ __forceinline__ double *lane_ws(double *ws) {
-  if constexpr (cp::lane_ws<NS>()) return ws + (size_t)(blockIdx.x * blockDim.x + threadIdx.x) * cp::LaneWs<NS>::size;
-  else return nullptr;
+__device__ __forceinline__ cp::LanePtr lane_ws(double *ws) {
+  if constexpr (cp::lane_ws<NS>()) return wave_block_ptr(ws, cp::LaneWs<NS>::size);
+  else return cp::LanePtr{nullptr, 1};
 }
 
 // pass 1: lane = qp (grid-stride), LU path only; singular local problems are queued
@@ -1566,9 +1575,8 @@ int dalloc(pa_context *c, T **p, size_t count) {
 
 int ns_template(int ns) { return ns <= 12 ? 12 : (ns <= 24 ? 24 : 48); }
 
-// One-wave blocks of the one-lane qp passes of the 48-system variant: each lane of them owns a
-// cp::LaneWs<48> block (80,640 bytes), 2.6 GB in all.
-constexpr unsigned WS_BLOCKS = 512;
+// One-wave blocks of the one-lane qp passes of the 48-system variant (WS_BLOCKS): each lane of them
+// owns a cp::LaneWs<48> array (80,640 bytes, lane-interleaved), 2.6 GB in all.
 unsigned lane_grid(const pa_context *c, unsigned g) { return c->ws ? std::min(g, WS_BLOCKS) : g; }
 
 bool mixed(const pa_context *c) { return c->precision == PA_PREC_MIXED; }
@@ -2194,10 +2202,12 @@ int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out) {
     pa_destroy(c);
     return rc;
   }
-  {  // singular-system store: as many entries as fit 768 MiB (at most one per qp)
+  {  // singular-system store: as many entries as fit 768 MiB (4 GiB for 48 systems, whose entries
+     // take 19 KB and whose qps meet singular systems more often; at most one per qp)
     const size_t ns_t = (size_t)ns_template(c->ns);
     const size_t per = sizeof(double) * (ns_t * ns_t + ns_t + SING_ROUNDS * ns_t) + SING_ROUNDS * 8 + 3 * sizeof(int);
-    const size_t cap = std::min<size_t>((size_t)nq, std::max<size_t>(64, ((size_t)768 << 20) / per));
+    const size_t budget = ns_t == 48 ? ((size_t)4 << 30) : ((size_t)768 << 20);
+    const size_t cap = std::min<size_t>((size_t)nq, std::max<size_t>(64, budget / per));
     c->sing.cap = (unsigned)cap;
     c->sing.miss = c->counters + 10;
     if ((rc = dalloc(c, &c->sing.A, cap * ns_t * ns_t)) || (rc = dalloc(c, &c->sing.b, cap * ns_t)) || (rc = dalloc(c, &c->sing.n, cap)) ||

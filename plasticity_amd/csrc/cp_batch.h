@@ -57,7 +57,7 @@ struct BArgs {
   double *Fe_iter, *Fp_iter, *s_iter, *slip_iter, *T;
   void *D;     // [81][nq] JxW dP/dF (DT)
   double *P;   // [9][nq] JxW P
-  double *ws;  // per-lane workspace, ws_lane doubles per lane
+  double *ws;  // 48-system workspace: cp::LaneWs<48> per lane, wavefront blocks interleaved by lane
   long long ws_lane;
   unsigned *work;  // work-queue counter (zeroed before the launch)
 };
@@ -94,13 +94,13 @@ __device__ __attribute__((noinline)) bool bq_one(const BArgs &a, long long Q, in
 }
 // U and V of the in-lane pseudo-inverse live in the lane's workspace for every class
 template <int NS>
-__device__ __forceinline__ SingAny<NS> bsing(const BSing &S, int mode, unsigned Q, unsigned e, int have, double *lw) {
+__device__ __forceinline__ SingAny<NS> bsing(const BSing &S, int mode, unsigned Q, unsigned e, int have, cp::LanePtr lw) {
   SingAny<NS> f{mode, S.st, S.count, S.queue, Q, e, have, 0, lw};
   return f;
 }
 
 template <int NS, typename DT>
-__device__ __forceinline__ void bq_claim(const BArgs &a, long long Q, int k, int flag, double *lw) {
+__device__ __forceinline__ void bq_claim(const BArgs &a, long long Q, int k, int flag, cp::LanePtr lw) {
   SingAny<NS> f = bsing<NS>(a.sing[k], SING_CLAIM, (unsigned)Q, 0u, 0, lw);
   bq_one<NS, DT>(a, Q, flag, f);  // a singular system defers the qp to the passes below
 }
@@ -109,7 +109,7 @@ __device__ __forceinline__ void bq_claim(const BArgs &a, long long Q, int k, int
 template <typename DT>
 __global__ void __launch_bounds__(64) QP_WAVES kb_qp_pass(BArgs a, int flag) {
   __shared__ unsigned sbase;
-  double *lw = a.ws + (size_t)(blockIdx.x * 64 + threadIdx.x) * a.ws_lane;
+  const cp::LanePtr lw = wave_block_ptr(a.ws, a.ws_lane);
   for (;;) {
     if (threadIdx.x == 0) sbase = atomicAdd(a.work, 64u);
     __syncthreads();
@@ -130,7 +130,7 @@ __global__ void __launch_bounds__(64) QP_WAVES kb_qp_pass(BArgs a, int flag) {
 template <int NS, typename DT>
 __global__ void __launch_bounds__(64) QP_WAVES kb_qp_replay(BArgs a, int k, int flag) {
   const BSing S = a.sing[k];
-  double *lw = a.ws + (size_t)(blockIdx.x * 64 + threadIdx.x) * a.ws_lane;
+  const cp::LanePtr lw = wave_block_ptr(a.ws, a.ws_lane);
   const unsigned n = min(*S.count, S.st.cap);
   for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
     if (S.st.done[e]) continue;
@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(64) QP_WAVES kb_qp_replay(BArgs a, int k, int 
 template <int NS, typename DT>
 __global__ void __launch_bounds__(64) QP_WAVES kb_qp_deferred(BArgs a, int k, int flag) {
   const BSing S = a.sing[k];
-  double *lw = a.ws + (size_t)(blockIdx.x * 64 + threadIdx.x) * a.ws_lane;
+  const cp::LanePtr lw = wave_block_ptr(a.ws, a.ws_lane);
   const unsigned n = *S.count;
   for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     if (i >= S.st.cap || !S.st.done[i]) {
@@ -1011,14 +1011,16 @@ int pa_batch_create(const pa_material *mats, int n_mats, const pa_problem *pb, i
       (rc = balloc(b, (double **)&b->K, 243 * b->nnode)) || (rc = balloc(b, (double **)&b->D, 81 * nq)) || (rc = balloc(b, &b->st, n_rve)) ||
       (rc = balloc(b, &b->ipart, 3 * (size_t)b->nkblk)) || (rc = balloc(b, &b->opart, (size_t)b->nkblk)) ||
       (rc = balloc(b, &b->upart, 2 * (size_t)b->nkblk)) || (rc = balloc(b, &b->npart, (size_t)b->nkblk)) ||
-      (rc = balloc(b, &b->ppart, 13 * (size_t)b->npblk)) || (rc = balloc(b, &b->ws, std::max<size_t>(1, (size_t)BATCH_MAX_GRID * 64 * b->ws_lane))) ||
+      (rc = balloc(b, &b->ppart, 13 * (size_t)b->npblk)) || (rc = balloc(b, &b->ws, std::max<size_t>(64, (size_t)BATCH_MAX_GRID * 64 * b->ws_lane))) ||
       (rc = balloc(b, &b->d_counts, 8))) {
     pa_batch_destroy(b);
     return rc;
   }
   for (int i = 0; i < 8; ++i)
     if ((rc = balloc(b, &b->kv[i], N))) { pa_batch_destroy(b); return rc; }
-  // per class: deferred-qp queue and singular-system store, as many entries as fit 512 MiB
+  // per class: deferred-qp queue and singular-system store, as many entries as fit 4 GiB (a qp the
+  // store cannot hold solves its singular systems in its own lane, with U and V in global memory for
+  // 48 systems: a 48-system entry takes 19 KB, 223 k entries)
   if (hipMemset(b->d_counts, 0, 8 * sizeof(unsigned)) != hipSuccess) { pa_batch_destroy(b); return fail(PA_ERR_HIP, "memset"); }
   b->hsing.resize(n_mats);
   for (int k = 0; k < n_mats; ++k) {
@@ -1027,7 +1029,7 @@ int pa_batch_create(const pa_material *mats, int n_mats, const pa_problem *pb, i
       if (rve_mat[r] == k) nqk += 8 * b->hrve[r].m.nelem;
     const size_t NS = (size_t)b->hcls[k].NS;
     const size_t per = sizeof(double) * (NS * NS + NS + SING_ROUNDS * NS) + SING_ROUNDS * 8 + 3 * sizeof(int);
-    const size_t cap = std::max<size_t>(1, std::min<size_t>((size_t)nqk, ((size_t)512 << 20) / per));
+    const size_t cap = std::max<size_t>(1, std::min<size_t>((size_t)nqk, ((size_t)4 << 30) / per));
     BSing &S = b->hsing[k];
     S.count = b->d_counts + k;
     S.st.cap = (unsigned)cap;

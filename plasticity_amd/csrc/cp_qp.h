@@ -23,6 +23,16 @@ namespace cp {
 
 #define CP_DEV __host__ __device__ __forceinline__
 
+// One lane's array in a workspace interleaved over s lanes: element k at p[k * s].  Consecutive
+// lanes hold consecutive addresses, so a wavefront's accesses to the same element of its lanes'
+// arrays coalesce into one contiguous span.
+struct LanePtr {
+  double *p;
+  long long s;
+  CP_DEV double &operator[](long long k) const { return p[k * s]; }
+  CP_DEV LanePtr operator+(long long k) const { return LanePtr{p + k * s, s}; }
+};
+
 // ---------------------------------------------------------------- 3x3 helpers
 CP_DEV void m3_mul(const double *A, const double *B, double *C) {  // C = A B
 #pragma unroll
@@ -114,8 +124,8 @@ CP_DEV void m3_expm(const double *A, double *E) {
 
 // LU with partial pivoting on an n x n row-major matrix (ld = NS) with m right-hand
 // sides (row-major, ld = m).  Returns false when a pivot is <= CP_PINV_RTOL * max|A|.
-template <int NS>
-CP_DEV bool lu_solve(int n, double *A, double *B, int m) {
+template <int NS, typename PA, typename PB>
+CP_DEV bool lu_solve(int n, PA A, PB B, int m) {
   double amax = 0;
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) amax = fmax(amax, fabs(A[i * NS + j]));
@@ -169,8 +179,8 @@ CP_DEV void rr_pair(int m, int r, int k, int &p, int &q) {
 // sigma_max are dropped unless there is exactly one and its step is determined by the data
 // (below).  Used only when LU finds A_PA numerically singular.
 // U, V: n x n work matrices (leading dimension NS) of the caller.
-template <int NS>
-__host__ __device__ __forceinline__ void svd_pinv_core(int n, const double *A, const double *b, double *x, double *U, double *V) {
+template <int NS, typename PA, typename PU>
+__host__ __device__ __forceinline__ void svd_pinv_core(int n, PA A, const double *b, double *x, PU U, PU V) {
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) { U[i * NS + j] = A[i * NS + j]; V[i * NS + j] = (i == j) ? 1.0 : 0.0; }
   const int m = n + (n & 1);
@@ -239,13 +249,13 @@ __host__ __device__ __forceinline__ void svd_pinv_core(int n, const double *A, c
 template <int NS>
 __host__ __device__ __attribute__((noinline)) void svd_pinv_solve(int n, const double *A, const double *b, double *x) {
   double U[NS * NS], V[NS * NS];
-  svd_pinv_core<NS>(n, A, b, x, U, V);
+  svd_pinv_core<NS, const double *, double *>(n, A, b, x, U, V);
 }
 // The same with U and V in the caller's workspace (2 NS^2 doubles): for the 48-system variant,
 // whose matrices do not fit the private segment the runtime grants a lane.
 template <int NS>
-__host__ __device__ __attribute__((noinline)) void svd_pinv_solve_ws(int n, const double *A, const double *b, double *x, double *UV) {
-  svd_pinv_core<NS>(n, A, b, x, UV, UV + NS * NS);
+__host__ __device__ __attribute__((noinline)) void svd_pinv_solve_ws(int n, LanePtr A, const double *b, double *x, LanePtr UV) {
+  svd_pinv_core<NS, LanePtr, LanePtr>(n, A, b, x, UV, UV + NS * NS);
 }
 
 // What qp_update does when LU finds A_PA numerically singular: the functor receives A_PA
@@ -253,21 +263,26 @@ __host__ __device__ __attribute__((noinline)) void svd_pinv_solve_ws(int n, cons
 // returns true, or returns false, which makes qp_update return -1 at once.
 template <int NS>
 struct SvdInLane {  // the pseudo-inverse in this lane
-  double *uv = nullptr;  // U and V in the lane's workspace (2 NS^2 doubles) when lane_ws<NS>()
+  LanePtr uv = {nullptr, 1};  // U and V in the lane's workspace (2 NS^2 doubles) when lane_ws<NS>()
   CP_DEV bool operator()(int n, const double *A, const double *b, double *x) const {
-    if (NS > 24) svd_pinv_solve_ws<NS>(n, A, b, x, uv);
-    else svd_pinv_solve<NS>(n, A, b, x);
+    svd_pinv_solve<NS>(n, A, b, x);
+    return true;
+  }
+  CP_DEV bool operator()(int n, LanePtr A, const double *b, double *x) const {
+    svd_pinv_solve_ws<NS>(n, A, b, x, uv);
     return true;
   }
 };
 struct LuOnly {  // give up: the caller handles the qp elsewhere
-  CP_DEV bool operator()(int, const double *, const double *, double *) const { return false; }
+  template <typename PA>
+  CP_DEV bool operator()(int, PA, const double *, double *) const { return false; }
 };
 
 // Per-lane workspace of the 48-system variant (doubles): its NS x NS matrices take 4 NS^2 doubles
-// (74 KB), more private memory than the runtime grants a lane, so they live in a block of global
-// memory the caller gives each lane: A over the potentially-active set, the compacted copy A_PA
-// (then A2 of the tangent), U and V of the pseudo-inverse, and the tangent's delgamma and S_PA.
+// (74 KB), more private memory than the runtime grants a lane, so they live in global memory the
+// caller shares among its lanes (LanePtr, lane-interleaved): A over the potentially-active set,
+// the compacted copy A_PA (then A2 of the tangent), U and V of the pseudo-inverse, and the
+// tangent's delgamma and S_PA, at these offsets of each lane's array.
 template <int NS> struct LaneWs {
   static constexpr int A = 0, Ap = NS * NS, UV = 2 * NS * NS, rhs = 4 * NS * NS, SV = 4 * NS * NS + 9 * NS,
                        size = 4 * NS * NS + 18 * NS;
@@ -357,16 +372,21 @@ CP_DEV void dexp_series(const double *X, const double *S, int count3, bool neg, 
 // start stands only if the loop converges on that set without dropping a system; otherwise the
 // loop restarts from zero on the trial set.  iters (may be null) receives the local solves of both
 // attempts.  Not combined with kinematic hardening.
-// ws: this lane's LaneWs<NS> block when lane_ws<NS>() (NS = 48), unused otherwise.
+// ws: this lane's LaneWs<NS> array when lane_ws<NS>() (NS = 48), unused otherwise.
 template <int NS, typename DT = double, bool KIN = false, typename SING = SvdInLane<NS>, bool WARM = false>
 CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ gt, int ns, int max_solver_iter,
                      const double *F, const double *Fp_t, double *s, double *x, int flag, double *P, double *T,
                      double *Fe_out, double *Fp_out, DT *D, long long dstride, bool *plastic_path = nullptr,
                      double dscale = 1.0, SING sing = SING(), double *W = nullptr, const float *x0 = nullptr,
-                     long long x0stride = 1, int *iters = nullptr, int nslip = 1 << 30, double *ws = nullptr) {
+                     long long x0stride = 1, int *iters = nullptr, int nslip = 1 << 30, LanePtr ws = LanePtr{nullptr, 1}) {
   static_assert(!(WARM && KIN), "the warm start does not cover kinematic hardening");
   constexpr bool WSP = lane_ws<NS>();
   typedef LaneWs<NS> LW;
+  // a matrix in the workspace (48 systems) or in this lane's private memory
+  auto wsa = [&](double *priv, int off) {
+    if constexpr (WSP) return ws + off;
+    else return priv;
+  };
   typedef MatLayout<NS> ML;
   typedef GrainLayout<NS> GL;
   const double tol1 = mat[ML::tol];
@@ -411,7 +431,7 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
 
   // ---- plastic correction (:250-468)
   double A_l[WSP ? 1 : NS * NS];
-  double *const A = WSP ? ws + LW::A : A_l;  // A over the initial potentially-active set, indexed by positions in PA0
+  const auto A = wsa(A_l, LW::A);  // A over the initial potentially-active set, indexed by positions in PA0
   int PA0[NS];
   int n0 = nPA;
   double h[NS];
@@ -459,7 +479,7 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
       for (int a = 0; a < ns; ++a) xb[a] = 0.0;
       while (true) {
         double Ap_l[WSP ? 1 : NS * NS], bp[NS];
-        double *const Ap = WSP ? ws + LW::Ap : Ap_l;
+        const auto Ap = wsa(Ap_l, LW::Ap);
         for (int i = 0; i < npos; ++i) {
           bp[i] = b[PA0[pos[i]]];
           for (int j = 0; j < npos; ++j) Ap[i * NS + j] = A[pos[i] * NS + pos[j]];
@@ -624,8 +644,8 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
   // delFp_delF = sum_j SV_j (x) delgamma_j (rank nPA): SV[j] (9) and delgamma (rhs, 9 per j) are
   // kept and each column kl of it is formed where the final tangent needs it
   double rhs_l[WSP ? 1 : NS * 9], SV_l[WSP ? 1 : NS * 9];
-  double *const rhs = WSP ? ws + LW::rhs : rhs_l;
-  double *const SV = WSP ? ws + LW::SV : SV_l;
+  const auto rhs = wsa(rhs_l, LW::rhs);
+  const auto SV = wsa(SV_l, LW::SV);
   const int nsv = (plastic && nPA > 0) ? nPA : 0;
   if (nsv > 0) {
     // count3 (:607-619): number of series terms until ||X^k / k!||_F <= max(tol/1e4, 1e-12)
@@ -677,7 +697,7 @@ CP_DEV int qp_update(const double *__restrict__ mat, const double *__restrict__ 
     }
     // A2 (:622-709)
     double A2_l[WSP ? 1 : NS * NS];
-    double *const A2 = WSP ? ws + LW::Ap : A2_l;  // the compacted A_PA is no longer needed
+    const auto A2 = wsa(A2_l, LW::Ap);  // the compacted A_PA is no longer needed
     double eneg[9], Xneg[9], t6[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) Xneg[i] = -XdelFp[i];

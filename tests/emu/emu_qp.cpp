@@ -166,8 +166,8 @@ int emu_qp_lane(const pa_material *mat, const double *rod, int nrows, long long 
       constexpr bool K = decltype(kinc)::value;
       return cp::qp_update<NSc, double, K>(mb.data(), gt, ns, mat->max_solver_iter, F + 9 * i, Fp + 9 * i, s_, x_, flag, P + 9 * i,
                                            T + 9 * i, Fe + 9 * i, Fpo + 9 * i, flag ? D + 81 * i : nullptr, 1, nullptr, 1.0,
-                                           cp::SvdInLane<NSc>{ws.data() + cp::LaneWs<NSc>::UV}, W_, nullptr, 1, nullptr, g_nslip,
-                                           ws.data());
+                                           cp::SvdInLane<NSc>{cp::LanePtr{ws.data() + cp::LaneWs<NSc>::UV, 1}}, W_, nullptr, 1,
+                                           nullptr, g_nslip, cp::LanePtr{ws.data(), 1});
     };
     using I12 = std::integral_constant<int, 12>;
     using I24 = std::integral_constant<int, 24>;
