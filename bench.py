@@ -119,6 +119,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-warm", action="store_true", help="skip the learned warm-start measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-RVE workload")
+    ap.add_argument("--only-batch", action="store_true", help="run the batched-RVE workload alone and print its record")
     ap.add_argument("--full-path", type=int, default=1,
                     help="after the timed steps, run the multigrid-PCG forms to the end of the load path (untimed for value)")
     args = ap.parse_args()
@@ -132,6 +133,9 @@ def main():
     from plasticity_amd import native, synth
 
     assert torch.cuda.is_available(), "no GPU"
+    if args.only_batch:
+        print(json.dumps({"batched_rves": batched_leg(pa, native, synth)}))
+        return
     n = args.n
     forms = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     solvers = ["bicg", "cg", "mgcg"] if args.solver == "all" else [args.solver]
@@ -199,41 +203,48 @@ def batched_leg(pa, native, synth):
     BCC with 48 slip systems and HCP with 24, 16 Voronoi grains each, tension in 10 steps of 2.5e-4
     (yield within the last steps; 4 Newton iterations per step, Jacobi-BiCG at 1e-10 per RVE), all
     RVEs in the same launches, Newton iterations replayed from HIP graphs after the autotuner's
-    pick.  Every Newton step of the load path is timed (host clock around each call, the
-    post-increment pass included)."""
+    pick; FP64 and the mixed form (FP32 tangents and matrices).  Every Newton step of the load path
+    is timed (host clock around each call, the post-increment pass included)."""
     eps = 2.5e-4
     mats, rve_mat, rve_nel, pb, egs, rods = synth.batch_workload(strain_per_step=eps)
-    b = pa.Batch(mats, rve_mat, rve_nel, pb)
-    for r in range(b.n_rve):
-        b.set_grains(r, egs[r], rods[r])
-    b.set_solver(native.PA_SOLVER_BICG, native.PA_PREC_FP64)
-    tune = b.tune()
-    steps, asm, kry, post, its, tot = 0, [], [], [], [], 0.0
-    rows = None
-    for inc in range(pb.total_increments):
-        for it in range(pb.max_newton):
+    res = {}
+    for form, prec in (("fp64", native.PA_PREC_FP64), ("mixed", native.PA_PREC_MIXED)):
+        b = pa.Batch(mats, rve_mat, rve_nel, pb)
+        for r in range(b.n_rve):
+            b.set_grains(r, egs[r], rods[r])
+        b.set_solver(native.PA_SOLVER_BICG, prec)
+        tune = b.tune()
+        steps, asm, kry, post, its, tot = 0, [], [], [], [], 0.0
+        rows = None
+        for inc in range(pb.total_increments):
+            for it in range(pb.max_newton):
+                t0 = time.perf_counter()
+                rn, k = b.newton_iteration(inc, it)
+                tot += time.perf_counter() - t0
+                ms = b.last_timings()
+                asm.append(ms[0]); kry.append(ms[1]); its.append(float(np.mean(k)))
+                steps += 1
             t0 = time.perf_counter()
-            rn, k = b.newton_iteration(inc, it)
+            rows = b.commit_increment()
             tot += time.perf_counter() - t0
-            ms = b.last_timings()
-            asm.append(ms[0]); kry.append(ms[1]); its.append(float(np.mean(k)))
-            steps += 1
-        t0 = time.perf_counter()
-        rows = b.commit_increment()
-        tot += time.perf_counter() - t0
-        post.append(b.last_timings()[2])
-    qp_ms = b.qp_pass_ms(1)
-    nq = b.nqp
-    b.close()
-    ms_step = 1000.0 * tot / steps
+            post.append(b.last_timings()[2])
+        qp_ms = b.qp_pass_ms(1)
+        nq = b.nqp
+        b.close()
+        ms_step = 1000.0 * tot / steps
+        res[form] = {"ms_per_step": ms_step, "qp_updates_per_s": nq / (ms_step * 1e-3), "assembly_ms_per_step": float(np.mean(asm)),
+                     "krylov_ms_per_step": float(np.mean(kry)), "post_ms_per_increment": float(np.mean(post)),
+                     "bicg_iters_per_step_mean_over_rves": float(np.mean(its)), "qp_pass_ms_last_state": qp_ms, "tuner": tune,
+                     "rows": rows}
+    a, m = res["fp64"], res["mixed"]
+    diff = float(np.abs(m["rows"][:, :12] - a["rows"][:, :12]).max() / np.abs(a["rows"][:, :12]).max())
+    for r in res.values():
+        del r["rows"]
     return {"workload": "64 RVEs, edge U{8..24} (seed 2024), %d qps, %d nodes; BCC 48 systems / HCP 24 systems alternating; "
-                        "16 grains each; 10 x %.1e tension; Jacobi-BiCG 1e-10; FP64" % (nq, sum(int(np.prod(np.asarray(n) + 1)) for n in rve_nel), eps),
-            "n_rve": int(len(rve_mat)), "qps": nq, "newton_steps": steps, "ms_per_step": ms_step,
-            "qp_updates_per_s": nq / (ms_step * 1e-3), "assembly_ms_per_step": float(np.mean(asm)),
-            "krylov_ms_per_step": float(np.mean(kry)), "post_ms_per_increment": float(np.mean(post)),
-            "bicg_iters_per_step_mean_over_rves": float(np.mean(its)), "qp_pass_ms_last_state": qp_ms,
-            "tuner": tune, "last_row_mean_Txx": float(rows[:, 6].mean()), "last_row_mean_slip": float(rows[:, 14].mean()),
-            "note": "the post-increment passes are in ms_per_step (spread over the 4 steps of their increment)"}
+                        "16 grains each; 10 x %.1e tension; Jacobi-BiCG 1e-10" % (nq, sum(int(np.prod(np.asarray(n) + 1)) for n in rve_nel), eps),
+            "n_rve": int(len(rve_mat)), "qps": nq, "newton_steps": 4 * pb.total_increments, "fp64": a, "mixed": m,
+            "speedup_mixed_vs_fp64": a["ms_per_step"] / m["ms_per_step"], "mixed_vs_fp64_final_rows_rel_diff": diff,
+            "note": "ms_per_step includes the post-increment passes (spread over the 4 steps of their increment)"}
 
 
 def warm_start_leg(pa, native):
