@@ -10,10 +10,13 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <ctime>
 #include <vector>
+#include <functional>
+#include <map>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -1168,6 +1171,299 @@ void reorient_qp(const M3 &Fe_old, const M3 &Fe_new, double r[3]) {
 
 }  // namespace
 
+// ---------------------------------------------------------------- periodic BCs
+// The affine constraints of Use Periodic BCs (BVP periodicBCs.cc), closed as deal.II's
+// AffineConstraints::close() leaves them: every constrained dof d reads u_d = sum_k w_k u_{m_k} + s g_d
+// over unconstrained masters m_k, s = the load scale of the Newton iteration (1 for
+// setPeriodicityConstraintsInc0, -1 for Inc0Neg, 0 for IncNot0: the inhomogeneities are linear in
+// the per-increment vertex values).
+struct PLine {
+  std::vector<std::pair<long long, double>> e;
+  double g = 0.0;
+};
+struct PeriodicCons {
+  std::vector<char> con;
+  std::vector<PLine> line;  // closed lines, indexed by dof
+  std::vector<double> tab_scale;  // tabular periodic BCs: load scale of increment 0..n-1
+};
+
+// A double converted to `unsigned int` as x86-64 code does it (truncation to a 64-bit integer, low
+// 32 bits kept): the reference accumulates the derived vertex value in an unsigned int
+// (periodicBCs.cc:56, 515-517), so values in (-1, 1) become 0.
+static double as_unsigned(double x) { return (double)(uint32_t)(int64_t)(long long)x; }
+
+// Node of vertex v (rows 3v..3v+2: V000 V100 V010 V001 V110 V101 V011 V111)
+static int pbc_vertex_node(const Mesh &m, int v) {
+  static const int c[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+  return m.node(c[v][0] * m.nx, c[v][1] * m.ny, c[v][2] * m.nz);
+}
+// Node t (0..n-2, sorted along the edge, periodicBCs.cc:775-800) of edge e (EX000 EX001 EX010 EX011
+// EY000 EY001 EY100 EY101 EZ000 EZ010 EZ100 EZ110: EXab at y = a, z = b; EYab at x = a, z = b;
+// EZab at x = a, y = b)
+static int pbc_edge_node(const Mesh &m, int e, int t) {
+  const int dir = e / 4, a = (e % 4) >> 1, b = e & 1;
+  if (dir == 0) return m.node(t + 1, a * m.ny, b * m.nz);
+  if (dir == 1) return m.node(a * m.nx, t + 1, b * m.nz);
+  return m.node(a * m.nx, b * m.ny, t + 1);
+}
+
+// setPeriodicityConstraintsInit (periodicBCs.cc:23-864, the parts that matter on one process) and
+// setFace/Edge/NodeConstraints (:866-1204) at unit load scale, then closed.  Returns false if
+// the input is outside what this restatement covers (unequal edge counts, a row out of range, a
+// constraint cycle).
+static bool build_periodic(const Mesh &m, const or_problem *pb, PeriodicCons &pc) {
+  const or_periodic *P = pb->periodic;
+  if (m.nx != m.ny || m.ny != m.nz || m.nx < 2) return false;  // one edge-node count for all edges (:49)
+  if (P->n_vertex_lines > OR_MAX_PBC_LINES || P->n_edge_lines > OR_MAX_PBC_LINES || P->n_face_lines > OR_MAX_PBC_LINES) return false;
+  const int n = m.nx;
+  // per-increment vertex values (:108-117)
+  double val[24];
+  int known[24];
+  const double incs = P->tabular ? (double)(unsigned)(P->tabular_time / pb->delT) : (double)pb->total_increments;
+  for (int i = 0; i < 24; ++i) { val[i] = P->vertex_total[i] / incs; known[i] = P->vertex_known[i]; }
+  // lines with a single unknown vertex dof fix it (:501-527); a line with all dofs known is done
+  const int nV = P->n_vertex_lines;
+  std::vector<int> vknown(nV, 0);
+  for (int j = 0; j < nV; ++j) {
+    const int cnt = P->vertex_dofs[j][0];
+    if (cnt < 1 || cnt > 4) return false;
+    for (int k = 0; k < cnt; ++k) if (P->vertex_dofs[j][k + 1] < 0 || P->vertex_dofs[j][k + 1] >= 24) return false;
+  }
+  for (int pass = 0; pass < nV; ++pass)
+    for (int j = 0; j < nV; ++j) {
+      const int cnt = P->vertex_dofs[j][0];
+      int check1 = 0;
+      for (int k = 0; k < cnt; ++k) check1 += known[P->vertex_dofs[j][k + 1]] == 1;
+      if (check1 == cnt) vknown[j] = 1;
+      if (check1 == cnt - 1) {
+        double check2 = 0.0;
+        int kf = 0;
+        for (int k = 0; k < cnt; ++k) {
+          const int d = P->vertex_dofs[j][k + 1];
+          if (known[d] == 1) check2 = as_unsigned(check2 + val[d] * P->vertex_coef[j][k + 1]);
+          else kf = k;
+        }
+        const int d = P->vertex_dofs[j][kf + 1];
+        known[d] = 1;
+        val[d] = check2 * (-1.0 / P->vertex_coef[j][kf + 1]);
+        vknown[j] = 1;
+      }
+    }
+  const long long N = m.ndof;
+  std::vector<char> con(N, 0);
+  std::vector<PLine> raw(N);
+  auto vdof = [&](int r) -> long long { return 3LL * pbc_vertex_node(m, r / 3) + r % 3; };
+  auto add_line = [&](long long d) { con[d] = 1; };  // AffineConstraints::add_line keeps an existing line
+  auto add_entry = [&](long long d, long long c, double w) {
+    for (auto &pr : raw[d].e) if (pr.first == c) return;  // an existing entry stays
+    raw[d].e.push_back({c, w});
+  };
+  // the vertex terms of a face or edge line: known vertex dofs give the inhomogeneity, unknown ones
+  // entries (periodicBCs.cc:945-1003, 1133-1203)
+  auto vertex_terms = [&](long long d, int v1, int v2, double c0, double c2, double c3) {
+    const bool f1 = known[v1] == 1, f2 = known[v2] == 1;
+    if (f1 && f2) raw[d].g = (-1 / c0) * c2 * val[v1] + (-1 / c0) * c3 * val[v2];
+    else if (f1) { raw[d].g = (-1 / c0) * c2 * val[v1]; add_entry(d, vdof(v2), (-1 / c0) * c3); }
+    else if (f2) { raw[d].g = (-1 / c0) * c3 * val[v2]; add_entry(d, vdof(v1), (-1 / c0) * c2); }
+    else { add_entry(d, vdof(v2), (-1 / c0) * c3); add_entry(d, vdof(v1), (-1 / c0) * c2); }
+  };
+  // faces: make_periodicity_constraints ties each dof of a positive face (cells away from the box
+  // edges, :120-150: the face's interior nodes) to its partner on the negative face (:1047-1075), then
+  // the vertex terms (:1077-1203)
+  for (int i = 0; i < P->n_face_lines; ++i) {
+    const int r = P->face_dofs[i][0];
+    if (r < 0 || r >= 18 || ((r / 3) & 1) == 0) return false;  // the line starts with FXP, FYP or FZP
+    for (int k = 2; k < 4; ++k) if (P->face_dofs[i][k] < 0 || P->face_dofs[i][k] >= 24) return false;
+  }
+  auto face_nodes = [&](int dir, int side, std::vector<int> &out) {
+    out.clear();
+    for (int b = 1; b < n; ++b)
+      for (int a = 1; a < n; ++a) {
+        const int x = side * n;
+        out.push_back(dir == 0 ? m.node(x, a, b) : dir == 1 ? m.node(a, x, b) : m.node(a, b, x));
+      }
+  };
+  std::vector<int> fp, fn;
+  for (int i = 0; i < P->n_face_lines; ++i) {
+    const int r = P->face_dofs[i][0], dir = (r / 3) / 2, c = r % 3;
+    face_nodes(dir, 1, fp);
+    face_nodes(dir, 0, fn);
+    for (size_t k = 0; k < fp.size(); ++k) {
+      const long long d = 3LL * fp[k] + c;
+      if (con[d]) continue;
+      add_line(d);
+      add_entry(d, 3LL * fn[k] + c, 1.0);
+    }
+  }
+  for (int i = 0; i < P->n_face_lines; ++i) {
+    const int r = P->face_dofs[i][0], dir = (r / 3) / 2, c = r % 3;
+    face_nodes(dir, 1, fp);
+    for (int node : fp) {
+      const long long d = 3LL * node + c;
+      if (con[d]) vertex_terms(d, P->face_dofs[i][2], P->face_dofs[i][3], P->face_coef[i][0], P->face_coef[i][2], P->face_coef[i][3]);
+    }
+  }
+  // edges (:922-1004): edge row E1 node t = -(c1/c0) E2 node t + vertex terms
+  for (int j = 0; j < P->n_edge_lines; ++j) {
+    const int *r = P->edge_dofs[j];
+    const double *c = P->edge_coef[j];
+    if (r[0] < 0 || r[0] >= 36 || r[1] < 0 || r[1] >= 36 || r[2] < 0 || r[2] >= 24 || r[3] < 0 || r[3] >= 24) return false;
+    for (int t = 0; t < n - 1; ++t) {
+      const long long d1 = 3LL * pbc_edge_node(m, r[0] / 3, t) + r[0] % 3, d2 = 3LL * pbc_edge_node(m, r[1] / 3, t) + r[1] % 3;
+      add_line(d1);
+      add_entry(d1, d2, (-1 / c[0]) * c[1]);
+      vertex_terms(d1, r[2], r[3], c[0], c[2], c[3]);
+    }
+  }
+  // vertices (:866-920): a line with unknown dofs constrains its first dof; known dofs take their value
+  for (int j = 0; j < nV; ++j) {
+    if (vknown[j]) continue;
+    const int cnt = P->vertex_dofs[j][0];
+    const long long d1 = vdof(P->vertex_dofs[j][1]);
+    if (known[P->vertex_dofs[j][1]] == 1) return false;  // the known value would overwrite the line's inhomogeneity
+    const double c1 = P->vertex_coef[j][1];
+    add_line(d1);
+    double inh = 0;
+    for (int k = 1; k < cnt; ++k) {
+      const int rv = P->vertex_dofs[j][k + 1];
+      if (known[rv] == 1) inh += P->vertex_coef[j][k + 1] * val[rv];
+      else add_entry(d1, vdof(rv), (-1 / c1) * P->vertex_coef[j][k + 1]);
+    }
+    raw[d1].g = (-1 / c1) * inh;
+  }
+  for (int i = 0; i < 24; ++i)
+    if (known[i] == 1) { const long long d = vdof(i); add_line(d); raw[d].g = val[i]; }
+  // AffineConstraints::close(): entries that are themselves constrained are replaced by their lines
+  pc.con = con;
+  pc.line.assign(N, PLine());
+  std::vector<char> state(N, 0);  // 0 open, 1 in progress, 2 closed
+  std::function<bool(long long)> close = [&](long long d) -> bool {
+    if (state[d] == 2) return true;
+    if (state[d] == 1) return false;  // cycle
+    state[d] = 1;
+    PLine out;
+    out.g = raw[d].g;
+    std::map<long long, double> acc;
+    for (auto &pr : raw[d].e) {
+      if (con[pr.first]) {
+        if (!close(pr.first)) return false;
+        const PLine &l = pc.line[pr.first];
+        out.g += pr.second * l.g;
+        for (auto &q : l.e) acc[q.first] += pr.second * q.second;
+      } else acc[pr.first] += pr.second;
+    }
+    for (auto &q : acc) out.e.push_back(q);
+    pc.line[d] = out;
+    state[d] = 2;
+    return true;
+  };
+  for (long long d = 0; d < N; ++d) if (con[d] && !close(d)) return false;
+  // tabular periodic BCs (setPeriodicityConstraints, :1276-1310): the coefficient of the table
+  // segment of each increment's first Newton iteration; the counter moves at most once per increment
+  if (P->tabular) {
+    pc.tab_scale.assign(pb->total_increments, 0.0);
+    int tc = 1;
+    for (int inc = 0; inc < pb->total_increments; ++inc) {
+      const double t = pb->delT * (inc + 1) - pb->delT / 1000;
+      if (inc == 0) tc = 1;
+      if (tc < P->n_table && t > P->table_time[tc]) ++tc;
+      pc.tab_scale[inc] = (tc - 1 < P->n_table) ? (double)P->table_coef[tc - 1] : 0.0;
+    }
+  }
+  return true;
+}
+
+// Load scale of the constraints at Newton iteration `it` of increment `inc`
+static double periodic_scale(const PeriodicCons &pc, const or_problem *pb, int inc, int it) {
+  if (it != 0) return 0.0;
+  if (pb->periodic->tabular) return pc.tab_scale[inc];
+  return 1.0;
+}
+
+// One Newton system under the closed periodic constraints: distribute_local_to_global of every
+// element (condensed matrix C^T K C and right-hand side C^T (R_e - K_e s g) on the unconstrained
+// rows; a constrained row keeps |K_ii| on its diagonal, or the mean |diag| of the element), the
+// Krylov solve (solveLinearSystem.cc:13-31) and constraints.distribute (:32).  Returns ||rhs||.
+static double periodic_newton_solve(const Mesh &m, const PeriodicCons &pc, double sc, const std::vector<double> &Kbuf,
+                                    const std::vector<double> &Rbuf, const or_problem *pb, std::vector<double> &du, int *iters) {
+  const long long N = m.ndof;
+  std::vector<std::map<long long, double>> J(N);
+  std::vector<double> b(N, 0.0);
+  std::vector<std::pair<long long, double>> ex[24];
+  for (long long e = 0; e < m.nelem; ++e) {
+    int dofs[24];
+    m.elem_dofs(e, dofs);
+    const double *Ke = &Kbuf[576 * e], *Re = &Rbuf[24 * e];
+    for (int i = 0; i < 24; ++i) {
+      ex[i].clear();
+      if (pc.con[dofs[i]]) ex[i] = pc.line[dofs[i]].e;
+      else ex[i].push_back({dofs[i], 1.0});
+    }
+    double avg = 0;
+    for (int i = 0; i < 24; ++i) avg += std::fabs(Ke[i * 24 + i]);
+    avg /= 24.0;
+    for (int i = 0; i < 24; ++i) {
+      double ri = Re[i];
+      for (int j = 0; j < 24; ++j)
+        if (pc.con[dofs[j]] && sc != 0.0 && pc.line[dofs[j]].g != 0.0) ri -= Ke[i * 24 + j] * sc * pc.line[dofs[j]].g;
+      for (auto &a : ex[i]) {
+        b[a.first] += a.second * ri;
+        for (int j = 0; j < 24; ++j)
+          for (auto &c : ex[j]) J[a.first][c.first] += a.second * Ke[i * 24 + j] * c.second;
+      }
+      if (pc.con[dofs[i]]) {
+        const double nd = std::fabs(Ke[i * 24 + i]) != 0 ? std::fabs(Ke[i * 24 + i]) : avg;
+        J[dofs[i]][dofs[i]] += nd;
+      }
+    }
+  }
+  CSR A;
+  A.rp.assign(N + 1, 0);
+  for (long long r = 0; r < N; ++r) {
+    J[r][r] += 0.0;  // every row holds its diagonal
+    A.rp[r + 1] = A.rp[r] + (long long)J[r].size();
+  }
+  A.ci.resize(A.rp[N]);
+  A.val.resize(A.rp[N]);
+  for (long long r = 0; r < N; ++r) {
+    long long k = A.rp[r];
+    for (auto &pr : J[r]) { A.ci[k] = (int)pr.first; A.val[k] = pr.second; ++k; }
+  }
+  const double rn = std::sqrt(dot(b, b));
+  std::vector<double> dx;
+  const int its = (pb->solver == 1) ? pcg(A, b, dx, pb->lin_tol * rn, pb->lin_maxit) : bicg(A, b, dx, pb->lin_tol * rn, pb->lin_maxit);
+  if (iters) *iters = its;
+  du.assign(N, 0.0);
+  for (long long d = 0; d < N; ++d) {
+    if (!pc.con[d]) { du[d] = dx[d]; continue; }
+    double v = sc * pc.line[d].g;
+    for (auto &pr : pc.line[d].e) v += pr.second * dx[pr.first];
+    du[d] = v;
+  }
+  return rn;
+}
+
+long long or_periodic_lines(const or_problem *pb, int *constrained, long long *rowptr, long long *master, double *weight,
+                            double *g, long long cap) {
+  if (!pb->periodic) return -1;
+  const Mesh m = make_mesh(pb);
+  PeriodicCons pc;
+  if (!build_periodic(m, pb, pc)) return -1;
+  long long k = 0;
+  for (long long d = 0; d < m.ndof; ++d) {
+    if (constrained) constrained[d] = pc.con[d];
+    if (rowptr) rowptr[d] = k;
+    if (g) g[d] = pc.con[d] ? pc.line[d].g : 0.0;
+    if (!pc.con[d]) continue;
+    for (auto &pr : pc.line[d].e) {
+      if (k < cap) { master[k] = pr.first; weight[k] = pr.second; }
+      ++k;
+    }
+  }
+  if (rowptr) rowptr[m.ndof] = k;
+  return k;
+}
+
 // ======================================================================= C API
 extern "C" {
 
@@ -1331,6 +1627,7 @@ int or_krylov(int method, long long n, const long long *rp, const int *ci, const
 int or_newton_system(const or_material *mat, const or_problem *pb, const unsigned *elem_grain, const double *grain_rod,
                      int n_grain_rows, int inc, int it, double *K_vals, double *R, double *du) {
   if (inc < 0 || it < 0 || it >= pb->max_newton) return 1;
+  if (pb->periodic) return 2;  // the condensed periodic matrix has another pattern
   g_dump_K = K_vals; g_dump_R = R; g_dump_du = du;
   g_dump_inc = inc; g_dump_it = it;
   or_problem p1 = *pb;
@@ -1344,6 +1641,7 @@ int or_newton_system(const or_material *mat, const or_problem *pb, const unsigne
 
 int or_first_newton(const or_material *mat, const or_problem *pb, const unsigned *elem_grain, const double *grain_rod,
                     int n_grain_rows, double *K_vals, double *R, double *du) {
+  if (pb->periodic) return 2;  // the condensed periodic matrix has another pattern
   g_dump_K = K_vals; g_dump_R = R; g_dump_du = du;
   g_dump_inc = g_dump_it = 0;
   or_problem p1 = *pb;
@@ -1427,6 +1725,9 @@ int simulate(const or_material *mats, int nph, const int *grain_phase, const or_
       for (int i = 0; i < 3; ++i) st.rotnew_iter[3 * Q + i] = st.rot_iter[3 * Q + i];
   };
 
+  const bool periodic = pb->periodic != nullptr;
+  PeriodicCons pcons;
+  if (periodic && !build_periodic(m, pb, pcons)) return -3;
   CSR A = make_csr(m);
   std::vector<double> x(m.ndof, 0.0), R(m.ndof), du;
   std::vector<double> Kbuf(576 * m.nelem), Rbuf(24 * m.nelem);
@@ -1435,7 +1736,8 @@ int simulate(const or_material *mats, int nph, const int *grain_phase, const or_
   double vol_total = 0;
 
   for (int inc = 0; inc < pb->n_increments; ++inc) {
-    dirichlet(m, pb, inc, con, delu);
+    if (periodic) { con.assign(m.ndof, 0); delu.assign(m.ndof, 0.0); }
+    else dirichlet(m, pb, inc, con, delu);
     for (int it = 0; it < pb->max_newton; ++it) {
       OR_FLOP_STEP(inc, it, pb->max_newton);
       const long long t_asm = now_ns();
@@ -1474,6 +1776,15 @@ int simulate(const or_material *mats, int nph, const int *grain_phase, const or_
             }
           }
         }
+      }
+      if (periodic) {
+        const double sc = periodic_scale(pcons, pb, inc, it);
+        const double rn = periodic_newton_solve(m, pcons, sc, Kbuf, Rbuf, pb, du,
+                                                (lin_iters ? &lin_iters[inc * pb->max_newton + it] : nullptr));
+        if (res_norms) res_norms[inc * pb->max_newton + it] = rn;
+        for (long long r = 0; r < m.ndof; ++r) x[r] += du[r];
+        g_phase_ns[1] += now_ns() - t_asm;
+        continue;
       }
       // AffineConstraints::distribute_local_to_global with inhomogeneous Dirichlet lines:
       // free rows get K_ff and R - K_fc g; constrained rows only a diagonal |K_ii| (or the mean |diag|).

@@ -47,6 +47,30 @@ typedef struct {
   double twin_shear, twin_threshold, twin_saturation;
 } or_material;
 
+/* Periodic boundary conditions (Use Periodic BCs; ellipticBVP init.cc:66-173, periodicBCs.cc).
+ * Rows of the constraint file (PeriodicBCsConstraints.txt): vertex rows 0..23 = V000 V100 V010 V001
+ * V110 V101 V011 V111 x 3 components; edge rows 0..35 = EX000 EX001 EX010 EX011 EY000 EY001 EY100
+ * EY101 EZ000 EZ010 EZ100 EZ110 x 3; face rows 0..17 = FXN FXP FYN FYP FZN FZP x 3. */
+#define OR_MAX_PBC_LINES 64
+typedef struct {
+  int n_vertex_lines;                            /* Number of Vertices Constraints */
+  int vertex_dofs[OR_MAX_PBC_LINES][5];          /* entry count, then vertex rows */
+  double vertex_coef[OR_MAX_PBC_LINES][5];       /* (count), then their coefficients */
+  int n_edge_lines;                              /* Number of Edges Constraints */
+  int edge_dofs[OR_MAX_PBC_LINES][4];            /* edge row, edge row, vertex row, vertex row */
+  double edge_coef[OR_MAX_PBC_LINES][4];
+  int n_face_lines;                              /* Number of Faces Constraints */
+  int face_dofs[OR_MAX_PBC_LINES][4];            /* positive-face row, face row, vertex row, vertex row */
+  double face_coef[OR_MAX_PBC_LINES][4];
+  int vertex_known[24];                          /* Vertices Periodic BCs row 1 */
+  double vertex_total[24];                       /* Vertices Periodic BCs row 2 */
+  int tabular;                                   /* Use Tabular Periodic BCs */
+  double tabular_time;                           /* periodic Tabular time */
+  int n_table;
+  double table_time[OR_MAX_TABLE];               /* Tabular Periodic Time Table */
+  int table_coef[OR_MAX_TABLE];                  /* Tabular Periodic Time Table Coefficient (1, -1, 0) */
+} or_periodic;
+
 typedef struct {
   int nx, ny, nz;                   /* elements per direction (after refinement) */
   double span[3];
@@ -69,6 +93,7 @@ typedef struct {
   double table_u[6][3][OR_MAX_TABLE];  /* displacement of each (face,dof) rule at the table times */
   int velgrad;                      /* Use velocity gradient BC */
   double vel_grad[3][3];            /* L */
+  const or_periodic *periodic;      /* Use Periodic BCs (NULL: off); the face rules above must be empty */
 } or_problem;
 
 /* Reorientation by twinning at the end of calculatePlasticity (calculatePlasticity.cc:962-1019),
@@ -151,6 +176,14 @@ int or_simulate(const or_material *mat, const or_problem *pb, const unsigned *el
 int or_simulate_phases(const or_material *mats, int n_phases, const int *grain_phase, const or_problem *pb,
                        const unsigned *elem_grain, const double *grain_rod, int n_grain_rows, double *rows, int *lin_iters,
                        double *res_norms, double *state_out, double *u_out);
+
+/* The closed periodic constraints of pb (pb->periodic set) at unit load scale (the increment-0
+ * inhomogeneities of setPeriodicityConstraintsInc0): for every dof d, constrained[d] = 1 if the
+ * dof is a constrained line, whose entries are master[rowptr[d]..rowptr[d+1]) with weight[...],
+ * and g[d] its inhomogeneity.  rowptr has ndof + 1 entries.  Returns the number of entries (call
+ * with cap 0 to size master/weight) or -1 if the input is refused. */
+long long or_periodic_lines(const or_problem *pb, int *constrained, long long *rowptr, long long *master, double *weight,
+                            double *g, long long cap);
 
 /* Increment 0, Newton iteration 0 only: the condensed CSR values (pattern of or_sparsity),
  * the condensed residual and the Newton correction du (after constraints.distribute). */

@@ -137,7 +137,72 @@ class Problem(ctypes.Structure):
         ("tabular", ctypes.c_int), ("n_table", ctypes.c_int), ("table_time", ctypes.c_double * 32),
         ("table_u", ctypes.c_double * 32 * 3 * 6),
         ("velgrad", ctypes.c_int), ("vel_grad", ctypes.c_double * 3 * 3),
+        ("periodic", ctypes.c_void_p),
     ]
+
+
+MAX_PBC = 64
+
+
+class Periodic(ctypes.Structure):
+    """or_periodic: the constraint file's vertex / edge / face lines, the vertex rows of the prm and
+    the tabular periodic table (ellipticBVP init.cc:66-131, userInputParameters.cc:126-136)."""
+    _fields_ = [
+        ("n_vertex_lines", ctypes.c_int), ("vertex_dofs", ctypes.c_int * 5 * MAX_PBC), ("vertex_coef", ctypes.c_double * 5 * MAX_PBC),
+        ("n_edge_lines", ctypes.c_int), ("edge_dofs", ctypes.c_int * 4 * MAX_PBC), ("edge_coef", ctypes.c_double * 4 * MAX_PBC),
+        ("n_face_lines", ctypes.c_int), ("face_dofs", ctypes.c_int * 4 * MAX_PBC), ("face_coef", ctypes.c_double * 4 * MAX_PBC),
+        ("vertex_known", ctypes.c_int * 24), ("vertex_total", ctypes.c_double * 24),
+        ("tabular", ctypes.c_int), ("tabular_time", ctypes.c_double), ("n_table", ctypes.c_int),
+        ("table_time", ctypes.c_double * 32), ("table_coef", ctypes.c_int * 32),
+    ]
+
+
+def read_periodic_file(path, n_vertex, n_edge, n_face):
+    """PeriodicBCsConstraints.txt as ellipticBVP init.cc:78-126 reads it: 4 header lines, n_vertex
+    pairs of lines (5 integers: entry count and vertex rows; 5 coefficients), 5 header lines,
+    n_edge pairs (4 rows, 4 coefficients), 5 header lines, n_face pairs."""
+    with open(path) as f:
+        lines = f.read().split("\n")
+    pos = [0]
+
+    def take():
+        line = lines[pos[0]]
+        pos[0] += 1
+        return line
+
+    def block(n_header, n_lines, width):
+        for _ in range(n_header):
+            take()
+        rows, coefs = [], []
+        for _ in range(n_lines):
+            rows.append([int(v) for v in take().split()[:width]])
+            coefs.append([float(v) for v in take().split()[:width]])
+        return rows, coefs
+    vr, vc = block(4, n_vertex, 5)
+    er, ec = block(5, n_edge, 4)
+    fr, fc = block(5, n_face, 4)
+    return dict(vertex=(vr, vc), edge=(er, ec), face=(fr, fc))
+
+
+def periodic_struct(per):
+    """or_periodic (and pa_periodic, the same layout) from a periodic dict."""
+    P = Periodic()
+    for name, (rows, coefs) in (("vertex", per["vertex"]), ("edge", per["edge"]), ("face", per["face"])):
+        setattr(P, "n_%s_lines" % name, len(rows))
+        R, Cf = getattr(P, "%s_dofs" % name), getattr(P, "%s_coef" % name)
+        for i, (r, c) in enumerate(zip(rows, coefs)):
+            for k in range(len(r)):
+                R[i][k] = r[k]
+                Cf[i][k] = c[k]
+    for i in range(24):
+        P.vertex_known[i] = int(per["known"][i])
+        P.vertex_total[i] = float(per["total"][i])
+    if per.get("tabular"):
+        P.tabular, P.tabular_time = 1, per["tabular_time"]
+        P.n_table = len(per["table_time"])
+        for i, (t, c) in enumerate(zip(per["table_time"], per["table_coef"])):
+            P.table_time[i], P.table_coef[i] = t, int(c)
+    return P
 
 
 class Case:
@@ -169,6 +234,7 @@ class Case:
         # twinning: dict(m, n (normalised), s0, h0, ss, pw per twin system, q (latent ratios over
         # slip + twin systems), shear, threshold, saturation); None = the inert twin system
         self.twin = None
+        self.periodic = None      # periodic BCs: dict of read_periodic_file + known, total (and the tabular table)
 
     # --- reference defaults for the inert twin system (userInputParameters.cc:300-311)
     def material(self, pinv_mode=1):
@@ -238,6 +304,9 @@ class Case:
             for a in range(3):
                 for b in range(3):
                     pb.vel_grad[a][b] = self.vel_grad[a][b]
+        if getattr(self, "periodic", None):
+            self._pbc = periodic_struct(self.periodic)  # kept alive with the case
+            pb.periodic = ctypes.cast(ctypes.pointer(self._pbc), ctypes.c_void_p)
         return pb
 
     @property
@@ -280,12 +349,12 @@ def _phase_case(prm, d, k, base):
     return c
 
 
-def load_reference_case(folder, twinning=False):
+def load_reference_case(folder, twinning=False, prm_name="prm.prm"):
     """A Case built from one of the reference application folders copied under tests/golden; with
     twinning=True the prm's twin systems and parameters are read (userInputParameters.cc twin
     block), otherwise the case runs with the inert twin system."""
     d = os.path.join(GOLDEN, folder)
-    prm = parse_prm(os.path.join(d, "prm.prm"))
+    prm = parse_prm(os.path.join(d, prm_name))
     ns = int(prm["Number of Slip Systems"])
     C = np.array([_floats(prm["Elastic Stiffness row %d" % (i + 1)]) for i in range(6)])
     m = read_vectors(os.path.join(d, prm["Slip Directions File"]), ns)
@@ -309,7 +378,7 @@ def load_reference_case(folder, twinning=False):
                     rules.append((int(v[0]), int(v[1]), [float(x) for x in v[2:2 + nt]]))
         tabular = (_floats(prm["Tabular Time Table"])[:nt], rules)
         bcs = []
-    elif _true(prm, "Use velocity gradient BC"):  # userInputParameters.cc:78
+    elif _true(prm, "Use velocity gradient BC") or _true(prm, "Use Periodic BCs"):  # userInputParameters.cc:78, 126
         bcs = []
     else:
         bcs = read_bcs(os.path.join(d, prm["Boundary condition filename"]), int(prm["BC file number of header lines"]),
@@ -339,6 +408,17 @@ def load_reference_case(folder, twinning=False):
     case.tabular = tabular
     if _true(prm, "Use velocity gradient BC"):
         case.vel_grad = np.array([_floats(prm["Velocity gradient row %d" % (i + 1)]) for i in range(3)])
+    case.periodic = None
+    if _true(prm, "Use Periodic BCs"):
+        case.periodic = read_periodic_file(os.path.join(d, prm["Periodic Boundary condition Constraint filename"]),
+                                           int(prm["Number of Vertices Constraints"]), int(prm["Number of Edges Constraints"]),
+                                           int(prm["Number of Faces Constraints"]))
+        case.periodic["known"] = [int(v) for v in prm["Vertices Periodic BCs row 1"].split(",")]
+        case.periodic["total"] = _floats(prm["Vertices Periodic BCs row 2"])
+        if _true(prm, "Use Tabular Periodic BCs"):
+            case.periodic.update(tabular=True, tabular_time=float(prm["periodic Tabular time"]),
+                                 table_time=_floats(prm["Tabular Periodic Time Table"]),
+                                 table_coef=[int(v) for v in prm["Tabular Periodic Time Table Coefficient"].split(",")])
     case.vox, case.nv = vox, nv
     case.grain_rod = rod
     case.phases = None  # multiphase: one Case per phase (materials), grain_phase[grain row] in 1..n
