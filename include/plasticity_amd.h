@@ -92,6 +92,32 @@ typedef struct {
   double vel_grad[3][3];                 /* Velocity gradient rows 1-3 (L) */
 } pa_problem;
 
+/* Periodic boundary conditions (Use Periodic BCs): the constraint file PeriodicBCsConstraints.txt
+ * as ellipticBVP::init reads it (src/ellipticBVP/init.cc:66-131) and the prm's vertex rows and
+ * tabular table (userInputParameters.cc:126-136).  Rows: vertex 0..23 = V000 V100 V010 V001 V110
+ * V101 V011 V111 x 3 components; edge 0..35 = EX000 EX001 EX010 EX011 EY000 EY001 EY100 EY101
+ * EZ000 EZ010 EZ100 EZ110 x 3 (EXab: y = a, z = b; EYab: x = a, z = b; EZab: x = a, y = b);
+ * face 0..17 = FXN FXP FYN FYP FZN FZP x 3. */
+#define PA_MAX_PBC_LINES 64
+typedef struct {
+  int n_vertex_lines;                        /* Number of Vertices Constraints */
+  int vertex_dofs[PA_MAX_PBC_LINES][5];      /* entry count, then the line's vertex rows */
+  double vertex_coef[PA_MAX_PBC_LINES][5];   /* (count), then their coefficients */
+  int n_edge_lines;                          /* Number of Edges Constraints */
+  int edge_dofs[PA_MAX_PBC_LINES][4];        /* constrained edge row, edge row, vertex row, vertex row */
+  double edge_coef[PA_MAX_PBC_LINES][4];
+  int n_face_lines;                          /* Number of Faces Constraints */
+  int face_dofs[PA_MAX_PBC_LINES][4];        /* positive-face row (FXP/FYP/FZP), face row, vertex row, vertex row */
+  double face_coef[PA_MAX_PBC_LINES][4];
+  int vertex_known[24];                      /* Vertices Periodic BCs row 1 */
+  double vertex_total[24];                   /* Vertices Periodic BCs row 2 (over the load path) */
+  int tabular;                               /* Use Tabular Periodic BCs */
+  double tabular_time;                       /* periodic Tabular time */
+  int n_table;                               /* entries of the two tables below (<= PA_MAX_TABLE) */
+  double table_time[PA_MAX_TABLE];           /* Tabular Periodic Time Table */
+  int table_coef[PA_MAX_TABLE];              /* Tabular Periodic Time Table Coefficient: 1 load, -1 unload, 0 hold */
+} pa_periodic;
+
 typedef struct pa_context pa_context;
 
 /* Last error message of the calling thread (never NULL). */
@@ -118,6 +144,27 @@ int pa_set_grains(pa_context *ctx, const unsigned *elem_grain, const double *gra
  * CG = Jacobi-PCG) and arithmetic form (FP64, or mixed: FP32 element tangents / matrix values,
  * FP64 residual, return map and Krylov recurrences). */
 int pa_set_solver(pa_context *ctx, int method, int precision);
+
+/* Periodic boundary conditions in place of the face rules: replaces setPeriodicityConstraintsInit
+ * and setPeriodicityConstraints (src/ellipticBVP/periodicBCs.cc:23-864, 1206-1310, with
+ * setFace/Edge/NodeConstraints :866-1204) and the condensed assembly and distribute of their
+ * affine constraints (assemble.cc:20-35, 78-82; solveLinearSystem.cc:32).  The dofs of the
+ * positive faces and of nine edges follow their periodic images plus corner terms, the corners
+ * follow the vertex lines and rows; the Newton iteration solves for the unconstrained dofs (the
+ * Jacobian applied as C^T K C, Jacobi on its diagonal).  Needs equal element counts nx = ny = nz
+ * >= 2 (the reference sizes every edge by Subdivisions Y - 1, periodicBCs.cc:49) and a Krylov
+ * method other than PA_SOLVER_MGCG; the face rules of the problem are ignored (as
+ * boundaryConditions.cc:455 does).  Call before the first Newton iteration. */
+int pa_set_periodic(pa_context *ctx, const pa_periodic *per);
+
+/* The closed constraint lines pa_set_periodic builds for the problem's mesh, on the host (no
+ * device call; no counterpart in the reference, whose lines live in an AffineConstraints object):
+ * constrained[d] = 1 for a constrained dof, whose line is u_d = sum weight[k] u_{master[k]} over
+ * k in [rowptr[d], rowptr[d+1]) plus g[d] at the first Newton iteration of a loading increment.
+ * rowptr has 3 (nx+1)(ny+1)(nz+1) + 1 entries; master and weight hold up to cap entries;
+ * *n_entries receives the number of entries (call with cap 0 to size the arrays). */
+int pa_periodic_lines(const pa_problem *pb, const pa_periodic *per, int *constrained, long long *rowptr, long long *master,
+                      double *weight, double *g, long long cap, long long *n_entries);
 
 /* Layout of the Newton step's quadrature-point passes (calculatePlasticity over every qp in
  * getElementalValues.cc:141 and updateAfterIncrement.cc:105).  No counterpart in the reference

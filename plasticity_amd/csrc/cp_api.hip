@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -1484,6 +1485,7 @@ __global__ void k_zero(long long N, double *__restrict__ v) {
 }
 
 #include "cp_mg.h"
+#include "cp_pbc.h"
 
 // ------------------------------------------------------------ host side
 inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -1547,6 +1549,12 @@ struct pa_context {
   double *d_out = nullptr;
   long long *d_rp = nullptr;
   int bc_inc = -1;
+  // periodic boundary conditions (pa_set_periodic): the closed constraint lines on the device and
+  // the load scale of each increment's first Newton iteration
+  bool pbc = false;
+  PbcDev pbcd = {};
+  std::vector<double> pbc_scale;
+  double pbc_s = 0.0;  // load scale of the current assembly
   // multigrid preconditioner (PA_SOLVER_MGCG): levels 0..mg_levels, mg_levels the coarsest;
   // level 0 is the Jacobian with the Krylov residual as right-hand side and z as solution
   int mg_levels = 0;
@@ -1753,7 +1761,40 @@ size_t partials_size(const pa_context *c) {
 
 size_t kbytes(const pa_context *c) { return (mixed(c) ? sizeof(float) : sizeof(double)) * 243 * (size_t)c->m.nnode; }
 
+// periodic BCs: y (and y2) folded onto the unconstrained dofs (cp_pbc.h); st: skip once the Krylov
+// solve has ended (null outside a solve)
+void pbc_fold(pa_context *c, double *y, double *y2, const KState *st) {
+  const PbcDev &p = c->pbcd;
+  if (p.nshort) hipLaunchKernelGGL(k_pbc_fold_short, dim3(nblk(p.nshort, VB)), dim3(VB), 0, c->stream, p, y, y2, st);
+  if (p.nlong) hipLaunchKernelGGL(k_pbc_fold_long, dim3(PBC_GL, p.nlong), dim3(VB), 0, c->stream, p, (const double *)y, (const double *)y2, st);
+  hipLaunchKernelGGL(k_pbc_fold_final, dim3(std::max(1u, nblk(std::max(p.ncon, p.nlong), VB))), dim3(VB), 0, c->stream, p, y, y2, st);
+}
+void pbc_expand(pa_context *c, double *x, double *x2, double scale, const KState *st) {
+  if (c->pbcd.ncon) hipLaunchKernelGGL(k_pbc_expand, dim3(nblk(c->pbcd.ncon, VB)), dim3(VB), 0, c->stream, c->pbcd, x, x2, scale, st);
+}
+// periodic BCs after the unconstrained assembly: R <- C^T (R - K s g), Jacobi entries of C^T K C
+template <typename KT>
+int pbc_assembled(pa_context *c, int inc, int it) {
+  const PbcDev &p = c->pbcd;
+  const double sc = (it == 0) ? c->pbc_scale[std::min<size_t>((size_t)inc, c->pbc_scale.size() - 1)] : 0.0;
+  c->pbc_s = sc;
+  const KT *K = (const KT *)c->K;
+  if (sc != 0.0) {
+    double *ge = c->kv[5];
+    hipLaunchKernelGGL(k_zero, dim3(nblk(c->ndof, 256)), dim3(256), 0, c->stream, c->ndof, ge);
+    pbc_expand(c, ge, nullptr, sc, nullptr);
+    hipLaunchKernelGGL(k_pbc_rhs<KT>, dim3(nblk(c->m.nnode, VB)), dim3(VB), 0, c->stream, c->m, K, (const double *)ge, c->R);
+  }
+  pbc_fold(c, c->R, nullptr, nullptr);
+  if (p.nshort) hipLaunchKernelGGL(k_pbc_diag_short<KT>, dim3(nblk(p.nshort, VB)), dim3(VB), 0, c->stream, p, c->m, K, c->dinv);
+  if (p.nlong) hipLaunchKernelGGL(k_pbc_diag_long<KT>, dim3(PBC_GL, p.nlong), dim3(VB), 0, c->stream, p, c->m, K);
+  hipLaunchKernelGGL(k_pbc_diag_final, dim3(std::max(1u, nblk(std::max(p.ncon, p.nlong), VB))), dim3(VB), 0, c->stream, p, c->dinv);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 int set_dirichlet(pa_context *c, int inc) {
+  if (c->pbc) return 0;  // boundaryConditions.cc:455: the face rules are not applied with periodic BCs
   if (c->bc_inc == inc) return 0;
   BCArgs bc;
   std::memset(&bc, 0, sizeof(bc));
@@ -1815,6 +1856,10 @@ int do_assemble(pa_context *c, int inc, int it, double *res_norm) {
   if (mixed(c)) hipLaunchKernelGGL(k_dinv<float>, dim3(nblk(c->ndof, 256)), dim3(256), 0, c->stream, c->m, (const float *)c->K, c->dinv);
   else hipLaunchKernelGGL(k_dinv<double>, dim3(nblk(c->ndof, 256)), dim3(256), 0, c->stream, c->m, (const double *)c->K, c->dinv);
   HIPCHK(hipGetLastError());
+  if (c->pbc) {
+    rc = mixed(c) ? pbc_assembled<float>(c, inc, it) : pbc_assembled<double>(c, inc, it);
+    if (rc) return rc;
+  }
   if (res_norm) {
     hipLaunchKernelGGL(k_norm2, dim3(vgrid(c->ndof)), dim3(256), 0, c->stream, c->ndof, c->R, c->d_out + 16,
                        c->partials + update_partials_end(c),
@@ -2005,23 +2050,27 @@ int launch_krylov_iteration(pa_context *c, int k) {
   } else if (c->method == PA_SOLVER_CG) {
     double *Q = Zl;
     hipLaunchKernelGGL(k_cg_p, dim3(gu), dim3(VB), 0, c->stream, N, (const double *)Zr, Pr, c->kst, (const double *)partU, gu, k);
+    if (c->pbc) pbc_expand(c, Pr, nullptr, 0.0, c->kst);
     if (mixed(c))
       hipLaunchKernelGGL(k_cg_spmv<float>, dim3(gn), dim3(VB), 0, c->stream, c->m, (const float *)c->K, (const double *)Pr, Q,
                          (const KState *)c->kst, partA);
     else
       hipLaunchKernelGGL(k_cg_spmv<double>, dim3(gn), dim3(VB), 0, c->stream, c->m, (const double *)c->K, (const double *)Pr, Q,
                          (const KState *)c->kst, partA);
+    if (c->pbc) pbc_fold(c, Q, nullptr, c->kst);
     hipLaunchKernelGGL(k_cg_update, dim3(gu), dim3(VB), 0, c->stream, N, X, (const double *)Pr, Rr, (const double *)Q, Zr,
                        (const double *)c->dinv, (const KState *)c->kst, (const double *)partA, gn, partU, k);
   } else {
     hipLaunchKernelGGL(k_bicg_p, dim3(gu), dim3(VB), 0, c->stream, N, (const double *)Zr, (const double *)Zl, Pr, Pl, c->kst,
                        (const double *)partU, gu, k);
+    if (c->pbc) pbc_expand(c, Pr, Pl, 0.0, c->kst);
     if (mixed(c))
       hipLaunchKernelGGL(k_bicg_spmv<float>, dim3(gn), dim3(VB), 0, c->stream, c->m, (const float *)c->K, (const double *)Pr,
                          (const double *)Pl, Zr, Zl, (const KState *)c->kst, partA);
     else
       hipLaunchKernelGGL(k_bicg_spmv<double>, dim3(gn), dim3(VB), 0, c->stream, c->m, (const double *)c->K, (const double *)Pr,
                          (const double *)Pl, Zr, Zl, (const KState *)c->kst, partA);
+    if (c->pbc) pbc_fold(c, Zr, Zl, c->kst);
     hipLaunchKernelGGL(k_bicg_update, dim3(gu), dim3(VB), 0, c->stream, N, X, (const double *)Pr, Rr, Rl, Zr, Zl,
                        (const double *)c->dinv, (const KState *)c->kst, (const double *)partA, gn, partU, k);
   }
@@ -2074,8 +2123,11 @@ int do_solve(pa_context *c, int it, int *iters) {
     return fail(PA_ERR_NONFINITE, "Krylov solve stopped on a NaN/Inf residual norm at iteration " + std::to_string(hs.it) +
                                       " (right-hand side norm " + std::to_string(hs.bnorm) + ")");
   // the CG iterate x is complete; BiCG likewise (x lives in kv[0] for both)
-  hipLaunchKernelGGL(k_apply_du, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, c->u, (const double *)X, (const unsigned char *)c->con,
-                     (const double *)c->delu, (int)(it == 0));
+  if (c->pbc)
+    hipLaunchKernelGGL(k_pbc_apply, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, c->u, (const double *)X, c->pbcd, c->pbc_s);
+  else
+    hipLaunchKernelGGL(k_apply_du, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, c->u, (const double *)X, (const unsigned char *)c->con,
+                       (const double *)c->delu, (int)(it == 0));
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -2357,12 +2409,78 @@ int pa_set_solver(pa_context *c, int method, int precision) {
   if (!c) return fail(PA_ERR_ARG, "null context");
   if (method != PA_SOLVER_BICG && method != PA_SOLVER_CG && method != PA_SOLVER_MGCG) return fail(PA_ERR_ARG, "unknown solver");
   if (precision != PA_PREC_FP64 && precision != PA_PREC_MIXED) return fail(PA_ERR_ARG, "unknown precision");
+  if (method == PA_SOLVER_MGCG && c->pbc) return fail(PA_ERR_UNSUPPORTED, "multigrid-PCG does not take periodic boundary conditions");
   if (method == PA_SOLVER_MGCG) {
     const int rc = mg_alloc(c);
     if (rc) return rc;
   }
   c->method = method;
   c->precision = precision;
+  return 0;
+}
+
+int pa_periodic_lines(const pa_problem *pb, const pa_periodic *per, int *constrained, long long *rowptr, long long *master,
+                      double *weight, double *g, long long cap, long long *n_entries) {
+  if (!pb || !per || !n_entries || cap < 0 || (cap > 0 && (!master || !weight))) return fail(PA_ERR_ARG, "null argument");
+  PbcLines L;
+  std::string err;
+  const int rc = pbc_build(*pb, *per, L, err);
+  if (rc) return fail(rc, err);
+  const long long N = (long long)L.crow.size();
+  long long k = 0;
+  for (long long d = 0; d < N; ++d) {
+    const int r = L.crow[d];
+    if (constrained) constrained[d] = r >= 0;
+    if (rowptr) rowptr[d] = k;
+    if (g) g[d] = r >= 0 ? L.cg[r] : 0.0;
+    if (r < 0) continue;
+    for (int q = L.cptr[r]; q < L.cptr[r + 1]; ++q, ++k)
+      if (k < cap) { master[k] = L.cidx[q]; weight[k] = L.cw[q]; }
+  }
+  if (rowptr) rowptr[N] = k;
+  *n_entries = k;
+  return 0;
+}
+
+int pa_set_periodic(pa_context *c, const pa_periodic *per) {
+  if (!c || !per) return fail(PA_ERR_ARG, "null argument");
+  if (c->pbc) return fail(PA_ERR_STATE, "periodic boundary conditions are already set");
+  if (c->method == PA_SOLVER_MGCG) return fail(PA_ERR_UNSUPPORTED, "multigrid-PCG does not take periodic boundary conditions");
+  PbcLines L;
+  std::string err;
+  int rc = pbc_build(c->pb, *per, L, err);
+  if (rc) return fail(rc, err);
+  PbcDev &p = c->pbcd;
+  p.ncon = (int)L.cdof.size();
+  p.nshort = (int)L.sdof.size();
+  p.nlong = (int)L.ldof.size();
+  auto up_i = [&](const std::vector<int> &v, const int **dst) -> int {
+    int *d = nullptr;
+    if (int r = dalloc(c, &d, std::max<size_t>(v.size(), 1))) return r;
+    if (!v.empty()) HIPCHK(hipMemcpy(d, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
+    *dst = d;
+    return 0;
+  };
+  auto up_d = [&](const std::vector<double> &v, const double **dst) -> int {
+    double *d = nullptr;
+    if (int r = dalloc(c, &d, std::max<size_t>(v.size(), 1))) return r;
+    if (!v.empty()) HIPCHK(hipMemcpy(d, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+    *dst = d;
+    return 0;
+  };
+  if ((rc = up_i(L.crow, &p.crow)) || (rc = up_i(L.cdof, &p.cdof)) || (rc = up_i(L.cptr, &p.cptr)) || (rc = up_i(L.cidx, &p.cidx)) ||
+      (rc = up_d(L.cw, &p.cw)) || (rc = up_d(L.cg, &p.cg)) || (rc = up_i(L.sdof, &p.sdof)) || (rc = up_i(L.sptr, &p.sptr)) ||
+      (rc = up_i(L.sidx, &p.sidx)) || (rc = up_d(L.sw, &p.sw)) || (rc = up_i(L.ldof, &p.ldof)) || (rc = up_i(L.lptr, &p.lptr)) ||
+      (rc = up_i(L.lidx, &p.lidx)) || (rc = up_d(L.lw, &p.lw)))
+    return rc;
+  if ((rc = dalloc(c, &p.part, std::max<size_t>(1, (size_t)p.nlong * PBC_GL * 2)))) return rc;
+  // no Dirichlet lines: the face rules are not applied (boundaryConditions.cc:455)
+  HIPCHK(hipMemsetAsync(c->con, 0, (size_t)c->ndof, c->stream));
+  HIPCHK(hipMemsetAsync(c->delu, 0, sizeof(double) * (size_t)c->ndof, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->pbc_scale = L.scale;
+  c->pbc = true;
+  c->bc_inc = -1;
   return 0;
 }
 
@@ -2492,6 +2610,7 @@ int pa_run_increment(pa_context *c, int inc, double row[15], int *lin_iters, dou
 
 int pa_last_residual(pa_context *c, double out[3]) {
   if (!c || !out) return fail(PA_ERR_ARG, "null argument");
+  if (c->pbc) return fail(PA_ERR_UNSUPPORTED, "the true residual is not formed under periodic boundary conditions");
   if (!c->solved || c->asm_precision != c->precision) return fail(PA_ERR_STATE, "no linear solve on the current assembly");
   const unsigned g = nblk(c->m.nnode, VB);
   if (!c->resid_part) {

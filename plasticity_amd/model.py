@@ -120,7 +120,21 @@ class userInputParameters:
         self.targetVelGrad = [fl(g("Velocity gradient row %d" % (i + 1), "0, 0, 0")) for i in range(3)]
         if self.useVelocityGrad and any(len(r) != 3 for r in self.targetVelGrad):
             raise native.PlasticityError("'Velocity gradient row' needs three values")
-        for key in ("Use Periodic BCs", "Use Neumann BCs",
+        # periodic BCs (userInputParameters.cc:126-136)
+        self.enablePeriodicBCs = b("Use Periodic BCs", "false")
+        self.Periodic_BCfilename = g("Periodic Boundary condition Constraint filename", "PeriodicBCsConstraints.txt")
+        self.numberVerticesConstraint = int(g("Number of Vertices Constraints", "12"))
+        self.numberEdgesConstraint = int(g("Number of Edges Constraints", "27"))
+        self.numberFacesConstraint = int(g("Number of Faces Constraints", "9"))
+        self.periodicBCsInput = [int(float(x)) for x in fl(g("Vertices Periodic BCs row 1", ""))]
+        self.periodicBCsInput2 = fl(g("Vertices Periodic BCs row 2", ""))
+        self.enableTabularPeriodicBCs = b("Use Tabular Periodic BCs", "false")
+        self.periodicTabularTime = float(g("periodic Tabular time", "-1"))
+        self.tabularPeriodicTimeInput = fl(g("Tabular Periodic Time Table", ""))
+        self.tabularPeriodicCoef = [int(float(x)) for x in fl(g("Tabular Periodic Time Table Coefficient", ""))]
+        if self.enablePeriodicBCs and (len(self.periodicBCsInput) != 24 or len(self.periodicBCsInput2) != 24):
+            raise native.PlasticityError("'Vertices Periodic BCs row 1/2' need 24 values")
+        for key in ("Use Neumann BCs",
                     "Use Indentation BCs", "Use Torsion BCs", "Flag To Use Taylor Model", "Use Element Deletion"):
             if b(key, "false"):
                 raise native.PlasticityError("'%s' is outside the implemented path" % key)
@@ -144,6 +158,52 @@ def read_vectors(path, n):
     if len(rows) < n:
         raise native.PlasticityError("%s: expected %d vectors" % (path, n))
     return np.array(rows)
+
+
+def read_periodic_constraints(path, n_vertex, n_edge, n_face):
+    """PeriodicBCsConstraints.txt (ellipticBVP init.cc:78-126): 4 header lines, then per vertex line a
+    line of 5 integers (entry count, vertex rows) and a line of 5 coefficients; 5 header lines, per
+    edge line 4 rows and 4 coefficients; 5 header lines, per face line 4 rows and 4 coefficients."""
+    try:
+        with open(path) as f:
+            lines = f.read().split("\n")
+    except OSError as e:
+        raise native.PlasticityError("Unable to open Periodic BCfilename %s" % path) from e
+    out, pos = {}, 0
+    for name, head, count, width in (("vertex", 4, n_vertex, 5), ("edge", 5, n_edge, 4), ("face", 5, n_face, 4)):
+        pos += head
+        rows, coefs = [], []
+        for _ in range(count):
+            rows.append([int(v) for v in lines[pos].split()[:width]])
+            coefs.append([float(v) for v in lines[pos + 1].split()[:width]])
+            pos += 2
+        out[name] = (rows, coefs)
+    return out
+
+
+def make_periodic(lines, known, total, tabular=None):
+    """pa_periodic from the constraint file's lines, the vertex rows 1 and 2 of the prm and, for
+    tabular periodic BCs, (periodic Tabular time, time table, coefficient table)."""
+    P = native.PaPeriodic()
+    for name in ("vertex", "edge", "face"):
+        rows, coefs = lines[name]
+        if len(rows) > native.MAX_PBC_LINES:
+            raise native.PlasticityError("more than %d %s constraint lines" % (native.MAX_PBC_LINES, name))
+        setattr(P, "n_%s_lines" % name, len(rows))
+        R, C = getattr(P, "%s_dofs" % name), getattr(P, "%s_coef" % name)
+        for i, (r, c) in enumerate(zip(rows, coefs)):
+            for k in range(len(r)):
+                R[i][k], C[i][k] = r[k], c[k]
+    for i in range(24):
+        P.vertex_known[i], P.vertex_total[i] = int(known[i]), float(total[i])
+    if tabular is not None:
+        t, times, coefs = tabular
+        if len(times) > native.MAX_TABLE or len(coefs) < len(times):
+            raise native.PlasticityError("tabular periodic table: at most %d times, one coefficient each" % native.MAX_TABLE)
+        P.tabular, P.tabular_time, P.n_table = 1, float(t), len(times)
+        for i, v in enumerate(times):
+            P.table_time[i], P.table_coef[i] = v, int(coefs[i])
+    return P
 
 
 def read_table(path, n):
@@ -240,6 +300,10 @@ class Context:
 
     def set_solver(self, method=native.PA_SOLVER_BICG, precision=native.PA_PREC_FP64):
         check(native.lib().pa_set_solver(self._h, ctypes.c_int(method), ctypes.c_int(precision)))
+
+    def set_periodic(self, per: native.PaPeriodic):
+        """Periodic boundary conditions in place of the face rules (pa_set_periodic)."""
+        check(native.lib().pa_set_periodic(self._h, ctypes.byref(per)))
 
     def last_residual(self):
         """(true ||B - K X|| / ||B||, ||D^-1(B - K X)|| / ||B||, the solver's own value) of the last solve."""
@@ -583,11 +647,20 @@ class crystalPlasticity:
         self.problem = make_problem(nel, ui.span, self.totalIncrements, ui.delT, bcs, ui.maxNonLinearIterations,
                                     ui.relLinearSolverTolerance, ui.maxLinearSolverIterations, cyc, tab,
                                     ui.targetVelGrad if ui.useVelocityGrad else None)
+        self.periodic = None
+        if ui.enablePeriodicBCs:  # ellipticBVP init.cc:66-131
+            lines = read_periodic_constraints(ui.file(ui.Periodic_BCfilename), ui.numberVerticesConstraint,
+                                              ui.numberEdgesConstraint, ui.numberFacesConstraint)
+            tabp = ((ui.periodicTabularTime, ui.tabularPeriodicTimeInput, ui.tabularPeriodicCoef)
+                    if ui.enableTabularPeriodicBCs else None)
+            self.periodic = make_periodic(lines, ui.periodicBCsInput, ui.periodicBCsInput2, tabp)
         self.orientations = crystalOrientationsIO()
         self.ctx = None
 
     def _init(self):
         self.ctx = Context(self.material, self.problem)
+        if self.periodic is not None:
+            self.ctx.set_periodic(self.periodic)
         eg = self.ctx.element_grains(self.orientations.vox)
         self.ctx.set_grains(eg, self.orientations.eulerAngles)
         self.cellOrientationMap = eg

@@ -27,7 +27,7 @@ EXPORTS = [
     "pa_mlp_predict", "pa_mlp_grad", "pa_mlp_train", "pa_qp_update_warm", "pa_grain_averages",
     "pa_batch_create", "pa_batch_destroy", "pa_batch_set_grains", "pa_batch_set_solver", "pa_batch_set_graphs",
     "pa_batch_newton_iteration", "pa_batch_commit_increment", "pa_batch_tune", "pa_batch_qp_pass_ms", "pa_batch_last_timings",
-    "pa_batch_get_state", "pa_batch_sizes",
+    "pa_batch_get_state", "pa_batch_sizes", "pa_set_periodic", "pa_periodic_lines",
 ]
 
 
@@ -57,6 +57,22 @@ class PaProblem(ctypes.Structure):
         ("tabular", ctypes.c_int), ("n_table", ctypes.c_int), ("table_time", ctypes.c_double * MAX_TABLE),
         ("table_u", ctypes.c_double * MAX_TABLE * 3 * 6),
         ("velgrad", ctypes.c_int), ("vel_grad", ctypes.c_double * 3 * 3),
+    ]
+
+
+MAX_PBC_LINES = 64
+
+
+class PaPeriodic(ctypes.Structure):
+    """pa_periodic: the periodic constraint file's lines, the prm's vertex rows and the tabular table."""
+    _fields_ = [
+        ("n_vertex_lines", ctypes.c_int), ("vertex_dofs", ctypes.c_int * 5 * MAX_PBC_LINES),
+        ("vertex_coef", ctypes.c_double * 5 * MAX_PBC_LINES),
+        ("n_edge_lines", ctypes.c_int), ("edge_dofs", ctypes.c_int * 4 * MAX_PBC_LINES), ("edge_coef", ctypes.c_double * 4 * MAX_PBC_LINES),
+        ("n_face_lines", ctypes.c_int), ("face_dofs", ctypes.c_int * 4 * MAX_PBC_LINES), ("face_coef", ctypes.c_double * 4 * MAX_PBC_LINES),
+        ("vertex_known", ctypes.c_int * 24), ("vertex_total", ctypes.c_double * 24),
+        ("tabular", ctypes.c_int), ("tabular_time", ctypes.c_double), ("n_table", ctypes.c_int),
+        ("table_time", ctypes.c_double * MAX_TABLE), ("table_coef", ctypes.c_int * MAX_TABLE),
     ]
 
 

@@ -510,6 +510,65 @@ def oracle_simulate(case, n_increments, pinv_mode=1, nthreads=1, want_state=Fals
     return out
 
 
+def qp_F(u, nx, ny, nz, h):
+    """F = I + sum_v u_v (x) grad N_v at every qp (Q = 8 e + q, element e with x fastest, Gauss
+    point q with bit d selecting 1/2 +- 1/(2 sqrt 3) along axis d), from nodal u (x fastest)."""
+    g = (0.5 - 0.5 / np.sqrt(3.0), 0.5 + 0.5 / np.sqrt(3.0))
+    nnx, nny = nx + 1, ny + 1
+    F = np.zeros((nx * ny * nz * 8, 3, 3))
+    for e in range(nx * ny * nz):
+        ex, ey, ez = e % nx, (e // nx) % ny, e // (nx * ny)
+        for q in range(8):
+            Fq = np.eye(3)
+            for v in range(8):
+                nd = (ex + (v & 1)) + nnx * ((ey + ((v >> 1) & 1)) + nny * (ez + ((v >> 2) & 1)))
+                N, dN = [], []
+                for d in range(3):
+                    xi = g[(q >> d) & 1]
+                    b = (v >> d) & 1
+                    N.append(xi if b else 1 - xi)
+                    dN.append((1.0 if b else -1.0) / h[d])
+                grad = np.array([dN[0] * N[1] * N[2], N[0] * dN[1] * N[2], N[0] * N[1] * dN[2]])
+                Fq += np.outer(u[3 * nd:3 * nd + 3], grad)
+            F[8 * e + q] = Fq
+    return F
+
+
+def check_committed_state(case, u1, T1, s1, Fp1, Fp0, s0, tol=1e-10, max_frac=0.01):
+    """Every qp's committed state after an increment (T1, s1, Fp1, from the run under test) against
+    the oracle's calculatePlasticity(.,.,0) on that run's own inputs: F from its displacement u1, Fp0
+    and s0 committed before the increment.  A qp whose numerically singular active set leaves a
+    singular value at round-off may take the other side of the pseudo-inverse's keep/drop decision;
+    it passes only if the run's state is the oracle's at an input within 1e-12 of its own (400 seeded
+    perturbations of F), and at most max_frac of the qps may need that.
+    Returns (qps, plastic qps, qps on the other side of a round-off decision)."""
+    n_s = case.n_slip
+    F = qp_F(u1, case.nx, case.ny, case.nz, tuple(case.span[d] / (case.nx, case.ny, case.nz)[d] for d in range(3)))
+    nq, skipped, plastic = len(F), 0, 0
+    rel = lambda a, b: np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(b).max(), 1e-300)
+    for Q in range(nq):
+        rod = case.grain_rod[case.elem_grain[Q // 8]]
+        Fpq = np.asarray(Fp0[Q]).reshape(3, 3)
+        sq = np.append(s0[Q][:n_s], 10e14)
+        o = oracle_qp(case, rod, F[Q], Fpq, sq, 0)
+        plastic += o["n_active"] > 0
+        if rel(T1[Q], o["T"].ravel()) < tol:
+            assert rel(s1[Q][:n_s], o["s"][:n_s]) < tol, Q
+            assert rel(Fp1[Q], o["Fp"].ravel()) < tol, Q
+            continue
+        r = np.random.default_rng(1000 + Q)
+        near = False
+        for _ in range(400):
+            o2 = oracle_qp(case, rod, F[Q] * (1 + 1e-12 * r.standard_normal((3, 3))), Fpq, sq, 0)
+            if rel(T1[Q], o2["T"].ravel()) < 1e-6 and rel(s1[Q][:n_s], o2["s"][:n_s]) < 1e-6:
+                near = True
+                break
+        assert near, (Q, rel(T1[Q], o["T"].ravel()))
+        skipped += 1
+    assert skipped <= max(1, int(max_frac * nq))
+    return nq, plastic, skipped
+
+
 def oracle_qp(case, rot, F, Fp=None, s=None, flag=1, pinv_mode=1, slip=None, W=None):
     lib = oracle()
     mat = case.material(pinv_mode)

@@ -126,3 +126,46 @@ def test_prm_with_velocity_gradient_bc_maps_onto_the_abi():
     assert [[pb.vel_grad[a][b] for b in range(3)] for a in range(3)] == [[-0.005, 0, 0], [0, -0.005, 0], [0, 0, 0.01]]
     assert all(pb.face_dof_constrained[f][d] == 0 for f in range(6) for d in range(3))
     assert cp.totalIncrements == 1000 and (pb.nx, pb.ny, pb.nz) == (8, 8, 8)
+
+
+def test_periodic_struct_layout_matches_header(tmp_path):
+    """ctypes mirror of pa_periodic: size and field offsets as the C compiler gives them."""
+    import ctypes
+    import shutil
+    import subprocess
+    import pytest
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    fields = ["vertex_dofs", "vertex_coef", "n_edge_lines", "edge_coef", "n_face_lines", "face_coef", "vertex_known",
+              "vertex_total", "tabular", "tabular_time", "n_table", "table_time", "table_coef"]
+    src = tmp_path / "layout.c"
+    body = "".join(' printf("%%zu\\n", offsetof(pa_periodic, %s));' % f for f in fields)
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "plasticity_amd.h"\n'
+                   'int main(void) { printf("%zu\\n", sizeof(pa_periodic));' + body + ' return 0; }\n')
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    want = [ctypes.sizeof(nat.PaPeriodic)] + [getattr(nat.PaPeriodic, f).offset for f in fields]
+    assert got == want
+
+
+def test_prm_with_periodic_bcs_maps_onto_the_abi():
+    """fcc/periodicBCs prm_UniaxialCyclicTabular.prm: the constraint file's 12 vertex, 27 edge and 9
+    face lines, the vertex rows and the tabular periodic table reach pa_periodic (ellipticBVP
+    init.cc:66-131, userInputParameters.cc:126-136), as the oracle's reader of the same file has them."""
+    import refcase
+    from plasticity_amd import model
+    d = os.path.join(ROOT, "tests", "golden", "fcc_periodicBCs")
+    ui = model.userInputParameters(os.path.join(d, "prm_UniaxialCyclicTabular.prm"))
+    cp = model.crystalPlasticity(ui)
+    P = cp.periodic
+    assert (P.n_vertex_lines, P.n_edge_lines, P.n_face_lines) == (12, 27, 9)
+    assert list(P.vertex_dofs[6]) == [4, 22, 4, 7, 1] and list(P.vertex_coef[6]) == [4.0, 1.0, 4.0, -1.0, 1.0]
+    assert list(P.edge_dofs[0]) == [3, 9, 18, 9] and list(P.face_dofs[8]) == [17, 14, 11, 2]
+    assert list(P.vertex_known[:6]) == [1, 1, 1, 1, 1, 1] and P.vertex_total[3] == 0.01
+    assert (P.tabular, P.tabular_time, P.n_table) == (1, 1.0, 7)
+    assert list(P.table_coef[:7]) == [1, -1, 1, -1, 1, -1, 1]
+    assert all(cp.problem.face_dof_constrained[f][k] == 0 for f in range(6) for k in range(3))
+    ref = refcase.periodic_struct(refcase.load_reference_case("fcc_periodicBCs", prm_name="prm_UniaxialCyclicTabular.prm").periodic)
+    assert bytes(P) == bytes(ref)

@@ -72,30 +72,6 @@ def _run(b, n_inc):
     return np.array(rows), np.array(its)
 
 
-def _qp_F(u, nx, ny, nz, h):
-    """F = I + sum_v u_v (x) grad N_v at every qp (Q = 8 e + q, element e with x fastest, Gauss
-    point q with bit d selecting 1/2 +- 1/(2 sqrt 3) along axis d), from nodal u (x fastest)."""
-    g = (0.5 - 0.5 / np.sqrt(3.0), 0.5 + 0.5 / np.sqrt(3.0))
-    nnx, nny = nx + 1, ny + 1
-    F = np.zeros((nx * ny * nz * 8, 3, 3))
-    for e in range(nx * ny * nz):
-        ex, ey, ez = e % nx, (e // nx) % ny, e // (nx * ny)
-        for q in range(8):
-            Fq = np.eye(3)
-            for v in range(8):
-                nd = (ex + (v & 1)) + nnx * ((ey + ((v >> 1) & 1)) + nny * (ez + ((v >> 2) & 1)))
-                N, dN = [], []
-                for d in range(3):
-                    xi = g[(q >> d) & 1]
-                    b = (v >> d) & 1
-                    N.append(xi if b else 1 - xi)
-                    dN.append((1.0 if b else -1.0) / h[d])
-                grad = np.array([dN[0] * N[1] * N[2], N[0] * dN[1] * N[2], N[0] * N[1] * dN[2]])
-                Fq += np.outer(u[3 * nd:3 * nd + 3], grad)
-            F[8 * e + q] = Fq
-    return F
-
-
 @pytest.mark.parametrize("solver", [0, 1], ids=["bicg", "pcg"])
 def test_batch_rves_match_their_own_oracle_runs(solver):
     """Every RVE against the oracle on that RVE alone.
@@ -150,7 +126,7 @@ def test_batch_rves_match_their_own_oracle_runs(solver):
         # per qp, increment 1, on the batch's own inputs
         Fp0, s0 = before[i]
         T1, s1, Fp1, u1 = after[i]
-        F = _qp_F(u1, c.nx, c.ny, c.nz, (1.0 / c.nx, 1.0 / c.ny, 1.0 / c.nz))
+        F = refcase.qp_F(u1, c.nx, c.ny, c.nz, (1.0 / c.nx, 1.0 / c.ny, 1.0 / c.nz))
         nq, skipped, plastic = len(F), 0, 0
         for Q in range(nq):
             rod = c.grain_rod[c.elem_grain[Q // 8]]
