@@ -539,13 +539,16 @@ __device__ __attribute__((noinline)) bool qp_tangent_one(const QpArgs &a, long l
   double Fp_t[9], s[NS], x[NS], P[9], T[9], Fe[9], Fp[9], W[KIN ? NS : 1];
 #pragma unroll
   for (int i = 0; i < 9; ++i) Fp_t[i] = a.Fp_conv[i * a.nq + Q];
-  for (int i = 0; i < a.ns; ++i) s[i] = a.s_conv[i * a.nq + Q];
+  const double *mat;
+  int ns;
+  qp_phase(a.ph, a.mat, a.ns, a.egrain[e], mat, ns);
+  for (int i = 0; i < ns; ++i) s[i] = a.s_conv[i * a.nq + Q];
   if constexpr (KIN)
-    for (int i = 0; i < a.ns; ++i) W[i] = a.W_conv[i * a.nq + Q];
+    for (int i = 0; i < ns; ++i) W[i] = a.W_conv[i * a.nq + Q];
   const double *gt = a.gtab + table_row(a.tw, a.egrain, Q) * a.gstride;
-  const int nslip = a.tw.on ? a.tw.nslip : a.ns;
+  const int nslip = a.tw.on ? a.tw.nslip : ns;
   bool plastic = false;
-  const int na = cp::qp_update<NS, DT, KIN>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 1, P, T, Fe, Fp, (DT *)a.D + Q,
+  const int na = cp::qp_update<NS, DT, KIN>(mat, gt, ns, a.max_solver_iter, F, Fp_t, s, x, 1, P, T, Fe, Fp, (DT *)a.D + Q,
                                             a.nq, &plastic, a.JxW, SingRef<NS>{&sing}, W, nullptr, 1, nullptr, nslip, sing.ws);
   if (na < 0) return false;
 #pragma unroll
@@ -554,9 +557,9 @@ __device__ __attribute__((noinline)) bool qp_tangent_one(const QpArgs &a, long l
     a.Fp_iter[i * a.nq + Q] = Fp[i];
     a.P[i * a.nq + Q] = a.JxW * P[i];
   }
-  for (int i = 0; i < a.ns; ++i) a.s_iter[i * a.nq + Q] = s[i];
+  for (int i = 0; i < ns; ++i) a.s_iter[i * a.nq + Q] = s[i];
   if constexpr (KIN)
-    for (int i = 0; i < a.ns; ++i) a.W_iter[i * a.nq + Q] = W[i];
+    for (int i = 0; i < ns; ++i) a.W_iter[i * a.nq + Q] = W[i];
   // slipfraction_iter is written only on the plastic path (calculatePlasticity.cc:454-456)
   if (plastic)
     for (int i = 0; i < nslip; ++i) a.slip_iter[i * a.nq + Q] = a.slip_conv[i * a.nq + Q] + x[i];
@@ -823,20 +826,23 @@ __device__ __attribute__((noinline)) bool post_qp_one(const PostArgs &a, long lo
   double Fp_t[9], s[NS], x[NS], P[9], T[9], Fe[9], Fp[9], W[KIN ? NS : 1];
 #pragma unroll
   for (int i = 0; i < 9; ++i) Fp_t[i] = a.Fp_conv[i * a.nq + Q];
-  for (int i = 0; i < a.ns; ++i) s[i] = a.s_conv[i * a.nq + Q];
+  const double *mat;
+  int ns;
+  qp_phase(a.ph, a.mat, a.ns, a.egrain[Q >> 3], mat, ns);
+  for (int i = 0; i < ns; ++i) s[i] = a.s_conv[i * a.nq + Q];
   if constexpr (KIN)
-    for (int i = 0; i < a.ns; ++i) W[i] = a.W_conv[i * a.nq + Q];
+    for (int i = 0; i < ns; ++i) W[i] = a.W_conv[i * a.nq + Q];
   const double *gt = a.gtab + table_row(a.tw, a.egrain, Q) * a.gstride;
-  const int nslip = a.tw.on ? a.tw.nslip : a.ns;
+  const int nslip = a.tw.on ? a.tw.nslip : ns;
   bool plastic = false;
-  const int na = cp::qp_update<NS, double, KIN>(a.mat, gt, a.ns, a.max_solver_iter, F, Fp_t, s, x, 0, P, T, Fe, Fp, (double *)nullptr,
+  const int na = cp::qp_update<NS, double, KIN>(mat, gt, ns, a.max_solver_iter, F, Fp_t, s, x, 0, P, T, Fe, Fp, (double *)nullptr,
                                                 0, &plastic, 1.0, SingRef<NS>{&sing}, W, nullptr, 1, nullptr, nslip, sing.ws);
   if (na < 0) return false;
 #pragma unroll
   for (int i = 0; i < 9; ++i) { a.Fe_iter[i * a.nq + Q] = Fe[i]; a.Fp_iter[i * a.nq + Q] = Fp[i]; a.T[i * a.nq + Q] = T[i]; }
-  for (int i = 0; i < a.ns; ++i) a.s_iter[i * a.nq + Q] = s[i];
+  for (int i = 0; i < ns; ++i) a.s_iter[i * a.nq + Q] = s[i];
   if constexpr (KIN)
-    for (int i = 0; i < a.ns; ++i) a.W_iter[i * a.nq + Q] = W[i];
+    for (int i = 0; i < ns; ++i) a.W_iter[i * a.nq + Q] = W[i];
   if (plastic)
     for (int i = 0; i < nslip; ++i) a.slip_iter[i * a.nq + Q] = a.slip_conv[i * a.nq + Q] + x[i];
   if (a.tw.on) twin_epilogue(a.tw, Q, a.nq, plastic, x, a.s_iter);
@@ -890,7 +896,8 @@ __global__ void __launch_bounds__(256) k_post_reduce(PostArgs a) {
       acc[6 + c] = T[vi[c]] * a.JxW;
     }
     double slip_sum = 0;
-    const int nslip = a.tw.on ? a.tw.nslip : a.ns;
+    // several phases: phase 1's slip-system count at every qp (updateAfterIncrement.cc:234-236)
+    const int nslip = a.ph.gphase ? a.ph.ns[0] : (a.tw.on ? a.tw.nslip : a.ns);
     for (int i = 0; i < nslip; ++i) slip_sum += a.slip_iter[i * a.nq + Q] * a.JxW;
     acc[12] = slip_sum;
     if (a.tw.on) {  // TwinRealVF and TwinMade numerators (updateAfterIncrement.cc:176, 222-228)
@@ -1551,6 +1558,13 @@ struct pa_context {
   int bc_inc = -1;
   // periodic boundary conditions (pa_set_periodic): the closed constraint lines on the device and
   // the load scale of each increment's first Newton iteration
+  // several phases (pa_create_phases): material of each phase, its block stride on the device and
+  // the phase of each grain row
+  int nph = 1;
+  pa_material phmat[4];
+  int mstride = 0;
+  std::vector<int> h_gphase;
+  int *d_gphase = nullptr;
   bool pbc = false;
   PbcDev pbcd = {};
   std::vector<double> pbc_scale;
@@ -1667,6 +1681,16 @@ TwinArgs twin_args(const pa_context *c) {
   return t;
 }
 
+PhaseArgs phase_args(const pa_context *c) {
+  PhaseArgs p;
+  std::memset(&p, 0, sizeof(p));
+  if (c->nph < 2) return p;
+  p.gphase = c->d_gphase;
+  p.mstride = c->mstride;
+  for (int k = 0; k < c->nph; ++k) p.ns[k] = c->phmat[k].n_slip;
+  return p;
+}
+
 // main_only: pass 1 alone (every qp on the LU path; the singular ones are only queued)
 int launch_qp_tangent(pa_context *c, bool main_only = false) {
   QpArgs a;
@@ -1681,6 +1705,7 @@ int launch_qp_tangent(pa_context *c, bool main_only = false) {
   a.queue = c->queue; a.qcount = c->counters + 8; a.sing = c->sing;
   a.nsing = c->counters + 8; a.work = c->counters + 11;
   a.tw = twin_args(c);
+  a.ph = phase_args(c);
   a.ws = c->ws;
   const int NS = ns_template(c->ns);
   const bool f = mixed(c);
@@ -2315,6 +2340,7 @@ int pa_destroy(pa_context *c) {
   for (void *p : c->allocs) hipFree(p);
   if (c->d_gtab) hipFree(c->d_gtab);
   if (c->d_vrod) hipFree(c->d_vrod);
+  if (c->d_gphase) hipFree(c->d_gphase);
   for (int i = 0; i < 4; ++i)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
   for (int i = 0; i < 2; ++i)
@@ -2356,10 +2382,119 @@ int pa_element_grains(pa_context *c, const unsigned *vox, const int nv[3], unsig
   return rc;
 }
 
+namespace {
+// grain tables of a context with several phases: each grain row with its phase's systems and
+// elastic moduli (no twinning variants); s starts at each qp's phase's s0 (init.cc:1125-1175)
+int set_grains_phases(pa_context *c, const unsigned *elem_grain, const double *grain_rod, int n_grain_rows) {
+  const int NS = ns_template(c->ns);
+  std::vector<double> tab((size_t)n_grain_rows * c->gstride, 0.0);
+  for (int g = 0; g < n_grain_rows; ++g) {
+    const pa_material &mt = c->phmat[c->h_gphase[g]];
+    std::vector<double> mv(3 * mt.n_slip), nv(3 * mt.n_slip);
+    for (int a = 0; a < mt.n_slip; ++a) {  // normalised slip vectors (init.cc:73-123)
+      double mn = 0, nn = 0;
+      for (int j = 0; j < 3; ++j) { mn += mt.m[a][j] * mt.m[a][j]; nn += mt.n[a][j] * mt.n[a][j]; }
+      mn = std::sqrt(mn); nn = std::sqrt(nn);
+      for (int j = 0; j < 3; ++j) { mv[3 * a + j] = mt.m[a][j] / mn; nv[3 * a + j] = mt.n[a][j] / nn; }
+    }
+    grain_table(mt, (const double(*)[3])mv.data(), (const double(*)[3])nv.data(), grain_rod + 3 * g, mt.n_slip, NS,
+                &tab[(size_t)g * c->gstride]);
+  }
+  if (c->d_gtab) hipFree(c->d_gtab);
+  c->d_gtab = nullptr;
+  HIPCHK(hipMalloc(&c->d_gtab, tab.size() * sizeof(double)));
+  HIPCHK(hipMemcpy(c->d_gtab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+  if (c->d_gphase) hipFree(c->d_gphase);
+  c->d_gphase = nullptr;
+  HIPCHK(hipMalloc(&c->d_gphase, (size_t)n_grain_rows * sizeof(int)));
+  HIPCHK(hipMemcpy(c->d_gphase, c->h_gphase.data(), (size_t)n_grain_rows * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_egrain, elem_grain, c->m.nelem * sizeof(unsigned), hipMemcpyHostToDevice));
+  const long long nq = c->nq;
+  std::vector<double> sb((size_t)c->ns * nq, 0.0), rn(3 * nq);
+  for (long long e = 0; e < c->m.nelem; ++e) {
+    const pa_material &mt = c->phmat[c->h_gphase[elem_grain[e]]];
+    for (int q = 0; q < 8; ++q) {
+      const long long Q = 8 * e + q;
+      for (int a = 0; a < mt.n_slip; ++a) sb[(size_t)a * nq + Q] = mt.s0[a];
+      for (int i = 0; i < 3; ++i) rn[i * nq + Q] = grain_rod[3 * elem_grain[e] + i];
+    }
+  }
+  HIPCHK(hipMemcpy(c->s_conv, sb.data(), sb.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->s_iter, sb.data(), sb.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->rotnew, rn.data(), rn.size() * sizeof(double), hipMemcpyHostToDevice));
+  c->n_grain_rows = n_grain_rows;
+  c->grains_set = true;
+  return 0;
+}
+}  // namespace
+
+int pa_create_phases(const pa_material *mats, int n_phases, const pa_problem *pb, pa_context **out) {
+  if (!mats || !pb || !out) return fail(PA_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (n_phases < 1 || n_phases > 4) return fail(PA_ERR_ARG, "1 to 4 phases");
+  // the envelope: the largest system count, kinematic hardening if any phase has it; the
+  // tolerance and local-loop settings are global prm keys, so the phases must agree on them
+  pa_material env = mats[0];
+  for (int k = 0; k < n_phases; ++k) {
+    const pa_material &m = mats[k];
+    if (m.n_slip < 1 || m.n_slip > PA_MAX_SLIP) return fail(PA_ERR_ARG, "n_slip out of range");
+    if (m.twinning) return fail(PA_ERR_UNSUPPORTED, "several phases with a twinning phase are not built");
+    if (m.max_slip_search != 1) return fail(PA_ERR_UNSUPPORTED, "Max Slip Search Iterations must be 1");
+    if (m.max_solver_iter != mats[0].max_solver_iter || m.stress_tol != mats[0].stress_tol)
+      return fail(PA_ERR_ARG, "the phases must share Max Solver Iterations and Stress Tolerance");
+    env.n_slip = std::max(env.n_slip, m.n_slip);
+    env.kinematic_hardening |= m.kinematic_hardening;
+  }
+  int rc = pa_create(&env, pb, out);
+  if (rc) return rc;
+  pa_context *c = *out;
+  if (n_phases == 1) { c->mat = mats[0]; return 0; }
+  const int NS = ns_template(c->ns);
+  const int tol = 3 * NS + NS * NS, c1 = tol + 1, c2 = c1 + NS;
+  c->mstride = c2 + NS;  // cp::MatLayout<NS>::size
+  std::vector<double> mb((size_t)n_phases * c->mstride, 0.0);
+  for (int k = 0; k < n_phases; ++k) {
+    const pa_material &m = mats[k];
+    double *b = &mb[(size_t)k * c->mstride];
+    for (int a = 0; a < m.n_slip; ++a) {
+      b[a] = m.h0[a]; b[NS + a] = m.ss[a]; b[2 * NS + a] = m.pw[a];
+      for (int q = 0; q < m.n_slip; ++q) b[3 * NS + a * NS + q] = m.q[a][q];
+      if (m.kinematic_hardening) { b[c1 + a] = m.C1[a]; b[c2 + a] = m.C2[a]; }
+    }
+    b[tol] = m.stress_tol;
+    c->phmat[k] = m;
+  }
+  double *d = nullptr;
+  if ((rc = dalloc(c, &d, mb.size())) || hipMemcpy(d, mb.data(), mb.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    pa_destroy(c);
+    *out = nullptr;
+    return rc ? rc : fail(PA_ERR_HIP, "copy materials");
+  }
+  c->d_mat = d;
+  c->nph = n_phases;
+  return 0;
+}
+
+int pa_set_grain_phases(pa_context *c, const int *grain_phase, int n_grain_rows) {
+  if (!c || !grain_phase || n_grain_rows < 1) return fail(PA_ERR_ARG, "null argument");
+  if (c->nph < 2) return fail(PA_ERR_STATE, "the context has one phase (pa_create_phases makes several)");
+  std::vector<int> gp(n_grain_rows);
+  for (int g = 0; g < n_grain_rows; ++g) {
+    if (grain_phase[g] < 1 || grain_phase[g] > c->nph) return fail(PA_ERR_ARG, "grain phase out of range");
+    gp[g] = grain_phase[g] - 1;
+  }
+  c->h_gphase = gp;
+  c->grains_set = false;  // the grain tables depend on the phases
+  return 0;
+}
+
 int pa_set_grains(pa_context *c, const unsigned *elem_grain, const double *grain_rod, int n_grain_rows) {
   if (!c || !elem_grain || !grain_rod || n_grain_rows < 1) return fail(PA_ERR_ARG, "null argument");
   for (long long e = 0; e < c->m.nelem; ++e)
     if ((int)elem_grain[e] >= n_grain_rows) return fail(PA_ERR_ARG, "grain id without an orientation row");
+  if (c->nph > 1 && (int)c->h_gphase.size() < n_grain_rows)
+    return fail(PA_ERR_STATE, "several phases: pa_set_grain_phases must cover every grain row first");
+  if (c->nph > 1) return set_grains_phases(c, elem_grain, grain_rod, n_grain_rows);
   const int NS = ns_template(c->ns);
   // normalised slip vectors (init.cc:73-123)
   std::vector<double> mv(3 * c->ns), nv(3 * c->ns);
@@ -2545,6 +2680,7 @@ int pa_commit_increment(pa_context *c, double row[15]) {
   a.queue = c->queue; a.qcount = c->counters + 9; a.sing = c->sing;
   a.nsing = c->counters + 9; a.work = c->counters + 12;
   a.tw = twin_args(c);
+  a.ph = phase_args(c);
   a.ws = c->ws;
   HIPCHK(hipEventRecord(c->ev[0], c->stream));
   const int NS = ns_template(c->ns);
@@ -2753,6 +2889,7 @@ int pa_get_csr(pa_context *c, long long *row_ptr, int *col_idx, double *values) 
 int pa_qp_update(pa_context *c, long long n, const double *F, const double *Fp_conv, const double *s_conv, const unsigned *grain,
                  int flag, double *P, double *T, double *dPdF, double *Fe, double *Fp, double *s_out, double *slip_inc, int *n_active,
                  const double *W_conv, double *W_out) {
+  if (c && c->nph > 1) return fail(PA_ERR_UNSUPPORTED, "the batch qp entry points take one phase");
   if (!c || n < 0 || !F || !Fp_conv || !s_conv || !grain || !P || !T || !Fe || !Fp || !s_out || !slip_inc || !n_active)
     return fail(PA_ERR_ARG, "null argument");
   if (flag && !dPdF) return fail(PA_ERR_ARG, "dPdF required for flag 1");
@@ -2994,6 +3131,7 @@ int pa_mlp_train(pa_mlp *m, long long n, const float *X, const float *Y, int epo
 int pa_qp_update_warm(pa_context *c, pa_mlp *net, long long n, const double *F, const double *Fp_conv, const double *Fe_conv,
                       const double *s_conv, const unsigned *grain, int flag, double *P, double *T, double *dPdF, double *Fe,
                       double *Fp, double *s_out, double *slip_inc, int *n_active, int *iters, float *feat, float *x0) {
+  if (c && c->nph > 1) return fail(PA_ERR_UNSUPPORTED, "the batch qp entry points take one phase");
   if (!c || n < 0 || !F || !Fp_conv || !Fe_conv || !s_conv || !grain || !P || !T || !Fe || !Fp || !s_out || !slip_inc || !n_active || !iters)
     return fail(PA_ERR_ARG, "null argument");
   if (flag && !dPdF) return fail(PA_ERR_ARG, "dPdF required for flag 1");

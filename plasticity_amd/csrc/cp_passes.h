@@ -127,6 +127,25 @@ __device__ __forceinline__ void twin_epilogue(const TwinArgs &t, long long Q, lo
     for (int k = 0; k < t.ntw; ++k) s_iter[(t.nslip + k) * nq + Q] = 100000.0;
   }
 }
+// Several phases (multiphaseInit.cc:68-361): each grain row's phase selects its material block
+// (hardening, latent ratios, kinematic hardening, tolerance) and its system count; the grain
+// tables already hold the phase's rotated Schmid tensors and elastic moduli.
+struct PhaseArgs {
+  const int *gphase;  // phase (0-based) of each grain row; null: one phase
+  int mstride;        // doubles per material block
+  int ns[4];          // systems of each phase
+};
+__device__ __forceinline__ void qp_phase(const PhaseArgs &ph, const double *mat0, int ns0, unsigned grain, const double *&mat, int &ns) {
+  if (ph.gphase) {
+    const int p = ph.gphase[grain];
+    mat = mat0 + (long long)p * ph.mstride;
+    ns = ph.ns[p];
+  } else {
+    mat = mat0;
+    ns = ns0;
+  }
+}
+
 // orientation-table row of qp Q in element e
 __device__ __forceinline__ long long table_row(const TwinArgs &t, const unsigned *egrain, long long Q) {
   return t.on ? (long long)egrain[Q >> 3] * (1 + t.ntw) + t.var_conv[Q] : (long long)egrain[Q >> 3];
@@ -156,6 +175,7 @@ struct QpArgs {
   unsigned *nsing;  // pseudo-inverse solves (numerically singular A_PA) of the pass
   TwinArgs tw;      // twinning (one-lane form)
   double *ws;       // one-lane form, 48 systems: per-lane matrix workspace (cp::LaneWs), else null
+  PhaseArgs ph;     // several phases (one-lane form)
 };
 
 template <int NS, typename DT, bool KIN, int G>
@@ -220,6 +240,7 @@ struct PostArgs {
   unsigned *nsing;  // pseudo-inverse solves of the pass
   TwinArgs tw;      // twinning (one-lane form)
   double *ws;       // one-lane form, 48 systems: per-lane matrix workspace (cp::LaneWs), else null
+  PhaseArgs ph;     // several phases (one-lane form)
 };
 
 __device__ __forceinline__ void post_F(const PostArgs &a, long long Q, double F[9]) { qp_F(a.m, a.h, a.u, Q, F); }

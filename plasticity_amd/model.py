@@ -263,10 +263,19 @@ class crystalOrientationsIO:
 class Context:
     """One device context (C-ABI handle)."""
 
-    def __init__(self, mat: PaMaterial, pb: PaProblem):
+    def __init__(self, mat, pb: PaProblem):
+        """mat: one PaMaterial, or a list of 2..4 (one per phase: pa_create_phases; then
+        set_grain_phases before set_grains)."""
         self._h = ctypes.c_void_p()
         L = native.lib()
-        check(L.pa_create(ctypes.byref(mat), ctypes.byref(pb), ctypes.byref(self._h)))
+        if isinstance(mat, (list, tuple)):
+            arr = (PaMaterial * len(mat))(*mat)
+            check(L.pa_create_phases(arr, ctypes.c_int(len(mat)), ctypes.byref(pb), ctypes.byref(self._h)))
+            self.phases = list(mat)
+            mat = max(mat, key=lambda m: m.n_slip)
+        else:
+            check(L.pa_create(ctypes.byref(mat), ctypes.byref(pb), ctypes.byref(self._h)))
+            self.phases = [mat]
         self.mat, self.pb = mat, pb
         ne, nq, nd, nz = (ctypes.c_longlong() for _ in range(4))
         check(L.pa_sizes(self._h, ctypes.byref(ne), ctypes.byref(nq), ctypes.byref(nd), ctypes.byref(nz)))
@@ -292,6 +301,11 @@ class Context:
         out = np.zeros(self.nelem, dtype=np.uint32)
         check(native.lib().pa_element_grains(self._h, ptr(vox), ptr(nv), ptr(out)))
         return out
+
+    def set_grain_phases(self, grain_phase):
+        """Phase (1-based) of each grain row, before set_grains (pa_set_grain_phases)."""
+        gp = np.ascontiguousarray(grain_phase, dtype=np.int32)
+        check(native.lib().pa_set_grain_phases(self._h, ptr(gp), ctypes.c_int(len(gp))))
 
     def set_grains(self, elem_grain, grain_rod):
         eg = np.ascontiguousarray(elem_grain, dtype=np.uint32)

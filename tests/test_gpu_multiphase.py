@@ -1,0 +1,151 @@
+"""Several phases on the GPU (pa_create_phases / pa_set_grain_phases; multiphaseInit.cc:68-361):
+each grain row's phase selects its material (slip systems, elastic moduli, hardening, latent
+ratios, kinematic hardening) against the oracle's several-phase run (or_simulate_phases), which
+tests/test_oracle_multiphase.py pins (one phase = the single-phase run bit for bit; the phase
+selects the material).
+
+* rows against the oracle at 1e-9 (Krylov at 1e-14) from the first increment up to the first one
+  where the two runs part, at least two increments past first yield (after yield round-off
+  active-set decisions make both depend on rounding);
+* at every increment every qp's committed state (T, s, Fp, backstress) equals the oracle's qp
+  update of that qp's phase on the GPU's own inputs at 1e-10, up to qps across a round-off decision;
+* a two-phase context whose grains all take phase 1 equals the one-phase context bit for bit;
+* a twinning phase, grains set before their phases and phase ids out of range are refused.
+"""
+import numpy as np
+import pytest
+
+import plasticity_amd as pa
+from plasticity_amd import native, synth
+import refcase
+
+pytestmark = pytest.mark.gpu
+
+
+def _phase_cases(n=4, strain=1e-3, n_inc=8):
+    bcs = ((1, 1, 0.0), (1, 2, 0.0), (1, 3, 0.0), (2, 1, strain * n_inc))
+    fcc = refcase.fcc_case(nx=n, ny=n, nz=n, delT=1.0, total_time=float(n_inc), lin_tol=1e-14, bcs=bcs)
+    t = synth.hcp24_tables()
+    hcp = refcase.Case(t["m"], t["n"], t["q"], t["C"], t["s0"], t["h0"], t["ss"], t["pw"], tol=t["tol"], nx=n, ny=n, nz=n,
+                       delT=1.0, total_time=float(n_inc), bcs=bcs, lin_tol=1e-14)
+    kin = refcase.fcc_case(nx=n, ny=n, nz=n, delT=1.0, total_time=float(n_inc), lin_tol=1e-14, bcs=bcs)
+    kin.C1, kin.C2 = np.full(12, 10.0), np.full(12, 1.0)
+    kin.h0 = kin.h0 * 0.8
+    case = refcase.fcc_case(nx=n, ny=n, nz=n, delT=1.0, total_time=float(n_inc), lin_tol=1e-14, bcs=bcs)
+    rng = np.random.default_rng(11)
+    ng = 6
+    case.grain_rod = rng.uniform(-0.5, 0.5, size=(ng, 3))
+    case.elem_grain = rng.integers(0, ng, size=n ** 3).astype(np.uint32)
+    case.phases = [fcc, hcp, kin]
+    case.grain_phase = np.array([1, 2, 3, 1, 2, 3], np.int32)
+    return case
+
+
+def _materials(case):
+    mats = []
+    for c in case.phases:
+        mats.append(pa.make_material(c.m, c.n, c.q, c.C, c.s0, c.h0, c.ss, c.pw, c.tol, C1=c.C1, C2=c.C2))
+    return mats
+
+
+def _problem(case):
+    return pa.make_problem((case.nx, case.ny, case.nz), case.span, case.total_increments, case.delT, case.bcs,
+                           lin_tol=case.lin_tol)
+
+
+def _rel(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _check_qps(case, ctx, u, Fp0, s0, W0):
+    """Per qp: the committed state against the oracle's update of the qp's phase on the GPU's inputs."""
+    F = refcase.qp_F(u, case.nx, case.ny, case.nz, tuple(case.span[d] / (case.nx, case.ny, case.nz)[d] for d in range(3)))
+    T1, s1, Fp1, W1 = ctx.get_state("T"), ctx.get_state("s"), ctx.get_state("Fp"), ctx.get_state("W")
+    rounded = 0
+    for Q in range(len(F)):
+        g = case.elem_grain[Q // 8]
+        pc = case.phases[case.grain_phase[g] - 1]
+        ns = pc.n_slip
+        rod = case.grain_rod[g]
+        W = W0[Q][:ns] if pc.C1 is not None else None
+        o = refcase.oracle_qp(pc, rod, F[Q], Fp0[Q].reshape(3, 3), np.append(s0[Q][:ns], 10e14), 0, W=W)
+        if _rel(T1[Q], o["T"].ravel()) < 1e-10:
+            assert _rel(s1[Q][:ns], o["s"][:ns]) < 1e-10 and _rel(Fp1[Q], o["Fp"].ravel()) < 1e-10, Q
+            if pc.C1 is not None:
+                assert np.abs(W1[Q][:ns] - o["W"][:ns]).max() <= 1e-10 * max(np.abs(o["W"][:ns]).max(), 1e-3), Q
+            assert not np.any(s1[Q][ns:]), Q  # the systems the phase does not have stay untouched
+            continue
+        r = np.random.default_rng(1000 + Q)
+        near = any(_rel(T1[Q], refcase.oracle_qp(pc, rod, F[Q] * (1 + 1e-12 * r.standard_normal((3, 3))), Fp0[Q].reshape(3, 3),
+                                                  np.append(s0[Q][:ns], 10e14), 0, W=W)["T"].ravel()) < 1e-6 for _ in range(400))
+        assert near, Q
+        rounded += 1
+    assert rounded <= max(1, len(F) // 100)
+    return rounded
+
+
+def test_phases_match_the_oracle():
+    n_inc = 8
+    case = _phase_cases(n_inc=n_inc)
+    ref = refcase.oracle_simulate(case, n_inc, want_state=True)
+    ctx = pa.Context(_materials(case), _problem(case))
+    ctx.set_grain_phases(case.grain_phase)
+    ctx.set_grains(case.elem_grain, case.grain_rod)
+    rows, rounded = [], 0
+    for inc in range(n_inc):
+        Fp0, s0, W0 = ctx.get_state("Fp"), ctx.get_state("s"), ctx.get_state("W")
+        rows.append(ctx.run_increment(inc)[0])
+        rounded += _check_qps(case, ctx, ctx.get_state("u"), Fp0, s0, W0)
+    rows = np.array(rows)
+    e = np.abs(rows[:, :12] - ref["rows"][:, :12]).max(axis=1) / np.abs(ref["rows"][:, :12]).max(axis=1)
+    parted = np.nonzero(e > 1e-9)[0]
+    n_same = int(parted[0]) if len(parted) else n_inc
+    first_plastic = int(np.nonzero(ref["rows"][:, 14] > 0)[0][0])
+    print("equal to the oracle (%.1e) through increment %d (first yield at %d); %d qp updates across a round-off decision"
+          % (e[:n_same].max(), n_same - 1, first_plastic, rounded), flush=True)
+    assert rows[-1, 14] > 0
+    assert n_same >= min(n_inc, first_plastic + 2)
+    if n_same == n_inc:
+        st = ref["state"]
+        assert _rel(ctx.get_state("u"), ref["u"]) < 1e-9 and _rel(ctx.get_state("T"), st[:, 0:9]) < 1e-9
+    ctx.close()
+
+
+def test_two_phase_context_with_one_phase_in_use_equals_the_one_phase_context():
+    """Two FCC phases (the second with other hardening; same system count and no kinematic
+    hardening, so both contexts run the same kernel instantiation), every grain in phase 1."""
+    case = _phase_cases(n_inc=4)
+    mats = _materials(case)
+    f = case.phases[0]
+    other = pa.make_material(f.m, f.n, f.q, f.C, f.s0, f.h0 * 1.2, f.ss, f.pw, f.tol)
+    pb = _problem(case)
+    out = []
+    for multi in (False, True):
+        ctx = pa.Context([mats[0], other] if multi else mats[0], pb)
+        if multi:
+            ctx.set_grain_phases(np.ones(len(case.grain_rod), np.int32))
+        ctx.set_grains(case.elem_grain, case.grain_rod)
+        out.append((np.array([ctx.run_increment(i)[0] for i in range(4)]), ctx.get_state("T"), ctx.get_state("s")[:, :12]))
+        ctx.close()
+    (r1, T1, s1), (r2, T2, s2) = out
+    assert r1[-1, 14] > 0
+    assert np.array_equal(r1, r2) and np.array_equal(T1, T2) and np.array_equal(s1, s2)
+
+
+def test_bad_phase_inputs_are_refused():
+    case = _phase_cases(n_inc=2)
+    mats = _materials(case)
+    pb = _problem(case)
+    t = synth.hcp24_tables()
+    twin = dict(m=t["m"][:2], n=t["n"][:2], s0=[50.0] * 2, h0=[10.0] * 2, ss=[220.0] * 2, pw=[0.75] * 2, shear=0.126, threshold=0.7,
+                saturation=0.0)
+    tw = pa.make_material(case.phases[0].m, case.phases[0].n, np.ones((14, 14)), case.phases[0].C, case.phases[0].s0,
+                          case.phases[0].h0, case.phases[0].ss, case.phases[0].pw, 1e-9, twin=twin)
+    with pytest.raises(native.PlasticityError, match="twinning"):
+        pa.Context([mats[0], tw], pb)
+    ctx = pa.Context(mats, pb)
+    with pytest.raises(native.PlasticityError, match="pa_set_grain_phases"):
+        ctx.set_grains(case.elem_grain, case.grain_rod)
+    with pytest.raises(native.PlasticityError, match="out of range"):
+        ctx.set_grain_phases(np.array([1, 2, 4, 1, 2, 3], np.int32))
+    ctx.close()
