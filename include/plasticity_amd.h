@@ -131,14 +131,16 @@ int pa_create(const pa_material *mat, const pa_problem *pb, pa_context **out);
 int pa_destroy(pa_context *ctx);
 
 /* Several phases (Enable Multiphase; crystalPlasticity::init's multiphase tables, init.cc:200-960,
- * and multiphaseInit.cc:68-361): mats[k] is phase k + 1's material (1..4 phases; no twinning
- * phase, PA_ERR_UNSUPPORTED; the phases share Max Solver Iterations and Stress Tolerance, which the
- * reference reads once).  Per-qp state is sized for the largest phase.  Then
- * pa_set_grain_phases (each grain row's phase, 1-based: the orientation file's fifth column,
- * crystalOrientationsIO.cc:42-45) before pa_set_grains, which builds each grain's tables from its
- * phase and starts s at its phase's s0.  SlipTotal sums phase 1's slip-system count at every qp
- * (updateAfterIncrement.cc:234-236).  The batch qp entry points (pa_qp_update, pa_qp_update_warm)
- * take one phase. */
+ * and multiphaseInit.cc:68-361): mats[k] is phase k + 1's material (1..4 phases, each with its own
+ * systems, moduli, hardening, kinematic hardening and twinning; the phases share Max Solver
+ * Iterations and Stress Tolerance, which the reference reads once).  Per-qp state is sized for the
+ * largest phase.  Then pa_set_grain_phases (each grain row's phase, 1-based: the orientation
+ * file's fifth column, crystalOrientationsIO.cc:42-45) before pa_set_grains, which builds each
+ * grain's tables (and twin variants) from its phase and starts s at its phase's s0.  As the
+ * reference does with several phases: a twinning phase reorients above its Twin Threshold
+ * Fraction alone (calculatePlasticity.cc:969-985), and SlipTotal / TwinRealVF sum phase 1's slip
+ * and twin counts (1 twin system without its twinning) at every qp (updateAfterIncrement.cc:
+ * 222-236).  The batch qp entry points (pa_qp_update, pa_qp_update_warm) take one phase. */
 int pa_create_phases(const pa_material *mats, int n_phases, const pa_problem *pb, pa_context **out);
 int pa_set_grain_phases(pa_context *ctx, const int *grain_phase, int n_grain_rows);
 

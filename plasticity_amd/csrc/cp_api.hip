@@ -546,7 +546,9 @@ __device__ __attribute__((noinline)) bool qp_tangent_one(const QpArgs &a, long l
   if constexpr (KIN)
     for (int i = 0; i < ns; ++i) W[i] = a.W_conv[i * a.nq + Q];
   const double *gt = a.gtab + table_row(a.tw, a.egrain, Q) * a.gstride;
-  const int nslip = a.tw.on ? a.tw.nslip : ns;
+  TwinArgs tw = a.tw;
+  if (a.ph.gphase && tw.on) phase_twin(a.ph, a.egrain[e], tw);
+  const int nslip = tw.on ? tw.nslip : ns;
   bool plastic = false;
   const int na = cp::qp_update<NS, DT, KIN>(mat, gt, ns, a.max_solver_iter, F, Fp_t, s, x, 1, P, T, Fe, Fp, (DT *)a.D + Q,
                                             a.nq, &plastic, a.JxW, SingRef<NS>{&sing}, W, nullptr, 1, nullptr, nslip, sing.ws);
@@ -563,7 +565,7 @@ __device__ __attribute__((noinline)) bool qp_tangent_one(const QpArgs &a, long l
   // slipfraction_iter is written only on the plastic path (calculatePlasticity.cc:454-456)
   if (plastic)
     for (int i = 0; i < nslip; ++i) a.slip_iter[i * a.nq + Q] = a.slip_conv[i * a.nq + Q] + x[i];
-  if (a.tw.on) twin_epilogue(a.tw, Q, a.nq, plastic, x, a.s_iter);
+  if (tw.on && tw.ntw > 0) twin_epilogue(tw, Q, a.nq, plastic, x, a.s_iter);
   return true;
 }
 
@@ -833,7 +835,9 @@ __device__ __attribute__((noinline)) bool post_qp_one(const PostArgs &a, long lo
   if constexpr (KIN)
     for (int i = 0; i < ns; ++i) W[i] = a.W_conv[i * a.nq + Q];
   const double *gt = a.gtab + table_row(a.tw, a.egrain, Q) * a.gstride;
-  const int nslip = a.tw.on ? a.tw.nslip : ns;
+  TwinArgs tw = a.tw;
+  if (a.ph.gphase && tw.on) phase_twin(a.ph, a.egrain[Q >> 3], tw);
+  const int nslip = tw.on ? tw.nslip : ns;
   bool plastic = false;
   const int na = cp::qp_update<NS, double, KIN>(mat, gt, ns, a.max_solver_iter, F, Fp_t, s, x, 0, P, T, Fe, Fp, (double *)nullptr,
                                                 0, &plastic, 1.0, SingRef<NS>{&sing}, W, nullptr, 1, nullptr, nslip, sing.ws);
@@ -845,7 +849,7 @@ __device__ __attribute__((noinline)) bool post_qp_one(const PostArgs &a, long lo
     for (int i = 0; i < ns; ++i) a.W_iter[i * a.nq + Q] = W[i];
   if (plastic)
     for (int i = 0; i < nslip; ++i) a.slip_iter[i * a.nq + Q] = a.slip_conv[i * a.nq + Q] + x[i];
-  if (a.tw.on) twin_epilogue(a.tw, Q, a.nq, plastic, x, a.s_iter);
+  if (tw.on && tw.ntw > 0) twin_epilogue(tw, Q, a.nq, plastic, x, a.s_iter);
   return true;
 }
 // lane = qp, grid-stride over one-wave blocks (as the first assembly pass)
@@ -897,12 +901,14 @@ __global__ void __launch_bounds__(256) k_post_reduce(PostArgs a) {
     }
     double slip_sum = 0;
     // several phases: phase 1's slip-system count at every qp (updateAfterIncrement.cc:234-236)
-    const int nslip = a.ph.gphase ? a.ph.ns[0] : (a.tw.on ? a.tw.nslip : a.ns);
+    const int nslip = a.ph.gphase ? a.ph.nslip[0] : (a.tw.on ? a.tw.nslip : a.ns);
     for (int i = 0; i < nslip; ++i) slip_sum += a.slip_iter[i * a.nq + Q] * a.JxW;
     acc[12] = slip_sum;
     if (a.tw.on) {  // TwinRealVF and TwinMade numerators (updateAfterIncrement.cc:176, 222-228)
       double fr = 0;
-      for (int k = 0; k < a.tw.ntw; ++k) fr += a.tw.tw_iter[k * a.nq + Q] * a.JxW;
+      // several phases: phase 1's twin-system count at every qp (1 without its twinning)
+      const int ntw = a.ph.gphase ? min(max(a.ph.ntw[0], 1), a.tw.ntw) : a.tw.ntw;
+      for (int k = 0; k < ntw; ++k) fr += a.tw.tw_iter[k * a.nq + Q] * a.JxW;
       acc[13] = fr;
       acc[14] = (a.tw.var_iter[Q] != 0 ? 1.0 : 0.0) * a.JxW;
     }
@@ -919,7 +925,7 @@ __global__ void __launch_bounds__(256) k_post_reduce(PostArgs a) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) r[i] = a.rotnew[i * a.nq + Q];
     if (a.tw.on && a.tw.var_iter[Q] != a.tw.var_conv[Q]) {  // reoriented by twinning in this increment (:1009-1010, :287)
-      const long long row = (long long)a.egrain[Q >> 3] * (1 + a.tw.ntw) + a.tw.var_iter[Q];
+      const long long row = (long long)a.egrain[Q >> 3] * a.tw.nvar + a.tw.var_iter[Q];
 #pragma unroll
       for (int i = 0; i < 3; ++i) r[i] = a.tw.vrod[3 * row + i];
     }
@@ -1678,6 +1684,7 @@ TwinArgs twin_args(const pa_context *c) {
   t.tw_conv = c->tw_conv; t.tw_iter = c->tw_iter;
   t.var_conv = c->var_conv; t.var_iter = c->var_iter;
   t.vrod = c->d_vrod;
+  t.nvar = 1 + c->ntw;
   return t;
 }
 
@@ -1687,7 +1694,14 @@ PhaseArgs phase_args(const pa_context *c) {
   if (c->nph < 2) return p;
   p.gphase = c->d_gphase;
   p.mstride = c->mstride;
-  for (int k = 0; k < c->nph; ++k) p.ns[k] = c->phmat[k].n_slip;
+  for (int k = 0; k < c->nph; ++k) {
+    const pa_material &m = c->phmat[k];
+    p.nslip[k] = m.n_slip;
+    p.ntw[k] = m.twinning ? m.n_twin : 0;
+    p.ns[k] = p.nslip[k] + p.ntw[k];
+    p.shear[k] = m.twinning ? m.twin_shear : 1.0;
+    p.F_T[k] = m.twinning ? std::min(1.0, m.twin_threshold) : 2.0;
+  }
   return p;
 }
 
@@ -2387,18 +2401,32 @@ namespace {
 // elastic moduli (no twinning variants); s starts at each qp's phase's s0 (init.cc:1125-1175)
 int set_grains_phases(pa_context *c, const unsigned *elem_grain, const double *grain_rod, int n_grain_rows) {
   const int NS = ns_template(c->ns);
-  std::vector<double> tab((size_t)n_grain_rows * c->gstride, 0.0);
+  const int nvar = 1 + c->ntw;  // orientation rows per grain: the grain's, then its phase's twin variants
+  std::vector<double> tab((size_t)n_grain_rows * nvar * c->gstride, 0.0);
+  c->h_vrod.assign((size_t)n_grain_rows * nvar * 3, 0.0);
   for (int g = 0; g < n_grain_rows; ++g) {
     const pa_material &mt = c->phmat[c->h_gphase[g]];
-    std::vector<double> mv(3 * mt.n_slip), nv(3 * mt.n_slip);
-    for (int a = 0; a < mt.n_slip; ++a) {  // normalised slip vectors (init.cc:73-123)
+    const int ntw = mt.twinning ? mt.n_twin : 0, nsys = mt.n_slip + ntw;
+    std::vector<double> mv(3 * nsys), nv(3 * nsys);
+    for (int a = 0; a < nsys; ++a) {  // normalised slip and twin vectors (init.cc:73-123)
       double mn = 0, nn = 0;
       for (int j = 0; j < 3; ++j) { mn += mt.m[a][j] * mt.m[a][j]; nn += mt.n[a][j] * mt.n[a][j]; }
       mn = std::sqrt(mn); nn = std::sqrt(nn);
       for (int j = 0; j < 3; ++j) { mv[3 * a + j] = mt.m[a][j] / mn; nv[3 * a + j] = mt.n[a][j] / nn; }
     }
-    grain_table(mt, (const double(*)[3])mv.data(), (const double(*)[3])nv.data(), grain_rod + 3 * g, mt.n_slip, NS,
-                &tab[(size_t)g * c->gstride]);
+    for (int v = 0; v < nvar; ++v) {
+      double *r = &c->h_vrod[((size_t)g * nvar + v) * 3];
+      if (v == 0 || v > ntw) for (int i = 0; i < 3; ++i) r[i] = grain_rod[3 * g + i];
+      else twin_variant_rod(grain_rod + 3 * g, &nv[3 * (mt.n_slip + v - 1)], r);
+      grain_table(mt, (const double(*)[3])mv.data(), (const double(*)[3])nv.data(), r, nsys, NS,
+                  &tab[((size_t)g * nvar + v) * c->gstride]);
+    }
+  }
+  if (c->ntw) {
+    if (c->d_vrod) hipFree(c->d_vrod);
+    c->d_vrod = nullptr;
+    HIPCHK(hipMalloc(&c->d_vrod, c->h_vrod.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(c->d_vrod, c->h_vrod.data(), c->h_vrod.size() * sizeof(double), hipMemcpyHostToDevice));
   }
   if (c->d_gtab) hipFree(c->d_gtab);
   c->d_gtab = nullptr;
@@ -2413,9 +2441,10 @@ int set_grains_phases(pa_context *c, const unsigned *elem_grain, const double *g
   std::vector<double> sb((size_t)c->ns * nq, 0.0), rn(3 * nq);
   for (long long e = 0; e < c->m.nelem; ++e) {
     const pa_material &mt = c->phmat[c->h_gphase[elem_grain[e]]];
+    const int nsys = mt.n_slip + (mt.twinning ? mt.n_twin : 0);
     for (int q = 0; q < 8; ++q) {
       const long long Q = 8 * e + q;
-      for (int a = 0; a < mt.n_slip; ++a) sb[(size_t)a * nq + Q] = mt.s0[a];
+      for (int a = 0; a < nsys; ++a) sb[(size_t)a * nq + Q] = mt.s0[a];
       for (int i = 0; i < 3; ++i) rn[i * nq + Q] = grain_rod[3 * elem_grain[e] + i];
     }
   }
@@ -2435,16 +2464,25 @@ int pa_create_phases(const pa_material *mats, int n_phases, const pa_problem *pb
   // the envelope: the largest system count, kinematic hardening if any phase has it; the
   // tolerance and local-loop settings are global prm keys, so the phases must agree on them
   pa_material env = mats[0];
+  int tot = 0, ntw = 0;
   for (int k = 0; k < n_phases; ++k) {
     const pa_material &m = mats[k];
-    if (m.n_slip < 1 || m.n_slip > PA_MAX_SLIP) return fail(PA_ERR_ARG, "n_slip out of range");
-    if (m.twinning) return fail(PA_ERR_UNSUPPORTED, "several phases with a twinning phase are not built");
+    const int mt = m.twinning ? m.n_twin : 0;
+    if (m.n_slip < 1 || m.n_slip + mt > PA_MAX_SLIP) return fail(PA_ERR_ARG, "n_slip out of range");
+    if (m.twinning && (m.n_twin < 1 || !(m.twin_shear > 0))) return fail(PA_ERR_ARG, "twinning: n_twin >= 1 and twin shear > 0");
     if (m.max_slip_search != 1) return fail(PA_ERR_UNSUPPORTED, "Max Slip Search Iterations must be 1");
     if (m.max_solver_iter != mats[0].max_solver_iter || m.stress_tol != mats[0].stress_tol)
       return fail(PA_ERR_ARG, "the phases must share Max Solver Iterations and Stress Tolerance");
-    env.n_slip = std::max(env.n_slip, m.n_slip);
+    tot = std::max(tot, m.n_slip + mt);
+    ntw = std::max(ntw, mt);
     env.kinematic_hardening |= m.kinematic_hardening;
   }
+  // state for the most systems and the most twin systems of any phase
+  env.twinning = ntw > 0;
+  env.n_twin = ntw;
+  env.n_slip = tot - ntw;
+  if (env.twinning && !(env.twin_shear > 0)) env.twin_shear = 1.0;  // per phase in the kernels
+  if (env.n_slip < 1) return fail(PA_ERR_ARG, "every phase needs slip systems");
   int rc = pa_create(&env, pb, out);
   if (rc) return rc;
   pa_context *c = *out;
@@ -2456,9 +2494,10 @@ int pa_create_phases(const pa_material *mats, int n_phases, const pa_problem *pb
   for (int k = 0; k < n_phases; ++k) {
     const pa_material &m = mats[k];
     double *b = &mb[(size_t)k * c->mstride];
-    for (int a = 0; a < m.n_slip; ++a) {
+    const int nsys = m.n_slip + (m.twinning ? m.n_twin : 0);
+    for (int a = 0; a < nsys; ++a) {
       b[a] = m.h0[a]; b[NS + a] = m.ss[a]; b[2 * NS + a] = m.pw[a];
-      for (int q = 0; q < m.n_slip; ++q) b[3 * NS + a * NS + q] = m.q[a][q];
+      for (int q = 0; q < nsys; ++q) b[3 * NS + a * NS + q] = m.q[a][q];
       if (m.kinematic_hardening) { b[c1 + a] = m.C1[a]; b[c2 + a] = m.C2[a]; }
     }
     b[tol] = m.stress_tol;

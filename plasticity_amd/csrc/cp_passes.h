@@ -101,7 +101,8 @@ struct TwinArgs {
   double *tw_iter;
   const int *var_conv;    // [nq] orientation variant
   int *var_iter;
-  const double *vrod;     // [grain][1 + ntw][3] Rodrigues vectors of the variants
+  const double *vrod;     // [grain][nvar][3] Rodrigues vectors of the variants
+  int nvar;               // orientation-table rows per grain: 1 + the largest twin-system count
 };
 
 // After a qp update: the twin fractions (written on the plastic path only, as the reference writes
@@ -133,8 +134,20 @@ __device__ __forceinline__ void twin_epilogue(const TwinArgs &t, long long Q, lo
 struct PhaseArgs {
   const int *gphase;  // phase (0-based) of each grain row; null: one phase
   int mstride;        // doubles per material block
-  int ns[4];          // systems of each phase
+  int ns[4];          // systems of each phase (slip, then twin)
+  int nslip[4], ntw[4];
+  double shear[4];    // Characteristic Twin Shear of each phase
+  double F_T[4];      // min(1, Twin Threshold Fraction): with several phases the saturation term is not
+                      // used (calculatePlasticity.cc:969-985)
 };
+// the twinning arguments of a qp's phase (the tables and fraction arrays are shared)
+__device__ __forceinline__ void phase_twin(const PhaseArgs &ph, unsigned grain, TwinArgs &tw) {
+  const int p = ph.gphase[grain];
+  tw.nslip = ph.nslip[p];
+  tw.ntw = ph.ntw[p];
+  tw.shear = ph.shear[p];
+  tw.F_T = ph.F_T[p];
+}
 __device__ __forceinline__ void qp_phase(const PhaseArgs &ph, const double *mat0, int ns0, unsigned grain, const double *&mat, int &ns) {
   if (ph.gphase) {
     const int p = ph.gphase[grain];
@@ -148,7 +161,7 @@ __device__ __forceinline__ void qp_phase(const PhaseArgs &ph, const double *mat0
 
 // orientation-table row of qp Q in element e
 __device__ __forceinline__ long long table_row(const TwinArgs &t, const unsigned *egrain, long long Q) {
-  return t.on ? (long long)egrain[Q >> 3] * (1 + t.ntw) + t.var_conv[Q] : (long long)egrain[Q >> 3];
+  return t.on ? (long long)egrain[Q >> 3] * t.nvar + t.var_conv[Q] : (long long)egrain[Q >> 3];
 }
 
 // ---- assembly, stage 1: F from u, calculatePlasticity(.,.,1); writes the iter history,

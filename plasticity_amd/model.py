@@ -138,8 +138,36 @@ class userInputParameters:
                     "Use Indentation BCs", "Use Torsion BCs", "Flag To Use Taylor Model", "Use Element Deletion"):
             if b(key, "false"):
                 raise native.PlasticityError("'%s' is outside the implemented path" % key)
-        if self.enableMultiphase or self.readExternalMesh:
-            raise native.PlasticityError("multiphase and external meshes are outside the implemented path")
+        if self.readExternalMesh:
+            raise native.PlasticityError("external meshes are outside the implemented path")
+        # several phases (userInputParameters.cc:286-545): phase k's keys carry the suffix " k"
+        self.numberofPhases = int(g("Number of Phases", "1")) if self.enableMultiphase else 1
+        if not 1 <= self.numberofPhases <= 4:
+            raise native.PlasticityError("'Number of Phases' must be 1 to 4")
+        self.phaseKeys = []
+        for k in range(2, self.numberofPhases + 1):
+            sfx = " %d" % k
+            ph = dict(elasticStiffness=[fl(g("Elastic Stiffness%s row %d" % (sfx, i + 1), "")) for i in range(6)],
+                      numSlipSystems=int(g("Number of Slip Systems" + sfx, "0")),
+                      latentHardeningRatioFileName=g("Latent Hardening Ratio filename" + sfx, ""),
+                      initialSlipResistance=fl(g("Initial Slip Resistance" + sfx, "")),
+                      initialHardeningModulus=fl(g("Initial Hardening Modulus" + sfx, "")),
+                      powerLawExponent=fl(g("Power Law Exponent" + sfx, "")), saturationStress=fl(g("Saturation Stress" + sfx, "")),
+                      slipDirectionsFile=g("Slip Directions File" + sfx, ""), slipNormalsFile=g("Slip Normals File" + sfx, ""),
+                      enableKinematicHardening=b("Enable Kinematic Hardening" + sfx, "false"),
+                      C_1=fl(g("C_1 Slip Kinematic Hardening" + sfx, "")), C_2=fl(g("C_2 Slip Kinematic Hardening" + sfx, "")),
+                      enableTwinning=b("Twinning enabled" + sfx, "false"))
+            if ph["enableTwinning"]:
+                ph.update(numTwinSystems=int(g("Number of Twin Systems" + sfx, "0")),
+                          initialSlipResistanceTwin=fl(g("Initial Slip Resistance Twin" + sfx, "")),
+                          initialHardeningModulusTwin=fl(g("Initial Hardening Modulus Twin" + sfx, "")),
+                          powerLawExponentTwin=fl(g("Power Law Exponent Twin" + sfx, "")),
+                          saturationStressTwin=fl(g("Saturation Stress Twin" + sfx, "")),
+                          twinSaturationFactor=float(g("Twin Saturation Factor" + sfx, "0")),
+                          twinThresholdFraction=float(g("Twin Threshold Fraction" + sfx, "0")),
+                          twinShear=float(g("Characteristic Twin Shear" + sfx, "0")),
+                          twinDirectionsFile=g("Twin Directions File" + sfx, ""), twinNormalsFile=g("Twin Normals File" + sfx, ""))
+            self.phaseKeys.append(ph)
 
     def file(self, name):
         return name if os.path.isabs(name) else os.path.join(self.dir, name)
@@ -228,6 +256,7 @@ class crystalOrientationsIO:
         self.numPts = None
         self.eulerAngles = None
         self.numberOfGrains = 0
+        self.phase = None
 
     def loadOrientations(self, voxelFile, headerLines, orientationFile, numPts, span):
         with open(voxelFile) as f:
@@ -243,7 +272,9 @@ class crystalOrientationsIO:
         self.vox, self.numPts = vox, nv
 
     def loadOrientationVector(self, eulerFile, enableMultiphase=False, numVoxelData=0):
-        table = {}
+        """id -> Rodrigues vector; with several phases the fifth column is the grain's phase
+        (crystalOrientationsIO.cc:11-62), kept in .phase (1-based; 1 for rows without an id)."""
+        table, phase = {}, {}
         last = 0
         with open(eulerFile) as f:
             f.readline()
@@ -251,12 +282,20 @@ class crystalOrientationsIO:
                 v = line.split()
                 if len(v) >= 4:
                     table[int(v[0])] = [float(x) for x in v[1:4]]
+                    if enableMultiphase:
+                        if len(v) < 5:
+                            raise native.PlasticityError("several phases: orientation row %s has no phase column" % v[0])
+                        phase[int(v[0])] = int(float(v[4]))
                     last = int(v[0])
         self.numberOfGrains = last  # the last id read (crystalOrientationsIO.cc:61)
         rod = np.zeros((max(table) + 1, 3))
+        ph = np.ones(max(table) + 1, dtype=np.int32)
         for k, v in table.items():
             rod[k] = v
+            if enableMultiphase:
+                ph[k] = phase[k]
         self.eulerAngles = rod
+        self.phase = ph if enableMultiphase else None
 
 
 # ------------------------------------------------------------------ context
@@ -639,6 +678,23 @@ class crystalPlasticity:
                                       ui.saturationStress1, ui.powerLawExponent1, ui.modelStressTolerance,
                                       ui.modelMaxSlipSearchIterations, ui.modelMaxSolverIterations,
                                       ui.C_1_slip1 if kin else None, ui.C_2_slip1 if kin else None, twin)
+        self.materials = [self.material]
+        for ph in ui.phaseKeys:  # phases 2.. (init.cc:200-960)
+            ns2 = ph["numSlipSystems"]
+            tw2 = None
+            if ph["enableTwinning"]:
+                ntw2 = ph["numTwinSystems"]
+                tw2 = dict(m=read_vectors(ui.file(ph["twinDirectionsFile"]), ntw2), n=read_vectors(ui.file(ph["twinNormalsFile"]), ntw2),
+                           s0=ph["initialSlipResistanceTwin"], h0=ph["initialHardeningModulusTwin"], ss=ph["saturationStressTwin"],
+                           pw=ph["powerLawExponentTwin"], shear=ph["twinShear"], threshold=ph["twinThresholdFraction"],
+                           saturation=ph["twinSaturationFactor"])
+            q2 = read_table(ui.file(ph["latentHardeningRatioFileName"]), ns2 + (len(tw2["m"]) if tw2 else 0))
+            k2 = ph["enableKinematicHardening"]
+            self.materials.append(make_material(read_vectors(ui.file(ph["slipDirectionsFile"]), ns2),
+                                                read_vectors(ui.file(ph["slipNormalsFile"]), ns2), q2, ph["elasticStiffness"],
+                                                ph["initialSlipResistance"], ph["initialHardeningModulus"], ph["saturationStress"],
+                                                ph["powerLawExponent"], ui.modelStressTolerance, ui.modelMaxSlipSearchIterations,
+                                                ui.modelMaxSolverIterations, ph["C_1"] if k2 else None, ph["C_2"] if k2 else None, tw2))
         nel = [s * 2 ** ui.meshRefineFactor for s in ui.subdivisions]
         self.totalIncrements = int(round(ui.totalTime / ui.delT))  # ellipticBVP.cc:40
         bcs = []
@@ -672,9 +728,14 @@ class crystalPlasticity:
         self.ctx = None
 
     def _init(self):
-        self.ctx = Context(self.material, self.problem)
+        multi = len(self.materials) > 1
+        self.ctx = Context(self.materials if multi else self.material, self.problem)
         if self.periodic is not None:
             self.ctx.set_periodic(self.periodic)
+        if multi:
+            if self.orientations.phase is None:
+                raise native.PlasticityError("several phases: load the orientations with enableMultiphase=True")
+            self.ctx.set_grain_phases(self.orientations.phase)
         eg = self.ctx.element_grains(self.orientations.vox)
         self.ctx.set_grains(eg, self.orientations.eulerAngles)
         self.cellOrientationMap = eg
@@ -807,5 +868,5 @@ def main(prm):
     problem = crystalPlasticity(ui)
     problem.orientations.loadOrientations(ui.file(ui.grainIDFile), ui.headerLinesGrainIDFile, ui.file(ui.grainOrientationsFile),
                                           ui.numPts, ui.span)
-    problem.orientations.loadOrientationVector(ui.file(ui.grainOrientationsFile))
+    problem.orientations.loadOrientationVector(ui.file(ui.grainOrientationsFile), ui.enableMultiphase)
     return problem.run()

@@ -2,6 +2,8 @@
 import os
 import re
 
+import numpy as np
+
 import plasticity_amd.native as nat
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -169,3 +171,26 @@ def test_prm_with_periodic_bcs_maps_onto_the_abi():
     assert all(cp.problem.face_dof_constrained[f][k] == 0 for f in range(6) for k in range(3))
     ref = refcase.periodic_struct(refcase.load_reference_case("fcc_periodicBCs", prm_name="prm_UniaxialCyclicTabular.prm").periodic)
     assert bytes(P) == bytes(ref)
+
+
+def test_prm_with_several_phases_maps_onto_the_abi():
+    """fcc/simpleCompression_Multiphase's prm.prm: three phases (userInputParameters.cc:286-545) reach
+    three pa_material (phase 2 with 18 slip and 6 twin systems, phase 3 with kinematic hardening) as
+    the oracle's reader has them, and the orientation file's fifth column gives the grain phases."""
+    import refcase
+    from plasticity_amd import model
+    d = os.path.join(ROOT, "tests", "golden", "fcc_simpleCompression_Multiphase")
+    ui = model.userInputParameters(os.path.join(d, "prm.prm"))
+    cp = model.crystalPlasticity(ui)
+    assert len(cp.materials) == 3
+    m1, m2, m3 = cp.materials
+    assert (m1.n_slip, m1.twinning, m1.kinematic_hardening) == (12, 0, 0)
+    assert (m2.n_slip, m2.twinning, m2.n_twin, m2.twin_shear, m2.twin_threshold) == (18, 1, 6, 0.126, 0.7)
+    assert (m3.n_slip, m3.kinematic_hardening, m3.C1[0], m3.C2[0]) == (12, 1, 10.0, 1.0)
+    assert m2.C[2][2] == 61.5e3 and m2.C[5][5] == 16.8e3 and m1.max_solver_iter == 4 == m2.max_solver_iter
+    ref = refcase.load_reference_case("fcc_simpleCompression_Multiphase", twinning=True)
+    for mat, case in zip(cp.materials, ref.phases):
+        nsys = case.nT if case.twin else case.n_slip
+        assert np.allclose([mat.s0[a] for a in range(nsys)], list(case.s0) + (list(case.twin["s0"]) if case.twin else []))
+    cp.orientations.loadOrientationVector(os.path.join(d, "orientations.txt"), True)
+    assert list(cp.orientations.phase[1:4]) == [1, 2, 3]

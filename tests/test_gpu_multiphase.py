@@ -10,7 +10,10 @@ selects the material).
 * at every increment every qp's committed state (T, s, Fp, backstress) equals the oracle's qp
   update of that qp's phase on the GPU's own inputs at 1e-10, up to qps across a round-off decision;
 * a two-phase context whose grains all take phase 1 equals the one-phase context bit for bit;
-* a twinning phase, grains set before their phases and phase ids out of range are refused.
+* the reference's multiphase example (FCC, HCP with twinning, FCC with kinematic hardening) runs as
+  the oracle's several-phase run does;
+* phases that disagree on the global local-loop keys, grains set before their phases and phase ids
+  out of range are refused.
 """
 import numpy as np
 import pytest
@@ -44,7 +47,9 @@ def _phase_cases(n=4, strain=1e-3, n_inc=8):
 def _materials(case):
     mats = []
     for c in case.phases:
-        mats.append(pa.make_material(c.m, c.n, c.q, c.C, c.s0, c.h0, c.ss, c.pw, c.tol, C1=c.C1, C2=c.C2))
+        q = c.twin["q"] if c.twin else c.q
+        mats.append(pa.make_material(c.m, c.n, q, c.C, c.s0, c.h0, c.ss, c.pw, c.tol, max_solver_iter=c.max_solver_iter,
+                                     C1=c.C1, C2=c.C2, twin=c.twin))
     return mats
 
 
@@ -65,10 +70,11 @@ def _check_qps(case, ctx, u, Fp0, s0, W0):
     for Q in range(len(F)):
         g = case.elem_grain[Q // 8]
         pc = case.phases[case.grain_phase[g] - 1]
-        ns = pc.n_slip
+        ns = pc.nT if pc.twin else pc.n_slip  # twin systems follow the slip systems; else the inert one
+        sq = s0[Q][:ns] if pc.twin else np.append(s0[Q][:ns], 10e14)
         rod = case.grain_rod[g]
         W = W0[Q][:ns] if pc.C1 is not None else None
-        o = refcase.oracle_qp(pc, rod, F[Q], Fp0[Q].reshape(3, 3), np.append(s0[Q][:ns], 10e14), 0, W=W)
+        o = refcase.oracle_qp(pc, rod, F[Q], Fp0[Q].reshape(3, 3), sq, 0, W=W)
         if _rel(T1[Q], o["T"].ravel()) < 1e-10:
             assert _rel(s1[Q][:ns], o["s"][:ns]) < 1e-10 and _rel(Fp1[Q], o["Fp"].ravel()) < 1e-10, Q
             if pc.C1 is not None:
@@ -77,7 +83,7 @@ def _check_qps(case, ctx, u, Fp0, s0, W0):
             continue
         r = np.random.default_rng(1000 + Q)
         near = any(_rel(T1[Q], refcase.oracle_qp(pc, rod, F[Q] * (1 + 1e-12 * r.standard_normal((3, 3))), Fp0[Q].reshape(3, 3),
-                                                  np.append(s0[Q][:ns], 10e14), 0, W=W)["T"].ravel()) < 1e-6 for _ in range(400))
+                                                  sq, 0, W=W)["T"].ravel()) < 1e-6 for _ in range(400))
         assert near, Q
         rounded += 1
     assert rounded <= max(1, len(F) // 100)
@@ -136,16 +142,64 @@ def test_bad_phase_inputs_are_refused():
     case = _phase_cases(n_inc=2)
     mats = _materials(case)
     pb = _problem(case)
-    t = synth.hcp24_tables()
-    twin = dict(m=t["m"][:2], n=t["n"][:2], s0=[50.0] * 2, h0=[10.0] * 2, ss=[220.0] * 2, pw=[0.75] * 2, shear=0.126, threshold=0.7,
-                saturation=0.0)
-    tw = pa.make_material(case.phases[0].m, case.phases[0].n, np.ones((14, 14)), case.phases[0].C, case.phases[0].s0,
-                          case.phases[0].h0, case.phases[0].ss, case.phases[0].pw, 1e-9, twin=twin)
-    with pytest.raises(native.PlasticityError, match="twinning"):
-        pa.Context([mats[0], tw], pb)
+    f = case.phases[0]
+    other = pa.make_material(f.m, f.n, f.q, f.C, f.s0, f.h0, f.ss, f.pw, f.tol, max_solver_iter=4)
+    with pytest.raises(native.PlasticityError, match="share"):
+        pa.Context([mats[0], other], pb)
     ctx = pa.Context(mats, pb)
     with pytest.raises(native.PlasticityError, match="pa_set_grain_phases"):
         ctx.set_grains(case.elem_grain, case.grain_rod)
     with pytest.raises(native.PlasticityError, match="out of range"):
         ctx.set_grain_phases(np.array([1, 2, 4, 1, 2, 3], np.int32))
     ctx.close()
+
+
+def test_reference_multiphase_example_matches_the_oracle():
+    """fcc/simpleCompression_Multiphase (4 x 4 x 12 elements; grains 1-3 along z in phases 1-3: FCC,
+    HCP with 18 slip and 6 twin systems, FCC with kinematic hardening; Max Solver Iterations 4),
+    40 increments of compression through first yield: rows against the oracle's several-phase run,
+    and the last increment per qp."""
+    case = refcase.load_reference_case("fcc_simpleCompression_Multiphase", twinning=True)
+    case.lin_tol = 1e-14
+    n_inc = 40
+    ref = refcase.oracle_simulate(case, n_inc, nthreads=8, want_state=True)
+    ctx = pa.Context(_materials(case), _problem(case))
+    ctx.set_grain_phases(case.grain_phase)
+    ctx.set_grains(case.elem_grain, case.grain_rod)
+    rows = [ctx.run_increment(i)[0] for i in range(n_inc - 1)]
+    Fp0, s0, W0 = ctx.get_state("Fp"), ctx.get_state("s"), ctx.get_state("W")
+    rows = np.array(rows + [ctx.run_increment(n_inc - 1)[0]])
+    e = np.abs(rows[:, :12] - ref["rows"][:, :12]).max(axis=1) / np.abs(ref["rows"][:, :12]).max(axis=1)
+    parted = np.nonzero(e > 1e-9)[0]
+    n_same = int(parted[0]) if len(parted) else n_inc
+    plastic = np.nonzero(ref["rows"][:, 14] > 0)[0]
+    first_plastic = int(plastic[0]) if len(plastic) else n_inc
+    print("equal to the oracle (%.1e) through increment %d (first yield at %d); SlipTotal %.3e, TwinRealVF %.3e"
+          % (e[:n_same].max(), n_same - 1, first_plastic, rows[-1, 14], rows[-1, 12]), flush=True)
+    assert first_plastic < n_inc and n_same >= min(n_inc, first_plastic + 2)
+    assert _rel(rows[:n_same, 12:15], ref["rows"][:n_same, 12:15]) < 1e-8
+    _check_qps(case, ctx, ctx.get_state("u"), Fp0, s0, W0)
+    ctx.close()
+
+
+def test_prm_driven_multiphase_run_equals_the_context_run(tmp_path):
+    """model.main's path on fcc/simpleCompression_Multiphase's prm.prm (the phases from the prm keys
+    with suffixes, the grain phases from the orientation file's fifth column), 10 increments, against
+    the Context run built from the oracle's reader of the same files."""
+    import os
+    from plasticity_amd import model
+    d = os.path.join(refcase.GOLDEN, "fcc_simpleCompression_Multiphase")
+    ui = model.userInputParameters(os.path.join(d, "prm.prm"))
+    cp = model.crystalPlasticity(ui)
+    cp.orientations.loadOrientations(ui.file(ui.grainIDFile), ui.headerLinesGrainIDFile, ui.file(ui.grainOrientationsFile),
+                                     ui.numPts, ui.span)
+    cp.orientations.loadOrientationVector(ui.file(ui.grainOrientationsFile), True)
+    rows = cp.run(10, write=False)
+    case = refcase.load_reference_case("fcc_simpleCompression_Multiphase", twinning=True)
+    ctx = pa.Context(_materials(case), _problem(case))
+    ctx.set_grain_phases(case.grain_phase)
+    ctx.set_grains(case.elem_grain, case.grain_rod)
+    rows2 = np.array([ctx.run_increment(i)[0] for i in range(10)])
+    ctx.close()
+    assert _rel(rows, rows2) < 1e-12
+
