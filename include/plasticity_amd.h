@@ -223,6 +223,16 @@ int pa_run_increment(pa_context *ctx, int increment, double row[15], int *lin_it
  * Output is qp-major (or node-major).  n = number of doubles the caller provides. */
 int pa_get_state(pa_context *ctx, int which, double *host, long long n);
 
+/* The post-processed fields of the last committed increment, L2-projected onto the nodes
+ * (ellipticBVP::projection, projectionFunctions.cc:143-223, with the values of
+ * updateAfterIncrement.cc:147-192): fields[0..nnode) von Mises stress sqrt(3/2)|dev T|
+ * ("Eqv_stress"), [nnode..2 nnode) equivalent strain sqrt(2/3)|dev E| ("Eqv_strain"),
+ * [2 nnode..3 nnode) the twinned flag ("Twin"); each the solution of M x = b (the consistent Q1
+ * mass matrix, Jacobi-PCG at the problem's Krylov tolerance, solveLinearSystem.cc:43-65);
+ * iters (optional, 3) the CG iterations.  Uses the Jacobian's storage: the next Newton step
+ * assembles again. */
+int pa_project_fields(pa_context *ctx, double *fields, int *iters);
+
 /* Grain-averaged output of updateAfterIncrement (src/materialModels/crystalPlasticity/
  * updateAfterIncrement.cc:611-707) over the Cauchy stress of the last pa_commit_increment: row r
  * (grain id r + 1 of the element grain map, the reference's cellOrientationMap - 1) holds

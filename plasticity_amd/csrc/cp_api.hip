@@ -1497,6 +1497,92 @@ __global__ void k_zero(long long N, double *__restrict__ v) {
   if (r < N) v[r] = 0.0;
 }
 
+// ---- post-processing projection (projectionFunctions.cc:143-223, updateAfterIncrement.cc:147-192):
+// per qp von Mises stress sqrt(3/2)|dev T|, equivalent strain sqrt(2/3)|dev E| with E = (F^T F - I)/2,
+// and the twinned flag; out[f * nq + Q]
+__global__ void k_proj_values(MeshDims m, double hx, double hy, double hz, long long nq, const double *__restrict__ u,
+                              const double *__restrict__ T, const int *__restrict__ var, double *__restrict__ out) {
+  const long long Q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (Q >= nq) return;
+  double F[9], C[9], t[9];
+  const double hh[3] = {hx, hy, hz};
+  qp_F(m, hh, u, Q, F);
+  cp::m3_tmul(F, F, C);
+  for (int i = 0; i < 9; ++i) t[i] = T[i * nq + Q];
+  double E[9];
+  for (int i = 0; i < 9; ++i) E[i] = 0.5 * (C[i] - ((i % 4 == 0) ? 1.0 : 0.0));
+  const double trE = E[0] + E[4] + E[8], trT = t[0] + t[4] + t[8];
+  double se = 0.0, st = 0.0;
+  for (int i = 0; i < 9; ++i) {
+    const double de = E[i] - ((i % 4 == 0) ? trE / 3 : 0.0), dt = t[i] - ((i % 4 == 0) ? trT / 3 : 0.0);
+    se += de * de;
+    st += dt * dt;
+  }
+  out[Q] = sqrt(3.0 / 2.0) * sqrt(st);
+  out[nq + Q] = sqrt(2.0 / 3.0) * sqrt(se);
+  out[2 * nq + Q] = (var && var[Q] != 0) ? 1.0 : 0.0;
+}
+
+// right-hand side of the L2 projection of field v: b_n = sum over the elements and Gauss points
+// around node n of N_n(q) v(q) JxW, in component 0 of node n (components 1, 2 zero)
+__global__ void k_proj_rhs(MeshDims m, double JxW, const double *__restrict__ v, double *__restrict__ b) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= m.nnode) return;
+  const int i = (int)(n % m.nnx), j = (int)((n / m.nnx) % m.nny), k = (int)(n / ((long long)m.nnx * m.nny));
+  const double g0 = 0.5 - 0.5 / sqrt(3.0), g1 = 0.5 + 0.5 / sqrt(3.0);
+  double acc = 0.0;
+  for (int c = 0; c < 8; ++c) {  // element c: the node is its vertex 7 - c
+    const int ex = i - 1 + (c & 1), ey = j - 1 + ((c >> 1) & 1), ez = k - 1 + ((c >> 2) & 1);
+    if (ex < 0 || ey < 0 || ez < 0 || ex >= m.nx || ey >= m.ny || ez >= m.nz) continue;
+    const int vtx = 7 - c;
+    const long long e = ex + (long long)m.nx * (ey + (long long)m.ny * ez);
+    for (int q = 0; q < 8; ++q) {
+      double N = 1.0;
+      for (int d = 0; d < 3; ++d) {
+        const double xi = ((q >> d) & 1) ? g1 : g0;
+        N *= ((vtx >> d) & 1) ? xi : 1.0 - xi;
+      }
+      acc += N * v[8 * e + q] * JxW;
+    }
+  }
+  b[3 * n] = acc;
+  b[3 * n + 1] = 0.0;
+  b[3 * n + 2] = 0.0;
+}
+
+// the consistent Q1 mass matrix (projectionFunctions.cc:99-135) as node blocks M_nm I: slot s of node
+// n sums N_n N_m JxW over the Gauss points of the elements holding both nodes
+__global__ void k_mass_blocks(MeshDims m, double JxW, double *__restrict__ K) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= m.nnode) return;
+  const int i = (int)(n % m.nnx), j = (int)((n / m.nnx) % m.nny), k = (int)(n / ((long long)m.nnx * m.nny));
+  const double g0 = 0.5 - 0.5 / sqrt(3.0), g1 = 0.5 + 0.5 / sqrt(3.0);
+  for (int s = 0; s < 27; ++s) {
+    const int dx = s % 3 - 1, dy = (s / 3) % 3 - 1, dz = s / 9 - 1;
+    double mv = 0.0;
+    for (int c = 0; c < 8; ++c) {
+      const int ex = i - 1 + (c & 1), ey = j - 1 + ((c >> 1) & 1), ez = k - 1 + ((c >> 2) & 1);
+      if (ex < 0 || ey < 0 || ez < 0 || ex >= m.nx || ey >= m.ny || ez >= m.nz) continue;
+      const int va = 7 - c;
+      // the neighbour's vertex in this element
+      const int bx = (va & 1) + dx, by = ((va >> 1) & 1) + dy, bz = ((va >> 2) & 1) + dz;
+      if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
+      const int vb = bx + 2 * by + 4 * bz;
+      for (int q = 0; q < 8; ++q) {
+        double Na = 1.0, Nb = 1.0;
+        for (int d = 0; d < 3; ++d) {
+          const double xi = ((q >> d) & 1) ? g1 : g0;
+          Na *= ((va >> d) & 1) ? xi : 1.0 - xi;
+          Nb *= ((vb >> d) & 1) ? xi : 1.0 - xi;
+        }
+        mv += Na * Nb * JxW;
+      }
+    }
+    double *B = K + ((long long)s * m.nnode + n) * 9;
+    for (int r = 0; r < 9; ++r) B[r] = (r % 4 == 0) ? mv : 0.0;
+  }
+}
+
 #include "cp_mg.h"
 #include "cp_pbc.h"
 
@@ -2117,9 +2203,33 @@ int launch_krylov_iteration(pa_context *c, int k) {
   return 0;
 }
 
+int krylov_run(pa_context *c, KState &hs);
+
 int do_solve(pa_context *c, int it, int *iters) {
   if (c->asm_precision != c->precision)
     return fail(PA_ERR_STATE, "no assembly in the current arithmetic form: call pa_assemble after pa_set_solver");
+  KState hs;
+  if (int rc = krylov_run(c, hs)) return rc;
+  if (iters) *iters = hs.it;
+  c->last_krylov = hs;
+  c->solved = true;
+  if (hs.breakdown == KRYLOV_NONFINITE)  // as PETSc's KSP_DIVERGED_NANORINF: no correction is applied
+    return fail(PA_ERR_NONFINITE, "Krylov solve stopped on a NaN/Inf residual norm at iteration " + std::to_string(hs.it) +
+                                      " (right-hand side norm " + std::to_string(hs.bnorm) + ")");
+  const long long N = c->ndof;
+  const double *X = c->kv[0];
+  // the CG iterate x is complete; BiCG likewise (x lives in kv[0] for both)
+  if (c->pbc)
+    hipLaunchKernelGGL(k_pbc_apply, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, c->u, X, c->pbcd, c->pbc_s);
+  else
+    hipLaunchKernelGGL(k_apply_du, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, c->u, X, (const unsigned char *)c->con,
+                       (const double *)c->delu, (int)(it == 0));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// the Krylov solve of K X = R (method and form of the context) into kv[0]; hs: the final state
+int krylov_run(pa_context *c, KState &hs) {
   const long long N = c->ndof;
   double *X = c->kv[0], *Rr = c->kv[1], *Rl = c->kv[2], *Zr = c->kv[3], *Zl = c->kv[4];
   const bool cg = c->method != PA_SOLVER_BICG;
@@ -2142,7 +2252,6 @@ int do_solve(pa_context *c, int it, int *iters) {
   // (multigrid: a few dozen iterations of ~30 launches each, so a short chunk bounds the
   // launches that run after convergence)
   const int chunk = mg ? 4 : 16;
-  KState hs;
   for (int j = 0;; ++j) {
     for (int k = 0; k < chunk; ++k) {
       const int rc = launch_krylov_iteration(c, j * chunk + k);
@@ -2155,19 +2264,6 @@ int do_solve(pa_context *c, int it, int *iters) {
     hs = c->hks[(j - 1) & 1];
     if (hs.done) break;
   }
-  if (iters) *iters = hs.it;
-  c->last_krylov = hs;
-  c->solved = true;
-  if (hs.breakdown == KRYLOV_NONFINITE)  // as PETSc's KSP_DIVERGED_NANORINF: no correction is applied
-    return fail(PA_ERR_NONFINITE, "Krylov solve stopped on a NaN/Inf residual norm at iteration " + std::to_string(hs.it) +
-                                      " (right-hand side norm " + std::to_string(hs.bnorm) + ")");
-  // the CG iterate x is complete; BiCG likewise (x lives in kv[0] for both)
-  if (c->pbc)
-    hipLaunchKernelGGL(k_pbc_apply, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, c->u, (const double *)X, c->pbcd, c->pbc_s);
-  else
-    hipLaunchKernelGGL(k_apply_du, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, c->u, (const double *)X, (const unsigned char *)c->con,
-                       (const double *)c->delu, (int)(it == 0));
-  HIPCHK(hipGetLastError());
   return 0;
 }
 
@@ -3223,6 +3319,42 @@ int pa_qp_update_warm(pa_context *c, pa_mlp *net, long long n, const double *F, 
   if (feat) HIPCHK(hipMemcpy(feat, dfeat, n * mlp::NIN * sizeof(float), hipMemcpyDeviceToHost));
   if (x0 && net) HIPCHK(hipMemcpy(x0, dx0, n * mlp::NOUT * sizeof(float), hipMemcpyDeviceToHost));
   return 0;
+}
+
+int pa_project_fields(pa_context *c, double *fields, int *iters) {
+  if (!c || !fields) return fail(PA_ERR_ARG, "null argument");
+  if (!c->grains_set) return fail(PA_ERR_STATE, "grains not set");
+  const long long nq = c->nq, nn = c->m.nnode;
+  // per-qp values into the staging of the assembly's P (9 per qp, 3 used)
+  hipLaunchKernelGGL(k_proj_values, dim3(nblk(nq, 256)), dim3(256), 0, c->stream, c->m, c->h[0], c->h[1], c->h[2], nq,
+                     (const double *)c->u, (const double *)c->T, (const int *)c->var_iter, c->Pq);
+  hipLaunchKernelGGL(k_mass_blocks, dim3(nblk(nn, 256)), dim3(256), 0, c->stream, c->m, c->JxW, (double *)c->K);
+  hipLaunchKernelGGL(k_dinv<double>, dim3(nblk(c->ndof, 256)), dim3(256), 0, c->stream, c->m, (const double *)c->K, c->dinv);
+  HIPCHK(hipGetLastError());
+  // solveLinearSystem2 (solveLinearSystem.cc:43-65): Jacobi-PCG in FP64, one field at a time, the
+  // field in component 0 of every node (the other components stay zero)
+  const int method = c->method, precision = c->precision;
+  const bool pbc = c->pbc;
+  c->method = PA_SOLVER_CG; c->precision = PA_PREC_FP64; c->pbc = false;
+  c->asm_precision = -1;  // K now holds the mass matrix: the next Newton step assembles again
+  c->solved = false;
+  std::vector<double> x(3 * nn);
+  int rc = 0;
+  for (int f = 0; f < 3 && !rc; ++f) {
+    hipLaunchKernelGGL(k_proj_rhs, dim3(nblk(nn, 256)), dim3(256), 0, c->stream, c->m, c->JxW, (const double *)(c->Pq + f * nq), c->R);
+    KState hs;
+    rc = krylov_run(c, hs);
+    if (rc) break;
+    if (iters) iters[f] = hs.it;
+    if (hipMemcpyAsync(x.data(), c->kv[0], sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+      rc = fail(PA_ERR_HIP, "copy projected field");
+      break;
+    }
+    for (long long n = 0; n < nn; ++n) fields[f * nn + n] = x[3 * n];
+  }
+  c->method = method; c->precision = precision; c->pbc = pbc;
+  return rc;
 }
 
 int pa_grain_averages(pa_context *c, int n_grains, double *out) {

@@ -51,6 +51,10 @@ class userInputParameters:
         self.externalMeshParameter = float(g("External mesh parameter", "0"))
         self.writeOutput = b("Write Output", "false")
         self.outputDirectory = g("Output Directory", "results")
+        self.skipOutputSteps = int(g("Skip Output Steps", "-1"))  # userInputParameters.cc:55, 161-167
+        self.output_Eqv_strain = b("Output Equivalent strain", "false")
+        self.output_Eqv_stress = b("Output Equivalent stress", "false")
+        self.output_Twin = b("Output Twin fractions", "false")
         self.delT = float(g("Time increments", "-1"))
         self.totalTime = float(g("Total time", "-1"))
         self.enableSimpleBCs = b("Use Simple BCs", "true")
@@ -412,6 +416,16 @@ class Context:
         check(native.lib().pa_get_state(self._h, ctypes.c_int(which), ptr(out), ctypes.c_longlong(out.size)))
         return out
 
+    def project_fields(self):
+        """The L2 projections onto the nodes of von Mises stress, equivalent strain and the twinned
+        flag of the last committed increment (pa_project_fields): array [3, nnode]."""
+        nn = self.ndof // 3
+        out = np.zeros((3, nn))
+        its = np.zeros(3, dtype=np.int32)
+        check(native.lib().pa_project_fields(self._h, ptr(out), ptr(its)))
+        self.last_projection_iters = its
+        return out
+
     def grain_averages(self, n_grains):
         """Grain-averaged output (updateAfterIncrement.cc:611-707): rows [volume, Txx, Tyy, Tzz,
         Txy, Txz, Tyz] for grain ids 1..n_grains, from the Cauchy stress of the last commit."""
@@ -758,6 +772,12 @@ class crystalPlasticity:
                 write_quadrature_output(os.path.join(outdir, "QuadratureOutputs%d.csv" % inc), self.ctx, self.cellOrientationMap,
                                         (self.problem.nx, self.problem.ny, self.problem.nz), ui.span,
                                         dealii_cell_order(ui.subdivisions, ui.meshRefineFactor))
+            if write and ui.writeOutput and self._output_due(inc, ui.skipOutputSteps):  # solve.cc:122-124
+                os.makedirs(outdir, exist_ok=True)
+                write_vtu_output(outdir, inc, self.totalIncrements, self.ctx, self.cellOrientationMap,
+                                 (self.problem.nx, self.problem.ny, self.problem.nz), ui.span,
+                                 [name for name, on in (("Eqv_stress", ui.output_Eqv_stress), ("Eqv_strain", ui.output_Eqv_strain),
+                                                        ("Twin", ui.output_Twin)) if on])
             if write and ui.writeGrainAveragedOutput and self._output_due(inc, ui.skipGrainAveragedOutputSteps):
                 if ui.numberOfGrainAverageDataOutput != 7:  # updateAfterIncrement.cc:657-660
                     raise native.PlasticityError("The size of numberOfGrainAverageDataOutput defined in prm.prm is not correct")
@@ -842,6 +862,79 @@ def write_quadrature_output(path, ctx, elem_grain, nel, span, cell_order):
     with open(path, "w") as f:
         for r in rows:
             f.write("".join("%8.5e ," % v for v in r) + "\n")
+
+
+# VTK cell type of a hexahedron, and its vertex order as lexicographic Q1 vertex ids (x fastest)
+_VTK_HEX = 12
+_VTK_HEX_ORDER = (0, 1, 3, 2, 4, 5, 7, 6)
+
+
+def _vtu_piece(path, nel, span, point_data, cell_data):
+    """One VTK XML UnstructuredGrid file of the box mesh (nodes x fastest, undeformed, as DataOut writes
+    the mesh) with ASCII point and cell arrays: point_data / cell_data are lists of (name, array
+    [n] or [n, 3])."""
+    nx, ny, nz = nel
+    xs = [np.linspace(0.0, span[d], n + 1) for d, n in enumerate(nel)]
+    Z, Y, X = np.meshgrid(xs[2], xs[1], xs[0], indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    ex, ey, ez = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    ex, ey, ez = ex.transpose(2, 1, 0).ravel(), ey.transpose(2, 1, 0).ravel(), ez.transpose(2, 1, 0).ravel()
+    conn = np.stack([(ex + (v & 1)) + (nx + 1) * ((ey + ((v >> 1) & 1)) + (ny + 1) * (ez + ((v >> 2) & 1)))
+                     for v in _VTK_HEX_ORDER], axis=1)
+    nel_t = nx * ny * nz
+
+    def arr(name, a):
+        a = np.asarray(a)
+        nc = 1 if a.ndim == 1 else a.shape[1]
+        body = " ".join("%.9g" % v for v in a.ravel())
+        return '<DataArray type="Float64" Name="%s" NumberOfComponents="%d" format="ascii">%s</DataArray>' % (name, nc, body)
+    with open(path, "w") as f:
+        f.write('<?xml version="1.0"?>\n<VTKFile type="UnstructuredGrid" version="0.1" byte_order="LittleEndian">\n'
+                "<UnstructuredGrid>\n")
+        f.write('<Piece NumberOfPoints="%d" NumberOfCells="%d">\n' % (len(pts), nel_t))
+        f.write("<Points>%s</Points>\n" % arr("Points", pts))
+        f.write('<Cells><DataArray type="Int64" Name="connectivity" format="ascii">%s</DataArray>'
+                % " ".join(str(v) for v in conn.ravel()))
+        f.write('<DataArray type="Int64" Name="offsets" format="ascii">%s</DataArray>' % " ".join(str(8 * (e + 1)) for e in range(nel_t)))
+        f.write('<DataArray type="UInt8" Name="types" format="ascii">%s</DataArray></Cells>\n' % " ".join([str(_VTK_HEX)] * nel_t))
+        f.write("<PointData>%s</PointData>\n" % "".join(arr(n, a) for n, a in point_data))
+        f.write("<CellData>%s</CellData>\n" % "".join(arr(n, a) for n, a in cell_data))
+        f.write("</Piece>\n</UnstructuredGrid>\n</VTKFile>\n")
+
+
+def _pvtu(path, piece, point_data, cell_data):
+    """The parallel record naming the one piece (DataOut::write_pvtu_record)."""
+    def decl(items):
+        return "".join('<PDataArray type="Float64" Name="%s" NumberOfComponents="%d"/>' % (n, 1 if np.asarray(a).ndim == 1 else 3)
+                       for n, a in items)
+    with open(path, "w") as f:
+        f.write('<?xml version="1.0"?>\n<VTKFile type="PUnstructuredGrid" version="0.1" byte_order="LittleEndian">\n'
+                '<PUnstructuredGrid GhostLevel="0">\n<PPointData>%s</PPointData>\n<PCellData>%s</PCellData>\n'
+                '<PPoints><PDataArray type="Float64" NumberOfComponents="3"/></PPoints>\n<Piece Source="%s"/>\n'
+                "</PUnstructuredGrid>\n</VTKFile>\n" % (decl(point_data), decl(cell_data), piece))
+
+
+def write_vtu_output(outdir, inc, total_increments, ctx, elem_grain, nel, span, projected):
+    """ellipticBVP::output (output.cc:4-270) on one process: solution-<inc>.0000.vtu with the nodal
+    displacement "u" and the cell data meshGrain_ID and subdomain, and when fields are selected
+    projectedFields-<inc>.0000.vtu with the L2-projected fields (pa_project_fields: Eqv_stress,
+    Eqv_strain, Twin), subdomain and meshGrain_ID; each with its .pvtu record.  The increment is
+    written with 4 digits below 10000 increments."""
+    digits = 4 if total_increments < 10000 else int(np.ceil(np.log10(total_increments))) + 1
+    tag = "%0*d" % (digits, inc)
+    u = ctx.get_state("u").reshape(-1, 3)
+    grain = np.asarray(elem_grain, dtype=np.float64)
+    sub = np.zeros(len(grain))
+    cells = [("meshGrain_ID", grain), ("subdomain", sub)]
+    _vtu_piece(os.path.join(outdir, "solution-%s.0000.vtu" % tag), nel, span, [("u", u)], cells)
+    _pvtu(os.path.join(outdir, "solution-%s.pvtu" % tag), "solution-%s.0000.vtu" % tag, [("u", u)], cells)
+    if projected:
+        fields = ctx.project_fields()
+        names = ("Eqv_stress", "Eqv_strain", "Twin")
+        pdata = [(n, fields[names.index(n)]) for n in projected]
+        pcells = [("subdomain", sub), ("meshGrain_ID", grain)]
+        _vtu_piece(os.path.join(outdir, "projectedFields-%s.0000.vtu" % tag), nel, span, pdata, pcells)
+        _pvtu(os.path.join(outdir, "projectedFields-%s.pvtu" % tag), "projectedFields-%s.0000.vtu" % tag, pdata, pcells)
 
 
 def write_grain_average(path, data):

@@ -27,7 +27,7 @@ EXPORTS = [
     "pa_mlp_predict", "pa_mlp_grad", "pa_mlp_train", "pa_qp_update_warm", "pa_grain_averages",
     "pa_batch_create", "pa_batch_destroy", "pa_batch_set_grains", "pa_batch_set_solver", "pa_batch_set_graphs",
     "pa_batch_newton_iteration", "pa_batch_commit_increment", "pa_batch_tune", "pa_batch_qp_pass_ms", "pa_batch_last_timings",
-    "pa_batch_get_state", "pa_batch_sizes", "pa_set_periodic", "pa_periodic_lines", "pa_create_phases", "pa_set_grain_phases",
+    "pa_batch_get_state", "pa_batch_sizes", "pa_set_periodic", "pa_periodic_lines", "pa_create_phases", "pa_set_grain_phases", "pa_project_fields",
 ]
 
 
