@@ -119,6 +119,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-warm", action="store_true", help="skip the learned warm-start measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-RVE workload")
+    ap.add_argument("--no-ext", action="store_true", help="skip the periodic-BC and three-phase workloads")
+    ap.add_argument("--only-ext", action="store_true", help="run the periodic-BC and three-phase workloads alone")
     ap.add_argument("--only-batch", action="store_true", help="run the batched-RVE workload alone and print its record")
     ap.add_argument("--full-path", type=int, default=1,
                     help="after the timed steps, run the multigrid-PCG forms to the end of the load path (untimed for value)")
@@ -135,6 +137,9 @@ def main():
     assert torch.cuda.is_available(), "no GPU"
     if args.only_batch:
         print(json.dumps({"batched_rves": batched_leg(pa, native, synth)}))
+        return
+    if args.only_ext:
+        print(json.dumps({"periodic_and_phases": extensions_leg(pa, native, synth, {}, n=args.n, grains=args.grains)}))
         return
     n = args.n
     forms = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
@@ -189,6 +194,8 @@ def main():
             err["crss"] = float(np.abs(b["s"] - a["s"]).max() / np.abs(a["s"]).max())
             err["displacement"] = float(np.abs(b["u"] - a["u"]).max() / np.abs(a["u"]).max())
             out["config"]["mixed_vs_fp64_gpu_diff_%s" % solver] = err
+    if not args.no_ext:
+        out["config"]["periodic_and_phases"] = extensions_leg(pa, native, synth, results, n=n, grains=args.grains)
     if not args.no_warm:
         out["config"]["learned_warm_start"] = warm_start_leg(pa, native)
     if not args.no_batch:
@@ -245,6 +252,70 @@ def batched_leg(pa, native, synth):
             "n_rve": int(len(rve_mat)), "qps": nq, "newton_steps": 4 * pb.total_increments, "fp64": a, "mixed": m,
             "speedup_mixed_vs_fp64": a["ms_per_step"] / m["ms_per_step"], "mixed_vs_fp64_final_rows_rel_diff": diff,
             "note": "ms_per_step includes the post-increment passes (spread over the 4 steps of their increment)"}
+
+
+def extensions_leg(pa, native, synth, results, n=64, grains=256, warmup=4, steps=8):
+    """The 64^3 FCC tension workload under periodic boundary conditions (the reference's fcc/periodicBCs
+    constraint lines and uniaxial-tension vertex rows, 5e-4 per step on the corner V100) and with
+    three phases (FCC; HCP with 24 systems; FCC with kinematic hardening: grain g in phase g % 3 + 1),
+    mixed form, Jacobi-PCG (multigrid does not take periodic lines) and multigrid-PCG for the phases:
+    ms per Newton step over `steps` steps after `warmup`, against the same form and solver with face
+    rules (the main legs)."""
+    from plasticity_amd import model
+    out = {}
+
+    def timed(ctx, nmax=4):
+        rows, its = [], []
+        for s in range(warmup + steps):
+            if s == warmup:
+                t0 = time.perf_counter()
+            inc, it = divmod(s, nmax)
+            _, k = ctx.newton_iteration(inc, it)
+            its.append(k)
+            if it == nmax - 1:
+                rows.append(ctx.commit_increment())
+        dt = time.perf_counter() - t0
+        return 1000.0 * dt / steps, float(np.mean(its[warmup:])), rows
+
+    # periodic: the constraint lines of the reference's input, the vertex rows of its uniaxial tension
+    d = os.path.join(ROOT, "tests", "golden", "fcc_periodicBCs")
+    mat, pb, vox, rod, _, _ = synth.fcc_workload(n, grains, 20, 5e-4, seed=2024)
+    lines = model.read_periodic_constraints(os.path.join(d, "PeriodicBCsConstraints.txt"), 12, 27, 9)
+    known = [1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1, 0, 1, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0, 0]
+    total = [0.0] * 24
+    for r in (3, 12, 15, 21):  # u_x of V100, V110, V101, V111
+        total[r] = 5e-4 * pb.total_increments
+    ctx = pa.Context(mat, pb)
+    ctx.set_grains(ctx.element_grains(vox), rod)
+    ctx.set_solver(native.PA_SOLVER_CG, native.PA_PREC_MIXED)
+    ctx.set_periodic(model.make_periodic(lines, known, total))
+    ms, its, rows = timed(ctx)
+    ctx.close()
+    ref = results.get(("mixed", "cg"))
+    out["periodic"] = {"workload": "%d^3 FCC, %d grains, periodic uniaxial tension 5e-4/step (fcc/periodicBCs lines), mixed, "
+                                   "Jacobi-PCG 1e-10" % (n, grains),
+                       "ms_per_step": ms, "qp_updates_per_s": 8 * n ** 3 / (ms * 1e-3), "krylov_iters_per_step": its,
+                       "face_rule_ms_per_step": ref["ms_per_step"] if ref else None,
+                       "last_row_Txx_Tyy_Tzz": [float(v) for v in rows[-1][6:9]]}
+    # three phases
+    t = synth.fcc_tables()
+    h = synth.hcp24_tables()
+    hcp = pa.make_material(h["m"], h["n"], h["q"], h["C"], h["s0"], h["h0"], h["ss"], h["pw"], h["tol"])
+    kin = pa.make_material(t["m"], t["n"], t["q"], t["C"], t["s0"], t["h0"], t["ss"], t["pw"], t["tol"],
+                           C1=[10.0] * 12, C2=[1.0] * 12)
+    ctx = pa.Context([mat, hcp, kin], pb)
+    ctx.set_grain_phases(np.arange(len(rod)) % 3 + 1)
+    ctx.set_grains(ctx.element_grains(vox), rod)
+    ctx.set_solver(native.PA_SOLVER_MGCG, native.PA_PREC_MIXED)
+    ms, its, rows = timed(ctx)
+    qp_ms = ctx.last_qp_pass_ms()
+    ctx.close()
+    ref = results.get(("mixed", "mgcg"))
+    out["phases"] = {"workload": "%d^3, %d grains in 3 phases (FCC 12, HCP 24, FCC 12 with kinematic hardening), tension "
+                                 "5e-4/step, mixed, multigrid-PCG 1e-10" % (n, grains),
+                     "ms_per_step": ms, "qp_updates_per_s": 8 * n ** 3 / (ms * 1e-3), "krylov_iters_per_step": its,
+                     "qp_main_pass_ms_last": qp_ms, "one_phase_ms_per_step": ref["ms_per_step"] if ref else None}
+    return out
 
 
 def warm_start_leg(pa, native):
