@@ -137,10 +137,12 @@ int pa_destroy(pa_context *ctx);
  * largest phase.  Then pa_set_grain_phases (each grain row's phase, 1-based: the orientation
  * file's fifth column, crystalOrientationsIO.cc:42-45) before pa_set_grains, which builds each
  * grain's tables (and twin variants) from its phase and starts s at its phase's s0.  As the
- * reference does with several phases: a twinning phase reorients above its Twin Threshold
- * Fraction alone (calculatePlasticity.cc:969-985), and SlipTotal / TwinRealVF sum phase 1's slip
- * and twin counts (1 twin system without its twinning) at every qp (updateAfterIncrement.cc:
- * 222-236).  The batch qp entry points (pa_qp_update, pa_qp_update_warm) take one phase. */
+ * reference does with Enable Multiphase: after each increment F_r, F_e and F_s are zero
+ * (updateAfterIncrement.cc:718-725), so the row's TwinRealVF, TwinMade and SlipTotal read 0 and
+ * no qp reorients by twinning (the reorientation needs F_r > 0, calculatePlasticity.cc:969-993);
+ * twin fractions still grow and harden in a twinning phase.  A context made here with one phase
+ * follows the same rule.  The batch qp entry points (pa_qp_update, pa_qp_update_warm) take one
+ * phase. */
 int pa_create_phases(const pa_material *mats, int n_phases, const pa_problem *pb, pa_context **out);
 int pa_set_grain_phases(pa_context *ctx, const int *grain_phase, int n_grain_rows);
 

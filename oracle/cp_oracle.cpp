@@ -1704,7 +1704,8 @@ int simulate(const or_material *mats, int nph, const int *grain_phase, const or_
     for (int q = 0; q < 8; ++q) for (int i = 0; i < 3; ++i) st.rotnew_conv[3 * (8 * e + q) + i] = grain_rod[3 * elem_grain[e] + i];
   double F_r = 0.0, F_e = 0.0;  // volume-averaged twin fraction / reoriented volume of the last increment
   // the averages sum the first numTwinSystems1 twin and numSlipSystems1 slip fractions of every qp,
-  // phase 1's counts whatever the qp's phase (updateAfterIncrement.cc:222-236)
+  // phase 1's counts whatever the qp's phase (updateAfterIncrement.cc:222-236); with several phases
+  // the sums are then discarded (:722-725)
   const int nTw1 = mats[0].twinning ? mats[0].n_twin : 1, nSl1 = mats[0].n_slip;
   std::vector<Grain> twin_grains;
   if (twinning) {
@@ -1886,6 +1887,9 @@ int simulate(const or_material *mats, int nph, const int *grain_phase, const or_
       st.twin_conv = st.twin_iter;
       F_r = Fr_sum / vol_total;                           // :719-720
       F_e = Fe_sum / vol_total;
+      // with Enable Multiphase both are set to zero instead (:722-725), so calculatePlasticity's
+      // F_r > 0 test (:991-993) never lets a qp reorient by twinning
+      if (multiphase) F_r = F_e = 0.0;
     }
     double *row = rows + 15 * inc;
     const int vi[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {0, 2}, {0, 1}};
@@ -1893,7 +1897,7 @@ int simulate(const or_material *mats, int nph, const int *grain_phase, const or_
     g_phase_ns[2] += now_ns() - t_post;
     row[12] = twinning ? F_r : 0.0;  // TwinRealVF (without twinning the inert system never activates)
     row[13] = twinning ? F_e : 0.0;  // TwinMade
-    row[14] = Fs / vol_total;
+    row[14] = multiphase ? 0.0 : Fs / vol_total;  // F_s = 0 with Enable Multiphase (:722-725)
   }
   if (state_out) {
     const int W = 9 + 9 + 9 + nT + 3 + ns + (twinning ? 4 + ntw : 0);

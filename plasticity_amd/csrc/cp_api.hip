@@ -899,16 +899,15 @@ __global__ void __launch_bounds__(256) k_post_reduce(PostArgs a) {
       acc[c] = 0.5 * (C[vi[c]] - ((vi[c] % 4 == 0) ? 1.0 : 0.0)) * a.JxW;
       acc[6 + c] = T[vi[c]] * a.JxW;
     }
+    // SlipTotal, TwinRealVF and TwinMade numerators (updateAfterIncrement.cc:176, 222-236); the
+    // host discards them with several phases, as the reference does (:722-725)
     double slip_sum = 0;
-    // several phases: phase 1's slip-system count at every qp (updateAfterIncrement.cc:234-236)
-    const int nslip = a.ph.gphase ? a.ph.nslip[0] : (a.tw.on ? a.tw.nslip : a.ns);
+    const int nslip = a.tw.on ? a.tw.nslip : a.ns;
     for (int i = 0; i < nslip; ++i) slip_sum += a.slip_iter[i * a.nq + Q] * a.JxW;
     acc[12] = slip_sum;
-    if (a.tw.on) {  // TwinRealVF and TwinMade numerators (updateAfterIncrement.cc:176, 222-228)
+    if (a.tw.on) {
       double fr = 0;
-      // several phases: phase 1's twin-system count at every qp (1 without its twinning)
-      const int ntw = a.ph.gphase ? min(max(a.ph.ntw[0], 1), a.tw.ntw) : a.tw.ntw;
-      for (int k = 0; k < ntw; ++k) fr += a.tw.tw_iter[k * a.nq + Q] * a.JxW;
+      for (int k = 0; k < a.tw.ntw; ++k) fr += a.tw.tw_iter[k * a.nq + Q] * a.JxW;
       acc[13] = fr;
       acc[14] = (a.tw.var_iter[Q] != 0 ? 1.0 : 0.0) * a.JxW;
     }
@@ -1651,8 +1650,9 @@ struct pa_context {
   // periodic boundary conditions (pa_set_periodic): the closed constraint lines on the device and
   // the load scale of each increment's first Newton iteration
   // several phases (pa_create_phases): material of each phase, its block stride on the device and
-  // the phase of each grain row
+  // the phase of each grain row; multiphase: Enable Multiphase (any phase count)
   int nph = 1;
+  bool multiphase = false;
   pa_material phmat[4];
   int mstride = 0;
   std::vector<int> h_gphase;
@@ -2582,6 +2582,7 @@ int pa_create_phases(const pa_material *mats, int n_phases, const pa_problem *pb
   int rc = pa_create(&env, pb, out);
   if (rc) return rc;
   pa_context *c = *out;
+  c->multiphase = true;
   if (n_phases == 1) { c->mat = mats[0]; return 0; }
   const int NS = ns_template(c->ns);
   const int tol = 3 * NS + NS * NS, c1 = tol + 1, c2 = c1 + NS;
@@ -2844,7 +2845,10 @@ int pa_commit_increment(pa_context *c, double row[15]) {
   float ms = 0;
   if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) c->last_ms[2] = ms;
   const double vol = c->microvol;
-  if (c->ntw) {  // updateAfterIncrement.cc:719-720: the F_T of the next increment uses these
+  // updateAfterIncrement.cc:719-725: the F_T of the next increment and its F_r > 0 reorientation
+  // test use these; with Enable Multiphase the reference sets F_r, F_e and F_s to zero, so no qp
+  // reorients by twinning and the three columns read 0
+  if (c->ntw && !c->multiphase) {
     c->F_r = h[13] / vol;
     c->F_e = h[14] / vol;
   }
@@ -2852,7 +2856,7 @@ int pa_commit_increment(pa_context *c, double row[15]) {
     for (int k = 0; k < 12; ++k) row[k] = h[k] / vol;
     row[12] = c->ntw ? c->F_r : 0.0;  // TwinRealVF (without twinning no twin system is ever active)
     row[13] = c->ntw ? c->F_e : 0.0;  // TwinMade
-    row[14] = h[12] / vol;
+    row[14] = c->multiphase ? 0.0 : h[12] / vol;
   }
   c->bc_inc = -1;
   for (int k = 0; k < 15; ++k)

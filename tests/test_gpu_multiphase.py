@@ -9,7 +9,9 @@ selects the material).
   active-set decisions make both depend on rounding);
 * at every increment every qp's committed state (T, s, Fp, backstress) equals the oracle's qp
   update of that qp's phase on the GPU's own inputs at 1e-10, up to qps across a round-off decision;
-* a two-phase context whose grains all take phase 1 equals the one-phase context bit for bit;
+* a two-phase context whose grains all take phase 1 equals the one-phase context bit for bit in
+  every stress and strain column and writes TwinRealVF, TwinMade and SlipTotal as 0, as the
+  reference does with Enable Multiphase (updateAfterIncrement.cc:718-725);
 * the reference's multiphase example (FCC, HCP with twinning, FCC with kinematic hardening) runs as
   the oracle's several-phase run does;
 * phases that disagree on the global local-loop keys, grains set before their phases and phase ids
@@ -97,19 +99,22 @@ def test_phases_match_the_oracle():
     ctx = pa.Context(_materials(case), _problem(case))
     ctx.set_grain_phases(case.grain_phase)
     ctx.set_grains(case.elem_grain, case.grain_rod)
-    rows, rounded = [], 0
+    rows, rounded, slipped = [], 0, []
     for inc in range(n_inc):
         Fp0, s0, W0 = ctx.get_state("Fp"), ctx.get_state("s"), ctx.get_state("W")
         rows.append(ctx.run_increment(inc)[0])
         rounded += _check_qps(case, ctx, ctx.get_state("u"), Fp0, s0, W0)
+        slipped.append(ctx.get_state("slip").max() > 0)
     rows = np.array(rows)
     e = np.abs(rows[:, :12] - ref["rows"][:, :12]).max(axis=1) / np.abs(ref["rows"][:, :12]).max(axis=1)
     parted = np.nonzero(e > 1e-9)[0]
     n_same = int(parted[0]) if len(parted) else n_inc
-    first_plastic = int(np.nonzero(ref["rows"][:, 14] > 0)[0][0])
+    # SlipTotal reads 0 with several phases: first yield from the slip state
+    first_plastic = int(np.argmax(slipped)) if any(slipped) else n_inc
     print("equal to the oracle (%.1e) through increment %d (first yield at %d); %d qp updates across a round-off decision"
           % (e[:n_same].max(), n_same - 1, first_plastic, rounded), flush=True)
-    assert rows[-1, 14] > 0
+    assert slipped[-1]
+    assert not rows[:, 12:15].any() and not ref["rows"][:, 12:15].any()
     assert n_same >= min(n_inc, first_plastic + 2)
     if n_same == n_inc:
         st = ref["state"]
@@ -135,7 +140,8 @@ def test_two_phase_context_with_one_phase_in_use_equals_the_one_phase_context():
         ctx.close()
     (r1, T1, s1), (r2, T2, s2) = out
     assert r1[-1, 14] > 0
-    assert np.array_equal(r1, r2) and np.array_equal(T1, T2) and np.array_equal(s1, s2)
+    assert np.array_equal(r1[:, :12], r2[:, :12]) and np.array_equal(T1, T2) and np.array_equal(s1, s2)
+    assert not r2[:, 12:15].any()  # Enable Multiphase zeroes them (updateAfterIncrement.cc:718-725)
 
 
 def test_bad_phase_inputs_are_refused():
@@ -166,18 +172,22 @@ def test_reference_multiphase_example_matches_the_oracle():
     ctx = pa.Context(_materials(case), _problem(case))
     ctx.set_grain_phases(case.grain_phase)
     ctx.set_grains(case.elem_grain, case.grain_rod)
-    rows = [ctx.run_increment(i)[0] for i in range(n_inc - 1)]
-    Fp0, s0, W0 = ctx.get_state("Fp"), ctx.get_state("s"), ctx.get_state("W")
-    rows = np.array(rows + [ctx.run_increment(n_inc - 1)[0]])
+    rows, slipped = [], []
+    for i in range(n_inc):
+        if i == n_inc - 1:
+            Fp0, s0, W0 = ctx.get_state("Fp"), ctx.get_state("s"), ctx.get_state("W")
+        rows.append(ctx.run_increment(i)[0])
+        slipped.append(ctx.get_state("slip").max() > 0)
+    rows = np.array(rows)
     e = np.abs(rows[:, :12] - ref["rows"][:, :12]).max(axis=1) / np.abs(ref["rows"][:, :12]).max(axis=1)
     parted = np.nonzero(e > 1e-9)[0]
     n_same = int(parted[0]) if len(parted) else n_inc
-    plastic = np.nonzero(ref["rows"][:, 14] > 0)[0]
-    first_plastic = int(plastic[0]) if len(plastic) else n_inc
-    print("equal to the oracle (%.1e) through increment %d (first yield at %d); SlipTotal %.3e, TwinRealVF %.3e"
-          % (e[:n_same].max(), n_same - 1, first_plastic, rows[-1, 14], rows[-1, 12]), flush=True)
+    first_plastic = int(np.argmax(slipped)) if any(slipped) else n_inc
+    print("equal to the oracle (%.1e) through increment %d (first yield at %d); largest twin fraction %.3e"
+          % (e[:n_same].max(), n_same - 1, first_plastic, ctx.get_state("twin_fraction").max()), flush=True)
     assert first_plastic < n_inc and n_same >= min(n_inc, first_plastic + 2)
-    assert _rel(rows[:n_same, 12:15], ref["rows"][:n_same, 12:15]) < 1e-8
+    # TwinRealVF, TwinMade and SlipTotal read 0 with Enable Multiphase, as in the golden's 1,000 rows
+    assert not rows[:, 12:15].any() and not ref["rows"][:, 12:15].any()
     _check_qps(case, ctx, ctx.get_state("u"), Fp0, s0, W0)
     ctx.close()
 

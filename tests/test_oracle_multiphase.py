@@ -41,7 +41,9 @@ def test_one_phase_equals_or_simulate_bit_for_bit():
     mp = _phases(case, case)
     mp.grain_phase = np.ones(4, np.int32)
     b = refcase.oracle_simulate(mp, 3, nthreads=2, want_state=True)
-    assert np.array_equal(a["rows"], b["rows"]) and np.array_equal(a["state"], b["state"])
+    assert a["rows"][-1, 14] > 0
+    assert np.array_equal(a["rows"][:, :12], b["rows"][:, :12]) and np.array_equal(a["state"], b["state"])
+    assert not b["rows"][:, 12:15].any()  # Enable Multiphase zeroes F_r, F_e, F_s (updateAfterIncrement.cc:718-725)
     assert np.array_equal(a["lin_iters"], b["lin_iters"])
 
 
@@ -55,7 +57,7 @@ def test_the_phase_selects_the_material():
     ref_case = refcase.oracle_simulate(case, 3, nthreads=2)["rows"]
     mp = _phases(case, other)
     mp.grain_phase = np.full(4, 2, np.int32)
-    assert np.array_equal(refcase.oracle_simulate(mp, 3, nthreads=2)["rows"], ref_other)
+    assert np.array_equal(refcase.oracle_simulate(mp, 3, nthreads=2)["rows"][:, :12], ref_other[:, :12])
     mp.grain_phase = np.array([1, 2, 1, 2], np.int32)
     mixed = refcase.oracle_simulate(mp, 3, nthreads=2)["rows"]
     lo, hi = np.minimum(ref_case[:, 6], ref_other[:, 6]), np.maximum(ref_case[:, 6], ref_other[:, 6])
@@ -76,3 +78,19 @@ def test_bad_phase_ids_are_refused():
     rc = lib.or_simulate_phases(mats, ctypes.c_int(2), refcase.ptr(gp), ctypes.byref(pb), refcase.ptr(eg), refcase.ptr(rod),
                                 ctypes.c_int(4), refcase.ptr(rows), None, None, None, None)
     assert rc == -1
+
+
+def test_multiphase_rows_write_zero_twin_and_slip_columns():
+    """With Enable Multiphase the reference sets F_r, F_e and F_s to zero after every increment
+    (updateAfterIncrement.cc:718-725): its golden for fcc/simpleCompression_Multiphase writes
+    TwinRealVF, TwinMade and SlipTotal as 0 in all 1,000 rows although the run is deep in plastic
+    flow by the end, and so does the oracle's several-phase run past its first yield."""
+    import os
+    g = np.loadtxt(os.path.join(refcase.GOLDEN, "fcc_simpleCompression_Multiphase", "correct_stressstrain.txt"), skiprows=1)
+    assert len(g) == 1000 and not g[:, 12:15].any()
+    assert abs(g[-1, 8]) < 0.2 * 1000 * abs(g[0, 8])  # far below the elastic line: plastic
+    case = refcase.load_reference_case("fcc_simpleCompression_Multiphase", twinning=True)
+    out = refcase.oracle_simulate(case, 30, nthreads=8, want_state=True)
+    Fp = out["state"][:, 9:18].reshape(-1, 3, 3)
+    assert np.abs(Fp - np.eye(3)).max() > 1e-6  # plastic flow at some qps
+    assert not out["rows"][:, 12:15].any()
