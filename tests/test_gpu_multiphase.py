@@ -213,3 +213,34 @@ def test_prm_driven_multiphase_run_equals_the_context_run(tmp_path):
     ctx.close()
     assert _rel(rows, rows2) < 1e-12
 
+
+
+def test_no_reorientation_by_twinning_with_several_phases():
+    """hcp/simpleTension with a Twin Threshold Fraction of 1e-6, 42 increments: the one-phase
+    context reorients qps by twinning; the same material as a two-phase context (every grain in
+    phase 1) reorients none, as the reference does with Enable Multiphase (F_r stays zero,
+    updateAfterIncrement.cc:718-725), and the two runs are equal until the one-phase run's first
+    reorientation."""
+    from types import SimpleNamespace
+    case = refcase.load_reference_case("hcp_simpleTension", twinning=True)
+    case.twin = dict(case.twin)
+    case.twin["threshold"], case.twin["saturation"] = 1e-6, 0.0
+    n = 42
+    out = []
+    for phases in ([case], [case, case]):
+        mats = _materials(SimpleNamespace(phases=phases))
+        ctx = pa.Context(mats if len(phases) > 1 else mats[0], _problem(case))
+        if len(phases) > 1:
+            ctx.set_grain_phases(np.ones(len(case.grain_rod), np.int32))
+        ctx.set_grains(case.elem_grain, case.grain_rod)
+        rows = np.array([ctx.run_increment(i)[0] for i in range(n)])
+        out.append((rows, int(ctx.get_state("twinned").sum()), float(ctx.get_state("twin_fraction").max())))
+        ctx.close()
+    (r1, tw1, _), (r2, tw2, f2) = out
+    e = np.abs(r1[:, :12] - r2[:, :12]).max(axis=1) / np.abs(r1[:, :12]).max(axis=1)
+    parted = np.nonzero(e > 1e-12)[0]
+    print("one phase: %d qps reoriented; two phases: %d (largest twin fraction %.2e); rows part at increment %s"
+          % (tw1, tw2, f2, parted[:1]), flush=True)
+    assert tw1 > 0 and tw2 == 0 and f2 > 1e-4
+    assert not r2[:, 12:15].any()
+    assert len(parted) and parted[0] >= 38  # equal until the one-phase run's first reorientation

@@ -94,3 +94,24 @@ def test_multiphase_rows_write_zero_twin_and_slip_columns():
     Fp = out["state"][:, 9:18].reshape(-1, 3, 3)
     assert np.abs(Fp - np.eye(3)).max() > 1e-6  # plastic flow at some qps
     assert not out["rows"][:, 12:15].any()
+
+
+def test_no_reorientation_by_twinning_with_several_phases():
+    """hcp/simpleTension with a Twin Threshold Fraction of 1e-6: the one-phase run reorients qps by
+    twinning within 42 increments; the same material as a several-phase run (every grain in phase
+    1) reorients none, although its twin fractions pass the threshold, because F_r stays zero
+    (updateAfterIncrement.cc:718-725, calculatePlasticity.cc:991-993).  Both runs are equal up to
+    the one-phase run's first reorientation."""
+    case = refcase.load_reference_case("hcp_simpleTension", twinning=True)
+    case.twin = dict(case.twin)
+    case.twin["threshold"], case.twin["saturation"] = 1e-6, 0.0
+    n = 42
+    col = 33 + case.nT + case.n_slip  # twin_conv in the state rows
+    one = refcase.oracle_simulate(case, n, nthreads=8, want_state=True)
+    mp = copy.copy(case)
+    mp.phases, mp.grain_phase = [case, case], np.ones(len(case.grain_rod), np.int32)
+    several = refcase.oracle_simulate(mp, n, nthreads=8, want_state=True)
+    assert one["state"][:, col].sum() > 0
+    assert several["state"][:, col].sum() == 0 and several["state"][:, col + 1:].max() > 1e-4
+    parted = np.nonzero(np.abs(one["rows"][:, :12] - several["rows"][:, :12]).max(axis=1) > 0)[0]
+    assert len(parted) and parted[0] >= 38  # the one-phase run twins from increment 38
